@@ -1,0 +1,150 @@
+/*
+ * pascal_btdma_amd: C ABI of the MI355X (gfx950) batched block-tridiagonal solver.
+ *
+ * Drop-in for the many-system plan-create / solve entry points of the reference:
+ *   CUDA C++ port   CUDA_CXX_Port/include/pascal_btdma_cuda.hpp:88-166
+ *   CUDA Fortran    Fortran_Original/src/mod_btdma_gpu_v2.f90:70-239
+ * One GPU, no MPI: the reference's communicator of nprocs row partitions becomes
+ * `npart` row partitions of every system held in the same HBM, and the reduced system
+ * of 2*npart rows is solved on the device (no all-to-all).
+ *
+ * Conventions (same as the reference):
+ *  - Row q of system s: A_q x_{q-1} + B_q x_q + C_q x_{q+1} = d_q; A of the first row and
+ *    C of the last row are ignored.
+ *  - Device arrays, solved in place: D returns x; A, C and D are overwritten (A only
+ *    when npart > 1); B is only read.
+ *  - Blocks are column-major: element (p,q) of a block is e = p + m*q.
+ *  - Layouts (element e of block row `row` of system `sys`):
+ *      BTD_LAYOUT_REF_GPU  sys + nsys*(row + nrow*e)   (pascal_btdma_cuda.hpp:46-65)
+ *      BTD_LAYOUT_REF_CPU  e + E*(sys + nsys*row)      (mod_btdma_cpu.f90:18; conversion only)
+ *      BTD_LAYOUT_TILED    ((tile*nrow + row)*E + e)*64 + lane, sys = 64*tile + lane,
+ *                          nsys padded up to a multiple of 64 (this library's native form)
+ *    with E = m*m for blocks and E = m for vectors.
+ *  - Every call is asynchronous on the given HIP stream (NULL = default stream) unless
+ *    stated otherwise. A plan holds device scratch and must not be used by two calls at
+ *    once.
+ * Errors are returned as btd_status; btd_last_error() gives a thread-local message.
+ */
+#ifndef PASCAL_BTDMA_AMD_H
+#define PASCAL_BTDMA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BTD_MAX_M 10
+
+typedef struct btd_plan_s *btd_plan_t;
+
+/* Status codes. The reference throws C++ exceptions; they map as noted. */
+typedef enum btd_status {
+    BTD_OK = 0,
+    BTD_ERR_INVALID_ARG = 1,     /* std::invalid_argument from create (pascal_btdma_cuda.cu:962-967) */
+    BTD_ERR_NOT_CREATED = 2,     /* std::runtime_error, uncreated plan (pascal_btdma_cuda.cu:865-867) */
+    BTD_ERR_DIM_MISMATCH = 3,    /* std::invalid_argument, dims != plan (pascal_btdma_cuda.cu:868-870) */
+    BTD_ERR_HIP = 4,             /* CudaError (pascal_btdma_cuda.hpp:20-22,43-44) */
+    BTD_ERR_SINGULAR = 5,        /* new: a zero pivot in some block LU (the reference ignores info) */
+    BTD_ERR_NOT_IMPLEMENTED = 6  /* std::logic_error from solve_cyclic (pascal_btdma_cuda.cu:1063-1075) */
+} btd_status;
+
+typedef enum btd_layout {
+    BTD_LAYOUT_REF_GPU = 0,
+    BTD_LAYOUT_REF_CPU = 1,
+    BTD_LAYOUT_TILED = 2
+} btd_layout;
+
+/* Phase times in seconds; the fields of BtdmaSolveTimings (pascal_btdma_cuda.hpp:24-39)
+ * and BTDMA_TIMING_gpu_v2 (mod_btdma_gpu_v2.f90:38-45). On one GPU the two exchange
+ * phases do not exist and read 0. Measured with HIP events on the solve stream. */
+typedef struct btd_timings {
+    double total;
+    double local_compute;
+    double forward_exchange;
+    double reduced_compute;
+    double backward_exchange;
+    double update_compute;
+} btd_timings;
+
+/* Replaces BtdmaGpuPlan::create(m, nsys, nrow, MPI_Comm, mode) (pascal_btdma_cuda.hpp:128-132,
+ * pascal_btdma_cuda.cu:956-1005) and btdma_makeplan_gpu_v2(plan, m, nsys, nrow_sub, comm)
+ * (mod_btdma_gpu_v2.f90:70-109). `npart` takes the place of the communicator size:
+ * npart = 1 is the single-partition solve; npart > 1 splits every system's rows with
+ * btd_partition_1d and needs nrow >= 3*npart. layout is BTD_LAYOUT_REF_GPU or
+ * BTD_LAYOUT_TILED. 1 <= m <= BTD_MAX_M, nsys >= 1, nrow >= 2. Allocates device scratch
+ * on the current device. */
+btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npart, int layout);
+
+/* Replaces BtdmaGpuPlan::destroy (pascal_btdma_cuda.hpp:134, pascal_btdma_cuda.cu:1007-1033)
+ * and btdma_cleanplan_gpu_v2 (mod_btdma_gpu_v2.f90:111-129). NULL is accepted. */
+btd_status btd_plan_destroy(btd_plan_t plan);
+
+/* Replaces solve_noncyclic(plan, a, b, c, d, m, nsys, nrow, stream)
+ * (pascal_btdma_cuda.hpp:137-145, pascal_btdma_cuda.cu:1035-1045) and
+ * btdma_many_mpi_gpu_v2(A,B,C,D,m,nsys,nrow_sub,plan) (mod_btdma_gpu_v2.f90:131-167).
+ * In place, asynchronous. Zero pivots are recorded in the plan; see btd_plan_check. */
+btd_status btd_solve_noncyclic(btd_plan_t plan, double *a, const double *b, double *c, double *d,
+                               int m, int nsys, int nrow, void *stream);
+
+/* Replaces solve_noncyclic_profiled (pascal_btdma_cuda.hpp:147-156, pascal_btdma_cuda.cu:1047-1061)
+ * and btdma_many_mpi_gpu_v2_profiled (mod_btdma_gpu_v2.f90:169-198). Synchronises the
+ * stream, fills *timings (may be NULL) and returns BTD_ERR_SINGULAR on a zero pivot. */
+btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double *a, const double *b, double *c,
+                                        double *d, int m, int nsys, int nrow, btd_timings *timings,
+                                        void *stream);
+
+/* Replaces solve_cyclic (pascal_btdma_cuda.hpp:158-166), which throws in the reference
+ * C++ port, and btdma_many_cycl_mpi_gpu_v2 (mod_btdma_gpu_v2.f90:200-239). Returns
+ * BTD_ERR_NOT_IMPLEMENTED, as the C++ port does. */
+btd_status btd_solve_cyclic(btd_plan_t plan, double *a, const double *b, double *c, double *d,
+                            int m, int nsys, int nrow, void *stream);
+
+/* New (the reference ignores LAPACK/elimination info): synchronises the plan's last
+ * stream use, stores 1 in *singular if any block LU met a zero pivot since the last
+ * check (0 otherwise), and clears the flag. */
+btd_status btd_plan_check(btd_plan_t plan, int *singular);
+
+/* Thread-local message for the last error on this thread ("" if none). */
+const char *btd_last_error(void);
+
+/* matrix_index / vector_index (pascal_btdma_cuda.hpp:46-65): offsets in the reference GPU
+ * layout. p is the block row, q the block column. */
+size_t btd_matrix_index(int sys, int row, int p, int q, int nsys, int nrow, int m);
+size_t btd_vector_index(int sys, int row, int p, int nsys, int nrow);
+
+/* partition_1d (pascal_btdma_cuda.cu:921-930) / mpiutil_para (mpiutil.f90:27-48). */
+void btd_partition_1d(int start, int end, int nprocs, int rank, int *first, int *last);
+
+/* Number of doubles an array occupies in a layout (E = m*m for blocks, m for vectors). */
+size_t btd_layout_elems(int layout, int nsys, int nrow, int elems_per_block);
+
+/* Layout conversion on the device (src and dst must not overlap). */
+btd_status btd_convert_layout(int src_layout, int dst_layout, int elems_per_block, int nsys,
+                              int nrow, const double *src, double *dst, void *stream);
+
+/* Seeded synthetic inputs written on the device in `layout`. kind: 0 = Study diagonal
+ * workload (A=0.25I, B=2I, C=0.25I, d=1), 1 = diagonally dominant rows with each whole
+ * block row permuted, 2 = Euler-like (J based, sigma 0.4), 3 = conditioned B blocks
+ * (cond log-uniform in [1e1,1e6]) with A, B, C rounded to FP32. Any pointer may be NULL. */
+btd_status btd_generate(int kind, uint64_t seed, int m, int nsys, int nrow, int layout, double *a,
+                        double *b, double *c, double *d, void *stream);
+
+/* Same values as btd_generate, rounded to FP32 for the A, B, C arrays (d stays FP64). */
+btd_status btd_generate_f32(int kind, uint64_t seed, int m, int nsys, int nrow, int layout,
+                            float *a, float *b, float *c, double *d, void *stream);
+
+/* Host copy of the generator for systems [sys0, sys0+count): natural row-major blocks,
+ * a[((row*count + s)*m + i)*m + j] = A_row(i,j) of system sys0+s; d[(row*count+s)*m + i].
+ * Bit-identical to btd_generate. Any pointer may be NULL. No device use. */
+btd_status btd_generate_host(int kind, uint64_t seed, int m, int nsys, int nrow, int sys0,
+                             int count, double *a, double *b, double *c, double *d);
+
+/* Library version string. */
+const char *btd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

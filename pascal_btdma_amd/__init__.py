@@ -1,0 +1,274 @@
+"""pascal_btdma_amd: batched block-tridiagonal (block Thomas) solver for one MI355X.
+
+Python mirror of the reference's C++ interface (CUDA_CXX_Port/include/pascal_btdma_cuda.hpp):
+``BtdmaGpuPlan.create / destroy``, ``solve_noncyclic``, ``solve_noncyclic_profiled``,
+``solve_cyclic``, ``BtdmaSolveTimings``, ``matrix_index``, ``vector_index`` and
+``partition_1d`` keep the reference's names, argument meaning and error behaviour
+(``ValueError`` where the reference throws ``std::invalid_argument``, ``RuntimeError``
+for ``std::runtime_error``, ``NotImplementedError`` for ``std::logic_error``).
+Arrays are torch CUDA tensors (float64) in device memory; all work runs in the HIP
+library through its C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+from dataclasses import dataclass
+
+from . import _capi
+
+__all__ = [
+    "Layout", "GenKind", "BtdmaGpuPlan", "BtdmaSolveTimings", "BtdmaError", "HipError",
+    "SingularError", "solve_noncyclic", "solve_noncyclic_profiled", "solve_cyclic",
+    "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
+    "generate", "generate_f32", "generate_host", "kMaxBlockSize",
+]
+
+kMaxBlockSize = 10
+
+
+class Layout(enum.IntEnum):
+    REF_GPU = 0   # sys + nsys*(row + nrow*e)   (pascal_btdma_cuda.hpp:46-65)
+    REF_CPU = 1   # e + E*(sys + nsys*row)      (mod_btdma_cpu.f90:18)
+    TILED = 2     # ((tile*nrow + row)*E + e)*64 + lane
+
+
+class GenKind(enum.IntEnum):
+    DIAG = 0
+    DDPERM = 1
+    EULER = 2
+    COND = 3
+
+
+class BtdmaError(RuntimeError):
+    pass
+
+
+class HipError(BtdmaError):
+    """A HIP runtime call failed (the reference's CudaError)."""
+
+
+class SingularError(BtdmaError, ArithmeticError):
+    """A block LU met a zero pivot."""
+
+
+_OK, _INVALID, _NOT_CREATED, _DIM, _HIP, _SINGULAR, _NOT_IMPL = range(7)
+
+
+def _check(status: int) -> None:
+    if status == _OK:
+        return
+    msg = _capi.last_error()
+    if status in (_INVALID, _DIM):
+        raise ValueError(msg)
+    if status == _NOT_CREATED:
+        raise RuntimeError(msg)
+    if status == _HIP:
+        raise HipError(msg)
+    if status == _SINGULAR:
+        raise SingularError(msg)
+    if status == _NOT_IMPL:
+        raise NotImplementedError(msg)
+    raise BtdmaError(f"status {status}: {msg}")
+
+
+@dataclass
+class BtdmaSolveTimings:
+    """Phase times in seconds (pascal_btdma_cuda.hpp:24-39)."""
+    total: float = 0.0
+    local_compute: float = 0.0
+    forward_exchange: float = 0.0
+    reduced_compute: float = 0.0
+    backward_exchange: float = 0.0
+    update_compute: float = 0.0
+
+    def computation(self) -> float:
+        return self.local_compute + self.reduced_compute + self.update_compute
+
+    def communication(self) -> float:
+        return self.forward_exchange + self.backward_exchange
+
+
+def _ptr(t) -> int:
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def _stream(stream) -> int | None:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _check_tensor(name, t, nelem, dtype):
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device (cuda) tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() < nelem:
+        raise ValueError(f"{name} holds {t.numel()} elements, the layout needs {nelem}")
+
+
+class BtdmaGpuPlan:
+    """Plan for a batch of nsys systems of nrow block rows (pascal_btdma_cuda.hpp:88-135).
+
+    ``npart`` replaces the reference's MPI communicator: the rows of every system are
+    split into npart partitions (partition_1d) solved by the two-stage method on the
+    same GPU. Move-only semantics become: one plan object, explicit destroy, and
+    ``with`` support.
+    """
+
+    def __init__(self):
+        self._h = ctypes.c_void_p(None)
+        self.m = self.nsys = self.nrow = 0
+        self.npart = 1
+        self.layout = Layout.REF_GPU
+        self.created = False
+
+    def create(self, m: int, nsys: int, nrow: int, npart: int = 1,
+               layout: Layout = Layout.REF_GPU) -> "BtdmaGpuPlan":
+        self.destroy()
+        h = ctypes.c_void_p(None)
+        _check(_capi.lib().btd_plan_create(ctypes.byref(h), int(m), int(nsys), int(nrow),
+                                           int(npart), int(layout)))
+        self._h = h
+        self.m, self.nsys, self.nrow, self.npart = int(m), int(nsys), int(nrow), int(npart)
+        self.layout = Layout(layout)
+        self.created = True
+        return self
+
+    def destroy(self) -> None:
+        if self._h.value:
+            _capi.lib().btd_plan_destroy(self._h)
+        self._h = ctypes.c_void_p(None)
+        self.created = False
+
+    def check(self) -> bool:
+        """Synchronise; True if any block LU met a zero pivot since the last check."""
+        s = ctypes.c_int(0)
+        _check(_capi.lib().btd_plan_check(self._h, ctypes.byref(s)))
+        return bool(s.value)
+
+    def matrix_elems(self) -> int:
+        return layout_elems(self.layout, self.nsys, self.nrow, self.m * self.m)
+
+    def vector_elems(self) -> int:
+        return layout_elems(self.layout, self.nsys, self.nrow, self.m)
+
+    @property
+    def handle(self) -> int:
+        return self._h.value or 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def _validate(plan, a, b, c, d, m, nsys, nrow):
+    import torch
+    if plan.created and (m, nsys, nrow) == (plan.m, plan.nsys, plan.nrow):
+        nm, nv = plan.matrix_elems(), plan.vector_elems()
+        for name, t, n in (("a", a, nm), ("b", b, nm), ("c", c, nm), ("d", d, nv)):
+            _check_tensor(name, t, n, torch.float64)
+
+
+def solve_noncyclic(plan: BtdmaGpuPlan, a, b, c, d, m: int, nsys: int, nrow: int,
+                    stream=None) -> None:
+    """In-place solve; x is returned in d (pascal_btdma_cuda.hpp:137-145)."""
+    _validate(plan, a, b, c, d, m, nsys, nrow)
+    _check(_capi.lib().btd_solve_noncyclic(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
+                                           int(m), int(nsys), int(nrow), _stream(stream)))
+
+
+def solve_noncyclic_profiled(plan: BtdmaGpuPlan, a, b, c, d, m: int, nsys: int, nrow: int,
+                             timings: BtdmaSolveTimings | None = None,
+                             stream=None) -> BtdmaSolveTimings:
+    """Solve and fill the six phase timings (pascal_btdma_cuda.hpp:147-156)."""
+    _validate(plan, a, b, c, d, m, nsys, nrow)
+    t = _capi.BtdTimings()
+    _check(_capi.lib().btd_solve_noncyclic_profiled(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
+                                                    int(m), int(nsys), int(nrow),
+                                                    ctypes.byref(t), _stream(stream)))
+    out = timings if timings is not None else BtdmaSolveTimings()
+    for f in ("total", "local_compute", "forward_exchange", "reduced_compute",
+              "backward_exchange", "update_compute"):
+        setattr(out, f, getattr(t, f))
+    return out
+
+
+def solve_cyclic(plan: BtdmaGpuPlan, a, b, c, d, m: int, nsys: int, nrow: int,
+                 stream=None) -> None:
+    """Periodic solve (pascal_btdma_cuda.hpp:158-166)."""
+    _check(_capi.lib().btd_solve_cyclic(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d), int(m),
+                                        int(nsys), int(nrow), _stream(stream)))
+
+
+def matrix_index(sys: int, row: int, p: int, q: int, nsys: int, nrow: int, m: int) -> int:
+    return int(_capi.lib().btd_matrix_index(sys, row, p, q, nsys, nrow, m))
+
+
+def vector_index(sys: int, row: int, p: int, nsys: int, nrow: int) -> int:
+    return int(_capi.lib().btd_vector_index(sys, row, p, nsys, nrow))
+
+
+def partition_1d(start: int, end: int, nprocs: int, rank: int) -> tuple[int, int]:
+    f, l = ctypes.c_int(), ctypes.c_int()
+    _capi.lib().btd_partition_1d(start, end, nprocs, rank, ctypes.byref(f), ctypes.byref(l))
+    return f.value, l.value
+
+
+def layout_elems(layout: Layout, nsys: int, nrow: int, elems_per_block: int) -> int:
+    return int(_capi.lib().btd_layout_elems(int(layout), nsys, nrow, elems_per_block))
+
+
+def convert_layout(src_layout: Layout, dst_layout: Layout, elems_per_block: int, nsys: int,
+                   nrow: int, src, dst, stream=None) -> None:
+    _check(_capi.lib().btd_convert_layout(int(src_layout), int(dst_layout), elems_per_block,
+                                          nsys, nrow, _ptr(src), _ptr(dst), _stream(stream)))
+
+
+def generate(kind: GenKind, seed: int, m: int, nsys: int, nrow: int, layout: Layout,
+             a=None, b=None, c=None, d=None, stream=None) -> None:
+    _check(_capi.lib().btd_generate(int(kind), int(seed), m, nsys, nrow, int(layout), _ptr(a),
+                                    _ptr(b), _ptr(c), _ptr(d), _stream(stream)))
+
+
+def generate_f32(kind: GenKind, seed: int, m: int, nsys: int, nrow: int, layout: Layout,
+                 a=None, b=None, c=None, d=None, stream=None) -> None:
+    _check(_capi.lib().btd_generate_f32(int(kind), int(seed), m, nsys, nrow, int(layout),
+                                        _ptr(a), _ptr(b), _ptr(c), _ptr(d), _stream(stream)))
+
+
+def generate_host(kind: GenKind, seed: int, m: int, nsys: int, nrow: int, sys0: int = 0,
+                  count: int | None = None):
+    """Host copy of the generator: numpy arrays A, B, C (nrow, count, m, m) natural
+    row-major blocks and d (nrow, count, m), bit-identical to the device generator."""
+    import numpy as np
+    count = nsys - sys0 if count is None else count
+    A = np.empty((nrow, count, m, m))
+    B = np.empty_like(A)
+    C = np.empty_like(A)
+    d = np.empty((nrow, count, m))
+    _check(_capi.lib().btd_generate_host(int(kind), int(seed), m, nsys, nrow, sys0, count,
+                                         A.ctypes.data, B.ctypes.data, C.ctypes.data,
+                                         d.ctypes.data))
+    return A, B, C, d

@@ -1,0 +1,71 @@
+"""ctypes binding of libpascal_btdma_amd.so (declared in include/pascal_btdma_amd.h).
+
+This is the binding a Python caller of the reference's C++ interface would add: the
+library's entry points take plain device pointers and sizes, so torch tensors are passed
+by data_ptr() and HIP streams by their handle. Loading fails loudly when the library has
+not been built; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpascal_btdma_amd.so")
+
+# Every symbol include/pascal_btdma_amd.h declares, with (restype, argtypes).
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_u64 = ctypes.c_uint64
+_dp = ctypes.c_void_p  # device or host double* passed as an address
+
+
+class BtdTimings(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "total", "local_compute", "forward_exchange", "reduced_compute",
+        "backward_exchange", "update_compute")]
+
+
+SIGNATURES = {
+    "btd_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _i]),
+    "btd_plan_destroy": (_i, [_vp]),
+    "btd_solve_noncyclic": (_i, [_vp, _dp, _dp, _dp, _dp, _i, _i, _i, _vp]),
+    "btd_solve_noncyclic_profiled": (_i, [_vp, _dp, _dp, _dp, _dp, _i, _i, _i,
+                                          ctypes.POINTER(BtdTimings), _vp]),
+    "btd_solve_cyclic": (_i, [_vp, _dp, _dp, _dp, _dp, _i, _i, _i, _vp]),
+    "btd_plan_check": (_i, [_vp, ctypes.POINTER(_i)]),
+    "btd_last_error": (ctypes.c_char_p, []),
+    "btd_matrix_index": (ctypes.c_size_t, [_i, _i, _i, _i, _i, _i, _i]),
+    "btd_vector_index": (ctypes.c_size_t, [_i, _i, _i, _i, _i]),
+    "btd_partition_1d": (None, [_i, _i, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "btd_layout_elems": (ctypes.c_size_t, [_i, _i, _i, _i]),
+    "btd_convert_layout": (_i, [_i, _i, _i, _i, _i, _dp, _dp, _vp]),
+    "btd_generate": (_i, [_i, _u64, _i, _i, _i, _i, _dp, _dp, _dp, _dp, _vp]),
+    "btd_generate_f32": (_i, [_i, _u64, _i, _i, _i, _i, _dp, _dp, _dp, _dp, _vp]),
+    "btd_generate_host": (_i, [_i, _u64, _i, _i, _i, _i, _i, _dp, _dp, _dp, _dp]),
+    "btd_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the library (once) and attach the declared signatures."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m pascal_btdma_amd.build` "
+                "(the HIP library is required; there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().btd_last_error()
+    return msg.decode() if msg else ""

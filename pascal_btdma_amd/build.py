@@ -1,0 +1,74 @@
+"""Builds libpascal_btdma_amd.so in-tree with hipcc for gfx950.
+
+Every HIP translation unit under csrc/ is compiled to an object in parallel, then the
+objects are linked into one shared library next to this file. Usage:
+    python -m pascal_btdma_amd.build [--verbose] [--resource-usage]
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libpascal_btdma_amd.so")
+ARCH = os.environ.get("BTD_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+# -packed-fp32-ops: keep v_pk_{add,mul,fma}_f32 out of the generated code.
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+
+# (source, extra defines, object name): the kernel templates are instantiated once per
+# block size m so the instantiations compile in parallel.
+UNITS = [("btd_inst.hip", [f"-DBTD_M={m}"], f"btd_inst_m{m}.o") for m in range(1, 11)] + [
+    ("btd_dispatch.hip", [], "btd_dispatch.o"),
+    ("btd_capi.hip", [], "btd_capi.o"),
+]
+
+
+def _compile(unit, extra: list[str], verbose: bool) -> tuple[str, float, str]:
+    src, defs, objname = unit
+    obj = os.path.join(BUILD, objname)
+    cmd = [HIPCC, *COMMON, *defs, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+    t0 = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    out = (r.stdout + r.stderr) if verbose else ""
+    return obj, time.time() - t0, out
+
+
+def build(verbose: bool = False, resource_usage: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
+    units = list(reversed(UNITS[:10])) + UNITS[10:]  # largest m first
+    with cf.ThreadPoolExecutor(max_workers=int(os.environ.get("BTD_BUILD_JOBS", "8"))) as ex:
+        results = list(ex.map(lambda u: _compile(u, extra, verbose or resource_usage), units))
+    objs = [r[0] for r in results]
+    for unit, (_, dt, out) in zip(units, results):
+        if verbose:
+            print(f"[build] {unit[2]}: {dt:.1f}s")
+        if out:
+            print(out)
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="--verbose" in sys.argv, resource_usage="--resource-usage" in sys.argv)
+    print(LIB)

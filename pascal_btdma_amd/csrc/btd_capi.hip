@@ -1,0 +1,319 @@
+// Host side of the C ABI: plan lifetime, argument checks, dispatch and timing.
+// The reference's equivalents are BtdmaGpuPlan::create/destroy and solve_noncyclic_impl
+// (CUDA_CXX_Port/src/pascal_btdma_cuda.cu:853-1075).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pascal_btdma_amd.h"
+#include "btd_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+btd_status fail(btd_status s, const std::string& msg) {
+    g_err = msg;
+    return s;
+}
+
+btd_status hip_fail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return BTD_ERR_HIP;
+}
+
+#define BTD_HIP(expr)                                  \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+}  // namespace
+
+struct btd_plan_s {
+    int m = 0, nsys = 0, nrow = 0, npart = 1, layout = BTD_LAYOUT_TILED;
+    int device = 0;
+    bool created = false;
+    std::vector<int> first, len;  // partition rows (host)
+    int* d_part = nullptr;        // [first..., len...] on the device
+    double* redA = nullptr;       // reduced system (tiled, 2*npart rows); B = I is implicit
+    double* redC = nullptr;
+    double* redD = nullptr;
+    int* d_flag = nullptr;  // zero-pivot flag
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipStream_t last_stream = nullptr;
+};
+
+static void plan_free(btd_plan_s* p) {
+    if (!p) return;
+    if (p->d_part) (void)hipFree(p->d_part);
+    if (p->redA) (void)hipFree(p->redA);
+    if (p->redC) (void)hipFree(p->redC);
+    if (p->redD) (void)hipFree(p->redD);
+    if (p->d_flag) (void)hipFree(p->d_flag);
+    for (auto& e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete p;
+}
+
+extern "C" {
+
+const char* btd_version(void) { return "pascal_btdma_amd 0.1 (gfx950)"; }
+
+const char* btd_last_error(void) { return g_err.c_str(); }
+
+size_t btd_matrix_index(int sys, int row, int p, int q, int nsys, int nrow, int m) {
+    return (size_t)sys + (size_t)nsys * ((size_t)row + (size_t)nrow * ((size_t)p + (size_t)m * q));
+}
+
+size_t btd_vector_index(int sys, int row, int p, int nsys, int nrow) {
+    return (size_t)sys + (size_t)nsys * ((size_t)row + (size_t)nrow * p);
+}
+
+void btd_partition_1d(int start, int end, int nprocs, int rank, int* first, int* last) {
+    const int n = end - start + 1;
+    const int base = n / nprocs, rem = n % nprocs;
+    *first = rank * base + start + std::min(rank, rem);
+    *last = *first + base - 1 + (rem > rank ? 1 : 0);
+}
+
+size_t btd_layout_elems(int layout, int nsys, int nrow, int elems_per_block) {
+    return (size_t)btd::layout_elems(layout, nsys, nrow, elems_per_block);
+}
+
+btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npart, int layout) {
+    g_err.clear();
+    if (!plan) return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: plan pointer is NULL");
+    *plan = nullptr;
+    if (m < 1 || m > BTD_MAX_M)
+        return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: requires 1 <= m <= 10");
+    if (nsys < 1 || nrow < 2)
+        return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: requires nsys > 0 and nrow >= 2");
+    if (npart < 1) return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: requires npart >= 1");
+    if (npart > 1 && nrow < 3 * npart)
+        return fail(BTD_ERR_INVALID_ARG,
+                    "btd_plan_create: the two-stage solve needs at least 3 rows per partition "
+                    "(nrow >= 3*npart)");
+    if (layout != BTD_LAYOUT_REF_GPU && layout != BTD_LAYOUT_TILED)
+        return fail(BTD_ERR_INVALID_ARG,
+                    "btd_plan_create: layout must be BTD_LAYOUT_REF_GPU or BTD_LAYOUT_TILED "
+                    "(convert CPU-layout arrays with btd_convert_layout)");
+    auto* p = new btd_plan_s;
+    p->m = m;
+    p->nsys = nsys;
+    p->nrow = nrow;
+    p->npart = npart;
+    p->layout = layout;
+    hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipMalloc(&p->d_flag, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(p->d_flag, 0, sizeof(int));
+    for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[i]);
+    if (e == hipSuccess && npart > 1) {
+        p->first.resize(npart);
+        p->len.resize(npart);
+        for (int r = 0; r < npart; ++r) {
+            int f, l;
+            btd_partition_1d(0, nrow - 1, npart, r, &f, &l);
+            p->first[r] = f;
+            p->len[r] = l - f + 1;
+        }
+        std::vector<int> tab(p->first);
+        tab.insert(tab.end(), p->len.begin(), p->len.end());
+        e = hipMalloc(&p->d_part, sizeof(int) * tab.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_part, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice);
+        const size_t tiles = (size_t)btd::pad_nsys(nsys);
+        const size_t rm = tiles * 2 * npart * m * m, rv = tiles * 2 * npart * m;
+        if (e == hipSuccess) e = hipMalloc(&p->redA, rm * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&p->redC, rm * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&p->redD, rv * sizeof(double));
+    }
+    if (e != hipSuccess) {
+        plan_free(p);
+        return hip_fail(e, "btd_plan_create");
+    }
+    p->created = true;
+    *plan = p;
+    return BTD_OK;
+}
+
+btd_status btd_plan_destroy(btd_plan_t plan) {
+    g_err.clear();
+    plan_free(plan);
+    return BTD_OK;
+}
+
+static btd_status check_solve(btd_plan_t plan, const void* a, const void* b, const void* c,
+                              const void* d, int m, int nsys, int nrow) {
+    if (!plan || !plan->created)
+        return fail(BTD_ERR_NOT_CREATED, "solve called with an uncreated plan");
+    if (m != plan->m || nsys != plan->nsys || nrow != plan->nrow)
+        return fail(BTD_ERR_DIM_MISMATCH, "solve dimensions do not match the plan");
+    if (!a || !b || !c || !d) return fail(BTD_ERR_INVALID_ARG, "solve: NULL array");
+    return BTD_OK;
+}
+
+static btd_status solve_impl(btd_plan_t plan, double* a, const double* b, double* c, double* d,
+                             hipStream_t s, bool timed) {
+    plan->last_stream = s;
+    if (plan->npart == 1) {
+        if (timed) BTD_HIP(hipEventRecord(plan->ev[0], s));
+        BTD_HIP(btd::launch_plain(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
+                                  plan->d_flag, s));
+        if (timed) BTD_HIP(hipEventRecord(plan->ev[1], s));
+    } else {
+        BTD_HIP(btd::launch_two_stage(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
+                                      plan->d_part, plan->d_part + plan->npart, plan->npart,
+                                      plan->redA, plan->redC, plan->redD, plan->d_flag, s,
+                                      timed ? plan->ev : nullptr));
+    }
+    return BTD_OK;
+}
+
+btd_status btd_solve_noncyclic(btd_plan_t plan, double* a, const double* b, double* c, double* d,
+                               int m, int nsys, int nrow, void* stream) {
+    g_err.clear();
+    btd_status st = check_solve(plan, a, b, c, d, m, nsys, nrow);
+    if (st != BTD_OK) return st;
+    return solve_impl(plan, a, b, c, d, (hipStream_t)stream, false);
+}
+
+static double ev_seconds(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms * 1e-3;
+}
+
+btd_status btd_plan_check(btd_plan_t plan, int* singular) {
+    g_err.clear();
+    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_plan_check: uncreated plan");
+    BTD_HIP(hipStreamSynchronize(plan->last_stream));
+    int h = 0;
+    BTD_HIP(hipMemcpy(&h, plan->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+    BTD_HIP(hipMemset(plan->d_flag, 0, sizeof(int)));
+    if (singular) *singular = h ? 1 : 0;
+    return BTD_OK;
+}
+
+btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double* a, const double* b, double* c,
+                                        double* d, int m, int nsys, int nrow, btd_timings* t,
+                                        void* stream) {
+    g_err.clear();
+    btd_status st = check_solve(plan, a, b, c, d, m, nsys, nrow);
+    if (st != BTD_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    st = solve_impl(plan, a, b, c, d, s, true);
+    if (st != BTD_OK) return st;
+    const int last = plan->npart == 1 ? 1 : 3;
+    BTD_HIP(hipEventSynchronize(plan->ev[last]));
+    if (t) {
+        *t = btd_timings{};
+        if (plan->npart == 1) {
+            t->local_compute = ev_seconds(plan->ev[0], plan->ev[1]);
+        } else {
+            t->local_compute = ev_seconds(plan->ev[0], plan->ev[1]);
+            t->reduced_compute = ev_seconds(plan->ev[1], plan->ev[2]);
+            t->update_compute = ev_seconds(plan->ev[2], plan->ev[3]);
+        }
+        t->total = ev_seconds(plan->ev[0], plan->ev[last]);
+    }
+    int sing = 0;
+    st = btd_plan_check(plan, &sing);
+    if (st != BTD_OK) return st;
+    if (sing) return fail(BTD_ERR_SINGULAR, "a block LU met a zero pivot");
+    return BTD_OK;
+}
+
+btd_status btd_solve_cyclic(btd_plan_t plan, double* a, const double* b, double* c, double* d,
+                            int m, int nsys, int nrow, void* stream) {
+    (void)plan;
+    (void)a;
+    (void)b;
+    (void)c;
+    (void)d;
+    (void)m;
+    (void)nsys;
+    (void)nrow;
+    (void)stream;
+    return fail(BTD_ERR_NOT_IMPLEMENTED, "btd_solve_cyclic is not implemented");
+}
+
+btd_status btd_convert_layout(int src_layout, int dst_layout, int E, int nsys, int nrow,
+                              const double* src, double* dst, void* stream) {
+    g_err.clear();
+    if (E < 1 || nsys < 1 || nrow < 1 || !src || !dst || src_layout < 0 || src_layout > 2 ||
+        dst_layout < 0 || dst_layout > 2)
+        return fail(BTD_ERR_INVALID_ARG, "btd_convert_layout: bad argument");
+    BTD_HIP(btd::launch_convert(src_layout, dst_layout, E, nsys, nrow, src, dst, (hipStream_t)stream));
+    return BTD_OK;
+}
+
+btd_status btd_generate(int kind, uint64_t seed, int m, int nsys, int nrow, int layout, double* a,
+                        double* b, double* c, double* d, void* stream) {
+    g_err.clear();
+    if (kind < 0 || kind > 3 || m < 1 || m > BTD_MAX_M || nsys < 1 || nrow < 1 || layout < 0 ||
+        layout > 2)
+        return fail(BTD_ERR_INVALID_ARG, "btd_generate: bad argument");
+    BTD_HIP(btd::launch_generate(m, kind, seed, layout, nsys, nrow, a, b, c, d, nullptr, nullptr,
+                                 nullptr, (hipStream_t)stream));
+    return BTD_OK;
+}
+
+btd_status btd_generate_f32(int kind, uint64_t seed, int m, int nsys, int nrow, int layout,
+                            float* a, float* b, float* c, double* d, void* stream) {
+    g_err.clear();
+    if (kind < 0 || kind > 3 || m < 1 || m > BTD_MAX_M || nsys < 1 || nrow < 1 || layout < 0 ||
+        layout > 2)
+        return fail(BTD_ERR_INVALID_ARG, "btd_generate_f32: bad argument");
+    BTD_HIP(btd::launch_generate(m, kind, seed, layout, nsys, nrow, nullptr, nullptr, nullptr, d, a,
+                                 b, c, (hipStream_t)stream));
+    return BTD_OK;
+}
+
+}  // extern "C"
+
+template <int M>
+static void gen_host_m(int kind, uint64_t seed, int nrow, int sys0, int count, double* a,
+                       double* b, double* c, double* d) {
+    for (int row = 0; row < nrow; ++row)
+        for (int s = 0; s < count; ++s) {
+            double A[M][M], B[M][M], C[M][M], dd[M];
+            btd::gen_row<M>(kind, seed, (int64_t)sys0 + s, row, nrow, A, B, C, dd);
+            const size_t o = ((size_t)row * count + s) * M * M;
+            for (int i = 0; i < M; ++i)
+                for (int j = 0; j < M; ++j) {
+                    if (a) a[o + i * M + j] = A[i][j];
+                    if (b) b[o + i * M + j] = B[i][j];
+                    if (c) c[o + i * M + j] = C[i][j];
+                }
+            if (d)
+                for (int i = 0; i < M; ++i) d[((size_t)row * count + s) * M + i] = dd[i];
+        }
+}
+
+extern "C" btd_status btd_generate_host(int kind, uint64_t seed, int m, int nsys, int nrow,
+                                        int sys0, int count, double* a, double* b, double* c,
+                                        double* d) {
+    g_err.clear();
+    if (kind < 0 || kind > 3 || m < 1 || m > BTD_MAX_M || nsys < 1 || nrow < 1 || sys0 < 0 ||
+        count < 0 || sys0 + count > nsys)
+        return fail(BTD_ERR_INVALID_ARG, "btd_generate_host: bad argument");
+    switch (m) {
+        case 1: gen_host_m<1>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        case 2: gen_host_m<2>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        case 3: gen_host_m<3>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        case 4: gen_host_m<4>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        case 5: gen_host_m<5>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        case 6: gen_host_m<6>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        case 7: gen_host_m<7>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        case 8: gen_host_m<8>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        case 9: gen_host_m<9>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        case 10: gen_host_m<10>(kind, seed, nrow, sys0, count, a, b, c, d); break;
+        default: break;
+    }
+    return BTD_OK;
+}

@@ -1,0 +1,149 @@
+// Host dispatch from a runtime block size m to the per-m instantiations (btd_inst.hip),
+// plus the layout-conversion kernel.
+#include "btd_internal.h"
+
+namespace btd {
+
+hipError_t launch_plain_1(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_1(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_1(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+hipError_t launch_plain_2(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_2(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_2(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+hipError_t launch_plain_3(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_3(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_3(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+hipError_t launch_plain_4(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_4(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_4(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+hipError_t launch_plain_5(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_5(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_5(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+hipError_t launch_plain_6(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_6(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_6(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+hipError_t launch_plain_7(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_7(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_7(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+hipError_t launch_plain_8(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_8(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_8(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+hipError_t launch_plain_9(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_9(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_9(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+hipError_t launch_plain_10(int, const double*, const double*, double*, double*, int, int, int*,
+                          hipStream_t);
+hipError_t launch_two_stage_10(int, double*, const double*, double*, double*, int, int, const int*,
+                              const int*, int, double*, double*, double*, int*, hipStream_t,
+                              hipEvent_t*);
+hipError_t launch_generate_10(int, uint64_t, int, int, int, double*, double*, double*, double*,
+                             float*, float*, float*, hipStream_t);
+
+// ---------------------------------------------------------------------------- convert --
+__global__ __launch_bounds__(256) void k_convert(int src_layout, int dst_layout, int E, int nsys,
+                                                 int nrow, const double* __restrict__ src,
+                                                 double* __restrict__ dst) {
+    const int64_t total = (int64_t)nsys * nrow * E;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= total) return;
+    // idx enumerates (e, row, sys) with sys fastest
+    const int64_t sys = idx % nsys, rest = idx / nsys;
+    const int64_t row = rest % nrow, e = rest / nrow;
+    dst[elem_offset(dst_layout, sys, row, e, E, nsys, nrow)] =
+        src[elem_offset(src_layout, sys, row, e, E, nsys, nrow)];
+}
+
+#define BTD_DISPATCH_M(m, CALL)                  \
+    switch (m) {                                 \
+        case 1: return CALL(1);                  \
+        case 2: return CALL(2);                  \
+        case 3: return CALL(3);                  \
+        case 4: return CALL(4);                  \
+        case 5: return CALL(5);                  \
+        case 6: return CALL(6);                  \
+        case 7: return CALL(7);                  \
+        case 8: return CALL(8);                  \
+        case 9: return CALL(9);                  \
+        case 10: return CALL(10);                \
+        default: return hipErrorInvalidValue;    \
+    }
+
+hipError_t launch_plain(int m, int layout, const double* A, const double* B, double* C, double* D,
+                        int nsys, int nrow, int* flag, hipStream_t s) {
+#define CALL(M) launch_plain_##M(layout, A, B, C, D, nsys, nrow, flag, s)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_two_stage(int m, int layout, double* A, const double* B, double* C, double* D,
+                            int nsys, int nrow, const int* pf, const int* pl, int npart,
+                            double* redA, double* redC, double* redD, int* flag, hipStream_t s,
+                            hipEvent_t* ev) {
+#define CALL(M) \
+    launch_two_stage_##M(layout, A, B, C, D, nsys, nrow, pf, pl, npart, redA, redC, redD, flag, s, ev)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_generate(int m, int kind, uint64_t seed, int layout, int nsys, int nrow, double* A,
+                           double* B, double* C, double* D, float* Af, float* Bf, float* Cf,
+                           hipStream_t s) {
+#define CALL(M) launch_generate_##M(kind, seed, layout, nsys, nrow, A, B, C, D, Af, Bf, Cf, s)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_convert(int src_layout, int dst_layout, int E, int nsys, int nrow,
+                          const double* src, double* dst, hipStream_t s) {
+    const int64_t total = (int64_t)nsys * nrow * E;
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_convert, grid, block, 0, s, src_layout, dst_layout, E, nsys, nrow, src, dst);
+    return hipGetLastError();
+}
+
+}  // namespace btd

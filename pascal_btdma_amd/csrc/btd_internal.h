@@ -1,0 +1,27 @@
+// Internal declarations shared by the kernel and host translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "btd_common.h"
+#include "btd_dense.h"
+
+namespace btd {
+
+hipError_t launch_plain(int m, int layout, const double* A, const double* B, double* C, double* D,
+                        int nsys, int nrow, int* flag, hipStream_t s);
+
+hipError_t launch_two_stage(int m, int layout, double* A, const double* B, double* C, double* D,
+                            int nsys, int nrow, const int* part_first, const int* part_len,
+                            int npart, double* redA, double* redC, double* redD, int* flag,
+                            hipStream_t s, hipEvent_t* ev);
+
+hipError_t launch_generate(int m, int kind, uint64_t seed, int layout, int nsys, int nrow, double* A,
+                           double* B, double* C, double* D, float* Af, float* Bf, float* Cf,
+                           hipStream_t s);
+
+hipError_t launch_convert(int src_layout, int dst_layout, int E, int nsys, int nrow,
+                          const double* src, double* dst, hipStream_t s);
+
+}  // namespace btd

@@ -1,0 +1,541 @@
+// gfx950 kernel templates of the batched block-Thomas solve, FP64 (instantiated per m
+// in btd_inst.hip).
+//
+// One lane owns one system (or one row partition of one system) and walks its rows in
+// order; the m x m blocks of the current row live in VGPRs. Consecutive lanes own
+// consecutive systems, so with either the reference GPU layout or the tiled layout a
+// wavefront's load of one block element is 64 consecutive doubles (512 B).
+//
+// Kernels:
+//   k_plain      single-partition solve, btdma_many (mod_btdma_cpu.f90:15-78;
+//                GPU: mod_btdma_gpu_v2.f90:241-350, pascal_btdma_cuda.cu:274-334).
+//   k_modi       stage 1 of the two-stage solve over one partition per lane,
+//                btdma_many_modi (mod_btdma_cpu.f90:305-443; mod_btdma_gpu_v2.f90:618-866).
+//   k_reduced    plain solve of the 2P-row reduced system (B = I), on-device.
+//   k_update     stage 3, btdma_many_update (mod_btdma_cpu.f90:511-535).
+//   k_generate   seeded input generator (bench and tests).
+#pragma once
+#include "btd_internal.h"
+
+namespace btd {
+
+// ------------------------------------------------------------------------ addressing --
+// Offset of element e of (sys,row) is base + row*rs + e*es. For the tiled layout the
+// element stride is the compile-time constant 64.
+template <int LAYOUT>
+struct Addr {
+    int64_t base, rs, es;
+    __device__ __forceinline__ Addr(int64_t sys, int64_t nsys, int64_t nrow, int64_t E) {
+        if constexpr (LAYOUT == LAYOUT_TILED) {
+            base = (sys / kTile) * nrow * E * kTile + (sys % kTile);
+            rs = E * kTile;
+            es = kTile;
+        } else {
+            base = sys;
+            rs = nsys;
+            es = nsys * nrow;
+        }
+    }
+    __device__ __forceinline__ int64_t estride() const {
+        if constexpr (LAYOUT == LAYOUT_TILED) return kTile;
+        return es;
+    }
+    __device__ __forceinline__ int64_t at(int64_t row, int e) const {
+        return base + row * rs + (int64_t)e * estride();
+    }
+};
+
+// ----------------------------------------------------------------------- row helpers --
+template <int M, int LAYOUT>
+__device__ __forceinline__ void load_mat(const double* __restrict__ p, const Addr<LAYOUT>& a,
+                                         int64_t row, double (&X)[M][M]) {
+    const double* r = p + a.at(row, 0);
+    const int64_t es = a.estride();
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int i = 0; i < M; ++i) X[i][j] = __builtin_nontemporal_load(r + (i + M * j) * es);
+}
+
+template <int M, int LAYOUT>
+__device__ __forceinline__ void load_col(const double* p, const Addr<LAYOUT>& a, int64_t row,
+                                         int j, double (&x)[M]) {
+    const double* r = p + a.at(row, 0);
+    const int64_t es = a.estride();
+#pragma unroll
+    for (int i = 0; i < M; ++i) x[i] = r[(i + M * j) * es];
+}
+
+template <int M, int LAYOUT>
+__device__ __forceinline__ void store_col(double* p, const Addr<LAYOUT>& a, int64_t row, int j,
+                                          const double (&x)[M]) {
+    double* r = p + a.at(row, 0);
+    const int64_t es = a.estride();
+#pragma unroll
+    for (int i = 0; i < M; ++i) r[(i + M * j) * es] = x[i];
+}
+
+template <int M, int LAYOUT>
+__device__ __forceinline__ void load_vec(const double* p, const Addr<LAYOUT>& a, int64_t row,
+                                         double (&x)[M]) {
+    const double* r = p + a.at(row, 0);
+    const int64_t es = a.estride();
+#pragma unroll
+    for (int i = 0; i < M; ++i) x[i] = r[i * es];
+}
+
+template <int M, int LAYOUT>
+__device__ __forceinline__ void store_vec(double* p, const Addr<LAYOUT>& a, int64_t row,
+                                          const double (&x)[M]) {
+    double* r = p + a.at(row, 0);
+    const int64_t es = a.estride();
+#pragma unroll
+    for (int i = 0; i < M; ++i) r[i * es] = x[i];
+}
+
+// R <- R - A*Y and v <- v - A*y, A streamed by columns.
+template <int M, int LAYOUT>
+__device__ __forceinline__ void gemm_sub(const double* __restrict__ Ap, const Addr<LAYOUT>& am,
+                                         int64_t row, const double (&Y)[M][M], const double (&y)[M],
+                                         double (&R)[M][M], double (&v)[M]) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        double ak[M];
+        load_col<M>(Ap, am, row, k, ak);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            v[i] = fma(-ak[i], y[k], v[i]);
+#pragma unroll
+            for (int j = 0; j < M; ++j) R[i][j] = fma(-ak[i], Y[k][j], R[i][j]);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------- plain --
+// Forward: [d'_q | C'_q] = (B_q - A_q C'_{q-1})^{-1} [d_q - A_q d'_{q-1} | C_q], written in
+// place over C_q and d_q; backward: x_q = d'_q - C'_q x_{q+1}. A and B are only read.
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_plain(const double* __restrict__ A,
+                                              const double* __restrict__ B, double* C, double* D,
+                                              int nsys, int nrow, int* __restrict__ flag) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    double Cp[M][M], dp[M];
+    bool ok = true;
+    for (int64_t q = 0; q < nrow; ++q) {
+        double R[M][M], v[M];
+        load_mat<M>(B, am, q, R);
+        load_vec<M>(D, av, q, v);
+        if (q > 0) gemm_sub<M>(A, am, q, Cp, dp, R, v);
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, v);
+#pragma unroll
+        for (int i = 0; i < M; ++i) dp[i] = v[i];
+        store_vec<M>(D, av, q, dp);
+        if (q + 1 < nrow) {
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double c[M];
+                load_col<M>(C, am, q, j, c);
+                lu_solve<M>(R, piv, c);
+#pragma unroll
+                for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
+                store_col<M>(C, am, q, j, c);
+            }
+        }
+    }
+    // back substitution; dp holds x_{n-1}
+    for (int64_t q = nrow - 2; q >= 0; --q) {
+        double x[M], c[M][M];
+        load_vec<M>(D, av, q, x);
+        load_mat<M>(C, am, q, c);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) x[i] = fma(-c[i][j], dp[j], x[i]);
+#pragma unroll
+        for (int i = 0; i < M; ++i) dp[i] = x[i];
+        store_vec<M>(D, av, q, x);
+    }
+    if (!ok) atomicOr(flag, 1);
+}
+
+// ------------------------------------------------------------------------------ modi --
+// Stage 1 over partition `part` (rows [r0, r0+N)) of each system, mod_btdma_cpu.f90:305-443.
+// Writes the partition's modified rows in place (Â over A, Ĉ over C, d̂ over D) and its
+// two reduced rows into the plan's reduced buffers (tiled, 2P rows per system):
+//   row 2*part   : (Â_1, I, Ĉ_1, d̂_1)  couples x_1 to x_0 (previous partition) and x_N
+//   row 2*part+1 : (Â_N, I, Ĉ_N, d̂_N)  couples x_N to x_1 and x_{N+1} (next partition)
+template <int M>
+__device__ __forceinline__ void solve_rows(const double (&R)[M][M], const int (&piv)[M],
+                                           double (&X)[M][M]) {
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        double c[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) c[i] = X[i][j];
+        lu_solve<M>(R, piv, c);
+#pragma unroll
+        for (int i = 0; i < M; ++i) X[i][j] = c[i];
+    }
+}
+
+template <int M, int LAYOUT>
+__device__ __forceinline__ void store_mat(double* p, const Addr<LAYOUT>& a, int64_t row,
+                                          const double (&X)[M][M]) {
+    double* r = p + a.at(row, 0);
+    const int64_t es = a.estride();
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int i = 0; i < M; ++i) r[(i + M * j) * es] = X[i][j];
+}
+
+template <int M, int LAYOUT>
+__device__ __forceinline__ void load_mat_c(const double* p, const Addr<LAYOUT>& a, int64_t row,
+                                           double (&X)[M][M]) {
+    const double* r = p + a.at(row, 0);
+    const int64_t es = a.estride();
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int i = 0; i < M; ++i) X[i][j] = r[(i + M * j) * es];
+}
+
+template <int M>
+__device__ __forceinline__ void mm(const double (&X)[M][M], const double (&Y)[M][M],
+                                   double (&Z)[M][M]) {
+#pragma unroll
+    for (int i = 0; i < M; ++i)
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double s = X[i][0] * Y[0][j];
+#pragma unroll
+            for (int k = 1; k < M; ++k) s = fma(X[i][k], Y[k][j], s);
+            Z[i][j] = s;
+        }
+}
+
+template <int M>
+__device__ __forceinline__ void mv(const double (&X)[M][M], const double (&y)[M], double (&z)[M]) {
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        double s = X[i][0] * y[0];
+#pragma unroll
+        for (int k = 1; k < M; ++k) s = fma(X[i][k], y[k], s);
+        z[i] = s;
+    }
+}
+
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict__ B, double* C,
+                                             double* D, int nsys, int nrow,
+                                             const int* __restrict__ part_first,
+                                             const int* __restrict__ part_len, int npart,
+                                             double* __restrict__ redA, double* __restrict__ redC,
+                                             double* __restrict__ redD, int* __restrict__ flag) {
+    const int ntiles = (nsys + 63) / 64;
+    const int tile = blockIdx.x % ntiles, part = blockIdx.x / ntiles;
+    const int64_t sys = (int64_t)tile * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const int64_t r0 = part_first[part];
+    const int N = part_len[part];
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    bool ok = true;
+    // forward, rows q = 0..N-1 (local): [d̂|Â|Ĉ]_q = R^{-1}[d - A d̂_{q-1} | -A Â_{q-1} | C]
+    double Ah[M][M], Ch[M][M], dh[M];
+    for (int q = 0; q < N; ++q) {
+        const int64_t g = r0 + q;
+        double R[M][M], v[M], Aq[M][M], X[M][M];
+        load_mat<M>(B, am, g, R);
+        load_vec<M>(D, av, g, v);
+        load_mat_c<M>(A, am, g, Aq);
+        if (q >= 2) {
+            double T[M][M];
+            mm<M>(Aq, Ch, T);
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) R[i][j] -= T[i][j];
+            double t[M];
+            mv<M>(Aq, dh, t);
+#pragma unroll
+            for (int i = 0; i < M; ++i) v[i] -= t[i];
+            mm<M>(Aq, Ah, X);
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) X[i][j] = -X[i][j];
+        } else {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) X[i][j] = Aq[i][j];
+        }
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, v);
+        solve_rows<M>(R, piv, X);
+        double Cq[M][M];
+        load_mat_c<M>(C, am, g, Cq);
+        solve_rows<M>(R, piv, Cq);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dh[i] = v[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                Ah[i][j] = X[i][j];
+                Ch[i][j] = Cq[i][j];
+            }
+        }
+        store_mat<M>(A, am, g, Ah);
+        store_mat<M>(C, am, g, Ch);
+        store_vec<M>(D, av, g, dh);
+    }
+    // bottom reduced row: the forward values of row N-1 (still in Ah, Ch, dh)
+    const int64_t rt = (int64_t)tile * (2 * npart);
+    const Addr<LAYOUT_TILED> rm((int64_t)tile * 64 + threadIdx.x, 0, 2 * npart, M * M);
+    const Addr<LAYOUT_TILED> rv((int64_t)tile * 64 + threadIdx.x, 0, 2 * npart, M);
+    (void)rt;
+    store_mat<M>(redA, rm, 2 * part + 1, Ah);
+    store_mat<M>(redC, rm, 2 * part + 1, Ch);
+    store_vec<M>(redD, rv, 2 * part + 1, dh);
+    // backward, q = N-3..1: d̂_q -= Ĉ_q d̂_{q+1}; Â_q -= Ĉ_q Â_{q+1}; Ĉ_q = -Ĉ_q Ĉ_{q+1}.
+    // (Ah, Ch, dh) hold row q+1's current values.
+    if (N >= 3) {
+        load_mat_c<M>(A, am, r0 + N - 2, Ah);
+        load_mat_c<M>(C, am, r0 + N - 2, Ch);
+        load_vec<M>(D, av, r0 + N - 2, dh);
+    }
+    for (int q = N - 3; q >= 1; --q) {
+        const int64_t g = r0 + q;
+        double Aq[M][M], Cq[M][M], dq[M], T[M][M], t[M];
+        load_mat_c<M>(A, am, g, Aq);
+        load_mat_c<M>(C, am, g, Cq);
+        load_vec<M>(D, av, g, dq);
+        mv<M>(Cq, dh, t);
+#pragma unroll
+        for (int i = 0; i < M; ++i) dq[i] -= t[i];
+        mm<M>(Cq, Ah, T);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) Aq[i][j] -= T[i][j];
+        mm<M>(Cq, Ch, T);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) Cq[i][j] = -T[i][j];
+        store_mat<M>(A, am, g, Aq);
+        store_mat<M>(C, am, g, Cq);
+        store_vec<M>(D, av, g, dq);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dh[i] = dq[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                Ah[i][j] = Aq[i][j];
+                Ch[i][j] = Cq[i][j];
+            }
+        }
+    }
+    // close, q = 0: R = I - Ĉ_0 Â_1; [d̂|Â|Ĉ]_0 = R^{-1}[d̂_0 - Ĉ_0 d̂_1 | Â_0 | -Ĉ_0 Ĉ_1]
+    // (Ah, Ch, dh) now hold row 1 after the backward pass.
+    {
+        double A0[M][M], C0[M][M], d0[M], R[M][M], X[M][M], T[M][M], t[M];
+        load_mat_c<M>(A, am, r0, A0);
+        load_mat_c<M>(C, am, r0, C0);
+        load_vec<M>(D, av, r0, d0);
+        mv<M>(C0, dh, t);
+#pragma unroll
+        for (int i = 0; i < M; ++i) d0[i] -= t[i];
+        mm<M>(C0, Ah, T);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) R[i][j] = (i == j ? 1.0 : 0.0) - T[i][j];
+        mm<M>(C0, Ch, X);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) X[i][j] = -X[i][j];
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, d0);
+        solve_rows<M>(R, piv, A0);
+        solve_rows<M>(R, piv, X);
+        store_mat<M>(redA, rm, 2 * part, A0);
+        store_mat<M>(redC, rm, 2 * part, X);
+        store_vec<M>(redD, rv, 2 * part, d0);
+    }
+    if (!ok) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------------------- reduced --
+// Plain solve of the reduced system, 2P rows with B = I, in the tiled reduced buffers.
+// The solution replaces redD.
+template <int M>
+__global__ __launch_bounds__(64) void k_reduced(const double* __restrict__ redA, double* redC,
+                                                double* redD, int nsys, int nrr,
+                                                int* __restrict__ flag) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const Addr<LAYOUT_TILED> am(sys, 0, nrr, M * M), av(sys, 0, nrr, M);
+    double Cp[M][M], dp[M];
+    bool ok = true;
+    for (int q = 0; q < nrr; ++q) {
+        double R[M][M], v[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) R[i][j] = (i == j) ? 1.0 : 0.0;
+        load_vec<M>(redD, av, q, v);
+        if (q > 0) gemm_sub<M>(redA, am, q, Cp, dp, R, v);
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, v);
+#pragma unroll
+        for (int i = 0; i < M; ++i) dp[i] = v[i];
+        store_vec<M>(redD, av, q, dp);
+        if (q + 1 < nrr) {
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double c[M];
+                load_col<M>(redC, am, q, j, c);
+                lu_solve<M>(R, piv, c);
+#pragma unroll
+                for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
+                store_col<M>(redC, am, q, j, c);
+            }
+        }
+    }
+    for (int q = nrr - 2; q >= 0; --q) {
+        double x[M], c[M][M];
+        load_vec<M>(redD, av, q, x);
+        load_mat_c<M>(redC, am, q, c);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) x[i] = fma(-c[i][j], dp[j], x[i]);
+#pragma unroll
+        for (int i = 0; i < M; ++i) dp[i] = x[i];
+        store_vec<M>(redD, av, q, x);
+    }
+    if (!ok) atomicOr(flag, 1);
+}
+
+// ----------------------------------------------------------------------------- update --
+// x_1, x_N of each partition from the reduced solution; x_q = d̂_q - Â_q x_1 - Ĉ_q x_N for
+// the interior rows (mod_btdma_cpu.f90:511-535).
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_update(const double* __restrict__ A,
+                                               const double* __restrict__ C, double* D, int nsys,
+                                               int nrow, const int* __restrict__ part_first,
+                                               const int* __restrict__ part_len, int npart,
+                                               const double* __restrict__ redD) {
+    const int ntiles = (nsys + 63) / 64;
+    const int tile = blockIdx.x % ntiles, part = blockIdx.x / ntiles;
+    const int64_t sys = (int64_t)tile * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const int64_t r0 = part_first[part];
+    const int N = part_len[part];
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    const Addr<LAYOUT_TILED> rv(sys, 0, 2 * npart, M);
+    double x1[M], xn[M];
+    load_vec<M>(redD, rv, 2 * part, x1);
+    load_vec<M>(redD, rv, 2 * part + 1, xn);
+    store_vec<M>(D, av, r0, x1);
+    store_vec<M>(D, av, r0 + N - 1, xn);
+    for (int q = 1; q < N - 1; ++q) {
+        const int64_t g = r0 + q;
+        double a[M][M], c[M][M], x[M];
+        load_mat<M>(A, am, g, a);
+        load_mat<M>(C, am, g, c);
+        load_vec<M>(D, av, g, x);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) x[i] = fma(-a[i][j], x1[j], fma(-c[i][j], xn[j], x[i]));
+        store_vec<M>(D, av, g, x);
+    }
+}
+
+// --------------------------------------------------------------------------- generate --
+template <int M>
+__global__ __launch_bounds__(256) void k_generate(int kind, uint64_t seed, int layout, int nsys,
+                                                  int nrow, double* A, double* B, double* C,
+                                                  double* D, float* Af, float* Bf, float* Cf) {
+    const int64_t npad = pad_nsys(nsys);
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= npad * nrow) return;
+    const int64_t row = idx / npad, sys = idx % npad;
+    if (sys >= nsys) return;
+    double a[M][M], b[M][M], c[M][M], d[M];
+    gen_row<M>(kind, seed, sys, row, nrow, a, b, c, d);
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            const int64_t o = elem_offset(layout, sys, row, i + M * j, M * M, nsys, nrow);
+            if (A) A[o] = a[i][j];
+            if (B) B[o] = b[i][j];
+            if (C) C[o] = c[i][j];
+            if (Af) Af[o] = (float)a[i][j];
+            if (Bf) Bf[o] = (float)b[i][j];
+            if (Cf) Cf[o] = (float)c[i][j];
+        }
+    if (D)
+        for (int i = 0; i < M; ++i) D[elem_offset(layout, sys, row, i, M, nsys, nrow)] = d[i];
+}
+
+// ---------------------------------------------------------------------- launch templates --
+template <int M>
+hipError_t launch_plain_m(int layout, const double* A, const double* B, double* C, double* D,
+                                 int nsys, int nrow, int* flag, hipStream_t s) {
+    const dim3 grid((nsys + 63) / 64), block(64);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_plain<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    else
+        hipLaunchKernelGGL((k_plain<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C, double* D,
+                                     int nsys, int nrow, const int* pf, const int* pl, int npart,
+                                     double* redA, double* redC, double* redD, int* flag,
+                                     hipStream_t s, hipEvent_t* ev) {
+    const int ntiles = (nsys + 63) / 64;
+    const dim3 block(64), gridp((unsigned)(ntiles * npart)), gridr((unsigned)ntiles);
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED>), gridp, block, 0, s, A, B, C, D, nsys, nrow, pf, pl,
+                           npart, redA, redC, redD, flag);
+    else
+        hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU>), gridp, block, 0, s, A, B, C, D, nsys, nrow, pf, pl,
+                           npart, redA, redC, redD, flag);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    hipLaunchKernelGGL((k_reduced<M>), gridr, block, 0, s, redA, redC, redD, nsys, 2 * npart, flag);
+    if (ev) (void)hipEventRecord(ev[2], s);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_update<M, LAYOUT_TILED>), gridp, block, 0, s, A, C, D, nsys, nrow, pf, pl,
+                           npart, redD);
+    else
+        hipLaunchKernelGGL((k_update<M, LAYOUT_REF_GPU>), gridp, block, 0, s, A, C, D, nsys, nrow, pf,
+                           pl, npart, redD);
+    if (ev) (void)hipEventRecord(ev[3], s);
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_generate_m(int kind, uint64_t seed, int layout, int nsys, int nrow,
+                                    double* A, double* B, double* C, double* D, float* Af,
+                                    float* Bf, float* Cf, hipStream_t s) {
+    const int64_t total = pad_nsys(nsys) * (int64_t)nrow;
+    const dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    hipLaunchKernelGGL((k_generate<M>), grid, block, 0, s, kind, seed, layout, nsys, nrow, A, B, C, D,
+                       Af, Bf, Cf);
+    return hipGetLastError();
+}
+
+}  // namespace btd

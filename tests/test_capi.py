@@ -1,0 +1,124 @@
+"""CPU-side checks of the product library: it loads, exports every symbol the public
+header declares, validates arguments before touching the device, and its host
+generator has the properties the workload definitions promise. No device compute."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pascal_btdma_amd as pb
+from pascal_btdma_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "pascal_btdma_amd.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(btd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _capi.lib()
+    syms = header_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+        assert s in _capi.SIGNATURES, f"{s} has no ctypes signature"
+    assert set(_capi.SIGNATURES) <= set(syms)
+    assert b"gfx950" in lib.btd_version()
+
+
+def test_index_helpers_match_reference_formula():
+    nsys, nrow, m = 7, 5, 3
+    for sys in range(nsys):
+        for row in range(nrow):
+            for p in range(m):
+                assert pb.vector_index(sys, row, p, nsys, nrow) == sys + nsys * (row + nrow * p)
+                for q in range(m):
+                    assert pb.matrix_index(sys, row, p, q, nsys, nrow, m) == \
+                        sys + nsys * (row + nrow * (p + m * q))
+
+
+def test_partition_matches_oracle_para(oracle):
+    for n in (1, 7, 64, 100, 65536):
+        for P in (1, 2, 3, 8, 32):
+            if P > n:
+                continue
+            for r in range(P):
+                f, l = pb.partition_1d(0, n - 1, P, r)
+                size, ia, ib = oracle.para(0, n - 1, r, P)
+                assert (f, l) == (ia, ib)
+
+
+def test_layout_elems():
+    assert pb.layout_elems(pb.Layout.REF_GPU, 100, 7, 9) == 100 * 7 * 9
+    assert pb.layout_elems(pb.Layout.TILED, 100, 7, 9) == 128 * 7 * 9
+
+
+@pytest.mark.parametrize("args,exc", [
+    ((0, 10, 10, 1, pb.Layout.TILED), ValueError),
+    ((11, 10, 10, 1, pb.Layout.TILED), ValueError),
+    ((3, 0, 10, 1, pb.Layout.TILED), ValueError),
+    ((3, 10, 1, 1, pb.Layout.TILED), ValueError),
+    ((3, 10, 10, 4, pb.Layout.TILED), ValueError),   # needs nrow >= 3*npart
+    ((3, 10, 10, 0, pb.Layout.TILED), ValueError),
+    ((3, 10, 10, 1, pb.Layout.REF_CPU), ValueError),
+])
+def test_plan_create_rejects_bad_arguments(args, exc):
+    with pytest.raises(exc):
+        pb.BtdmaGpuPlan().create(*args)
+
+
+def test_uncreated_plan_and_cyclic_errors():
+    plan = pb.BtdmaGpuPlan()
+    with pytest.raises(RuntimeError):
+        pb.solve_noncyclic(plan, 1, 1, 1, 1, 3, 10, 10, stream=0)
+    with pytest.raises(NotImplementedError):
+        pb.solve_cyclic(plan, 1, 1, 1, 1, 3, 10, 10, stream=0)
+
+
+def test_generator_subset_is_consistent():
+    full = pb.generate_host(pb.GenKind.DDPERM, 3, 3, 20, 9)
+    part = pb.generate_host(pb.GenKind.DDPERM, 3, 3, 20, 9, sys0=5, count=4)
+    for f, p in zip(full, part):
+        np.testing.assert_array_equal(f[:, 5:9], p)
+
+
+def test_generator_ddperm_rows_are_dominant_after_unpermuting():
+    for m in (2, 3, 5, 8, 10):
+        A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 11, m, 6, 7)
+        assert np.all(A[0] == 0) and np.all(C[-1] == 0)
+        for q in range(7):
+            for s in range(6):
+                jmax = np.abs(B[q, s]).argmax(axis=1)
+                assert sorted(jmax) == list(range(m))  # a permutation of the columns
+                for i in range(m):
+                    j = jmax[i]
+                    off = np.abs(B[q, s, i]).sum() - abs(B[q, s, i, j])
+                    margin = abs(B[q, s, i, j]) - off - np.abs(A[q, s, i]).sum() - np.abs(C[q, s, i]).sum()
+                    assert margin >= 1.0 - 1e-12
+
+
+def test_generator_euler_couples_neighbouring_rows():
+    A, B, C, d = pb.generate_host(pb.GenKind.EULER, 5, 5, 4, 9)
+    # A_{q+1} = -0.2 J_q and C_{q-1} = 0.2 J_q, so C_{q-1} = -A_{q+1} exactly
+    np.testing.assert_array_equal(C[:-2], -A[2:])
+    assert np.all(A[0] == 0) and np.all(C[-1] == 0)
+
+
+def test_generator_cond_blocks():
+    A, B, C, d = pb.generate_host(pb.GenKind.COND, 9, 5, 16, 8)
+    for X in (A, B, C):
+        np.testing.assert_array_equal(X.astype(np.float32).astype(np.float64), X)
+    k = np.linalg.cond(B.reshape(-1, 5, 5))
+    assert k.min() > 5 and k.max() < 2e6
+    assert np.log10(k).std() > 0.8  # spread over the log-uniform range
+
+
+def test_generator_diag_is_study_workload():
+    A, B, C, d = pb.generate_host(pb.GenKind.DIAG, 0, 3, 2, 4)
+    np.testing.assert_array_equal(B, np.broadcast_to(2 * np.eye(3), B.shape))
+    np.testing.assert_array_equal(A, np.broadcast_to(0.25 * np.eye(3), A.shape))
+    np.testing.assert_array_equal(d, 1.0)

@@ -1,0 +1,171 @@
+"""Parity of the HIP solver (through the C ABI) with the CPU oracle, on the device.
+
+Tolerance: FP64 solves must match the oracle to a per-system normwise relative error
+max|x - x_oracle| / max|x_oracle| <= 1e-12 on the same inputs.
+"""
+import csv
+import os
+
+import numpy as np
+import pytest
+
+import pascal_btdma_amd as pb
+from helpers import cpu_layout, gpu_solve, nat_to_layout, layout_to_vec, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def oracle_plain(oracle, A, B, C, d):
+    x, info = oracle.solve_plain(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    return x
+
+
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_config0_full_batch(cuda, oracle, layout):
+    """1024 systems, m = 3, n = 64, diagonally dominant rows with permuted block rows."""
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 2024, 3, 1024, 64)
+    x, sing = gpu_solve(A, B, C, d, int(layout))
+    assert not sing
+    err = rel_err(x, oracle_plain(oracle, A, B, C, d))
+    print(f"config0 layout={layout.name} max rel err {err.max():.3e}")
+    assert err.max() <= TOL
+
+
+@pytest.mark.parametrize("m", list(range(1, 11)))
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_plain_every_block_size(cuda, oracle, m, layout):
+    nsys = 70  # not a multiple of 64: a partial tile
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 77 + m, m, nsys, 13)
+    x, sing = gpu_solve(A, B, C, d, int(layout))
+    assert not sing
+    assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
+
+
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_euler_m5(cuda, oracle, layout):
+    A, B, C, d = pb.generate_host(pb.GenKind.EULER, 5, 5, 256, 512)
+    x, sing = gpu_solve(A, B, C, d, int(layout))
+    assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
+
+
+def test_two_rows(cuda, oracle):
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 1, 4, 65, 2)
+    x, _ = gpu_solve(A, B, C, d, int(pb.Layout.TILED))
+    assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
+
+
+@pytest.mark.parametrize("m,n,npart", [(2, 96, 2), (3, 96, 4), (5, 96, 8), (3, 100, 7),
+                                        (1, 30, 10), (8, 48, 4), (10, 36, 3), (5, 9, 3)])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_two_stage_matches_oracle(cuda, oracle, m, n, npart, layout):
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 900 + m + n, m, 67, n)
+    x, t, sing = gpu_solve(A, B, C, d, int(layout), npart=npart, profiled=True)
+    assert not sing
+    xo, info = oracle.solve_two_stage(cpu_layout(A), cpu_layout(B), cpu_layout(C), d, npart)
+    assert info == 0
+    assert rel_err(x, xo).max() <= TOL
+    assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
+    assert t.forward_exchange == 0 and t.backward_exchange == 0
+    assert t.total >= t.local_compute > 0 and t.reduced_compute > 0 and t.update_compute > 0
+
+
+@pytest.mark.parametrize("kind", [pb.GenKind.DDPERM, pb.GenKind.EULER, pb.GenKind.COND,
+                                  pb.GenKind.DIAG])
+@pytest.mark.parametrize("m", [2, 3, 5, 8, 10])
+def test_device_generator_matches_host(cuda, kind, m):
+    import torch
+    nsys, n = 70, 6
+    ref = pb.generate_host(kind, 31, m, nsys, n)
+    for layout in (pb.Layout.REF_GPU, pb.Layout.TILED, pb.Layout.REF_CPU):
+        nm = pb.layout_elems(layout, nsys, n, m * m)
+        nv = pb.layout_elems(layout, nsys, n, m)
+        t = [torch.zeros(nm, dtype=torch.float64, device=cuda) for _ in range(3)]
+        td = torch.zeros(nv, dtype=torch.float64, device=cuda)
+        pb.generate(kind, 31, m, nsys, n, layout, *t, td)
+        torch.cuda.synchronize()
+        for X, tt in zip(ref[:3], t):
+            np.testing.assert_array_equal(tt.cpu().numpy(), nat_to_layout(X, int(layout)))
+        np.testing.assert_array_equal(td.cpu().numpy(), nat_to_layout(ref[3], int(layout)))
+
+
+def test_layout_conversion_round_trip(cuda):
+    import torch
+    m, nsys, n = 3, 70, 5
+    A = pb.generate_host(pb.GenKind.DDPERM, 4, m, nsys, n)[0]
+    src = torch.from_numpy(nat_to_layout(A, 1)).to(cuda)
+    tiled = torch.zeros(pb.layout_elems(pb.Layout.TILED, nsys, n, m * m), dtype=torch.float64, device=cuda)
+    refg = torch.zeros(nsys * n * m * m, dtype=torch.float64, device=cuda)
+    back = torch.zeros_like(refg)
+    pb.convert_layout(pb.Layout.REF_CPU, pb.Layout.TILED, m * m, nsys, n, src, tiled)
+    pb.convert_layout(pb.Layout.TILED, pb.Layout.REF_GPU, m * m, nsys, n, tiled, refg)
+    pb.convert_layout(pb.Layout.REF_GPU, pb.Layout.REF_CPU, m * m, nsys, n, refg, back)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(tiled.cpu().numpy()[: tiled.numel()], nat_to_layout(A, 2))
+    np.testing.assert_array_equal(refg.cpu().numpy(), nat_to_layout(A, 0))
+    np.testing.assert_array_equal(back.cpu().numpy(), src.cpu().numpy())
+
+
+def study_rows():
+    with open(os.path.join(GOLDEN, "study_signatures.csv")) as f:
+        return [r for r in csv.DictReader(f)
+                if int(r["nsys"]) * int(r["n3"]) * int(r["m"]) ** 2 <= 4096 * 2048 * 25]
+
+
+@pytest.mark.parametrize("row", study_rows(), ids=lambda r: f"np{r['nranks']}_{r['n1']}x{r['n2']}x{r['n3']}_m{r['m']}")
+def test_study_signatures_on_device(cuda, row):
+    """The reference's published signatures of the diagonal workload, reproduced by the
+    device solve on the full batch (np ranks -> npart partitions)."""
+    import torch
+    m, nsys, n, P = int(row["m"]), int(row["nsys"]), int(row["n3"]), int(row["nranks"])
+    lay = pb.Layout.REF_GPU
+    nm, nv = nsys * n * m * m, nsys * n * m
+    a, b, c = (torch.empty(nm, dtype=torch.float64, device=cuda) for _ in range(3))
+    d = torch.empty(nv, dtype=torch.float64, device=cuda)
+    pb.generate(pb.GenKind.DIAG, 0, m, nsys, n, lay, a, b, c, d)
+    with pb.BtdmaGpuPlan().create(m, nsys, n, P, lay) as plan:
+        pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    x = d.cpu().numpy()
+    s, l2, linf = x.sum(), np.sqrt((x * x).sum()), np.abs(x).max()
+    assert abs(s / float(row["solution_sum"]) - 1) < 1e-9
+    assert abs(l2 / float(row["solution_l2"]) - 1) < 1e-9
+    assert abs(linf - float(row["solution_linf"])) < 1e-14
+    for z, key in ((0, "sample_z0"), (n // 2, "sample_zmid"), (n - 1, "sample_zlast")):
+        assert abs(x[pb.vector_index(0, z, 0, nsys, n)] - float(row[key])) < 1e-14
+
+
+def test_singular_block_is_reported(cuda):
+    import torch
+    m, nsys, n = 3, 64, 8
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 5, m, nsys, n)
+    B[4, 17] = 0.0
+    A[4, 17] = 0.0  # row 4 of system 17 is then all zero in its diagonal block
+    lay = 2
+    ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d))
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, pb.Layout.TILED) as plan:
+        with pytest.raises(pb.SingularError):
+            pb.solve_noncyclic_profiled(plan, ta, tb, tc, td, m, nsys, n)
+        ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d))
+        pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
+        assert plan.check()
+        assert not plan.check()  # the check clears the flag
+    # the other systems are unaffected
+    x = layout_to_vec(td.cpu().numpy(), lay, n, nsys, m)
+    assert np.isfinite(x[:, :17]).all() and np.isfinite(x[:, 18:]).all()
+
+
+def test_api_errors_on_device(cuda):
+    import torch
+    with pb.BtdmaGpuPlan().create(3, 64, 8) as plan:
+        t = torch.zeros(64 * 8 * 9, dtype=torch.float64, device=cuda)
+        v = torch.zeros(64 * 8 * 3, dtype=torch.float64, device=cuda)
+        with pytest.raises(ValueError):
+            pb.solve_noncyclic(plan, t, t, t, v, 3, 64, 9)  # dimension mismatch
+        with pytest.raises(ValueError):
+            pb.solve_noncyclic(plan, t[:10], t, t, v, 3, 64, 8)  # too small
+        with pytest.raises(NotImplementedError):
+            pb.solve_cyclic(plan, t, t, t, v, 3, 64, 8)
