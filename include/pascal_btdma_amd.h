@@ -101,6 +101,35 @@ btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double *a, const double
 btd_status btd_solve_cyclic(btd_plan_t plan, double *a, const double *b, double *c, double *d,
                             int m, int nsys, int nrow, void *stream);
 
+/* Forward solve that also keeps the block factorization for btd_solve_adjoint (no
+ * counterpart in the reference; the differentiable-solve extension). Same in-place
+ * contract as btd_solve_noncyclic (npart must be 1): c returns C'_q, d returns x.
+ * factors: btd_layout_elems(layout, nsys, nrow, m*m) doubles, the LU factors of
+ * R_q = B_q - A_q C'_{q-1} per row. pivots: btd_layout_elems(layout, nsys, nrow, 1)
+ * 64-bit words, the pivot rows of R_q packed 4 bits per elimination step. */
+btd_status btd_solve_noncyclic_save(btd_plan_t plan, const double *a, const double *b, double *c,
+                                    double *d, double *factors, uint64_t *pivots, void *stream);
+
+/* Adjoint of the solve for L(x) with dL/dx = w (no counterpart in the reference):
+ * lambda = T^{-T} w from the factors of btd_solve_noncyclic_save (a: the original A,
+ * c_factored: C' as returned in c, x: the solution), then dd = lambda and the banded
+ * outer products dA_q = -lambda_q x_{q-1}^T, dB_q = -lambda_q x_q^T,
+ * dC_q = -lambda_q x_{q+1}^T (dA of the first row and dC of the last row are 0). */
+btd_status btd_solve_adjoint(btd_plan_t plan, const double *a, const double *c_factored,
+                             const double *factors, const uint64_t *pivots, const double *x,
+                             const double *w, double *lambda, double *da, double *db, double *dc,
+                             void *stream);
+
+/* Mixed precision (no counterpart in the reference): blocks a, b, c stored in FP32,
+ * d and x in FP64. An FP32 block-Thomas solve followed by `sweeps` (0..8) iterative-
+ * refinement sweeps, each with the residual r = d - T x formed in FP64 from the
+ * FP32-stored blocks and the correction x += T^{-1} r from the FP32 factorization.
+ * a, b, c, d are only read. work: btd_layout_elems(layout, nsys, nrow, m*m) +
+ * btd_layout_elems(layout, nsys, nrow, m) floats of device scratch. npart must be 1. */
+btd_status btd_solve_noncyclic_f32r(btd_plan_t plan, const float *a, const float *b, const float *c,
+                                    const double *d, double *x, float *work, int sweeps,
+                                    void *stream);
+
 /* New (the reference ignores LAPACK/elimination info): synchronises the plan's last
  * stream use, stores 1 in *singular if any block LU met a zero pivot since the last
  * check (0 otherwise), and clears the flag. */

@@ -21,7 +21,8 @@ __all__ = [
     "Layout", "GenKind", "BtdmaGpuPlan", "BtdmaSolveTimings", "BtdmaError", "HipError",
     "SingularError", "solve_noncyclic", "solve_noncyclic_profiled", "solve_cyclic",
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
-    "generate", "generate_f32", "generate_host", "kMaxBlockSize",
+    "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
+    "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems",
 ]
 
 kMaxBlockSize = 10
@@ -220,6 +221,55 @@ def solve_cyclic(plan: BtdmaGpuPlan, a, b, c, d, m: int, nsys: int, nrow: int,
     """Periodic solve (pascal_btdma_cuda.hpp:158-166)."""
     _check(_capi.lib().btd_solve_cyclic(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d), int(m),
                                         int(nsys), int(nrow), _stream(stream)))
+
+
+def solve_noncyclic_save(plan: BtdmaGpuPlan, a, b, c, d, factors, pivots, stream=None) -> None:
+    """Forward solve that keeps the block factorization (LU of R_q and pivots) for the
+    adjoint. In place like solve_noncyclic: c returns C', d returns x."""
+    import torch
+    if plan.created:
+        nm, nv1 = plan.matrix_elems(), layout_elems(plan.layout, plan.nsys, plan.nrow, 1)
+        _check_tensor("factors", factors, nm, torch.float64)
+        _check_tensor("pivots", pivots, nv1, torch.int64)
+        _validate(plan, a, b, c, d, plan.m, plan.nsys, plan.nrow)
+    _check(_capi.lib().btd_solve_noncyclic_save(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
+                                                _ptr(factors), _ptr(pivots), _stream(stream)))
+
+
+def solve_adjoint(plan: BtdmaGpuPlan, a, c_factored, factors, pivots, x, w, lam, da, db, dc,
+                  stream=None) -> None:
+    """lambda = T^{-T} w and the gradients dA, dB, dC of L with dL/dx = w."""
+    import torch
+    if plan.created:
+        nm, nv = plan.matrix_elems(), plan.vector_elems()
+        for name, t, n in (("a", a, nm), ("c_factored", c_factored, nm), ("factors", factors, nm),
+                           ("x", x, nv), ("w", w, nv), ("lam", lam, nv), ("da", da, nm),
+                           ("db", db, nm), ("dc", dc, nm)):
+            _check_tensor(name, t, n, torch.float64)
+    _check(_capi.lib().btd_solve_adjoint(plan._h, _ptr(a), _ptr(c_factored), _ptr(factors),
+                                         _ptr(pivots), _ptr(x), _ptr(w), _ptr(lam), _ptr(da),
+                                         _ptr(db), _ptr(dc), _stream(stream)))
+
+
+def f32r_work_elems(plan: BtdmaGpuPlan) -> int:
+    """Floats of device scratch btd_solve_noncyclic_f32r needs for this plan."""
+    return plan.matrix_elems() + plan.vector_elems()
+
+
+def solve_noncyclic_f32r(plan: BtdmaGpuPlan, a, b, c, d, x, work, sweeps: int = 2,
+                         stream=None) -> None:
+    """FP32-stored blocks (a, b, c float32), FP64 d and x: FP32 solve plus `sweeps`
+    refinement sweeps with FP64 residuals. a, b, c, d are only read."""
+    import torch
+    if plan.created:
+        nm, nv = plan.matrix_elems(), plan.vector_elems()
+        for name, t, n, dt in (("a", a, nm, torch.float32), ("b", b, nm, torch.float32),
+                               ("c", c, nm, torch.float32), ("d", d, nv, torch.float64),
+                               ("x", x, nv, torch.float64),
+                               ("work", work, f32r_work_elems(plan), torch.float32)):
+            _check_tensor(name, t, n, dt)
+    _check(_capi.lib().btd_solve_noncyclic_f32r(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
+                                                _ptr(x), _ptr(work), int(sweeps), _stream(stream)))
 
 
 def matrix_index(sys: int, row: int, p: int, q: int, nsys: int, nrow: int, m: int) -> int:

@@ -228,6 +228,59 @@ btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double* a, const double
     return BTD_OK;
 }
 
+static btd_status check_single_partition(btd_plan_t plan, const char* what) {
+    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, std::string(what) + ": uncreated plan");
+    if (plan->npart != 1)
+        return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": needs a plan with npart == 1");
+    return BTD_OK;
+}
+
+btd_status btd_solve_noncyclic_save(btd_plan_t plan, const double* a, const double* b, double* c,
+                                    double* d, double* factors, uint64_t* pivots, void* stream) {
+    g_err.clear();
+    btd_status st = check_single_partition(plan, "btd_solve_noncyclic_save");
+    if (st != BTD_OK) return st;
+    if (!a || !b || !c || !d || !factors || !pivots)
+        return fail(BTD_ERR_INVALID_ARG, "btd_solve_noncyclic_save: NULL array");
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_plain_save(plan->m, plan->layout, a, b, c, d, factors, pivots, plan->nsys,
+                                   plan->nrow, plan->d_flag, (hipStream_t)stream));
+    return BTD_OK;
+}
+
+btd_status btd_solve_adjoint(btd_plan_t plan, const double* a, const double* c_factored,
+                             const double* factors, const uint64_t* pivots, const double* x,
+                             const double* w, double* lambda, double* da, double* db, double* dc,
+                             void* stream) {
+    g_err.clear();
+    btd_status st = check_single_partition(plan, "btd_solve_adjoint");
+    if (st != BTD_OK) return st;
+    if (!a || !c_factored || !factors || !pivots || !x || !w || !lambda || !da || !db || !dc)
+        return fail(BTD_ERR_INVALID_ARG, "btd_solve_adjoint: NULL array");
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_adjoint(plan->m, plan->layout, a, c_factored, factors, pivots, x, w, lambda,
+                                da, db, dc, plan->nsys, plan->nrow, (hipStream_t)stream));
+    return BTD_OK;
+}
+
+btd_status btd_solve_noncyclic_f32r(btd_plan_t plan, const float* a, const float* b, const float* c,
+                                    const double* d, double* x, float* work, int sweeps,
+                                    void* stream) {
+    g_err.clear();
+    btd_status st = check_single_partition(plan, "btd_solve_noncyclic_f32r");
+    if (st != BTD_OK) return st;
+    if (!a || !b || !c || !d || !x || !work)
+        return fail(BTD_ERR_INVALID_ARG, "btd_solve_noncyclic_f32r: NULL array");
+    if (sweeps < 0 || sweeps > 8)
+        return fail(BTD_ERR_INVALID_ARG, "btd_solve_noncyclic_f32r: sweeps must be in [0, 8]");
+    plan->last_stream = (hipStream_t)stream;
+    float* wc = work;
+    float* wv = work + btd::layout_elems(plan->layout, plan->nsys, plan->nrow, plan->m * plan->m);
+    BTD_HIP(btd::launch_f32r(plan->m, plan->layout, a, b, c, d, x, wc, wv, plan->nsys, plan->nrow,
+                             sweeps, plan->d_flag, (hipStream_t)stream));
+    return BTD_OK;
+}
+
 btd_status btd_solve_cyclic(btd_plan_t plan, double* a, const double* b, double* c, double* d,
                             int m, int nsys, int nrow, void* stream) {
     (void)plan;

@@ -84,6 +84,86 @@ hipError_t launch_two_stage_10(int, double*, const double*, double*, double*, in
 hipError_t launch_generate_10(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
+hipError_t launch_plain_save_1(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_1(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_1(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
+hipError_t launch_plain_save_2(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_2(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_2(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
+hipError_t launch_plain_save_3(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_3(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_3(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
+hipError_t launch_plain_save_4(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_4(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_4(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
+hipError_t launch_plain_save_5(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_5(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_5(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
+hipError_t launch_plain_save_6(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_6(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_6(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
+hipError_t launch_plain_save_7(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_7(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_7(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
+hipError_t launch_plain_save_8(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_8(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_8(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
+hipError_t launch_plain_save_9(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_9(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_9(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
+hipError_t launch_plain_save_10(int, const double*, const double*, double*, double*, double*,
+                               uint64_t*, int, int, int*, hipStream_t);
+hipError_t launch_adjoint_10(int, const double*, const double*, const double*, const uint64_t*,
+                            const double*, const double*, double*, double*, double*, double*, int,
+                            int, hipStream_t);
+hipError_t launch_f32r_10(int, const float*, const float*, const float*, const double*, double*,
+                         float*, float*, int, int, int, int*, hipStream_t);
+
 // ---------------------------------------------------------------------------- convert --
 __global__ __launch_bounds__(256) void k_convert(int src_layout, int dst_layout, int E, int nsys,
                                                  int nrow, const double* __restrict__ src,
@@ -134,6 +214,30 @@ hipError_t launch_generate(int m, int kind, uint64_t seed, int layout, int nsys,
                            double* B, double* C, double* D, float* Af, float* Bf, float* Cf,
                            hipStream_t s) {
 #define CALL(M) launch_generate_##M(kind, seed, layout, nsys, nrow, A, B, C, D, Af, Bf, Cf, s)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_plain_save(int m, int layout, const double* A, const double* B, double* C,
+                             double* D, double* F, uint64_t* PV, int nsys, int nrow, int* flag,
+                             hipStream_t s) {
+#define CALL(M) launch_plain_save_##M(layout, A, B, C, D, F, PV, nsys, nrow, flag, s)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_adjoint(int m, int layout, const double* A, const double* Cp, const double* F,
+                          const uint64_t* PV, const double* X, const double* W, double* Lam,
+                          double* dA, double* dB, double* dC, int nsys, int nrow, hipStream_t s) {
+#define CALL(M) launch_adjoint_##M(layout, A, Cp, F, PV, X, W, Lam, dA, dB, dC, nsys, nrow, s)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_f32r(int m, int layout, const float* A, const float* B, const float* C,
+                       const double* D, double* X, float* Wc, float* Wv, int nsys, int nrow,
+                       int sweeps, int* flag, hipStream_t s) {
+#define CALL(M) launch_f32r_##M(layout, A, B, C, D, X, Wc, Wv, nsys, nrow, sweeps, flag, s)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }

@@ -31,4 +31,23 @@ hipError_t BTD_NAME(launch_generate_)(int kind, uint64_t seed, int layout, int n
     return launch_generate_m<BTD_M>(kind, seed, layout, nsys, nrow, A, B, C, D, Af, Bf, Cf, s);
 }
 
+hipError_t BTD_NAME(launch_plain_save_)(int layout, const double* A, const double* B, double* C,
+                                        double* D, double* F, uint64_t* PV, int nsys, int nrow,
+                                        int* flag, hipStream_t s) {
+    return launch_plain_save_m<BTD_M>(layout, A, B, C, D, F, PV, nsys, nrow, flag, s);
+}
+
+hipError_t BTD_NAME(launch_adjoint_)(int layout, const double* A, const double* Cp,
+                                     const double* F, const uint64_t* PV, const double* X,
+                                     const double* W, double* Lam, double* dA, double* dB,
+                                     double* dC, int nsys, int nrow, hipStream_t s) {
+    return launch_adjoint_m<BTD_M>(layout, A, Cp, F, PV, X, W, Lam, dA, dB, dC, nsys, nrow, s);
+}
+
+hipError_t BTD_NAME(launch_f32r_)(int layout, const float* A, const float* B, const float* C,
+                                  const double* D, double* X, float* Wc, float* Wv, int nsys,
+                                  int nrow, int sweeps, int* flag, hipStream_t s) {
+    return launch_f32r_m<BTD_M>(layout, A, B, C, D, X, Wc, Wv, nsys, nrow, sweeps, flag, s);
+}
+
 }  // namespace btd

@@ -12,6 +12,18 @@ namespace btd {
 hipError_t launch_plain(int m, int layout, const double* A, const double* B, double* C, double* D,
                         int nsys, int nrow, int* flag, hipStream_t s);
 
+hipError_t launch_plain_save(int m, int layout, const double* A, const double* B, double* C,
+                             double* D, double* F, uint64_t* PV, int nsys, int nrow, int* flag,
+                             hipStream_t s);
+
+hipError_t launch_adjoint(int m, int layout, const double* A, const double* Cp, const double* F,
+                          const uint64_t* PV, const double* X, const double* W, double* Lam,
+                          double* dA, double* dB, double* dC, int nsys, int nrow, hipStream_t s);
+
+hipError_t launch_f32r(int m, int layout, const float* A, const float* B, const float* C,
+                       const double* D, double* X, float* Wc, float* Wv, int nsys, int nrow,
+                       int sweeps, int* flag, hipStream_t s);
+
 hipError_t launch_two_stage(int m, int layout, double* A, const double* B, double* C, double* D,
                             int nsys, int nrow, const int* part_first, const int* part_len,
                             int npart, double* redA, double* redC, double* redD, int* flag,

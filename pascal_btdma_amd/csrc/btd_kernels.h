@@ -111,63 +111,6 @@ __device__ __forceinline__ void gemm_sub(const double* __restrict__ Ap, const Ad
     }
 }
 
-// ----------------------------------------------------------------------------- plain --
-// Forward: [d'_q | C'_q] = (B_q - A_q C'_{q-1})^{-1} [d_q - A_q d'_{q-1} | C_q], written in
-// place over C_q and d_q; backward: x_q = d'_q - C'_q x_{q+1}. A and B are only read.
-template <int M, int LAYOUT>
-__global__ __launch_bounds__(64) void k_plain(const double* __restrict__ A,
-                                              const double* __restrict__ B, double* C, double* D,
-                                              int nsys, int nrow, int* __restrict__ flag) {
-    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (sys >= nsys) return;
-    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
-    double Cp[M][M], dp[M];
-    bool ok = true;
-    for (int64_t q = 0; q < nrow; ++q) {
-        double R[M][M], v[M];
-        load_mat<M>(B, am, q, R);
-        load_vec<M>(D, av, q, v);
-        if (q > 0) gemm_sub<M>(A, am, q, Cp, dp, R, v);
-        int piv[M];
-        ok = lu_factor<M>(R, piv) && ok;
-        lu_solve<M>(R, piv, v);
-#pragma unroll
-        for (int i = 0; i < M; ++i) dp[i] = v[i];
-        store_vec<M>(D, av, q, dp);
-        if (q + 1 < nrow) {
-#pragma unroll
-            for (int j = 0; j < M; ++j) {
-                double c[M];
-                load_col<M>(C, am, q, j, c);
-                lu_solve<M>(R, piv, c);
-#pragma unroll
-                for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
-                store_col<M>(C, am, q, j, c);
-            }
-        }
-    }
-    // back substitution; dp holds x_{n-1}
-    for (int64_t q = nrow - 2; q >= 0; --q) {
-        double x[M], c[M][M];
-        load_vec<M>(D, av, q, x);
-        load_mat<M>(C, am, q, c);
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int j = 0; j < M; ++j) x[i] = fma(-c[i][j], dp[j], x[i]);
-#pragma unroll
-        for (int i = 0; i < M; ++i) dp[i] = x[i];
-        store_vec<M>(D, av, q, x);
-    }
-    if (!ok) atomicOr(flag, 1);
-}
-
-// ------------------------------------------------------------------------------ modi --
-// Stage 1 over partition `part` (rows [r0, r0+N)) of each system, mod_btdma_cpu.f90:305-443.
-// Writes the partition's modified rows in place (Â over A, Ĉ over C, d̂ over D) and its
-// two reduced rows into the plan's reduced buffers (tiled, 2P rows per system):
-//   row 2*part   : (Â_1, I, Ĉ_1, d̂_1)  couples x_1 to x_0 (previous partition) and x_N
-//   row 2*part+1 : (Â_N, I, Ĉ_N, d̂_N)  couples x_N to x_1 and x_{N+1} (next partition)
 template <int M>
 __device__ __forceinline__ void solve_rows(const double (&R)[M][M], const int (&piv)[M],
                                            double (&X)[M][M]) {
@@ -229,6 +172,396 @@ __device__ __forceinline__ void mv(const double (&X)[M][M], const double (&y)[M]
     }
 }
 
+// ----------------------------------------------------------------------------- plain --
+// Forward: [d'_q | C'_q] = (B_q - A_q C'_{q-1})^{-1} [d_q - A_q d'_{q-1} | C_q], written in
+// place over C_q and d_q; backward: x_q = d'_q - C'_q x_{q+1}. A and B are only read.
+// SAVE additionally keeps, per row, the LU factors of R_q = B_q - A_q C'_{q-1} (matrix
+// layout, reciprocal pivots on the diagonal) and the pivot rows packed 4 bits per step in
+// one 64-bit word (vector layout with one element per row), for the adjoint solve.
+template <int M>
+__device__ __forceinline__ uint64_t pack_piv(const int (&piv)[M]) {
+    uint64_t w = 0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) w |= (uint64_t)piv[k] << (4 * k);
+    return w;
+}
+
+template <int M>
+__device__ __forceinline__ void unpack_piv(uint64_t w, int (&piv)[M]) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) piv[k] = (int)((w >> (4 * k)) & 15u);
+}
+
+template <int M, int LAYOUT, bool SAVE>
+__device__ __forceinline__ void plain_solve(const double* __restrict__ A,
+                                            const double* __restrict__ B, double* C, double* D,
+                                            double* __restrict__ F, uint64_t* __restrict__ PV,
+                                            int nsys, int nrow, int* __restrict__ flag) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M), ap(sys, nsys, nrow, 1);
+    double Cp[M][M], dp[M];
+    bool ok = true;
+    for (int64_t q = 0; q < nrow; ++q) {
+        double R[M][M], v[M];
+        load_mat<M>(B, am, q, R);
+        load_vec<M>(D, av, q, v);
+        if (q > 0) gemm_sub<M>(A, am, q, Cp, dp, R, v);
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        if constexpr (SAVE) {
+            double* r = F + am.at(q, 0);
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int i = 0; i < M; ++i) r[(i + M * j) * am.estride()] = R[i][j];
+            PV[ap.at(q, 0)] = pack_piv<M>(piv);
+        }
+        lu_solve<M>(R, piv, v);
+#pragma unroll
+        for (int i = 0; i < M; ++i) dp[i] = v[i];
+        store_vec<M>(D, av, q, dp);
+        if (q + 1 < nrow) {
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double c[M];
+                load_col<M>(C, am, q, j, c);
+                lu_solve<M>(R, piv, c);
+#pragma unroll
+                for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
+                store_col<M>(C, am, q, j, c);
+            }
+        }
+    }
+    // back substitution; dp holds x_{n-1}
+    for (int64_t q = nrow - 2; q >= 0; --q) {
+        double x[M], c[M][M];
+        load_vec<M>(D, av, q, x);
+        load_mat<M>(C, am, q, c);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) x[i] = fma(-c[i][j], dp[j], x[i]);
+#pragma unroll
+        for (int i = 0; i < M; ++i) dp[i] = x[i];
+        store_vec<M>(D, av, q, x);
+    }
+    if (!ok) atomicOr(flag, 1);
+}
+
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_plain(const double* __restrict__ A,
+                                              const double* __restrict__ B, double* C, double* D,
+                                              int nsys, int nrow, int* __restrict__ flag) {
+    plain_solve<M, LAYOUT, false>(A, B, C, D, nullptr, nullptr, nsys, nrow, flag);
+}
+
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_plain_save(const double* __restrict__ A,
+                                                   const double* __restrict__ B, double* C,
+                                                   double* D, double* __restrict__ F,
+                                                   uint64_t* __restrict__ PV, int nsys, int nrow,
+                                                   int* __restrict__ flag) {
+    plain_solve<M, LAYOUT, true>(A, B, C, D, F, PV, nsys, nrow, flag);
+}
+
+// --------------------------------------------------------------------------- adjoint --
+// T^T lambda = w with the block factorization T = L U of the forward solve (L: diagonal
+// R_q, sub-diagonal A_q; U: identity diagonal, super-diagonal C'_q):
+//   U^T y = w:       y_0 = w_0,  y_q = w_q - C'_{q-1}^T y_{q-1}
+//   L^T lambda = y:  R_q^T lambda_q = y_q - A_{q+1}^T lambda_{q+1}   (q = n-1 .. 0)
+// then the gradients of L(x) with dL/dx = w:
+//   dd = lambda, dA_q = -lambda_q x_{q-1}^T, dB_q = -lambda_q x_q^T, dC_q = -lambda_q x_{q+1}^T
+// (dA_0 = 0 and dC_{n-1} = 0: those blocks are ignored by the solve).
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_adjoint(const double* __restrict__ A,
+                                                const double* __restrict__ Cp,
+                                                const double* __restrict__ F,
+                                                const uint64_t* __restrict__ PV,
+                                                const double* __restrict__ X,
+                                                const double* __restrict__ W, double* Lam,
+                                                double* __restrict__ dA, double* __restrict__ dB,
+                                                double* __restrict__ dC, int nsys, int nrow) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M), ap(sys, nsys, nrow, 1);
+    double y[M];
+    load_vec<M>(W, av, 0, y);
+    store_vec<M>(Lam, av, 0, y);
+    for (int64_t q = 1; q < nrow; ++q) {
+        double c[M][M], w[M];
+        load_mat_c<M>(Cp, am, q - 1, c);
+        load_vec<M>(W, av, q, w);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) w[i] = fma(-c[j][i], y[j], w[i]);
+#pragma unroll
+        for (int i = 0; i < M; ++i) y[i] = w[i];
+        store_vec<M>(Lam, av, q, y);
+    }
+    double t[M], xq[M], xq1[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) t[i] = xq1[i] = 0.0;
+    load_vec<M>(X, av, nrow - 1, xq);
+    for (int64_t q = nrow - 1; q >= 0; --q) {
+        double z[M], f[M][M], xm[M];
+        int piv[M];
+        load_vec<M>(Lam, av, q, z);
+        load_mat_c<M>(F, am, q, f);
+        unpack_piv<M>(PV[ap.at(q, 0)], piv);
+#pragma unroll
+        for (int i = 0; i < M; ++i) z[i] -= t[i];
+        lu_solve_t<M>(f, piv, z);
+        store_vec<M>(Lam, av, q, z);
+        if (q > 0) {
+            load_vec<M>(X, av, q - 1, xm);
+        } else {
+#pragma unroll
+            for (int i = 0; i < M; ++i) xm[i] = 0.0;
+        }
+        double* ra = dA + am.at(q, 0);
+        double* rb = dB + am.at(q, 0);
+        double* rc = dC + am.at(q, 0);
+        const int64_t es = am.estride();
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                ra[(i + M * j) * es] = -z[i] * xm[j];
+                rb[(i + M * j) * es] = -z[i] * xq[j];
+                rc[(i + M * j) * es] = -z[i] * xq1[j];
+            }
+        if (q > 0) {
+            double a[M][M];
+            load_mat_c<M>(A, am, q, a);
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < M; ++j) s = fma(a[j][i], z[j], s);
+                t[i] = s;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            xq1[i] = xq[i];
+            xq[i] = xm[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------- FP32 + refine --
+// Blocks stored in FP32; d and x in FP64. Sweep 0 is an FP32 block-Thomas solve (its
+// C'_q kept in the FP32 workspace Wc, d'_q in Wv) with FP64 back substitution into x.
+// Each refinement sweep forms the FP64 residual r_q = d_q - A_q x_{q-1} - B_q x_q -
+// C_q x_{q+1} from the FP32-stored blocks, solves T delta = r with the same FP32 block
+// factorization (R_q recomputed, C'_q recomputed in registers for the next row and read
+// back from Wc for the back substitution) and updates x += delta in FP64.
+template <int M>
+__device__ __forceinline__ bool lu_factor_f(float (&a)[M][M], int (&piv)[M]) {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        float best = fabsf(a[k][k]);
+        int p = k;
+#pragma unroll
+        for (int i = k + 1; i < M; ++i) {
+            const float v = fabsf(a[i][k]);
+            const bool c = v > best;
+            best = c ? v : best;
+            p = c ? i : p;
+        }
+        piv[k] = p;
+#pragma unroll
+        for (int i = k + 1; i < M; ++i) {
+            const bool c = (p == i);
+#pragma unroll
+            for (int j = k; j < M; ++j) {
+                const float t = a[k][j];
+                a[k][j] = c ? a[i][j] : t;
+                a[i][j] = c ? t : a[i][j];
+            }
+        }
+        ok = ok && (a[k][k] != 0.0f);
+        const float r = 1.0f / a[k][k];
+#pragma unroll
+        for (int i = k + 1; i < M; ++i) {
+            const float l = a[i][k] * r;
+            a[i][k] = l;
+#pragma unroll
+            for (int j = k + 1; j < M; ++j) a[i][j] = fmaf(-l, a[k][j], a[i][j]);
+        }
+        a[k][k] = r;
+    }
+    return ok;
+}
+
+template <int M>
+__device__ __forceinline__ void lu_solve_f(const float (&a)[M][M], const int (&piv)[M], float (&x)[M]) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+#pragma unroll
+        for (int i = k + 1; i < M; ++i) {
+            const bool c = (piv[k] == i);
+            const float t = x[k];
+            x[k] = c ? x[i] : t;
+            x[i] = c ? t : x[i];
+        }
+#pragma unroll
+        for (int i = k + 1; i < M; ++i) x[i] = fmaf(-a[i][k], x[k], x[i]);
+    }
+#pragma unroll
+    for (int k = M - 1; k >= 0; --k) {
+        float s = x[k];
+#pragma unroll
+        for (int j = k + 1; j < M; ++j) s = fmaf(-a[k][j], x[j], s);
+        x[k] = s * a[k][k];
+    }
+}
+
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_f32r(const float* __restrict__ A, const float* __restrict__ B,
+                                             const float* __restrict__ C,
+                                             const double* __restrict__ D, double* X,
+                                             float* Wc, float* Wv, int nsys, int nrow, int sweeps,
+                                             int* __restrict__ flag) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    const int64_t es = am.estride(), vs = av.estride();
+    bool ok = true;
+    for (int sweep = 0; sweep <= sweeps; ++sweep) {
+        // forward: FP32 factorization, FP32 right-hand side (d, or the FP64 residual rounded)
+        float Cp[M][M], dp[M];
+        double xm[M], xc[M];  // x_{q-1}, x_q of the previous iterate (sweeps >= 1)
+        if (sweep > 0) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) xm[i] = 0.0;
+            load_vec<M>(X, av, 0, xc);
+        }
+        for (int64_t q = 0; q < nrow; ++q) {
+            float R[M][M], a[M][M], v[M];
+            const float* rb = B + am.at(q, 0);
+            const float* ra = A + am.at(q, 0);
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    R[i][j] = rb[(i + M * j) * es];
+                    a[i][j] = q > 0 ? ra[(i + M * j) * es] : 0.0f;
+                }
+            double r[M];
+            load_vec<M>(D, av, q, r);
+            float cq[M][M];
+            const float* rc = C + am.at(q, 0);
+            const bool has_next = q + 1 < nrow;
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int i = 0; i < M; ++i) cq[i][j] = has_next ? rc[(i + M * j) * es] : 0.0f;
+            if (sweep > 0) {
+                double xp[M];
+                if (has_next) {
+                    load_vec<M>(X, av, q + 1, xp);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < M; ++i) xp[i] = 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    double s = r[i];
+#pragma unroll
+                    for (int j = 0; j < M; ++j) {
+                        s = fma(-(double)a[i][j], xm[j], s);
+                        s = fma(-(double)R[i][j], xc[j], s);
+                        s = fma(-(double)cq[i][j], xp[j], s);
+                    }
+                    r[i] = s;
+                }
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    xm[i] = xc[i];
+                    xc[i] = xp[i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < M; ++i) v[i] = (float)r[i];
+            if (q > 0) {
+#pragma unroll
+                for (int k = 0; k < M; ++k)
+#pragma unroll
+                    for (int i = 0; i < M; ++i) {
+                        v[i] = fmaf(-a[i][k], dp[k], v[i]);
+#pragma unroll
+                        for (int j = 0; j < M; ++j) R[i][j] = fmaf(-a[i][k], Cp[k][j], R[i][j]);
+                    }
+            }
+            int piv[M];
+            const bool okq = lu_factor_f<M>(R, piv);
+            if (sweep == 0) ok = ok && okq;
+            lu_solve_f<M>(R, piv, v);
+#pragma unroll
+            for (int i = 0; i < M; ++i) dp[i] = v[i];
+            float* wv = Wv + av.at(q, 0);
+#pragma unroll
+            for (int i = 0; i < M; ++i) wv[i * vs] = v[i];
+            if (has_next) {
+                float* wc = Wc + am.at(q, 0);
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    float c[M];
+#pragma unroll
+                    for (int i = 0; i < M; ++i) c[i] = cq[i][j];
+                    lu_solve_f<M>(R, piv, c);
+#pragma unroll
+                    for (int i = 0; i < M; ++i) {
+                        Cp[i][j] = c[i];
+                        if (sweep == 0) wc[(i + M * j) * es] = c[i];
+                    }
+                }
+            }
+        }
+        // backward in FP64: delta_q = d'_q - C'_q delta_{q+1}; x = delta (sweep 0) or x += delta
+        double dn[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) dn[i] = (double)dp[i];
+        {
+            double x[M];
+            if (sweep > 0) load_vec<M>(X, av, nrow - 1, x);
+#pragma unroll
+            for (int i = 0; i < M; ++i) x[i] = sweep > 0 ? x[i] + dn[i] : dn[i];
+            store_vec<M>(X, av, nrow - 1, x);
+        }
+        for (int64_t q = nrow - 2; q >= 0; --q) {
+            const float* wc = Wc + am.at(q, 0);
+            const float* wv = Wv + av.at(q, 0);
+            double dl[M];
+#pragma unroll
+            for (int i = 0; i < M; ++i) dl[i] = (double)wv[i * vs];
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+#pragma unroll
+                for (int i = 0; i < M; ++i) dl[i] = fma(-(double)wc[(i + M * j) * es], dn[j], dl[i]);
+            double x[M];
+            if (sweep > 0) load_vec<M>(X, av, q, x);
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                dn[i] = dl[i];
+                x[i] = sweep > 0 ? x[i] + dl[i] : dl[i];
+            }
+            store_vec<M>(X, av, q, x);
+        }
+    }
+    if (!ok) atomicOr(flag, 1);
+}
+
+// ------------------------------------------------------------------------------ modi --
+// Stage 1 over partition `part` (rows [r0, r0+N)) of each system, mod_btdma_cpu.f90:305-443.
+// Writes the partition's modified rows in place (Â over A, Ĉ over C, d̂ over D) and its
+// two reduced rows into the plan's reduced buffers (tiled, 2P rows per system):
+//   row 2*part   : (Â_1, I, Ĉ_1, d̂_1)  couples x_1 to x_0 (previous partition) and x_N
+//   row 2*part+1 : (Â_N, I, Ĉ_N, d̂_N)  couples x_N to x_1 and x_{N+1} (next partition)
 template <int M, int LAYOUT>
 __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict__ B, double* C,
                                              double* D, int nsys, int nrow,
@@ -497,6 +830,48 @@ hipError_t launch_plain_m(int layout, const double* A, const double* B, double* 
         hipLaunchKernelGGL((k_plain<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
     else
         hipLaunchKernelGGL((k_plain<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_plain_save_m(int layout, const double* A, const double* B, double* C, double* D,
+                               double* F, uint64_t* PV, int nsys, int nrow, int* flag,
+                               hipStream_t s) {
+    const dim3 grid((nsys + 63) / 64), block(64);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_plain_save<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, F, PV, nsys,
+                           nrow, flag);
+    else
+        hipLaunchKernelGGL((k_plain_save<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, F, PV, nsys,
+                           nrow, flag);
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_adjoint_m(int layout, const double* A, const double* Cp, const double* F,
+                            const uint64_t* PV, const double* X, const double* W, double* Lam,
+                            double* dA, double* dB, double* dC, int nsys, int nrow, hipStream_t s) {
+    const dim3 grid((nsys + 63) / 64), block(64);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_adjoint<M, LAYOUT_TILED>), grid, block, 0, s, A, Cp, F, PV, X, W, Lam, dA,
+                           dB, dC, nsys, nrow);
+    else
+        hipLaunchKernelGGL((k_adjoint<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, Cp, F, PV, X, W, Lam,
+                           dA, dB, dC, nsys, nrow);
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_f32r_m(int layout, const float* A, const float* B, const float* C,
+                         const double* D, double* X, float* Wc, float* Wv, int nsys, int nrow,
+                         int sweeps, int* flag, hipStream_t s) {
+    const dim3 grid((nsys + 63) / 64), block(64);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_f32r<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, X, Wc, Wv, nsys,
+                           nrow, sweeps, flag);
+    else
+        hipLaunchKernelGGL((k_f32r<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, X, Wc, Wv, nsys,
+                           nrow, sweeps, flag);
     return hipGetLastError();
 }
 
