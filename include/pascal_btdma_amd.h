@@ -170,6 +170,40 @@ btd_status btd_generate_f32(int kind, uint64_t seed, int m, int nsys, int nrow, 
 btd_status btd_generate_host(int kind, uint64_t seed, int m, int nsys, int nrow, int sys0,
                              int count, double *a, double *b, double *c, double *d);
 
+/* ---- Mixed-size batches (no counterpart in the reference, whose plan has one m and one
+ * nrow): systems of different block size m (1..10) and row count n (>= 2) solved by one
+ * persistent kernel that takes tiles of 64 same-m systems from a work queue in
+ * longest-first order. Packed layout: element e of row `row` of system s is at
+ * mat_off(s) + (row*m*m + e)*64 for blocks and vec_off(s) + (row*m + e)*64 for vectors
+ * (btd_batch_locate). In place like btd_solve_noncyclic. */
+typedef struct btd_batch_s *btd_batch_t;
+
+/* m[s], n[s] for s < nsys (host arrays, copied). */
+btd_status btd_batch_create(btd_batch_t *batch, int nsys, const int *m, const int *n);
+btd_status btd_batch_destroy(btd_batch_t batch);
+/* Doubles needed by the packed block arrays (A, B, C) and vector arrays (D). */
+btd_status btd_batch_sizes(btd_batch_t batch, size_t *mat_elems, size_t *vec_elems, int *ntiles);
+btd_status btd_batch_locate(btd_batch_t batch, int sys, size_t *mat_off, size_t *vec_off);
+/* Launch parameters (what the autotuner picks): placement 0 keeps all blocks in VGPRs,
+ * 1 moves m >= 6 to LDS; grid = persistent workgroups of 64 lanes. */
+btd_status btd_batch_set_params(btd_batch_t batch, int placement, int grid);
+/* The seeded generators of btd_generate, per system (system index s, its own n). */
+btd_status btd_batch_generate(btd_batch_t batch, int kind, uint64_t seed, double *a, double *b,
+                              double *c, double *d, void *stream);
+/* Asynchronous, capturable in a graph (the work counter is reset by a kernel). */
+btd_status btd_batch_solve(btd_batch_t batch, double *a, const double *b, double *c, double *d,
+                           void *stream);
+btd_status btd_batch_check(btd_batch_t batch, int *singular);
+
+/* ---- Graph capture of any sequence of the asynchronous calls above on one stream
+ * (hipStreamBeginCapture / hipGraphInstantiate / hipGraphLaunch); other streams may join
+ * the capture through events. The stream must not be the default stream. */
+typedef struct btd_graph_s *btd_graph_t;
+btd_status btd_graph_begin(void *stream);
+btd_status btd_graph_end(void *stream, btd_graph_t *graph);
+btd_status btd_graph_launch(btd_graph_t graph, void *stream);
+btd_status btd_graph_destroy(btd_graph_t graph);
+
 /* Library version string. */
 const char *btd_version(void);
 

@@ -22,7 +22,7 @@ __all__ = [
     "SingularError", "solve_noncyclic", "solve_noncyclic_profiled", "solve_cyclic",
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
     "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
-    "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems",
+    "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "BtdmaBatch", "Graph",
 ]
 
 kMaxBlockSize = 10
@@ -270,6 +270,108 @@ def solve_noncyclic_f32r(plan: BtdmaGpuPlan, a, b, c, d, x, work, sweeps: int = 
             _check_tensor(name, t, n, dt)
     _check(_capi.lib().btd_solve_noncyclic_f32r(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
                                                 _ptr(x), _ptr(work), int(sweeps), _stream(stream)))
+
+
+class BtdmaBatch:
+    """A batch of systems with their own block size m[s] (1..10) and row count n[s] (>= 2),
+    packed into 64-system tiles of equal m and solved by one persistent kernel.
+    Element e of block row `row` of system s sits at locate(s)[0] + (row*m*m + e)*64 in the
+    block arrays and at locate(s)[1] + (row*m + e)*64 in the vector array."""
+
+    def __init__(self, m, n):
+        import numpy as np
+        self.m = np.ascontiguousarray(m, dtype=np.int32)
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        if self.m.shape != self.n.shape or self.m.ndim != 1:
+            raise ValueError("m and n must be 1-D arrays of the same length")
+        self._h = ctypes.c_void_p(None)
+        ip = ctypes.POINTER(ctypes.c_int)
+        _check(_capi.lib().btd_batch_create(ctypes.byref(self._h), len(self.m),
+                                            self.m.ctypes.data_as(ip), self.n.ctypes.data_as(ip)))
+        me, ve, nt = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
+        _check(_capi.lib().btd_batch_sizes(self._h, ctypes.byref(me), ctypes.byref(ve), ctypes.byref(nt)))
+        self.mat_elems, self.vec_elems, self.ntiles = me.value, ve.value, nt.value
+        self.placement, self.grid = 0, None
+
+    @property
+    def nsys(self) -> int:
+        return len(self.m)
+
+    def locate(self, sys: int) -> tuple[int, int]:
+        mo, vo = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(_capi.lib().btd_batch_locate(self._h, int(sys), ctypes.byref(mo), ctypes.byref(vo)))
+        return mo.value, vo.value
+
+    def set_params(self, placement: int, grid: int) -> None:
+        _check(_capi.lib().btd_batch_set_params(self._h, int(placement), int(grid)))
+        self.placement, self.grid = int(placement), int(grid)
+
+    def generate(self, kind: GenKind, seed: int, a, b, c, d, stream=None) -> None:
+        _check(_capi.lib().btd_batch_generate(self._h, int(kind), int(seed), _ptr(a), _ptr(b),
+                                              _ptr(c), _ptr(d), _stream(stream)))
+
+    def solve(self, a, b, c, d, stream=None) -> None:
+        import torch
+        for name, t, n in (("a", a, self.mat_elems), ("b", b, self.mat_elems),
+                           ("c", c, self.mat_elems), ("d", d, self.vec_elems)):
+            _check_tensor(name, t, n, torch.float64)
+        _check(_capi.lib().btd_batch_solve(self._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
+                                           _stream(stream)))
+
+    def check(self) -> bool:
+        s = ctypes.c_int(0)
+        _check(_capi.lib().btd_batch_check(self._h, ctypes.byref(s)))
+        return bool(s.value)
+
+    def extract(self, flat, sys: int, vector: bool = True):
+        """Host copy of one system's vector (n, m) or blocks (n, m, m) from a packed array."""
+        import numpy as np
+        m, n = int(self.m[sys]), int(self.n[sys])
+        mo, vo = self.locate(sys)
+        E = m if vector else m * m
+        base = vo if vector else mo
+        idx = base + (np.arange(n)[:, None] * E + np.arange(E)[None, :]) * 64
+        vals = flat[idx]
+        return vals if vector else np.swapaxes(vals.reshape(n, m, m), -1, -2)
+
+    def destroy(self) -> None:
+        if self._h.value:
+            _capi.lib().btd_batch_destroy(self._h)
+        self._h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured sequence of asynchronous solves replayed with one launch
+    (btd_graph_begin / btd_graph_end / btd_graph_launch)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p(None)
+
+    def begin(self, stream) -> None:
+        _check(_capi.lib().btd_graph_begin(_stream(stream)))
+
+    def end(self, stream) -> None:
+        _check(_capi.lib().btd_graph_end(_stream(stream), ctypes.byref(self._h)))
+
+    def launch(self, stream=None) -> None:
+        _check(_capi.lib().btd_graph_launch(self._h, _stream(stream)))
+
+    def destroy(self) -> None:
+        if self._h.value:
+            _capi.lib().btd_graph_destroy(self._h)
+        self._h = ctypes.c_void_p(None)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def matrix_index(sys: int, row: int, p: int, q: int, nsys: int, nrow: int, m: int) -> int:

@@ -47,6 +47,20 @@ SIGNATURES = {
     "btd_generate_f32": (_i, [_i, _u64, _i, _i, _i, _i, _dp, _dp, _dp, _dp, _vp]),
     "btd_generate_host": (_i, [_i, _u64, _i, _i, _i, _i, _i, _dp, _dp, _dp, _dp]),
     "btd_version": (ctypes.c_char_p, []),
+    "btd_batch_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "btd_batch_destroy": (_i, [_vp]),
+    "btd_batch_sizes": (_i, [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                             ctypes.POINTER(_i)]),
+    "btd_batch_locate": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_size_t),
+                              ctypes.POINTER(ctypes.c_size_t)]),
+    "btd_batch_set_params": (_i, [_vp, _i, _i]),
+    "btd_batch_generate": (_i, [_vp, _i, _u64, _dp, _dp, _dp, _dp, _vp]),
+    "btd_batch_solve": (_i, [_vp, _dp, _dp, _dp, _dp, _vp]),
+    "btd_batch_check": (_i, [_vp, ctypes.POINTER(_i)]),
+    "btd_graph_begin": (_i, [_vp]),
+    "btd_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "btd_graph_launch": (_i, [_vp, _vp]),
+    "btd_graph_destroy": (_i, [_vp]),
 }
 
 _lib = None

@@ -33,6 +33,8 @@ COMMON = [
 UNITS = [("btd_inst.hip", [f"-DBTD_M={m}"], f"btd_inst_m{m}.o") for m in range(1, 11)] + [
     ("btd_dispatch.hip", [], "btd_dispatch.o"),
     ("btd_capi.hip", [], "btd_capi.o"),
+    ("btd_batch.hip", [], "btd_batch.o"),
+    ("btd_batch_api.hip", [], "btd_batch_api.o"),
 ]
 
 
@@ -51,7 +53,7 @@ def _compile(unit, extra: list[str], verbose: bool) -> tuple[str, float, str]:
 def build(verbose: bool = False, resource_usage: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
-    units = list(reversed(UNITS[:10])) + UNITS[10:]  # largest m first
+    units = [UNITS[-2]] + list(reversed(UNITS[:10])) + UNITS[10:-2] + UNITS[-1:]  # slowest first
     with cf.ThreadPoolExecutor(max_workers=int(os.environ.get("BTD_BUILD_JOBS", "8"))) as ex:
         results = list(ex.map(lambda u: _compile(u, extra, verbose or resource_usage), units))
     objs = [r[0] for r in results]
@@ -66,7 +68,30 @@ def build(verbose: bool = False, resource_usage: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpp_examples(verbose)
     return LIB
+
+
+def build_cpp_examples(verbose: bool = False) -> None:
+    """C++ programs under tests/cpp that use the C++ mirror header (linked against the
+    library with an $ORIGIN-relative rpath, so they run from any copy of the tree)."""
+    root = os.path.dirname(HERE)
+    src_dir = os.path.join(root, "tests", "cpp")
+    if not os.path.isdir(src_dir):
+        return
+    out_dir = os.path.join(src_dir, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    for f in sorted(os.listdir(src_dir)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, os.path.splitext(f)[0])
+        cmd = [HIPCC, "-O2", "-std=c++17", os.path.join(src_dir, f), "-I", os.path.join(root, "include"),
+               "-L", HERE, "-lpascal_btdma_amd", "-Wl,-rpath,$ORIGIN/../../../pascal_btdma_amd", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {f} failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"[build] {exe}")
 
 
 if __name__ == "__main__":

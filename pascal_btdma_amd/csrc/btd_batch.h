@@ -1,0 +1,312 @@
+// Mixed-size batches: systems of different block size m and row count n solved by one
+// persistent kernel.
+//
+// Packing: systems are grouped by m, sorted by n (longest first) and cut into tiles of up
+// to 64 systems of the same m. Each tile is stored like the tiled layout with its own row
+// count (the longest system of the tile): element e of row `row` of the system in lane l of
+// tile t is at tile_mat_off[t] + (row*E + e)*64 + l (E = m*m), vectors likewise with E = m.
+// Work order: tiles by decreasing estimated cost n_max * m^3 (longest-processing-time
+// first), consumed through an atomic counter by a grid sized to the device; every wave
+// exits when the counter passes the number of tiles.
+#pragma once
+
+#include "btd_kernels.h"
+
+namespace btd {
+
+
+// Plain block Thomas for one lane of a tile (rows 0..n-1), blocks in VGPRs.
+template <int M>
+__device__ __forceinline__ bool plain_lane(const double* __restrict__ A, const double* __restrict__ B,
+                                           double* C, double* D, int64_t mo, int64_t vo, int n) {
+    constexpr int E = M * M;
+    const int64_t es = kTile;
+    double Cp[M][M], dp[M];
+    bool ok = true;
+    for (int q = 0; q < n; ++q) {
+        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        double R[M][M], v[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int i = 0; i < M; ++i) R[i][j] = __builtin_nontemporal_load(B + rm + (i + M * j) * es);
+#pragma unroll
+        for (int i = 0; i < M; ++i) v[i] = D[rv + i * es];
+        if (q > 0) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                double ak[M];
+#pragma unroll
+                for (int i = 0; i < M; ++i) ak[i] = __builtin_nontemporal_load(A + rm + (i + M * k) * es);
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    v[i] = fma(-ak[i], dp[k], v[i]);
+#pragma unroll
+                    for (int j = 0; j < M; ++j) R[i][j] = fma(-ak[i], Cp[k][j], R[i][j]);
+                }
+            }
+        }
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, v);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dp[i] = v[i];
+            D[rv + i * es] = v[i];
+        }
+        if (q + 1 < n) {
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double c[M];
+#pragma unroll
+                for (int i = 0; i < M; ++i) c[i] = C[rm + (i + M * j) * es];
+                lu_solve<M>(R, piv, c);
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    Cp[i][j] = c[i];
+                    C[rm + (i + M * j) * es] = c[i];
+                }
+            }
+        }
+    }
+    for (int q = n - 2; q >= 0; --q) {
+        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        double x[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) x[i] = D[rv + i * es];
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int i = 0; i < M; ++i) x[i] = fma(-C[rm + (i + M * j) * es], dp[j], x[i]);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dp[i] = x[i];
+            D[rv + i * es] = x[i];
+        }
+    }
+    return ok;
+}
+
+// The same solve with the row's block R, its pivot order and the carried C' in LDS
+// (lane-interleaved, [element][lane]); the pivot exchange is then a swap of two row indices
+// instead of register selects over whole rows. Used for large m when the autotuner finds
+// it faster. sR: M*M*64 doubles, sC: M*M*64 doubles, sP: M*64 ints.
+template <int M>
+__device__ bool plain_lane_lds(const double* __restrict__ A, const double* __restrict__ B,
+                               double* C, double* D, int64_t mo, int64_t vo, int n,
+                               double* __restrict__ sR, double* __restrict__ sC,
+                               int* __restrict__ sP) {
+    constexpr int E = M * M;
+    const int64_t es = kTile;
+    const int lane = threadIdx.x;
+#define BTD_R(i, j) sR[((i) + M * (j)) * kTile + lane]
+#define BTD_CP(i, j) sC[((i) + M * (j)) * kTile + lane]
+#define BTD_P(i) sP[(i) * kTile + lane]
+    double dp[M];
+    bool ok = true;
+    for (int q = 0; q < n; ++q) {
+        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        double v[M];
+        for (int e = 0; e < E; ++e) sR[e * kTile + lane] = __builtin_nontemporal_load(B + rm + e * es);
+#pragma unroll
+        for (int i = 0; i < M; ++i) v[i] = D[rv + i * es];
+        if (q > 0) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                double ak[M];
+#pragma unroll
+                for (int i = 0; i < M; ++i) ak[i] = __builtin_nontemporal_load(A + rm + (i + M * k) * es);
+#pragma unroll
+                for (int i = 0; i < M; ++i) v[i] = fma(-ak[i], dp[k], v[i]);
+                for (int j = 0; j < M; ++j) {
+                    const double ckj = BTD_CP(k, j);
+#pragma unroll
+                    for (int i = 0; i < M; ++i) BTD_R(i, j) = fma(-ak[i], ckj, BTD_R(i, j));
+                }
+            }
+        }
+        for (int i = 0; i < M; ++i) BTD_P(i) = i;
+        for (int k = 0; k < M; ++k) {
+            int p = k;
+            double best = fabs(BTD_R(BTD_P(k), k));
+            for (int i = k + 1; i < M; ++i) {
+                const double t = fabs(BTD_R(BTD_P(i), k));
+                if (t > best) {
+                    best = t;
+                    p = i;
+                }
+            }
+            const int pk = BTD_P(p);
+            BTD_P(p) = BTD_P(k);
+            BTD_P(k) = pk;
+            ok = ok && (BTD_R(pk, k) != 0.0);
+            const double r = 1.0 / BTD_R(pk, k);
+            BTD_R(pk, k) = r;
+            for (int i = k + 1; i < M; ++i) {
+                const int pi = BTD_P(i);
+                const double l = BTD_R(pi, k) * r;
+                BTD_R(pi, k) = l;
+                for (int j = k + 1; j < M; ++j) BTD_R(pi, j) = fma(-l, BTD_R(pk, j), BTD_R(pi, j));
+            }
+        }
+        int perm[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) perm[i] = BTD_P(i);
+        // P R = L U with L, U stored in the physical rows perm[i]
+        auto solve = [&](double (&x)[M]) {
+            double y[M];
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                double s = 0.0;
+#pragma unroll
+                for (int r = 0; r < M; ++r) s = (perm[i] == r) ? x[r] : s;
+#pragma unroll
+                for (int k = 0; k < i; ++k) s = fma(-BTD_R(perm[i], k), y[k], s);
+                y[i] = s;
+            }
+#pragma unroll
+            for (int i = M - 1; i >= 0; --i) {
+                double s = y[i];
+#pragma unroll
+                for (int j = i + 1; j < M; ++j) s = fma(-BTD_R(perm[i], j), x[j], s);
+                x[i] = s * BTD_R(perm[i], i);
+            }
+        };
+        solve(v);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dp[i] = v[i];
+            D[rv + i * es] = v[i];
+        }
+        if (q + 1 < n) {
+            for (int j = 0; j < M; ++j) {
+                double c[M];
+#pragma unroll
+                for (int i = 0; i < M; ++i) c[i] = C[rm + (i + M * j) * es];
+                solve(c);
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    BTD_CP(i, j) = c[i];
+                    C[rm + (i + M * j) * es] = c[i];
+                }
+            }
+        }
+    }
+#undef BTD_R
+#undef BTD_CP
+#undef BTD_P
+    for (int q = n - 2; q >= 0; --q) {
+        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        double x[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) x[i] = D[rv + i * es];
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int i = 0; i < M; ++i) x[i] = fma(-C[rm + (i + M * j) * es], dp[j], x[i]);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dp[i] = x[i];
+            D[rv + i * es] = x[i];
+        }
+    }
+    return ok;
+}
+
+// Persistent solve of a packed batch. PLACEMENT 0 keeps every block size in VGPRs;
+// PLACEMENT 1 moves m >= 6 to the LDS variant (105 KB of LDS per 64-lane workgroup).
+template <int PLACEMENT>
+__global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restrict__ A,
+                                              const double* __restrict__ B, double* C, double* D,
+                                              int* __restrict__ counter, int* __restrict__ flag) {
+    bool ok = true;
+    const int lane = threadIdx.x;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1);
+        t = __shfl(t, 0);
+        if (t >= b.ntiles) break;
+        const int tile = b.order[t];
+        const int m = b.tile_m[tile];
+        if (lane < b.tile_count[tile]) {
+            const int n = b.lane_rows[tile * kTile + lane];
+            const int64_t mo = b.mat_off[tile] + lane, vo = b.vec_off[tile] + lane;
+            if constexpr (PLACEMENT == 1) {
+                __shared__ double sR[kMaxM * kMaxM * kTile];
+                __shared__ double sC[kMaxM * kMaxM * kTile];
+                __shared__ int sP[kMaxM * kTile];
+                switch (m) {
+                    case 6: ok = plain_lane_lds<6>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                    case 7: ok = plain_lane_lds<7>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                    case 8: ok = plain_lane_lds<8>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                    case 9: ok = plain_lane_lds<9>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                    case 10: ok = plain_lane_lds<10>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                    default: break;
+                }
+            }
+            if (PLACEMENT == 0 || m < 6) {
+                switch (m) {
+                    case 1: ok = plain_lane<1>(A, B, C, D, mo, vo, n) && ok; break;
+                    case 2: ok = plain_lane<2>(A, B, C, D, mo, vo, n) && ok; break;
+                    case 3: ok = plain_lane<3>(A, B, C, D, mo, vo, n) && ok; break;
+                    case 4: ok = plain_lane<4>(A, B, C, D, mo, vo, n) && ok; break;
+                    case 5: ok = plain_lane<5>(A, B, C, D, mo, vo, n) && ok; break;
+                    case 6: ok = plain_lane<6>(A, B, C, D, mo, vo, n) && ok; break;
+                    case 7: ok = plain_lane<7>(A, B, C, D, mo, vo, n) && ok; break;
+                    case 8: ok = plain_lane<8>(A, B, C, D, mo, vo, n) && ok; break;
+                    case 9: ok = plain_lane<9>(A, B, C, D, mo, vo, n) && ok; break;
+                    case 10: ok = plain_lane<10>(A, B, C, D, mo, vo, n) && ok; break;
+                    default: break;
+                }
+            }
+        }
+    }
+    if (!ok) atomicOr(flag, 1);
+}
+
+// Work-counter reset as a kernel (a memset node would not replay reliably in a graph).
+__global__ void k_batch_reset(int* counter) { counter[0] = 0; }
+
+// Generator for a packed batch: one thread per (tile, row, lane).
+template <int M>
+__device__ __forceinline__ void gen_batch_elem(int kind, uint64_t seed, int64_t sys, int row, int n,
+                                               int64_t mo, int64_t vo, double* A, double* B,
+                                               double* C, double* D) {
+    double a[M][M], bb[M][M], c[M][M], d[M];
+    gen_row<M>(kind, seed, sys, row, n, a, bb, c, d);
+    const int64_t rm = mo + (int64_t)row * M * M * kTile, rv = vo + (int64_t)row * M * kTile;
+    for (int j = 0; j < M; ++j)
+        for (int i = 0; i < M; ++i) {
+            A[rm + (i + M * j) * kTile] = a[i][j];
+            B[rm + (i + M * j) * kTile] = bb[i][j];
+            C[rm + (i + M * j) * kTile] = c[i][j];
+        }
+    for (int i = 0; i < M; ++i) D[rv + i * kTile] = d[i];
+}
+
+__global__ __launch_bounds__(256) void k_batch_generate(BatchDev b, const int* __restrict__ lane_sys,
+                                                        int kind, uint64_t seed, double* A, double* B,
+                                                        double* C, double* D) {
+    const int tile = blockIdx.y;
+    const int row = blockIdx.x * 4 + threadIdx.x / kTile, lane = threadIdx.x % kTile;
+    if (lane >= b.tile_count[tile]) return;
+    const int n = b.lane_rows[tile * kTile + lane];
+    if (row >= n) return;
+    const int64_t sys = lane_sys[tile * kTile + lane];
+    const int64_t mo = b.mat_off[tile] + lane, vo = b.vec_off[tile] + lane;
+    switch (b.tile_m[tile]) {
+        case 1: gen_batch_elem<1>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        case 2: gen_batch_elem<2>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        case 3: gen_batch_elem<3>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        case 4: gen_batch_elem<4>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        case 5: gen_batch_elem<5>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        case 6: gen_batch_elem<6>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        case 7: gen_batch_elem<7>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        case 8: gen_batch_elem<8>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        case 9: gen_batch_elem<9>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        case 10: gen_batch_elem<10>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+        default: break;
+    }
+}
+
+}  // namespace btd

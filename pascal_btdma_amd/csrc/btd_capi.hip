@@ -34,6 +34,8 @@ btd_status hip_fail(hipError_t e, const char* what) {
 
 }  // namespace
 
+void btd::set_last_error(const char* msg) { g_err = msg; }
+
 struct btd_plan_s {
     int m = 0, nsys = 0, nrow = 0, npart = 1, layout = BTD_LAYOUT_TILED;
     int device = 0;

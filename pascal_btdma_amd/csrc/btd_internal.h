@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "btd_common.h"
 #include "btd_dense.h"
 
@@ -32,6 +34,28 @@ hipError_t launch_two_stage(int m, int layout, double* A, const double* B, doubl
 hipError_t launch_generate(int m, int kind, uint64_t seed, int layout, int nsys, int nrow, double* A,
                            double* B, double* C, double* D, float* Af, float* Bf, float* Cf,
                            hipStream_t s);
+
+// Device tables of a packed mixed-size batch (see btd_batch.h).
+struct BatchDev {
+    const int* order;          // tile ids in work order
+    const int* tile_m;         // block size of each tile
+    const int* tile_rows;      // rows of the longest system of each tile
+    const int* tile_count;     // systems in the tile (<= 64)
+    const int* lane_rows;      // rows of each (tile, lane): [tile*64 + lane]
+    const int64_t* mat_off;    // first matrix element of each tile
+    const int64_t* vec_off;    // first vector element of each tile
+    int ntiles;
+};
+
+hipError_t launch_batch(const BatchDev& b, int placement, int grid, const double* A, const double* B,
+                        double* C, double* D, int* counter, int* flag, hipStream_t s);
+
+hipError_t launch_batch_generate(const BatchDev& b, const int* lane_sys, int max_rows, int kind,
+                                 uint64_t seed, double* A, double* B, double* C, double* D,
+                                 hipStream_t s);
+
+// Sets the thread-local message returned by btd_last_error().
+void set_last_error(const char* msg);
 
 hipError_t launch_convert(int src_layout, int dst_layout, int E, int nsys, int nrow,
                           const double* src, double* dst, hipStream_t s);
