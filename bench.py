@@ -1,15 +1,18 @@
 """Benchmark of the batched block-Thomas solve on one MI355X.
 
-Workload (default): the Euler-3D-like sweep, m = 5, n = 512 block rows, 131072 systems,
+Headline workload: the Euler-3D-like sweep, m = 5, n = 512 block rows, 131072 systems,
 FP64, single partition, tiled layout; inputs generated on the device and resident in
 HBM before the timed region. A step is one solve of the whole batch. The solve is in
 place (C and D are overwritten), so C and D are restored from pristine copies before
-every step, outside the event-timed region.
+every step, outside the event-timed region (events are recorded on the solve's stream).
 
 Prints one JSON line: block-rows/s, the HBM roofline of the solve kernel (algorithmic
 bytes 8*(3m^2+2m) per block row over the HIP-event time of the kernel), the maximum
-relative error vs the CPU oracle on a subset of systems, and the CPU baseline (the
-oracle on a bounded sample of the same workload, on the host cores).
+relative error vs the CPU oracle on a subset of systems, the CPU baseline (the oracle on
+a bounded sample of the same workload, on the host cores), and under "extras" the other
+configurations: the reference Study's single-GPU case, the small CPU-sized case, the
+mixed-size batch with long partitioned systems in one graph, FP32 storage with FP64
+refinement, and the differentiable solve (forward and adjoint).
 """
 from __future__ import annotations
 
@@ -18,6 +21,7 @@ import json
 import os
 import sys
 import time
+import traceback
 
 import numpy as np
 
@@ -34,6 +38,11 @@ WORKLOADS = {
                       name="diagonally dominant permuted rows, m=3, n=64, 1024 systems, FP64 plain"),
 }
 
+# The reference's only published numbers for this path: Study, 1 x H200, CUDA C++ port,
+# diagonal workload 32 x 32 x 128 (nsys 1024, n 128), solve time in ms
+# (Study/result/tables/summary_by_case.csv:2-5).
+STUDY_H200_MS = {2: 2.134, 3: 3.901, 5: 11.581, 8: 28.550}
+
 
 def algorithmic_bytes(m: int, rows: int) -> float:
     """FP64 plain solve: read A, B, C, d once and write x once per block row."""
@@ -46,6 +55,24 @@ def tiled_vec_subset(flat: np.ndarray, n: int, m: int, tiles: list[int]) -> np.n
         blk = flat[t * n * m * 64:(t + 1) * n * m * 64].reshape(n, m, 64)
         out.append(np.transpose(blk, (0, 2, 1)))
     return np.concatenate(out, axis=1)  # (n, 64*len(tiles), m)
+
+
+def rel_err_sys(x: np.ndarray, xo: np.ndarray) -> np.ndarray:
+    return np.abs(x - xo).max(axis=(0, 2)) / np.abs(xo).max(axis=(0, 2))
+
+
+def event_time(fn, restore, steps: int, stream) -> np.ndarray:
+    """Per-step HIP-event times (ms) of fn on `stream`, restore() outside the events."""
+    import torch
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    for e0, e1 in evs:
+        restore()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    return np.array([e0.elapsed_time(e1) for e0, e1 in evs])
 
 
 def cpu_baseline(w: dict, target_s: float = 12.0) -> dict:
@@ -70,7 +97,7 @@ def cpu_baseline(w: dict, target_s: float = 12.0) -> dict:
 
     S = max(threads, 64)
     dt = run(gen(S))
-    S = int(min(w["nsys"], max(S, S * target_s / 3.0 / max(dt, 1e-3))))
+    S = int(min(w["nsys"], 16384, max(S, S * target_s / 3.0 / max(dt, 1e-3))))
     S = max(threads, S - S % threads)
     arrs = gen(S)
     total_t, total_rows, reps = 0.0, 0, 0
@@ -80,9 +107,266 @@ def cpu_baseline(w: dict, target_s: float = 12.0) -> dict:
         reps += 1
     return {"value": total_rows / total_t, "unit": "block-rows/s", "cores": pyoracle.last_threads(),
             "kind": "port",
-            "sample": f"first {S} systems of the same workload, solved {reps}x "
-                      f"({total_rows} block rows, {total_t:.1f} s) with the C oracle, "
-                      f"OpenMP over systems"}
+            "sample": f"first {S} systems of the same workload ({S * n} block rows), solved {reps}x "
+                      f"in {total_t:.1f} s with the C oracle, OpenMP over systems"}
+
+
+# ------------------------------------------------------------------------------ extras --
+
+def extra_study(dev) -> dict:
+    """The reference Study's single-GPU case (diagonal workload, 32 x 32 x 128) for m = 2, 3,
+    5, 8: time of one solve at npart = 1 and at the best npart, vs the published H200 time."""
+    import torch
+    import pascal_btdma_amd as pb
+    nsys, n = 1024, 128
+    lay = pb.Layout.REF_GPU
+    out = {}
+    for m, h200 in STUDY_H200_MS.items():
+        a, b, c, d = (torch.empty(nsys * n * (m * m if i < 3 else m), dtype=torch.float64, device=dev)
+                      for i in range(4))
+        pb.generate(pb.GenKind.DIAG, 0, m, nsys, n, lay, a, b, c, d)
+        c0, d0 = c.clone(), d.clone()
+        a0 = a.clone()
+        res = {}
+        for P in (1, 2, 4, 8, 16):
+            with pb.BtdmaGpuPlan().create(m, nsys, n, P, lay) as plan:
+                def restore():
+                    a.copy_(a0)
+                    c.copy_(c0)
+                    d.copy_(d0)
+                s = torch.cuda.current_stream()
+                restore()
+                pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n)
+                t = event_time(lambda: pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n), restore, 5, s)
+                res[P] = float(np.median(t))
+        best = min(res, key=res.get)
+        out[f"m{m}"] = {"ms_npart1": res[1], "best_npart": best, "ms_best": res[best],
+                        "h200_ms_published": h200, "speedup_npart1": h200 / res[1],
+                        "speedup_best": h200 / res[best],
+                        "block_rows_per_s_best": nsys * n / (res[best] * 1e-3)}
+    return out
+
+
+def extra_config0(dev) -> dict:
+    """Config 0: 1024 systems, m = 3, n = 64 (the CPU-runnable case), FP64 plain."""
+    import torch
+    import pascal_btdma_amd as pb
+    from oracle import pyoracle
+    w = WORKLOADS["ddperm_m3"]
+    m, n, nsys = w["m"], w["n"], w["nsys"]
+    lay = pb.Layout.TILED
+    a, b, c = (torch.empty(pb.layout_elems(lay, nsys, n, m * m), dtype=torch.float64, device=dev)
+               for _ in range(3))
+    d = torch.empty(pb.layout_elems(lay, nsys, n, m), dtype=torch.float64, device=dev)
+    pb.generate(w["kind"], w["seed"], m, nsys, n, lay, a, b, c, d)
+    c0, d0 = c.clone(), d.clone()
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, lay) as plan:
+        def restore():
+            c.copy_(c0)
+            d.copy_(d0)
+        restore()
+        pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n)
+        t = event_time(lambda: pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n), restore, 10,
+                       torch.cuda.current_stream())
+    x = tiled_vec_subset(d.cpu().numpy(), n, m, list(range(nsys // 64)))
+    A, B, C, dd = pb.generate_host(w["kind"], w["seed"], m, nsys, n)
+    xo, _ = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
+                                 pyoracle.to_cpu_layout(C), dd)
+    ms = float(np.median(t))
+    return {"workload": w["name"], "ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3),
+            "max_rel_err_vs_oracle_all_systems": float(rel_err_sys(x, xo).max())}
+
+
+def extra_batch(dev, steps: int) -> dict:
+    """Config 2: 20000 systems with m uniform in {2,3,5,8,10}, n log-uniform in [32,1024],
+    plus 512 long systems (m = 5, n = 65536) split into P partitions, all replayed as one
+    captured graph (the batch's persistent kernel and the long systems' three kernels on a
+    forked branch). The autotuner picks the batch placement/grid and P in {8,16,32}."""
+    import torch
+    import pascal_btdma_amd as pb
+    from oracle import pyoracle
+    from pascal_btdma_amd.tuner import tune_batch, tune_partitions
+    rng = np.random.default_rng(2026)
+    nb = 20000
+    m = rng.choice(np.array([2, 3, 5, 8, 10]), nb)
+    n = np.rint(np.exp(rng.uniform(np.log(32), np.log(1024), nb))).astype(np.int32)
+    bt = pb.BtdmaBatch(m, n)
+    a, b, c0 = (torch.empty(bt.mat_elems, dtype=torch.float64, device=dev) for _ in range(3))
+    d0 = torch.empty(bt.vec_elems, dtype=torch.float64, device=dev)
+    bt.generate(pb.GenKind.DDPERM, 7, a, b, c0, d0)
+    tb = tune_batch(bt, a, b, c0, d0, reps=2)
+    lm, lnsys, ln = 5, 512, 65536
+    lay = pb.Layout.TILED
+    la0, lb, lc0 = (torch.empty(pb.layout_elems(lay, lnsys, ln, lm * lm), dtype=torch.float64, device=dev)
+                    for _ in range(3))
+    ld0 = torch.empty(pb.layout_elems(lay, lnsys, ln, lm), dtype=torch.float64, device=dev)
+    pb.generate(pb.GenKind.DDPERM, 8, lm, lnsys, ln, lay, la0, lb, lc0, ld0)
+    tp = tune_partitions(lm, lnsys, ln, la0, lb, lc0, ld0, lay, candidates=(8, 16, 32), reps=2)
+    plan = pb.BtdmaGpuPlan().create(lm, lnsys, ln, tp["npart"], lay)
+    c, d, la, lc, ld = c0.clone(), d0.clone(), la0.clone(), lc0.clone(), ld0.clone()
+
+    def restore():
+        c.copy_(c0)
+        d.copy_(d0)
+        la.copy_(la0)
+        lc.copy_(lc0)
+        ld.copy_(ld0)
+
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    g = pb.Graph()
+    fork, join = torch.cuda.Event(), torch.cuda.Event()
+    torch.cuda.synchronize()
+    g.begin(s1)
+    fork.record(s1)
+    s2.wait_event(fork)
+    bt.solve(a, b, c, d, stream=s1)
+    pb.solve_noncyclic(plan, la, lb, lc, ld, lm, lnsys, ln, stream=s2)
+    join.record(s2)
+    s1.wait_event(join)
+    g.end(s1)
+    restore()
+    torch.cuda.synchronize()
+    g.launch(s1)
+    s1.synchronize()
+    t = event_time(lambda: g.launch(s1), lambda: (restore(), torch.cuda.synchronize()), steps, s1)
+    # the batch alone and the long systems alone, for the record
+    tb_only = event_time(lambda: bt.solve(a, b, c, d), restore, 3, torch.cuda.current_stream())
+    tl_only = event_time(lambda: pb.solve_noncyclic(plan, la, lb, lc, ld, lm, lnsys, ln), restore, 3,
+                         torch.cuda.current_stream())
+    # parity on a sample of batch systems and 2 long systems
+    restore()
+    torch.cuda.synchronize()
+    g.launch(s1)
+    s1.synchronize()
+    hd = d.cpu().numpy()
+    worst = 0.0
+    for s in range(0, nb, 997):
+        A, B, C, dd = pb.generate_host(pb.GenKind.DDPERM, 7, int(m[s]), nb, int(n[s]), s, 1)
+        xo, _ = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
+                                     pyoracle.to_cpu_layout(C), dd)
+        x = bt.extract(hd, s)
+        worst = max(worst, float(np.abs(x - xo[:, 0]).max() / np.abs(xo[:, 0]).max()))
+    A, B, C, dd = pb.generate_host(pb.GenKind.DDPERM, 8, lm, lnsys, ln, 0, 2)
+    xo, _ = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
+                                 pyoracle.to_cpu_layout(C), dd)
+    xl = tiled_vec_subset(ld[: ln * lm * 64].cpu().numpy(), ln, lm, [0])[:, :2]
+    worst_long = float(rel_err_sys(xl, xo).max())
+    g.destroy()
+    plan.destroy()
+    rows_b = int(n.sum())
+    rows_l = lnsys * ln
+    bytes_b = float(sum(8.0 * (3 * mm * mm + 2 * mm) * nn for mm, nn in zip(m.tolist(), n.tolist())))
+    bytes_l = algorithmic_bytes(lm, rows_l)
+    ms = float(np.median(t))
+    return {"workload": "20000 systems m in {2,3,5,8,10}, n log-uniform [32,1024] + 512 systems m=5 "
+                        "n=65536 in P partitions, one graph launch",
+            "ms": ms, "block_rows": rows_b + rows_l,
+            "block_rows_per_s": (rows_b + rows_l) / (ms * 1e-3),
+            "algorithmic_GBs": (bytes_b + bytes_l) / (ms * 1e-3) / 1e9,
+            "hbm_frac": (bytes_b + bytes_l) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "batch_only_ms": float(np.median(tb_only)), "long_only_ms": float(np.median(tl_only)),
+            "long_only_GBs": bytes_l / (np.median(tl_only) * 1e-3) / 1e9,
+            "tuned_batch": tb, "tuned_partitions": tp, "batch_tiles": bt.ntiles,
+            "max_rel_err_vs_oracle_sampled": max(worst, worst_long)}
+
+
+def extra_f32r(dev, steps: int, fp64_ms: float) -> dict:
+    """Config 3: m = 5, n = 512, 131072 systems, blocks with cond log-uniform [1e1, 1e6]
+    stored in FP32; FP32 solve + s FP64-residual refinement sweeps, s = 0..3, vs the FP64
+    plain kernel's time on config 1, with the error vs the oracle on the same FP32 inputs."""
+    import torch
+    import pascal_btdma_amd as pb
+    from oracle import pyoracle
+    m, n, nsys = 5, 512, 131072
+    lay = pb.Layout.TILED
+    a, b, c = (torch.empty(pb.layout_elems(lay, nsys, n, m * m), dtype=torch.float32, device=dev)
+               for _ in range(3))
+    d = torch.empty(pb.layout_elems(lay, nsys, n, m), dtype=torch.float64, device=dev)
+    pb.generate_f32(pb.GenKind.COND, 3, m, nsys, n, lay, a, b, c, d)
+    x = torch.empty_like(d)
+    out = {"workload": "cond-controlled B (1e1..1e6), m=5, n=512, 131072 systems, FP32 blocks",
+           "fp64_plain_ms_config1": fp64_ms}
+    A, B, C, dd = pb.generate_host(pb.GenKind.COND, 3, m, nsys, n, 0, 128)
+    xo, _ = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
+                                 pyoracle.to_cpu_layout(C), dd)
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, lay) as plan:
+        work = torch.empty(pb.f32r_work_elems(plan), dtype=torch.float32, device=dev)
+        for sweeps in (0, 1, 2, 3):
+            pb.solve_noncyclic_f32r(plan, a, b, c, d, x, work, sweeps)
+            t = event_time(lambda: pb.solve_noncyclic_f32r(plan, a, b, c, d, x, work, sweeps),
+                           lambda: None, steps, torch.cuda.current_stream())
+            xs = tiled_vec_subset(x[: 2 * n * m * 64].cpu().numpy(), n, m, [0, 1])
+            err = rel_err_sys(xs, xo)
+            ms = float(np.median(t))
+            out[f"sweeps{sweeps}"] = {"ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3),
+                                      "speedup_vs_fp64_plain": fp64_ms / ms,
+                                      "max_rel_err_vs_oracle": float(err.max()),
+                                      "median_rel_err_vs_oracle": float(np.median(err)),
+                                      "frac_systems_le_1e-12": float((err <= 1e-12).mean())}
+        # the FP64 plain kernel on the same (FP32-rounded) inputs: the two-FP64-solver floor
+        a64, b64, c64 = a.double(), b.double(), c.double()
+        d64 = d.clone()
+        pb.solve_noncyclic(plan, a64, b64, c64, d64, m, nsys, n)
+        xs = tiled_vec_subset(d64[: 2 * n * m * 64].cpu().numpy(), n, m, [0, 1])
+        out["fp64_plain_same_inputs_max_rel_err"] = float(rel_err_sys(xs, xo).max())
+    out["error_sample"] = "first 128 systems vs the oracle on the same FP32-rounded blocks"
+    return out
+
+
+def extra_adjoint(dev, steps: int) -> dict:
+    """Config 4: WENO-like m = 3, n = 256, 262144 systems through the autograd Function:
+    forward (with factors kept) and backward (adjoint kernel) times, raw kernel times."""
+    import torch
+    import pascal_btdma_amd as pb
+    from pascal_btdma_amd.autograd import block_tridiagonal_solve
+    m, n, nsys = 3, 256, 262144
+    lay = pb.Layout.TILED
+    a, b, c = (torch.empty(pb.layout_elems(lay, nsys, n, m * m), dtype=torch.float64, device=dev)
+               for _ in range(3))
+    d = torch.empty(pb.layout_elems(lay, nsys, n, m), dtype=torch.float64, device=dev)
+    pb.generate(pb.GenKind.EULER, 4, m, nsys, n, lay, a, b, c, d)
+    w = torch.rand_like(d) * 2 - 1
+    plan = pb.BtdmaGpuPlan().create(m, nsys, n, 1, lay)
+    for t_ in (a, b, c, d):
+        t_.requires_grad_(True)
+    s = torch.cuda.current_stream()
+    fwd, bwd = [], []
+    for it in range(steps + 1):
+        for t_ in (a, b, c, d):
+            t_.grad = None
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(s)
+        x = block_tridiagonal_solve(plan, a, b, c, d)
+        e1.record(s)
+        x.backward(w)
+        e2.record(s)
+        torch.cuda.synchronize()
+        if it > 0:
+            fwd.append(e0.elapsed_time(e1))
+            bwd.append(e1.elapsed_time(e2))
+        del x
+    with torch.no_grad():
+        cw, xw = c.detach().clone(), d.detach().clone()
+        F = torch.empty_like(cw)
+        P = torch.empty(pb.layout_elems(lay, nsys, n, 1), dtype=torch.int64, device=dev)
+        ad = a.detach()
+        kf = event_time(lambda: pb.solve_noncyclic_save(plan, ad, b.detach(), cw, xw, F, P),
+                        lambda: (cw.copy_(c.detach()), xw.copy_(d.detach())), steps, s)
+        lam = torch.empty_like(xw)
+        da, db, dc = (torch.empty_like(ad) for _ in range(3))
+        kb = event_time(lambda: pb.solve_adjoint(plan, ad, cw, F, P, xw, w, lam, da, db, dc),
+                        lambda: None, steps, s)
+    plan.destroy()
+    rows = nsys * n
+    f_ms, b_ms = float(np.median(fwd)), float(np.median(bwd))
+    kf_ms, kb_ms = float(np.median(kf)), float(np.median(kb))
+    return {"workload": "WENO-like m=3, n=256, 262144 systems, FP64, autograd",
+            "forward_ms": f_ms, "backward_ms": b_ms, "backward_over_forward": b_ms / f_ms,
+            "kernel_forward_save_ms": kf_ms, "kernel_adjoint_ms": kb_ms,
+            "kernel_backward_over_forward": kb_ms / kf_ms,
+            "forward_block_rows_per_s": rows / (f_ms * 1e-3),
+            "kernel_adjoint_alg_GBs": 8.0 * (6 * m * m + 5 * m) * rows / (kb_ms * 1e-3) / 1e9,
+            "fp32_refine_variant": "not built (the adjoint runs in FP64)"}
 
 
 def main() -> int:
@@ -93,8 +377,22 @@ def main() -> int:
     ap.add_argument("--workload", default="euler_m5", choices=sorted(WORKLOADS))
     ap.add_argument("--layout", default="tiled", choices=["tiled", "ref_gpu"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extras", action="store_true", help="headline workload only")
+    ap.add_argument("--only", default="", help="comma list of extras to run (study,config0,batch,f32r,adjoint)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="only copy one buffer of this many GiB to another (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_gib > 0:
+        import torch
+        nel = int(args.copy_probe_gib * 2**30) // 8
+        src = torch.empty(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": nel * 8, "copy_probe_bytes_written": nel * 8}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(independent systems shard across GPUs with no communication; see DESIGN.md)",
@@ -127,29 +425,30 @@ def main() -> int:
         restore()
         pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n)
     torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for e0, e1 in evs:
-        restore()
-        e0.record(stream)
-        pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n)
-        e1.record(stream)
-    torch.cuda.synchronize()
+    times = event_time(lambda: pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n), restore,
+                       args.steps, stream)
     wall = time.perf_counter() - t0
-    times = np.array([e0.elapsed_time(e1) for e0, e1 in evs])  # ms
     singular = plan.check()
     ms = float(times.mean())
     rows = nsys * n
     value = rows / (ms * 1e-3)
     achieved = algorithmic_bytes(m, rows) / (ms * 1e-3) / 1e9
 
+    # measured copy bandwidth on this device (torch D2D copy of C: read + write)
+    tc = event_time(lambda: c.copy_(c0), lambda: None, 3, stream)
+    copy_gbs = 2 * nm * 8 / (float(np.median(tc)) * 1e-3) / 1e9
+    restore()
+    pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n)
+    torch.cuda.synchronize()
+
     # parity on a subset of systems: the first two and the last tiles
     ntiles = (nsys + 63) // 64
     tiles = sorted({0, 1, ntiles - 1})
     if lay == pb.Layout.TILED:
-        x = tiled_vec_subset(d.cpu().numpy(), n, m, tiles)
+        tile_v = n * m * 64
+        x = np.concatenate([tiled_vec_subset(d[t * tile_v:(t + 1) * tile_v].cpu().numpy(), n, m, [0])
+                            for t in tiles], axis=1)
         sys_ids = [t * 64 + j for t in tiles for j in range(64)]
     else:
         full = d.view(m, n, nsys)
@@ -161,15 +460,39 @@ def main() -> int:
         A, B, C, dd = pb.generate_host(w["kind"], w["seed"], m, nsys, n, s0, 64)
         xo, info = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
                                         pyoracle.to_cpu_layout(C), dd)
-        xs = x[:, lo:lo + 64]
-        errs.append((np.abs(xs - xo).max(axis=(0, 2)) / np.abs(xo).max(axis=(0, 2))).max())
+        errs.append(rel_err_sys(x[:, lo:lo + 64], xo).max())
     max_err = float(max(errs))
+    plan.destroy()
+    del a, b, c, d, c0, d0
+    torch.cuda.empty_cache()
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_plain_{args.workload}_{args.layout}.json")
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("bytes_per_launch")
+
+    extras = {}
+    if not args.no_extras:
+        only = set(filter(None, args.only.split(",")))
+        jobs = [("study", lambda: extra_study(dev)),
+                ("config0", lambda: extra_config0(dev)),
+                ("batch", lambda: extra_batch(dev, 3)),
+                ("f32r", lambda: extra_f32r(dev, 3, ms)),
+                ("adjoint", lambda: extra_adjoint(dev, 3))]
+        for name, job in jobs:
+            if only and name not in only:
+                continue
+            t_start = time.perf_counter()
+            try:
+                extras[name] = job()
+            except Exception as e:  # report, keep the headline line
+                extras[name] = {"error": f"{type(e).__name__}: {e}",
+                                "trace": traceback.format_exc(limit=3)}
+            extras[name]["bench_seconds"] = round(time.perf_counter() - t_start, 1)
+            torch.cuda.empty_cache()
+            print(f"[bench] extra {name} done in {extras[name]['bench_seconds']} s", file=sys.stderr,
+                  flush=True)
 
     cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_seconds)
     line = {
@@ -185,10 +508,11 @@ def main() -> int:
         "max_rel_err_vs_oracle": max_err, "parity_systems": len(sys_ids),
         "singular": singular, "ms_min": float(times.min()), "ms_max": float(times.max()),
         "wall_s_timed_region_incl_restore": wall,
+        "measured_copy_GBs": copy_gbs,
         "kernel": f"k_plain<{m},{'TILED' if lay == pb.Layout.TILED else 'REF_GPU'}>",
+        "extras": extras,
     }
     print(json.dumps(line), flush=True)
-    plan.destroy()
     return 0
 
 
