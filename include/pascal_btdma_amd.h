@@ -184,8 +184,10 @@ btd_status btd_batch_destroy(btd_batch_t batch);
 /* Doubles needed by the packed block arrays (A, B, C) and vector arrays (D). */
 btd_status btd_batch_sizes(btd_batch_t batch, size_t *mat_elems, size_t *vec_elems, int *ntiles);
 btd_status btd_batch_locate(btd_batch_t batch, int sys, size_t *mat_off, size_t *vec_off);
-/* Launch parameters (what the autotuner picks): placement 0 keeps all blocks in VGPRs,
- * 1 moves m >= 6 to LDS; grid = persistent workgroups of 64 lanes. */
+/* Launch parameters (what the autotuner picks): placement 0 = one lane per system, blocks
+ * in VGPRs; 1 = as 0 but m >= 6 with blocks in LDS; 2, 3, 4 = m lanes per system (one block
+ * row per lane, pivot search and row exchange through LDS) for m >= 6, m >= 4, m >= 2;
+ * grid = persistent workgroups of 64 lanes. */
 btd_status btd_batch_set_params(btd_batch_t batch, int placement, int grid);
 /* The seeded generators of btd_generate, per system (system index s, its own n). */
 btd_status btd_batch_generate(btd_batch_t batch, int kind, uint64_t seed, double *a, double *b,

@@ -5,9 +5,11 @@
 // to 64 systems of the same m. Each tile is stored like the tiled layout with its own row
 // count (the longest system of the tile): element e of row `row` of the system in lane l of
 // tile t is at tile_mat_off[t] + (row*E + e)*64 + l (E = m*m), vectors likewise with E = m.
-// Work order: tiles by decreasing estimated cost n_max * m^3 (longest-processing-time
-// first), consumed through an atomic counter by a grid sized to the device; every wave
-// exits when the counter passes the number of tiles.
+// Work items: a whole tile (one lane per system) or, for the cooperative placement, a
+// chunk of floor(64/m) systems of a tile with m lanes per system. Items are ordered by
+// decreasing estimated cost (longest-processing-time first) and consumed through an
+// atomic counter by a grid sized to the device; every wave exits when the counter passes
+// the number of items.
 #pragma once
 
 #include "btd_kernels.h"
@@ -213,38 +215,206 @@ __device__ bool plain_lane_lds(const double* __restrict__ A, const double* __res
     return ok;
 }
 
-// Persistent solve of a packed batch. PLACEMENT 0 keeps every block size in VGPRs;
-// PLACEMENT 1 moves m >= 6 to the LDS variant (105 KB of LDS per 64-lane workgroup).
-template <int PLACEMENT>
+// Cooperative solve: m lanes per system, lane i owns block row i of every block. The row
+// product A_q C'_{q-1}, the pivot search, the pivot row and the back-substitution values are
+// exchanged through LDS (sh: W = 2m+2 doubles per lane, [w][lane]). Partial pivoting: the
+// pivot of column k is the largest |r(k)| among the rows not yet chosen (lowest lane on a
+// tie). All lanes of the wave run the same number of row steps (nmax) so every barrier is
+// reached by the whole workgroup; lanes past their system's n, and lanes of no system
+// (n = 0), compute on zeros and store nothing.
+template <int M>
+__device__ bool coop_rows(const double* __restrict__ A, const double* __restrict__ B, double* C,
+                          double* D, int64_t mo, int64_t vo, int i, int base, int n, int nmax,
+                          double* __restrict__ sh) {
+    constexpr int E = M * M;
+    const int lane = threadIdx.x;
+    const int64_t es = kTile;
+    bool ok = true;
+    double xlast = 0.0;
+    for (int q = 0; q < nmax; ++q) {
+        const bool act = q < n;
+        const bool has_next = q + 1 < n;
+        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        double r[M], rhs[M + 1];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            r[j] = act ? __builtin_nontemporal_load(B + rm + (i + M * j) * es) : 0.0;
+            rhs[1 + j] = has_next ? C[rm + (i + M * j) * es] : 0.0;
+        }
+        rhs[0] = act ? D[rv + i * es] : 0.0;
+        if (q > 0) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const double ak = act ? __builtin_nontemporal_load(A + rm + (i + M * k) * es) : 0.0;
+                rhs[0] = fma(-ak, sh[M * kTile + base + k], rhs[0]);
+#pragma unroll
+                for (int j = 0; j < M; ++j) r[j] = fma(-ak, sh[j * kTile + base + k], r[j]);
+            }
+        }
+        __syncthreads();
+        int rank = -1;
+        int plane[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            sh[lane] = rank < 0 ? fabs(r[k]) : -1.0;
+            __syncthreads();
+            int p = 0;
+            double best = -2.0;
+#pragma unroll
+            for (int t = 0; t < M; ++t) {
+                const double v = sh[base + t];
+                const bool c = v > best;
+                best = c ? v : best;
+                p = c ? t : p;
+            }
+            plane[k] = p;
+            __syncthreads();
+            if (i == p) {
+#pragma unroll
+                for (int j = 0; j < M; ++j) sh[j * kTile + lane] = r[j];
+#pragma unroll
+                for (int j = 0; j <= M; ++j) sh[(M + j) * kTile + lane] = rhs[j];
+                rank = k;
+            }
+            __syncthreads();
+            const int pl = base + p;
+            const double pk = sh[k * kTile + pl];
+            ok = ok && (!act || pk != 0.0);
+            if (rank < 0) {
+                const double l = r[k] / pk;
+#pragma unroll
+                for (int j = k + 1; j < M; ++j) r[j] = fma(-l, sh[j * kTile + pl], r[j]);
+#pragma unroll
+                for (int j = 0; j <= M; ++j) rhs[j] = fma(-l, sh[(M + j) * kTile + pl], rhs[j]);
+            }
+            __syncthreads();
+        }
+        // back substitution: the lane of rank k holds sum_{j>=k} r(j) x_j = rhs
+#pragma unroll
+        for (int k = M - 1; k >= 0; --k) {
+            if (rank == k) {
+                const double rk = 1.0 / r[k];
+#pragma unroll
+                for (int j = 0; j <= M; ++j) {
+                    rhs[j] *= rk;
+                    sh[(M + j) * kTile + lane] = rhs[j];
+                }
+            }
+            __syncthreads();
+            const int pl = base + plane[k];
+            if (rank < k) {
+#pragma unroll
+                for (int j = 0; j <= M; ++j) rhs[j] = fma(-r[k], sh[(M + j) * kTile + pl], rhs[j]);
+            }
+            __syncthreads();
+        }
+        // unknown i is held by the lane that pivoted column i
+        int src = base;
+#pragma unroll
+        for (int t = 0; t < M; ++t) src = (i == t) ? base + plane[t] : src;
+        double xr[M + 1];
+#pragma unroll
+        for (int j = 0; j <= M; ++j) xr[j] = sh[(M + j) * kTile + src];
+        __syncthreads();
+        if (act) {
+            D[rv + i * es] = xr[0];
+            xlast = xr[0];
+        }
+        if (has_next) {
+#pragma unroll
+            for (int j = 0; j < M; ++j) C[rm + (i + M * j) * es] = xr[1 + j];
+        }
+#pragma unroll
+        for (int j = 0; j < M; ++j) sh[j * kTile + lane] = xr[1 + j];
+        sh[M * kTile + lane] = xr[0];
+        __syncthreads();
+    }
+    // back substitution over the rows: x_q(i) = d'_q(i) - sum_j C'_q(i,j) x_{q+1}(j)
+    double xnext = xlast;
+    for (int q = nmax - 2; q >= 0; --q) {
+        const bool act = q < n - 1;
+        sh[lane] = xnext;
+        __syncthreads();
+        if (act) {
+            const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+            double x = D[rv + i * es];
+#pragma unroll
+            for (int j = 0; j < M; ++j) x = fma(-C[rm + (i + M * j) * es], sh[base + j], x);
+            D[rv + i * es] = x;
+            xnext = x;
+        }
+        __syncthreads();
+    }
+    return ok;
+}
+
+template <int M>
+__device__ __forceinline__ bool coop_item(const BatchDev& b, const double* A, const double* B,
+                                          double* C, double* D, int tile, int lane0, double* sh) {
+    constexpr int G = kTile / M;  // systems per wave
+    const int lane = threadIdx.x;
+    const int g = lane / M, i = lane % M;
+    const int slot = lane0 + g;
+    const int count = b.tile_count[tile];
+    const bool mine = g < G && slot < count;
+    const int n = mine ? b.lane_rows[tile * kTile + slot] : 0;
+    const int nmax = b.lane_rows[tile * kTile + lane0];  // systems are sorted longest first
+    const int64_t mo = b.mat_off[tile] + (mine ? slot : 0), vo = b.vec_off[tile] + (mine ? slot : 0);
+    return coop_rows<M>(A, B, C, D, mo, vo, mine ? i : 0, mine ? g * M : lane, n, nmax, sh);
+}
+
+// Persistent solve of a packed batch. LDS_LANE adds the LDS-resident per-lane variant
+// (105 KB of LDS per workgroup); without it the kernel needs 64*(2*10+2)*8 B of LDS for
+// the cooperative items only.
+template <bool LDS_LANE>
 __global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restrict__ A,
                                               const double* __restrict__ B, double* C, double* D,
                                               int* __restrict__ counter, int* __restrict__ flag) {
+    __shared__ double sh[(2 * kMaxM + 2) * kTile];
     bool ok = true;
     const int lane = threadIdx.x;
     for (;;) {
         int t = 0;
         if (lane == 0) t = atomicAdd(counter, 1);
         t = __shfl(t, 0);
-        if (t >= b.ntiles) break;
-        const int tile = b.order[t];
+        if (t >= b.nitems) break;
+        const int tile = b.item_tile[t];
+        const int mode = b.item_mode[t];
         const int m = b.tile_m[tile];
+        if (mode == 2) {
+            switch (m) {
+                case 2: ok = coop_item<2>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
+                case 3: ok = coop_item<3>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
+                case 4: ok = coop_item<4>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
+                case 5: ok = coop_item<5>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
+                case 6: ok = coop_item<6>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
+                case 7: ok = coop_item<7>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
+                case 8: ok = coop_item<8>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
+                case 9: ok = coop_item<9>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
+                case 10: ok = coop_item<10>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
+                default: break;
+            }
+            continue;
+        }
         if (lane < b.tile_count[tile]) {
             const int n = b.lane_rows[tile * kTile + lane];
             const int64_t mo = b.mat_off[tile] + lane, vo = b.vec_off[tile] + lane;
-            if constexpr (PLACEMENT == 1) {
-                __shared__ double sR[kMaxM * kMaxM * kTile];
-                __shared__ double sC[kMaxM * kMaxM * kTile];
-                __shared__ int sP[kMaxM * kTile];
-                switch (m) {
-                    case 6: ok = plain_lane_lds<6>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
-                    case 7: ok = plain_lane_lds<7>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
-                    case 8: ok = plain_lane_lds<8>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
-                    case 9: ok = plain_lane_lds<9>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
-                    case 10: ok = plain_lane_lds<10>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
-                    default: break;
+            if constexpr (LDS_LANE) {
+                if (mode == 1) {
+                    __shared__ double sR[kMaxM * kMaxM * kTile];
+                    __shared__ double sC[kMaxM * kMaxM * kTile];
+                    __shared__ int sP[kMaxM * kTile];
+                    switch (m) {
+                        case 6: ok = plain_lane_lds<6>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                        case 7: ok = plain_lane_lds<7>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                        case 8: ok = plain_lane_lds<8>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                        case 9: ok = plain_lane_lds<9>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                        case 10: ok = plain_lane_lds<10>(A, B, C, D, mo, vo, n, sR, sC, sP) && ok; break;
+                        default: break;
+                    }
                 }
             }
-            if (PLACEMENT == 0 || m < 6) {
+            if (mode == 0) {
                 switch (m) {
                     case 1: ok = plain_lane<1>(A, B, C, D, mo, vo, n) && ok; break;
                     case 2: ok = plain_lane<2>(A, B, C, D, mo, vo, n) && ok; break;

@@ -23,7 +23,7 @@ btd_status bfail(btd_status s, const std::string& m) {
 struct btd_batch_s {
     int nsys = 0, ntiles = 0, max_rows = 0;
     std::vector<int> m, n;
-    std::vector<int> tile_m, tile_rows, tile_count, lane_rows, lane_sys, order;
+    std::vector<int> tile_m, tile_rows, tile_count, lane_rows, lane_sys;
     std::vector<int64_t> mat_off, vec_off;
     std::vector<int> sys_tile, sys_lane;
     int64_t mat_elems = 0, vec_elems = 0;
@@ -32,6 +32,8 @@ struct btd_batch_s {
     const int* d_lane_sys = nullptr;
     int* d_counter = nullptr;
     int* d_flag = nullptr;
+    int* d_items = nullptr;  // [tile | lane0 | mode] x max_items
+    int max_items = 0;
     int placement = 0;
     int grid = 0;
     hipStream_t last_stream = nullptr;
@@ -41,6 +43,45 @@ static void batch_free(btd_batch_s* b) {
     if (!b) return;
     if (b->dev) (void)hipFree(b->dev);
     delete b;
+}
+
+// Work items for a placement policy: 0 every tile one lane per system in VGPRs; 1 m >= 6 in
+// LDS (one lane per system); 2 m >= 6 cooperative (m lanes per system, floor(64/m) systems
+// per item); 3 m >= 4 cooperative; 4 every m >= 2 cooperative. Longest estimated work first.
+static btd_status build_items(btd_batch_s* b, int placement) {
+    struct Item {
+        int tile, lane0, mode;
+        double cost;
+    };
+    std::vector<Item> items;
+    const int coop_from = placement == 2 ? 6 : placement == 3 ? 4 : placement == 4 ? 2 : 1000;
+    for (int t = 0; t < b->ntiles; ++t) {
+        const int m = b->tile_m[t];
+        const double mm = m;
+        if (m >= coop_from) {
+            const int G = btd::kTile / m;
+            for (int l0 = 0; l0 < b->tile_count[t]; l0 += G) {
+                const double rows = b->lane_rows[t * btd::kTile + l0];
+                items.push_back({t, l0, 2, rows * mm * (mm + 4) * 4.0});
+            }
+        } else {
+            const int mode = (placement == 1 && m >= 6) ? 1 : 0;
+            items.push_back({t, 0, mode, (double)b->tile_rows[t] * mm * mm * (mm + 4)});
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+    if ((int)items.size() > b->max_items) return bfail(BTD_ERR_INVALID_ARG, "batch: too many work items");
+    std::vector<int> host(3 * (size_t)b->max_items, 0);
+    for (size_t k = 0; k < items.size(); ++k) {
+        host[k] = items[k].tile;
+        host[b->max_items + k] = items[k].lane0;
+        host[2 * b->max_items + k] = items[k].mode;
+    }
+    hipError_t e = hipMemcpy(b->d_items, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bfail(BTD_ERR_HIP, std::string("batch items: ") + hipGetErrorString(e));
+    b->bd.nitems = (int)items.size();
+    b->placement = placement;
+    return BTD_OK;
 }
 
 extern "C" {
@@ -94,19 +135,10 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
     b->ntiles = (int)b->tile_m.size();
     b->mat_elems = mo;
     b->vec_elems = vo;
-    // longest-processing-time first: estimated cost rows * m^2 * (m + 4)
-    b->order.resize(b->ntiles);
-    std::iota(b->order.begin(), b->order.end(), 0);
-    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) {
-        auto cost = [&](int t) {
-            const double mm = b->tile_m[t];
-            return (double)b->tile_rows[t] * mm * mm * (mm + 4);
-        };
-        return cost(a) > cost(c);
-    });
     // device tables in one allocation
     const size_t T = b->ntiles;
-    const size_t bytes_i = sizeof(int) * (T * 4 + T * btd::kTile * 2 + 2);
+    b->max_items = (int)T * ((btd::kTile + 1) / 2);  // cooperative m = 2 cuts a tile into 2 items
+    const size_t bytes_i = sizeof(int) * (T * 3 + T * btd::kTile * 2 + 2 + 3 * (size_t)b->max_items);
     const size_t bytes = ((bytes_i + 15) / 16) * 16 + sizeof(int64_t) * 2 * T;
     std::vector<char> host(bytes, 0);
     int* hi = (int*)host.data();
@@ -117,9 +149,9 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
         o += v.size();
         return at;
     };
-    const size_t o_order = put(b->order), o_m = put(b->tile_m), o_rows = put(b->tile_rows),
-                 o_cnt = put(b->tile_count), o_lr = put(b->lane_rows), o_ls = put(b->lane_sys);
-    const size_t o_counter = o, o_flag = o + 1;
+    const size_t o_m = put(b->tile_m), o_rows = put(b->tile_rows), o_cnt = put(b->tile_count),
+                 o_lr = put(b->lane_rows), o_ls = put(b->lane_sys);
+    const size_t o_counter = o, o_flag = o + 1, o_items = o + 2;
     int64_t* h64 = (int64_t*)(host.data() + ((bytes_i + 15) / 16) * 16);
     std::memcpy(h64, b->mat_off.data(), sizeof(int64_t) * T);
     std::memcpy(h64 + T, b->vec_off.data(), sizeof(int64_t) * T);
@@ -131,8 +163,9 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
     }
     int* di = (int*)b->dev;
     int64_t* d64 = (int64_t*)((char*)b->dev + ((bytes_i + 15) / 16) * 16);
-    b->bd = btd::BatchDev{di + o_order, di + o_m, di + o_rows, di + o_cnt, di + o_lr,
-                          d64, d64 + T, b->ntiles};
+    b->d_items = di + o_items;
+    b->bd = btd::BatchDev{b->d_items, b->d_items + b->max_items, b->d_items + 2 * b->max_items,
+                          di + o_m, di + o_rows, di + o_cnt, di + o_lr, d64, d64 + T, b->ntiles, 0};
     b->d_lane_sys = di + o_ls;
     b->d_counter = di + o_counter;
     b->d_flag = di + o_flag;
@@ -141,7 +174,12 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
     }
-    b->grid = std::min(b->ntiles, 4 * cus);  // one 64-lane wave per SIMD
+    b->grid = 4 * cus;  // one 64-lane wave per SIMD
+    btd_status st = build_items(b, 0);
+    if (st != BTD_OK) {
+        batch_free(b);
+        return st;
+    }
     *batch = b;
     return BTD_OK;
 }
@@ -170,9 +208,10 @@ btd_status btd_batch_locate(btd_batch_t b, int sys, size_t* mat_off, size_t* vec
 
 btd_status btd_batch_set_params(btd_batch_t b, int placement, int grid) {
     if (!b) return bfail(BTD_ERR_NOT_CREATED, "btd_batch_set_params: NULL batch");
-    if (placement < 0 || placement > 1 || grid < 1)
-        return bfail(BTD_ERR_INVALID_ARG, "btd_batch_set_params: placement is 0 or 1, grid >= 1");
-    b->placement = placement;
+    if (placement < 0 || placement > 4 || grid < 1)
+        return bfail(BTD_ERR_INVALID_ARG, "btd_batch_set_params: placement in 0..4, grid >= 1");
+    btd_status st = build_items(b, placement);
+    if (st != BTD_OK) return st;
     b->grid = grid;
     return BTD_OK;
 }
@@ -193,8 +232,8 @@ btd_status btd_batch_solve(btd_batch_t b, double* a, const double* bb, double* c
     if (!b) return bfail(BTD_ERR_NOT_CREATED, "btd_batch_solve: NULL batch");
     if (!a || !bb || !c || !d) return bfail(BTD_ERR_INVALID_ARG, "btd_batch_solve: NULL array");
     b->last_stream = (hipStream_t)stream;
-    hipError_t e = btd::launch_batch(b->bd, b->placement, b->grid, a, bb, c, d, b->d_counter, b->d_flag,
-                                     (hipStream_t)stream);
+    hipError_t e = btd::launch_batch(b->bd, b->placement == 1, b->grid, a, bb, c, d, b->d_counter,
+                                     b->d_flag, (hipStream_t)stream);
     if (e != hipSuccess) return bfail(BTD_ERR_HIP, std::string("btd_batch_solve: ") + hipGetErrorString(e));
     return BTD_OK;
 }

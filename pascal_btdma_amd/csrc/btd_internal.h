@@ -37,7 +37,10 @@ hipError_t launch_generate(int m, int kind, uint64_t seed, int layout, int nsys,
 
 // Device tables of a packed mixed-size batch (see btd_batch.h).
 struct BatchDev {
-    const int* order;          // tile ids in work order
+    const int* item_tile;      // work items in work order: tile id,
+    const int* item_lane0;     // first system slot of the item in its tile,
+    const int* item_mode;      // 0 lane per system (VGPR), 1 lane per system (LDS),
+                               // 2 m lanes per system (one block row each)
     const int* tile_m;         // block size of each tile
     const int* tile_rows;      // rows of the longest system of each tile
     const int* tile_count;     // systems in the tile (<= 64)
@@ -45,9 +48,10 @@ struct BatchDev {
     const int64_t* mat_off;    // first matrix element of each tile
     const int64_t* vec_off;    // first vector element of each tile
     int ntiles;
+    int nitems;
 };
 
-hipError_t launch_batch(const BatchDev& b, int placement, int grid, const double* A, const double* B,
+hipError_t launch_batch(const BatchDev& b, bool lds_lane, int grid, const double* A, const double* B,
                         double* C, double* D, int* counter, int* flag, hipStream_t s);
 
 hipError_t launch_batch_generate(const BatchDev& b, const int* lane_sys, int max_rows, int kind,

@@ -1,7 +1,8 @@
 """Autotuner for the launch choices of the solver, by measurement on the device.
 
-* ``tune_batch``: for a mixed-size batch, the block placement of large m (VGPRs, or LDS
-  for m >= 6) and the persistent grid size (workgroups of 64 lanes per CU).
+* ``tune_batch``: for a mixed-size batch, the placement (one lane per system with blocks
+  in VGPRs or in LDS, or m lanes per system from a block size up) and the persistent grid
+  size (64-lane workgroups per CU).
 * ``tune_partitions``: for long systems, the number of row partitions P of the two-stage
   solve (P partitions per system become P lanes; the reduced system has 2P rows).
 
@@ -37,7 +38,7 @@ def _time(fn, restore, reps: int) -> float:
 
 
 def tune_batch(batch: BtdmaBatch, a, b, c, d, reps: int = 3,
-               placements=(0, 1), grids_per_cu=(1, 2, 4)) -> dict:
+               placements=(0, 1, 2, 3, 4), grids_per_cu=(1, 2, 4)) -> dict:
     """Pick (placement, grid) for ``batch``; a, b, c, d are its packed inputs (unchanged)."""
     cw, dw = c.clone(), d.clone()
 
@@ -51,7 +52,7 @@ def tune_batch(batch: BtdmaBatch, a, b, c, d, reps: int = 3,
         if pl == 1 and int(batch.m.max()) < 6:
             continue
         for g in grids_per_cu:
-            grid = max(1, min(batch.ntiles, g * cus))
+            grid = max(1, g * cus)
             batch.set_params(pl, grid)
             table[(pl, grid)] = _time(lambda: batch.solve(a, b, cw, dw), restore, reps)
     best = min(table, key=table.get)

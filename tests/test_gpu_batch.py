@@ -46,7 +46,7 @@ def test_batch_generator_matches_host(cuda):
         np.testing.assert_array_equal(bt.extract(hd, s), dd[:, 0])
 
 
-@pytest.mark.parametrize("placement", [0, 1])
+@pytest.mark.parametrize("placement", [0, 1, 2, 3, 4])
 def test_batch_matches_oracle(cuda, oracle, placement):
     rng = np.random.default_rng(placement + 1)
     m, n = rand_sizes(rng, 300)
