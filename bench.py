@@ -201,7 +201,8 @@ def extra_batch(dev, steps: int) -> dict:
                     for _ in range(3))
     ld0 = torch.empty(pb.layout_elems(lay, lnsys, ln, lm), dtype=torch.float64, device=dev)
     pb.generate(pb.GenKind.DDPERM, 8, lm, lnsys, ln, lay, la0, lb, lc0, ld0)
-    tp = tune_partitions(lm, lnsys, ln, la0, lb, lc0, ld0, lay, candidates=(8, 16, 32), reps=2)
+    tp = tune_partitions(lm, lnsys, ln, la0, lb, lc0, ld0, lay, candidates=(8, 16, 32, 64, 128, 256),
+                         reps=2)
     plan = pb.BtdmaGpuPlan().create(lm, lnsys, ln, tp["npart"], lay)
     c, d, la, lc, ld = c0.clone(), d0.clone(), la0.clone(), lc0.clone(), ld0.clone()
 

@@ -215,21 +215,63 @@ __device__ bool plain_lane_lds(const double* __restrict__ A, const double* __res
     return ok;
 }
 
-// Cooperative solve: m lanes per system, lane i owns block row i of every block. The row
-// product A_q C'_{q-1}, the pivot search, the pivot row and the back-substitution values are
-// exchanged through LDS (sh: W = 2m+2 doubles per lane, [w][lane]). Partial pivoting: the
-// pivot of column k is the largest |r(k)| among the rows not yet chosen (lowest lane on a
-// tie). All lanes of the wave run the same number of row steps (nmax) so every barrier is
-// reached by the whole workgroup; lanes past their system's n, and lanes of no system
-// (n = 0), compute on zeros and store nothing.
+// Orders the LDS accesses of the lanes of one wavefront. The batch kernel runs one-wave
+// (64-lane) workgroups, so LDS-only acquire/release fences around a wave barrier suffice and
+// outstanding global loads (the prefetched next row) are not waited for, unlike with a full
+// workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// 1/x from the hardware reciprocal refined by two Newton steps (error ~1 ulp); shorter on
+// the pivot's critical path than an IEEE division.
+__device__ __forceinline__ double fast_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Cooperative solve: m lanes per system, lane i owns block row i of every block. LDS
+// (sh, [slot][lane], 5m+5 slots) holds the carried C'_{q-1} (column j in lane base+j) and
+// d'_{q-1}, and two alternating exchange buffers, so every pivot step needs one barrier:
+// each lane posts its candidate |r(k)| with its whole augmented row, every lane picks the
+// pivot (largest |r(k)| among rows not yet chosen, lowest lane on a tie) and eliminates
+// with it; the buffer written at step k is rewritten only at step k+2, after the barrier
+// of step k+1 that every lane passes once done reading. After the elimination each lane
+// posts its row once more and back-substitutes one right-hand-side column on its own
+// (lane i: column i of C'_q; every lane: d'_q). The next row's blocks are loaded while the
+// current row is eliminated. All lanes of the wave run the same number of row steps (nmax),
+// so every barrier is reached by the whole workgroup; lanes past their system's n, and
+// lanes of no system (n = 0), compute on zeros and store nothing.
 template <int M>
 __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict__ B, double* C,
                           double* D, int64_t mo, int64_t vo, int i, int base, int n, int nmax,
                           double* __restrict__ sh) {
     constexpr int E = M * M;
+    constexpr int W = 2 * M + 2;          // exchange slots: row (M), rhs (M+1), candidate (1)
+    double* carry = sh;                   // slots [0, M]: C'(k, lane-base) and d'(lane-base)
+    double* xbuf0 = sh + (M + 1) * kTile; // two exchange buffers of W slots
+    double* xbuf1 = xbuf0 + W * kTile;
     const int lane = threadIdx.x;
     const int64_t es = kTile;
     bool ok = true;
+    double pa[M], pb[M], pc[M], pd;
+    auto fetch = [&](int q) {
+        const bool act = q < n, has_next = q + 1 < n;
+        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            pa[k] = (act && q > 0) ? __builtin_nontemporal_load(A + rm + (i + M * k) * es) : 0.0;
+            pb[k] = act ? __builtin_nontemporal_load(B + rm + (i + M * k) * es) : 0.0;
+            pc[k] = has_next ? C[rm + (i + M * k) * es] : 0.0;
+        }
+        pd = act ? D[rv + i * es] : 0.0;
+    };
+    fetch(0);
     double xlast = 0.0;
     for (int q = 0; q < nmax; ++q) {
         const bool act = q < n;
@@ -238,112 +280,121 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
         double r[M], rhs[M + 1];
 #pragma unroll
         for (int j = 0; j < M; ++j) {
-            r[j] = act ? __builtin_nontemporal_load(B + rm + (i + M * j) * es) : 0.0;
-            rhs[1 + j] = has_next ? C[rm + (i + M * j) * es] : 0.0;
+            r[j] = pb[j];
+            rhs[1 + j] = pc[j];
         }
-        rhs[0] = act ? D[rv + i * es] : 0.0;
+        rhs[0] = pd;
         if (q > 0) {
 #pragma unroll
             for (int k = 0; k < M; ++k) {
-                const double ak = act ? __builtin_nontemporal_load(A + rm + (i + M * k) * es) : 0.0;
-                rhs[0] = fma(-ak, sh[M * kTile + base + k], rhs[0]);
+                rhs[0] = fma(-pa[k], carry[M * kTile + base + k], rhs[0]);
 #pragma unroll
-                for (int j = 0; j < M; ++j) r[j] = fma(-ak, sh[j * kTile + base + k], r[j]);
+                for (int j = 0; j < M; ++j) r[j] = fma(-pa[k], carry[k * kTile + base + j], r[j]);
             }
         }
-        __syncthreads();
+        if (q + 1 < nmax) fetch(q + 1);
         int rank = -1;
         int plane[M];
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            sh[lane] = rank < 0 ? fabs(r[k]) : -1.0;
-            __syncthreads();
+            double* xb = (k & 1) ? xbuf1 : xbuf0;
+#pragma unroll
+            for (int j = 0; j < M; ++j) xb[j * kTile + lane] = r[j];
+#pragma unroll
+            for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
+            xb[(2 * M + 1) * kTile + lane] = rank < 0 ? fabs(r[k]) : -1.0;
+            wave_lds_sync();
             int p = 0;
             double best = -2.0;
 #pragma unroll
             for (int t = 0; t < M; ++t) {
-                const double v = sh[base + t];
+                const double v = xb[(2 * M + 1) * kTile + base + t];
                 const bool c = v > best;
                 best = c ? v : best;
                 p = c ? t : p;
             }
             plane[k] = p;
-            __syncthreads();
-            if (i == p) {
-#pragma unroll
-                for (int j = 0; j < M; ++j) sh[j * kTile + lane] = r[j];
-#pragma unroll
-                for (int j = 0; j <= M; ++j) sh[(M + j) * kTile + lane] = rhs[j];
-                rank = k;
-            }
-            __syncthreads();
             const int pl = base + p;
-            const double pk = sh[k * kTile + pl];
+            const double pk = xb[k * kTile + pl];
             ok = ok && (!act || pk != 0.0);
-            if (rank < 0) {
-                const double l = r[k] / pk;
+            if (i == p) {
+                rank = k;
+            } else if (rank < 0) {
+                const double l = r[k] * fast_rcp(pk);
 #pragma unroll
-                for (int j = k + 1; j < M; ++j) r[j] = fma(-l, sh[j * kTile + pl], r[j]);
+                for (int j = k + 1; j < M; ++j) r[j] = fma(-l, xb[j * kTile + pl], r[j]);
 #pragma unroll
-                for (int j = 0; j <= M; ++j) rhs[j] = fma(-l, sh[(M + j) * kTile + pl], rhs[j]);
+                for (int j = 0; j <= M; ++j) rhs[j] = fma(-l, xb[(M + j) * kTile + pl], rhs[j]);
             }
-            __syncthreads();
         }
-        // back substitution: the lane of rank k holds sum_{j>=k} r(j) x_j = rhs
+        // post the eliminated rows; lane i solves column i+1 (C'(:, i)), every lane column 0
+        double* xb = (M & 1) ? xbuf1 : xbuf0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) xb[j * kTile + lane] = r[j];
+#pragma unroll
+        for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
+        wave_lds_sync();
+        double xc[M], xd[M];
 #pragma unroll
         for (int k = M - 1; k >= 0; --k) {
-            if (rank == k) {
-                const double rk = 1.0 / r[k];
-#pragma unroll
-                for (int j = 0; j <= M; ++j) {
-                    rhs[j] *= rk;
-                    sh[(M + j) * kTile + lane] = rhs[j];
-                }
-            }
-            __syncthreads();
             const int pl = base + plane[k];
-            if (rank < k) {
+            double sc = 0.0;
 #pragma unroll
-                for (int j = 0; j <= M; ++j) rhs[j] = fma(-r[k], sh[(M + j) * kTile + pl], rhs[j]);
+            for (int t = 0; t < M; ++t) sc = (i == t) ? xb[(M + 1 + t) * kTile + pl] : sc;
+            double sd = xb[M * kTile + pl];
+#pragma unroll
+            for (int j = k + 1; j < M; ++j) {
+                const double u = xb[j * kTile + pl];
+                sc = fma(-u, xc[j], sc);
+                sd = fma(-u, xd[j], sd);
             }
-            __syncthreads();
+            const double rk = fast_rcp(xb[k * kTile + pl]);
+            xc[k] = sc * rk;
+            xd[k] = sd * rk;
         }
-        // unknown i is held by the lane that pivoted column i
-        int src = base;
+        double di = xd[0];
 #pragma unroll
-        for (int t = 0; t < M; ++t) src = (i == t) ? base + plane[t] : src;
-        double xr[M + 1];
-#pragma unroll
-        for (int j = 0; j <= M; ++j) xr[j] = sh[(M + j) * kTile + src];
-        __syncthreads();
+        for (int t = 1; t < M; ++t) di = (i == t) ? xd[t] : di;
         if (act) {
-            D[rv + i * es] = xr[0];
-            xlast = xr[0];
+            D[rv + i * es] = di;
+            xlast = di;
         }
         if (has_next) {
 #pragma unroll
-            for (int j = 0; j < M; ++j) C[rm + (i + M * j) * es] = xr[1 + j];
+            for (int k = 0; k < M; ++k) C[rm + (k + M * i) * es] = xc[k];
         }
 #pragma unroll
-        for (int j = 0; j < M; ++j) sh[j * kTile + lane] = xr[1 + j];
-        sh[M * kTile + lane] = xr[0];
-        __syncthreads();
+        for (int k = 0; k < M; ++k) carry[k * kTile + lane] = xc[k];  // C'(k, i) in lane base+i
+        carry[M * kTile + lane] = di;                                   // d'(i) in lane base+i
+        wave_lds_sync();
     }
-    // back substitution over the rows: x_q(i) = d'_q(i) - sum_j C'_q(i,j) x_{q+1}(j)
+    // back substitution over the rows: x_q(i) = d'_q(i) - sum_j C'_q(i,j) x_{q+1}(j); the
+    // next row's C'(i,:) and d'(i) are loaded while the current row is finished.
     double xnext = xlast;
+    double cn[M], dn = 0.0;
+    auto bfetch = [&](int q) {
+        const bool act = q >= 0 && q < n - 1;
+        const int64_t rm = mo + (int64_t)(act ? q : 0) * E * kTile, rv = vo + (int64_t)(act ? q : 0) * M * kTile;
+#pragma unroll
+        for (int j = 0; j < M; ++j) cn[j] = act ? C[rm + (i + M * j) * es] : 0.0;
+        dn = act ? D[rv + i * es] : 0.0;
+    };
+    bfetch(nmax - 2);
     for (int q = nmax - 2; q >= 0; --q) {
         const bool act = q < n - 1;
-        sh[lane] = xnext;
-        __syncthreads();
-        if (act) {
-            const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
-            double x = D[rv + i * es];
+        double cq[M], x = dn;
 #pragma unroll
-            for (int j = 0; j < M; ++j) x = fma(-C[rm + (i + M * j) * es], sh[base + j], x);
-            D[rv + i * es] = x;
+        for (int j = 0; j < M; ++j) cq[j] = cn[j];
+        bfetch(q - 1);
+        sh[lane] = xnext;
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
+        if (act) {
+            D[vo + (int64_t)q * M * kTile + i * es] = x;
             xnext = x;
         }
-        __syncthreads();
+        wave_lds_sync();
     }
     return ok;
 }
@@ -363,14 +414,14 @@ __device__ __forceinline__ bool coop_item(const BatchDev& b, const double* A, co
     return coop_rows<M>(A, B, C, D, mo, vo, mine ? i : 0, mine ? g * M : lane, n, nmax, sh);
 }
 
-// Persistent solve of a packed batch. LDS_LANE adds the LDS-resident per-lane variant
-// (105 KB of LDS per workgroup); without it the kernel needs 64*(2*10+2)*8 B of LDS for
-// the cooperative items only.
-template <bool LDS_LANE>
+// Persistent solve of a packed batch. KIND selects the code a placement needs, so that the
+// register allocation of the kernel is that of the paths it can run: 0 one lane per system
+// for every m (VGPRs); 1 one lane per system, m >= 6 with blocks in LDS (105 KB of LDS);
+// 2 one lane per system for m <= 5 and m lanes per system (cooperative items).
+template <int KIND>
 __global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restrict__ A,
                                               const double* __restrict__ B, double* C, double* D,
                                               int* __restrict__ counter, int* __restrict__ flag) {
-    __shared__ double sh[(2 * kMaxM + 2) * kTile];
     bool ok = true;
     const int lane = threadIdx.x;
     for (;;) {
@@ -381,25 +432,29 @@ __global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restri
         const int tile = b.item_tile[t];
         const int mode = b.item_mode[t];
         const int m = b.tile_m[tile];
-        if (mode == 2) {
-            switch (m) {
-                case 2: ok = coop_item<2>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
-                case 3: ok = coop_item<3>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
-                case 4: ok = coop_item<4>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
-                case 5: ok = coop_item<5>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
-                case 6: ok = coop_item<6>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
-                case 7: ok = coop_item<7>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
-                case 8: ok = coop_item<8>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
-                case 9: ok = coop_item<9>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
-                case 10: ok = coop_item<10>(b, A, B, C, D, tile, b.item_lane0[t], sh) && ok; break;
-                default: break;
+        if constexpr (KIND == 2) {
+            if (mode == 2) {
+                __shared__ double sh[(5 * kMaxM + 5) * kTile];
+                const int l0 = b.item_lane0[t];
+                switch (m) {
+                    case 2: ok = coop_item<2>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                    case 3: ok = coop_item<3>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                    case 4: ok = coop_item<4>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                    case 5: ok = coop_item<5>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                    case 6: ok = coop_item<6>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                    case 7: ok = coop_item<7>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                    case 8: ok = coop_item<8>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                    case 9: ok = coop_item<9>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                    case 10: ok = coop_item<10>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                    default: break;
+                }
+                continue;
             }
-            continue;
         }
         if (lane < b.tile_count[tile]) {
             const int n = b.lane_rows[tile * kTile + lane];
             const int64_t mo = b.mat_off[tile] + lane, vo = b.vec_off[tile] + lane;
-            if constexpr (LDS_LANE) {
+            if constexpr (KIND == 1) {
                 if (mode == 1) {
                     __shared__ double sR[kMaxM * kMaxM * kTile];
                     __shared__ double sC[kMaxM * kMaxM * kTile];
@@ -421,12 +476,17 @@ __global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restri
                     case 3: ok = plain_lane<3>(A, B, C, D, mo, vo, n) && ok; break;
                     case 4: ok = plain_lane<4>(A, B, C, D, mo, vo, n) && ok; break;
                     case 5: ok = plain_lane<5>(A, B, C, D, mo, vo, n) && ok; break;
-                    case 6: ok = plain_lane<6>(A, B, C, D, mo, vo, n) && ok; break;
-                    case 7: ok = plain_lane<7>(A, B, C, D, mo, vo, n) && ok; break;
-                    case 8: ok = plain_lane<8>(A, B, C, D, mo, vo, n) && ok; break;
-                    case 9: ok = plain_lane<9>(A, B, C, D, mo, vo, n) && ok; break;
-                    case 10: ok = plain_lane<10>(A, B, C, D, mo, vo, n) && ok; break;
                     default: break;
+                }
+                if constexpr (KIND == 0) {
+                    switch (m) {
+                        case 6: ok = plain_lane<6>(A, B, C, D, mo, vo, n) && ok; break;
+                        case 7: ok = plain_lane<7>(A, B, C, D, mo, vo, n) && ok; break;
+                        case 8: ok = plain_lane<8>(A, B, C, D, mo, vo, n) && ok; break;
+                        case 9: ok = plain_lane<9>(A, B, C, D, mo, vo, n) && ok; break;
+                        case 10: ok = plain_lane<10>(A, B, C, D, mo, vo, n) && ok; break;
+                        default: break;
+                    }
                 }
             }
         }

@@ -3,13 +3,15 @@
 
 namespace btd {
 
-hipError_t launch_batch(const BatchDev& b, bool lds_lane, int grid, const double* A, const double* B,
+hipError_t launch_batch(const BatchDev& b, int kind, int grid, const double* A, const double* B,
                         double* C, double* D, int* counter, int* flag, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, s, counter);
-    if (lds_lane)
-        hipLaunchKernelGGL((k_batch<true>), dim3(grid), dim3(64), 0, s, b, A, B, C, D, counter, flag);
+    if (kind == 1)
+        hipLaunchKernelGGL((k_batch<1>), dim3(grid), dim3(64), 0, s, b, A, B, C, D, counter, flag);
+    else if (kind == 2)
+        hipLaunchKernelGGL((k_batch<2>), dim3(grid), dim3(64), 0, s, b, A, B, C, D, counter, flag);
     else
-        hipLaunchKernelGGL((k_batch<false>), dim3(grid), dim3(64), 0, s, b, A, B, C, D, counter, flag);
+        hipLaunchKernelGGL((k_batch<0>), dim3(grid), dim3(64), 0, s, b, A, B, C, D, counter, flag);
     return hipGetLastError();
 }
 

@@ -232,8 +232,9 @@ btd_status btd_batch_solve(btd_batch_t b, double* a, const double* bb, double* c
     if (!b) return bfail(BTD_ERR_NOT_CREATED, "btd_batch_solve: NULL batch");
     if (!a || !bb || !c || !d) return bfail(BTD_ERR_INVALID_ARG, "btd_batch_solve: NULL array");
     b->last_stream = (hipStream_t)stream;
-    hipError_t e = btd::launch_batch(b->bd, b->placement == 1, b->grid, a, bb, c, d, b->d_counter,
-                                     b->d_flag, (hipStream_t)stream);
+    const int kind = b->placement == 0 ? 0 : b->placement == 1 ? 1 : 2;
+    hipError_t e = btd::launch_batch(b->bd, kind, b->grid, a, bb, c, d, b->d_counter, b->d_flag,
+                                     (hipStream_t)stream);
     if (e != hipSuccess) return bfail(BTD_ERR_HIP, std::string("btd_batch_solve: ") + hipGetErrorString(e));
     return BTD_OK;
 }

@@ -51,7 +51,7 @@ struct BatchDev {
     int nitems;
 };
 
-hipError_t launch_batch(const BatchDev& b, bool lds_lane, int grid, const double* A, const double* B,
+hipError_t launch_batch(const BatchDev& b, int kind, int grid, const double* A, const double* B,
                         double* C, double* D, int* counter, int* flag, hipStream_t s);
 
 hipError_t launch_batch_generate(const BatchDev& b, const int* lane_sys, int max_rows, int kind,

@@ -558,10 +558,16 @@ __global__ __launch_bounds__(64) void k_f32r(const float* __restrict__ A, const 
 
 // ------------------------------------------------------------------------------ modi --
 // Stage 1 over partition `part` (rows [r0, r0+N)) of each system, mod_btdma_cpu.f90:305-443.
-// Writes the partition's modified rows in place (Â over A, Ĉ over C, d̂ over D) and its
-// two reduced rows into the plan's reduced buffers (tiled, 2P rows per system):
-//   row 2*part   : (Â_1, I, Ĉ_1, d̂_1)  couples x_1 to x_0 (previous partition) and x_N
-//   row 2*part+1 : (Â_N, I, Ĉ_N, d̂_N)  couples x_N to x_1 and x_{N+1} (next partition)
+// Forward sweep (in place: Â over A, Ĉ over C, d̂ over D for rows 0..N-2):
+//   [d̂|Â|Ĉ]_q = R^{-1}[d_q - A_q d̂_{q-1} | -A_q Â_{q-1} | C_q],  R = B_q - A_q Ĉ_{q-1}
+// (rows 0 and 1: R = B_q and the right-hand side [d_q | A_q | C_q]), so that
+//   x_q + Â_q x_0' + Ĉ_q x_{q+1} = d̂_q   (x_0' = first row of the partition, q >= 1).
+// The reference then rewrites every row in a backward sweep; here the backward sweep only
+// reads the forward values and carries the composed coefficients of row 1 in registers,
+// which is all the closing step needs (stage 3 uses the forward values directly), so it
+// stores nothing. Reduced rows go to the plan's buffers (tiled, 2P rows per system):
+//   row 2*part   : (Â_1, I, Ĉ_1, d̂_1) after the close, couples x_1 to x_0 and x_N
+//   row 2*part+1 : (Â_N, I, Ĉ_N, d̂_N) forward values, couples x_N to x_1 and x_{N+1}
 template <int M, int LAYOUT>
 __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict__ B, double* C,
                                              double* D, int nsys, int nrow,
@@ -577,105 +583,102 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
     const int N = part_len[part];
     const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
     bool ok = true;
-    // forward, rows q = 0..N-1 (local): [d̂|Â|Ĉ]_q = R^{-1}[d - A d̂_{q-1} | -A Â_{q-1} | C]
     double Ah[M][M], Ch[M][M], dh[M];
     for (int q = 0; q < N; ++q) {
         const int64_t g = r0 + q;
-        double R[M][M], v[M], Aq[M][M], X[M][M];
+        double R[M][M], v[M], X[M][M];
         load_mat<M>(B, am, g, R);
         load_vec<M>(D, av, g, v);
-        load_mat_c<M>(A, am, g, Aq);
         if (q >= 2) {
-            double T[M][M];
-            mm<M>(Aq, Ch, T);
 #pragma unroll
             for (int i = 0; i < M; ++i)
 #pragma unroll
-                for (int j = 0; j < M; ++j) R[i][j] -= T[i][j];
-            double t[M];
-            mv<M>(Aq, dh, t);
+                for (int j = 0; j < M; ++j) X[i][j] = 0.0;
 #pragma unroll
-            for (int i = 0; i < M; ++i) v[i] -= t[i];
-            mm<M>(Aq, Ah, X);
+            for (int k = 0; k < M; ++k) {
+                double ak[M];
+                load_col<M>(A, am, g, k, ak);
 #pragma unroll
-            for (int i = 0; i < M; ++i)
+                for (int i = 0; i < M; ++i) {
+                    v[i] = fma(-ak[i], dh[k], v[i]);
 #pragma unroll
-                for (int j = 0; j < M; ++j) X[i][j] = -X[i][j];
+                    for (int j = 0; j < M; ++j) {
+                        R[i][j] = fma(-ak[i], Ch[k][j], R[i][j]);
+                        X[i][j] = fma(-ak[i], Ah[k][j], X[i][j]);
+                    }
+                }
+            }
         } else {
-#pragma unroll
-            for (int i = 0; i < M; ++i)
-#pragma unroll
-                for (int j = 0; j < M; ++j) X[i][j] = Aq[i][j];
+            load_mat_c<M>(A, am, g, X);
         }
         int piv[M];
         ok = lu_factor<M>(R, piv) && ok;
         lu_solve<M>(R, piv, v);
         solve_rows<M>(R, piv, X);
-        double Cq[M][M];
-        load_mat_c<M>(C, am, g, Cq);
-        solve_rows<M>(R, piv, Cq);
+        const bool keep = q + 1 < N;  // the last row only feeds the reduced system
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double c[M];
+            load_col<M>(C, am, g, j, c);
+            lu_solve<M>(R, piv, c);
+#pragma unroll
+            for (int i = 0; i < M; ++i) Ch[i][j] = c[i];
+            if (keep) store_col<M>(C, am, g, j, c);
+        }
 #pragma unroll
         for (int i = 0; i < M; ++i) {
             dh[i] = v[i];
 #pragma unroll
-            for (int j = 0; j < M; ++j) {
-                Ah[i][j] = X[i][j];
-                Ch[i][j] = Cq[i][j];
-            }
+            for (int j = 0; j < M; ++j) Ah[i][j] = X[i][j];
         }
-        store_mat<M>(A, am, g, Ah);
-        store_mat<M>(C, am, g, Ch);
-        store_vec<M>(D, av, g, dh);
+        if (keep) {
+            store_mat<M>(A, am, g, Ah);
+            store_vec<M>(D, av, g, dh);
+        }
     }
-    // bottom reduced row: the forward values of row N-1 (still in Ah, Ch, dh)
-    const int64_t rt = (int64_t)tile * (2 * npart);
-    const Addr<LAYOUT_TILED> rm((int64_t)tile * 64 + threadIdx.x, 0, 2 * npart, M * M);
-    const Addr<LAYOUT_TILED> rv((int64_t)tile * 64 + threadIdx.x, 0, 2 * npart, M);
-    (void)rt;
+    const Addr<LAYOUT_TILED> rm(sys, 0, 2 * npart, M * M), rv(sys, 0, 2 * npart, M);
     store_mat<M>(redA, rm, 2 * part + 1, Ah);
     store_mat<M>(redC, rm, 2 * part + 1, Ch);
     store_vec<M>(redD, rv, 2 * part + 1, dh);
-    // backward, q = N-3..1: d̂_q -= Ĉ_q d̂_{q+1}; Â_q -= Ĉ_q Â_{q+1}; Ĉ_q = -Ĉ_q Ĉ_{q+1}.
-    // (Ah, Ch, dh) hold row q+1's current values.
-    if (N >= 3) {
-        load_mat_c<M>(A, am, r0 + N - 2, Ah);
-        load_mat_c<M>(C, am, r0 + N - 2, Ch);
-        load_vec<M>(D, av, r0 + N - 2, dh);
-    }
+    // backward composition, q = N-3..1, read only: with (Ah, Ch, dh) the coefficients of
+    // row q+1 in terms of x_0' and x_N, row q becomes d̂_q - Ĉ_q dh, Â_q - Ĉ_q Ah, -Ĉ_q Ch.
+    load_mat_c<M>(A, am, r0 + N - 2, Ah);
+    load_mat_c<M>(C, am, r0 + N - 2, Ch);
+    load_vec<M>(D, av, r0 + N - 2, dh);
     for (int q = N - 3; q >= 1; --q) {
         const int64_t g = r0 + q;
-        double Aq[M][M], Cq[M][M], dq[M], T[M][M], t[M];
-        load_mat_c<M>(A, am, g, Aq);
+        double Cq[M][M], An[M][M], dn[M], Cn[M][M];
         load_mat_c<M>(C, am, g, Cq);
-        load_vec<M>(D, av, g, dq);
-        mv<M>(Cq, dh, t);
-#pragma unroll
-        for (int i = 0; i < M; ++i) dq[i] -= t[i];
-        mm<M>(Cq, Ah, T);
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int j = 0; j < M; ++j) Aq[i][j] -= T[i][j];
-        mm<M>(Cq, Ch, T);
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int j = 0; j < M; ++j) Cq[i][j] = -T[i][j];
-        store_mat<M>(A, am, g, Aq);
-        store_mat<M>(C, am, g, Cq);
-        store_vec<M>(D, av, g, dq);
+        load_mat_c<M>(A, am, g, An);
+        load_vec<M>(D, av, g, dn);
 #pragma unroll
         for (int i = 0; i < M; ++i) {
-            dh[i] = dq[i];
+#pragma unroll
+            for (int k = 0; k < M; ++k) dn[i] = fma(-Cq[i][k], dh[k], dn[i]);
 #pragma unroll
             for (int j = 0; j < M; ++j) {
-                Ah[i][j] = Aq[i][j];
-                Ch[i][j] = Cq[i][j];
+                double sa = An[i][j], sc = 0.0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    sa = fma(-Cq[i][k], Ah[k][j], sa);
+                    sc = fma(-Cq[i][k], Ch[k][j], sc);
+                }
+                An[i][j] = sa;
+                Cn[i][j] = sc;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dh[i] = dn[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                Ah[i][j] = An[i][j];
+                Ch[i][j] = Cn[i][j];
             }
         }
     }
     // close, q = 0: R = I - Ĉ_0 Â_1; [d̂|Â|Ĉ]_0 = R^{-1}[d̂_0 - Ĉ_0 d̂_1 | Â_0 | -Ĉ_0 Ĉ_1]
-    // (Ah, Ch, dh) now hold row 1 after the backward pass.
+    // with (Ah, Ch, dh) row 1 composed over the whole partition.
     {
         double A0[M][M], C0[M][M], d0[M], R[M][M], X[M][M], T[M][M], t[M];
         load_mat_c<M>(A, am, r0, A0);
@@ -760,8 +763,10 @@ __global__ __launch_bounds__(64) void k_reduced(const double* __restrict__ redA,
 }
 
 // ----------------------------------------------------------------------------- update --
-// x_1, x_N of each partition from the reduced solution; x_q = d̂_q - Â_q x_1 - Ĉ_q x_N for
-// the interior rows (mod_btdma_cpu.f90:511-535).
+// Stage 3 (btdma_many_update, mod_btdma_cpu.f90:511-535): x_1 and x_N of each partition come
+// from the reduced solution; the interior rows follow from the forward values of stage 1 by
+// x_q = d̂_q - Â_q x_1 - Ĉ_q x_{q+1}, q = N-2 .. 1 (the reference uses the backward-rewritten
+// rows and x_N instead; the two are equal in exact arithmetic).
 template <int M, int LAYOUT>
 __global__ __launch_bounds__(64) void k_update(const double* __restrict__ A,
                                                const double* __restrict__ C, double* D, int nsys,
@@ -781,7 +786,7 @@ __global__ __launch_bounds__(64) void k_update(const double* __restrict__ A,
     load_vec<M>(redD, rv, 2 * part + 1, xn);
     store_vec<M>(D, av, r0, x1);
     store_vec<M>(D, av, r0 + N - 1, xn);
-    for (int q = 1; q < N - 1; ++q) {
+    for (int q = N - 2; q >= 1; --q) {
         const int64_t g = r0 + q;
         double a[M][M], c[M][M], x[M];
         load_mat<M>(A, am, g, a);
@@ -791,6 +796,8 @@ __global__ __launch_bounds__(64) void k_update(const double* __restrict__ A,
         for (int i = 0; i < M; ++i)
 #pragma unroll
             for (int j = 0; j < M; ++j) x[i] = fma(-a[i][j], x1[j], fma(-c[i][j], xn[j], x[i]));
+#pragma unroll
+        for (int i = 0; i < M; ++i) xn[i] = x[i];
         store_vec<M>(D, av, g, x);
     }
 }
