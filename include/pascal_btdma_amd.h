@@ -95,9 +95,12 @@ btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double *a, const double
                                         double *d, int m, int nsys, int nrow, btd_timings *timings,
                                         void *stream);
 
-/* Replaces solve_cyclic (pascal_btdma_cuda.hpp:158-166), which throws in the reference
- * C++ port, and btdma_many_cycl_mpi_gpu_v2 (mod_btdma_gpu_v2.f90:200-239). Returns
- * BTD_ERR_NOT_IMPLEMENTED, as the C++ port does. */
+/* Periodic solve: replaces btdma_many_cycl_mpi_gpu_v2 (mod_btdma_gpu_v2.f90:200-239) and
+ * solve_cyclic (pascal_btdma_cuda.hpp:158-166), which throws logic_error in the reference
+ * C++ port. Row 0 couples to x_{n-1} through A of row 0 and row n-1 to x_0 through C of
+ * row n-1. npart = 1: one periodic block-Thomas sweep per system (needs nrow >= 3; A, C, D
+ * overwritten); npart > 1: stage 1 as in btd_solve_noncyclic, a periodic reduced system
+ * of 2*npart rows, stage 3 (btdma_many_cyclic_mpi, mod_btdma_cpu.f90:273-302). */
 btd_status btd_solve_cyclic(btd_plan_t plan, double *a, const double *b, double *c, double *d,
                             int m, int nsys, int nrow, void *stream);
 

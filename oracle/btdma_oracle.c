@@ -520,6 +520,54 @@ done:
     return info;
 }
 
+/* btdma_many_cyclic_mpi (mod_btdma_cpu.f90:273-302) with the npart row partitions held in
+ * one process: stage 1 and stage 3 as in ora_btdma_many_mpi, the reduced system of 2*npart
+ * rows solved periodically (btdma_many_cycl). Requires n/npart >= 3 and 2*npart >= 3. */
+int ora_btdma_many_cyclic_mpi(int m, int nsys, int n, int npart, double *A, double *B, double *C,
+                              double *D) {
+    if (m < 1 || nsys < 1 || npart < 2 || n < 3 * npart) return -1;
+    size_t mm2 = (size_t)m * m;
+    size_t rmat = mm2 * (size_t)nsys * 2, rvec = (size_t)m * nsys * 2;
+    size_t tmat = mm2 * (size_t)nsys * 2 * npart, tvec = (size_t)m * nsys * 2 * npart;
+    double *rdA = malloc(sizeof(double) * rmat), *rdB = malloc(sizeof(double) * rmat);
+    double *rdC = malloc(sizeof(double) * rmat), *rdD = malloc(sizeof(double) * rvec);
+    double *trA = malloc(sizeof(double) * tmat), *trB = malloc(sizeof(double) * tmat);
+    double *trC = malloc(sizeof(double) * tmat), *trD = malloc(sizeof(double) * tvec);
+    int info = 0;
+    if (!rdA || !rdB || !rdC || !rdD || !trA || !trB || !trC || !trD) {
+        info = -2;
+        goto done;
+    }
+    for (int r = 0; r < npart; ++r) {
+        int ia, ib;
+        int nrow = ora_para(0, n - 1, r, npart, &ia, &ib);
+        size_t om = mm2 * (size_t)nsys * ia, ov = (size_t)m * nsys * ia;
+        info = imax(info, ora_btdma_many_modi(m, nsys, nrow, A + om, B + om, C + om, D + ov, rdA, rdB,
+                                              rdC, rdD));
+        memcpy(trA + mm2 * (size_t)nsys * 2 * r, rdA, sizeof(double) * rmat);
+        memcpy(trB + mm2 * (size_t)nsys * 2 * r, rdB, sizeof(double) * rmat);
+        memcpy(trC + mm2 * (size_t)nsys * 2 * r, rdC, sizeof(double) * rmat);
+        memcpy(trD + (size_t)m * nsys * 2 * r, rdD, sizeof(double) * rvec);
+    }
+    info = imax(info, ora_btdma_many_cycl(2 * npart, nsys, m, trA, trB, trC, trD));
+    for (int r = 0; r < npart; ++r) {
+        int ia, ib;
+        int nrow = ora_para(0, n - 1, r, npart, &ia, &ib);
+        size_t om = mm2 * (size_t)nsys * ia, ov = (size_t)m * nsys * ia;
+        ora_btdma_many_update(m, nsys, nrow, A + om, C + om, D + ov, trD + (size_t)m * nsys * 2 * r);
+    }
+done:
+    free(rdA);
+    free(rdB);
+    free(rdC);
+    free(rdD);
+    free(trA);
+    free(trB);
+    free(trC);
+    free(trD);
+    return info;
+}
+
 int ora_btdma_many_cycl(int n, int nsys, int m, double *a, double *b, double *c, double *d) {
     if (n < 3 || nsys < 1 || m < 1) return -1;
     double *e = (double *)malloc(sizeof(double) * (size_t)m * m * nsys * n);

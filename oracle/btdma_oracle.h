@@ -20,6 +20,7 @@
  *                        whole-block-row permutation invariance
  *   ora_btdma_many_mpi   pinned: dense solve, P invariance, Study signatures at np=2,4,8
  *   ora_btdma_many_cycl  pinned: dense periodic solve (n >= 3)
+ *   ora_btdma_many_cyclic_mpi  pinned: dense periodic solve, P invariance
  *   ora_btdma_many_t     pinned: dense transposed solve
  */
 #ifndef BTDMA_ORACLE_H
@@ -60,6 +61,11 @@ int ora_btdma_many_mpi(int m, int nsys, int n, int npart, double *A, double *B, 
 /* btdma_many_cycl (mod_btdma_cpu.f90:80-183): periodic block Thomas. A(row 1) couples to
  * x_n, C(row n) couples to x_1. Requires n >= 3. */
 int ora_btdma_many_cycl(int n, int nsys, int m, double *a, double *b, double *c, double *d);
+
+/* btdma_many_cyclic_mpi (mod_btdma_cpu.f90:273-302) with npart in-process row partitions:
+ * stage 1, a periodic reduced system of 2*npart rows, stage 3. npart >= 2, n/npart >= 3. */
+int ora_btdma_many_cyclic_mpi(int m, int nsys, int n, int npart, double *A, double *B, double *C,
+                              double *D);
 
 /* Transposed solve T^T lambda = w for the (non-periodic) system given by a, b, c,
  * assembled as the block-tridiagonal system with blocks a'_q = c_{q-1}^T,

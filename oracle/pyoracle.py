@@ -40,6 +40,7 @@ def lib():
         L.ora_btdma_many.argtypes = [ctypes.c_int] * 3 + [_dp] * 4
         L.ora_btdma_many_mpi.argtypes = [ctypes.c_int] * 4 + [_dp] * 4
         L.ora_btdma_many_cycl.argtypes = [ctypes.c_int] * 3 + [_dp] * 4
+        L.ora_btdma_many_cyclic_mpi.argtypes = [ctypes.c_int] * 4 + [_dp] * 4
         L.ora_btdma_many_t.argtypes = [ctypes.c_int] * 3 + [_dp] * 4
         L.ora_set_num_threads.argtypes = [ctypes.c_int]
         L.ora_num_threads.restype = ctypes.c_int
@@ -103,6 +104,14 @@ def solve_cyclic(A, B, C, D):
     A, B, C, D = _prep(A, B, C, D)
     n, nsys, m = D.shape
     info = lib().ora_btdma_many_cycl(n, nsys, m, _p(A), _p(B), _p(C), _p(D))
+    return D, info
+
+
+def solve_cyclic_two_stage(A, B, C, D, npart: int):
+    """btdma_many_cyclic_mpi with npart in-process row partitions (npart >= 2)."""
+    A, B, C, D = _prep(A, B, C, D)
+    n, nsys, m = D.shape
+    info = lib().ora_btdma_many_cyclic_mpi(m, nsys, n, npart, _p(A), _p(B), _p(C), _p(D))
     return D, info
 
 

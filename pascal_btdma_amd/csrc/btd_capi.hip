@@ -160,18 +160,22 @@ static btd_status check_solve(btd_plan_t plan, const void* a, const void* b, con
 }
 
 static btd_status solve_impl(btd_plan_t plan, double* a, const double* b, double* c, double* d,
-                             hipStream_t s, bool timed) {
+                             hipStream_t s, bool timed, bool cyclic = false) {
     plan->last_stream = s;
     if (plan->npart == 1) {
         if (timed) BTD_HIP(hipEventRecord(plan->ev[0], s));
-        BTD_HIP(btd::launch_plain(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
-                                  plan->d_flag, s));
+        if (cyclic)
+            BTD_HIP(btd::launch_cyclic(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
+                                       plan->d_flag, s));
+        else
+            BTD_HIP(btd::launch_plain(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
+                                      plan->d_flag, s));
         if (timed) BTD_HIP(hipEventRecord(plan->ev[1], s));
     } else {
         BTD_HIP(btd::launch_two_stage(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                       plan->d_part, plan->d_part + plan->npart, plan->npart,
                                       plan->redA, plan->redC, plan->redD, plan->d_flag, s,
-                                      timed ? plan->ev : nullptr));
+                                      timed ? plan->ev : nullptr, cyclic));
     }
     return BTD_OK;
 }
@@ -285,16 +289,12 @@ btd_status btd_solve_noncyclic_f32r(btd_plan_t plan, const float* a, const float
 
 btd_status btd_solve_cyclic(btd_plan_t plan, double* a, const double* b, double* c, double* d,
                             int m, int nsys, int nrow, void* stream) {
-    (void)plan;
-    (void)a;
-    (void)b;
-    (void)c;
-    (void)d;
-    (void)m;
-    (void)nsys;
-    (void)nrow;
-    (void)stream;
-    return fail(BTD_ERR_NOT_IMPLEMENTED, "btd_solve_cyclic is not implemented");
+    g_err.clear();
+    btd_status st = check_solve(plan, a, b, c, d, m, nsys, nrow);
+    if (st != BTD_OK) return st;
+    if (plan->npart == 1 && nrow < 3)
+        return fail(BTD_ERR_INVALID_ARG, "btd_solve_cyclic: the periodic solve needs nrow >= 3");
+    return solve_impl(plan, a, b, c, d, (hipStream_t)stream, false, true);
 }
 
 btd_status btd_convert_layout(int src_layout, int dst_layout, int E, int nsys, int nrow,

@@ -8,7 +8,8 @@ hipError_t launch_plain_1(int, const double*, const double*, double*, double*, i
                           hipStream_t);
 hipError_t launch_two_stage_1(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_1(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_1(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -16,7 +17,8 @@ hipError_t launch_plain_2(int, const double*, const double*, double*, double*, i
                           hipStream_t);
 hipError_t launch_two_stage_2(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_2(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_2(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -24,7 +26,8 @@ hipError_t launch_plain_3(int, const double*, const double*, double*, double*, i
                           hipStream_t);
 hipError_t launch_two_stage_3(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_3(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_3(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -32,7 +35,8 @@ hipError_t launch_plain_4(int, const double*, const double*, double*, double*, i
                           hipStream_t);
 hipError_t launch_two_stage_4(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_4(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_4(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -40,7 +44,8 @@ hipError_t launch_plain_5(int, const double*, const double*, double*, double*, i
                           hipStream_t);
 hipError_t launch_two_stage_5(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_5(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_5(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -48,7 +53,8 @@ hipError_t launch_plain_6(int, const double*, const double*, double*, double*, i
                           hipStream_t);
 hipError_t launch_two_stage_6(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_6(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_6(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -56,7 +62,8 @@ hipError_t launch_plain_7(int, const double*, const double*, double*, double*, i
                           hipStream_t);
 hipError_t launch_two_stage_7(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_7(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_7(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -64,7 +71,8 @@ hipError_t launch_plain_8(int, const double*, const double*, double*, double*, i
                           hipStream_t);
 hipError_t launch_two_stage_8(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_8(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_8(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -72,7 +80,8 @@ hipError_t launch_plain_9(int, const double*, const double*, double*, double*, i
                           hipStream_t);
 hipError_t launch_two_stage_9(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_9(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_9(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -80,7 +89,8 @@ hipError_t launch_plain_10(int, const double*, const double*, double*, double*, 
                           hipStream_t);
 hipError_t launch_two_stage_10(int, double*, const double*, double*, double*, int, int, const int*,
                               const int*, int, double*, double*, double*, int*, hipStream_t,
-                              hipEvent_t*);
+                              hipEvent_t*, bool);
+hipError_t launch_cyclic_10(int, double*, const double*, double*, double*, int, int, int*, hipStream_t);
 hipError_t launch_generate_10(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
@@ -203,9 +213,9 @@ hipError_t launch_plain(int m, int layout, const double* A, const double* B, dou
 hipError_t launch_two_stage(int m, int layout, double* A, const double* B, double* C, double* D,
                             int nsys, int nrow, const int* pf, const int* pl, int npart,
                             double* redA, double* redC, double* redD, int* flag, hipStream_t s,
-                            hipEvent_t* ev) {
+                            hipEvent_t* ev, bool cyclic) {
 #define CALL(M) \
-    launch_two_stage_##M(layout, A, B, C, D, nsys, nrow, pf, pl, npart, redA, redC, redD, flag, s, ev)
+    launch_two_stage_##M(layout, A, B, C, D, nsys, nrow, pf, pl, npart, redA, redC, redD, flag, s, ev, cyclic)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }
@@ -238,6 +248,13 @@ hipError_t launch_f32r(int m, int layout, const float* A, const float* B, const 
                        const double* D, double* X, float* Wc, float* Wv, int nsys, int nrow,
                        int sweeps, int* flag, hipStream_t s) {
 #define CALL(M) launch_f32r_##M(layout, A, B, C, D, X, Wc, Wv, nsys, nrow, sweeps, flag, s)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_cyclic(int m, int layout, double* A, const double* B, double* C, double* D,
+                         int nsys, int nrow, int* flag, hipStream_t s) {
+#define CALL(M) launch_cyclic_##M(layout, A, B, C, D, nsys, nrow, flag, s)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }

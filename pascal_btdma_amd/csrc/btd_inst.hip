@@ -20,9 +20,15 @@ hipError_t BTD_NAME(launch_plain_)(int layout, const double* A, const double* B,
 hipError_t BTD_NAME(launch_two_stage_)(int layout, double* A, const double* B, double* C,
                                        double* D, int nsys, int nrow, const int* pf,
                                        const int* pl, int npart, double* redA, double* redC,
-                                       double* redD, int* flag, hipStream_t s, hipEvent_t* ev) {
+                                       double* redD, int* flag, hipStream_t s, hipEvent_t* ev,
+                                       bool cyclic) {
     return launch_two_stage_m<BTD_M>(layout, A, B, C, D, nsys, nrow, pf, pl, npart, redA, redC,
-                                     redD, flag, s, ev);
+                                     redD, flag, s, ev, cyclic);
+}
+
+hipError_t BTD_NAME(launch_cyclic_)(int layout, double* A, const double* B, double* C, double* D,
+                                    int nsys, int nrow, int* flag, hipStream_t s) {
+    return launch_cyclic_m<BTD_M>(layout, A, B, C, D, nsys, nrow, flag, s);
 }
 
 hipError_t BTD_NAME(launch_generate_)(int kind, uint64_t seed, int layout, int nsys, int nrow,

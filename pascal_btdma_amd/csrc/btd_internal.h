@@ -29,7 +29,10 @@ hipError_t launch_f32r(int m, int layout, const float* A, const float* B, const 
 hipError_t launch_two_stage(int m, int layout, double* A, const double* B, double* C, double* D,
                             int nsys, int nrow, const int* part_first, const int* part_len,
                             int npart, double* redA, double* redC, double* redD, int* flag,
-                            hipStream_t s, hipEvent_t* ev);
+                            hipStream_t s, hipEvent_t* ev, bool cyclic);
+
+hipError_t launch_cyclic(int m, int layout, double* A, const double* B, double* C, double* D,
+                         int nsys, int nrow, int* flag, hipStream_t s);
 
 hipError_t launch_generate(int m, int kind, uint64_t seed, int layout, int nsys, int nrow, double* A,
                            double* B, double* C, double* D, float* Af, float* Bf, float* Cf,

@@ -762,6 +762,187 @@ __global__ __launch_bounds__(64) void k_reduced(const double* __restrict__ redA,
     if (!ok) atomicOr(flag, 1);
 }
 
+// ----------------------------------------------------------------------------- cyclic --
+// Periodic block Thomas, btdma_many_cycl (mod_btdma_cpu.f90:80-183; GPU
+// mod_btdma_gpu_v2.f90:352-616): row 0 couples to x_{n-1} through A_0 and row n-1 to x_0
+// through C_{n-1}. x_0 is kept as a separate unknown: rows 1..n-1 are eliminated with a fill-in
+// block column E (E_1 = -A_1, E_{n-1} = -C_{n-1}, otherwise 0), so that after the forward and
+// backward sweeps x_q = d'_q + E'_q x_0; then row 0 gives
+//   (B_0 + A_0 E'_{n-1} + C_0 E'_1) x_0 = d_0 - A_0 d'_{n-1} - C_0 d'_1.
+// In place: E'_q over A_q, C'_q over C_q, d'_q over D (rows >= 1); needs n >= 3. BI = true
+// solves a system whose B blocks are the identity (the reduced system of the two-stage
+// solve), B is then not read.
+template <int M, int LAYOUT, bool BI>
+__device__ __forceinline__ bool cyclic_solve(double* A, const double* __restrict__ B, double* C,
+                                             double* D, const Addr<LAYOUT>& am,
+                                             const Addr<LAYOUT>& av, int nrow) {
+    bool ok = true;
+    double Cp[M][M], Ep[M][M], dp[M];
+    for (int q = 1; q < nrow; ++q) {
+        const bool last = q == nrow - 1;
+        double R[M][M], v[M], X[M][M];
+        if constexpr (BI) {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) R[i][j] = (i == j) ? 1.0 : 0.0;
+        } else {
+            load_mat<M>(B, am, q, R);
+        }
+        load_vec<M>(D, av, q, v);
+        if (q == 1) {
+            load_mat_c<M>(A, am, q, X);
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) X[i][j] = -X[i][j];
+        } else {
+            if (last) {
+                load_mat_c<M>(C, am, q, X);
+#pragma unroll
+                for (int i = 0; i < M; ++i)
+#pragma unroll
+                    for (int j = 0; j < M; ++j) X[i][j] = -X[i][j];
+            } else {
+#pragma unroll
+                for (int i = 0; i < M; ++i)
+#pragma unroll
+                    for (int j = 0; j < M; ++j) X[i][j] = 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                double ak[M];
+                load_col<M>(A, am, q, k, ak);
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    v[i] = fma(-ak[i], dp[k], v[i]);
+#pragma unroll
+                    for (int j = 0; j < M; ++j) {
+                        R[i][j] = fma(-ak[i], Cp[k][j], R[i][j]);
+                        X[i][j] = fma(-ak[i], Ep[k][j], X[i][j]);
+                    }
+                }
+            }
+        }
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, v);
+        solve_rows<M>(R, piv, X);
+        if (!last) {
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double c[M];
+                load_col<M>(C, am, q, j, c);
+                lu_solve<M>(R, piv, c);
+#pragma unroll
+                for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
+                store_col<M>(C, am, q, j, c);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dp[i] = v[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j) Ep[i][j] = X[i][j];
+        }
+        store_mat<M>(A, am, q, Ep);
+        store_vec<M>(D, av, q, dp);
+    }
+    // backward, q = n-2..1: d'_q -= C'_q d'_{q+1}; E'_q -= C'_q E'_{q+1} ((Ep, dp) = row q+1)
+    for (int q = nrow - 2; q >= 1; --q) {
+        double c[M][M], e[M][M], d[M];
+        load_mat_c<M>(C, am, q, c);
+        load_mat_c<M>(A, am, q, e);
+        load_vec<M>(D, av, q, d);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) d[i] = fma(-c[i][k], dp[k], d[i]);
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double s = e[i][j];
+#pragma unroll
+                for (int k = 0; k < M; ++k) s = fma(-c[i][k], Ep[k][j], s);
+                e[i][j] = s;
+            }
+        }
+        store_mat<M>(A, am, q, e);
+        store_vec<M>(D, av, q, d);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dp[i] = d[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j) Ep[i][j] = e[i][j];
+        }
+    }
+    // row 0; (Ep, dp) now hold row 1
+    double x0[M];
+    {
+        double a0[M][M], c0[M][M], en[M][M], dn[M], R[M][M];
+        load_mat_c<M>(A, am, 0, a0);
+        load_mat_c<M>(C, am, 0, c0);
+        load_mat_c<M>(A, am, nrow - 1, en);
+        load_vec<M>(D, av, nrow - 1, dn);
+        load_vec<M>(D, av, 0, x0);
+        if constexpr (BI) {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) R[i][j] = (i == j) ? 1.0 : 0.0;
+        } else {
+            load_mat_c<M>(B, am, 0, R);
+        }
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) x0[i] = fma(-a0[i][k], dn[k], fma(-c0[i][k], dp[k], x0[i]));
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double s = R[i][j];
+#pragma unroll
+                for (int k = 0; k < M; ++k) s = fma(a0[i][k], en[k][j], fma(c0[i][k], Ep[k][j], s));
+                R[i][j] = s;
+            }
+        }
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, x0);
+        store_vec<M>(D, av, 0, x0);
+    }
+    for (int q = 1; q < nrow; ++q) {
+        double e[M][M], d[M];
+        load_mat_c<M>(A, am, q, e);
+        load_vec<M>(D, av, q, d);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int k = 0; k < M; ++k) d[i] = fma(e[i][k], x0[k], d[i]);
+        store_vec<M>(D, av, q, d);
+    }
+    return ok;
+}
+
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_cyclic(double* A, const double* __restrict__ B, double* C,
+                                               double* D, int nsys, int nrow,
+                                               int* __restrict__ flag) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    if (!cyclic_solve<M, LAYOUT, false>(A, B, C, D, am, av, nrow)) atomicOr(flag, 1);
+}
+
+// Periodic reduced system (2P rows, B = I) of the two-stage cyclic solve
+// (btdma_many_cyclic_mpi, mod_btdma_cpu.f90:273-302).
+template <int M>
+__global__ __launch_bounds__(64) void k_reduced_cyclic(double* redA, double* redC, double* redD,
+                                                       int nsys, int nrr, int* __restrict__ flag) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const Addr<LAYOUT_TILED> am(sys, 0, nrr, M * M), av(sys, 0, nrr, M);
+    if (!cyclic_solve<M, LAYOUT_TILED, true>(redA, nullptr, redC, redD, am, av, nrr)) atomicOr(flag, 1);
+}
+
 // ----------------------------------------------------------------------------- update --
 // Stage 3 (btdma_many_update, mod_btdma_cpu.f90:511-535): x_1 and x_N of each partition come
 // from the reduced solution; the interior rows follow from the forward values of stage 1 by
@@ -883,10 +1064,21 @@ hipError_t launch_f32r_m(int layout, const float* A, const float* B, const float
 }
 
 template <int M>
+hipError_t launch_cyclic_m(int layout, double* A, const double* B, double* C, double* D, int nsys,
+                           int nrow, int* flag, hipStream_t s) {
+    const dim3 grid((nsys + 63) / 64), block(64);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_cyclic<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    else
+        hipLaunchKernelGGL((k_cyclic<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    return hipGetLastError();
+}
+
+template <int M>
 hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C, double* D,
-                                     int nsys, int nrow, const int* pf, const int* pl, int npart,
-                                     double* redA, double* redC, double* redD, int* flag,
-                                     hipStream_t s, hipEvent_t* ev) {
+                              int nsys, int nrow, const int* pf, const int* pl, int npart,
+                              double* redA, double* redC, double* redD, int* flag,
+                              hipStream_t s, hipEvent_t* ev, bool cyclic) {
     const int ntiles = (nsys + 63) / 64;
     const dim3 block(64), gridp((unsigned)(ntiles * npart)), gridr((unsigned)ntiles);
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -897,7 +1089,10 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
         hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU>), gridp, block, 0, s, A, B, C, D, nsys, nrow, pf, pl,
                            npart, redA, redC, redD, flag);
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL((k_reduced<M>), gridr, block, 0, s, redA, redC, redD, nsys, 2 * npart, flag);
+    if (cyclic)
+        hipLaunchKernelGGL((k_reduced_cyclic<M>), gridr, block, 0, s, redA, redC, redD, nsys, 2 * npart, flag);
+    else
+        hipLaunchKernelGGL((k_reduced<M>), gridr, block, 0, s, redA, redC, redD, nsys, 2 * npart, flag);
     if (ev) (void)hipEventRecord(ev[2], s);
     if (layout == LAYOUT_TILED)
         hipLaunchKernelGGL((k_update<M, LAYOUT_TILED>), gridp, block, 0, s, A, C, D, nsys, nrow, pf, pl,

@@ -2,7 +2,7 @@
 // C++ mirror header: the Study's diagonal workload (A = 0.25 I, B = 2 I, C = 0.25 I, d = 1),
 // 32 x 32 systems of 128 block rows, solved with npart = 1 and npart = 4, checked against
 // the reference's published signature (sum, L2, end and middle samples) and against the
-// error behaviour of the reference (invalid sizes, dimension mismatch, cyclic).
+// error behaviour of the reference (invalid sizes, dimension mismatch, uncreated plan).
 // Usage: ex_btdma_noncyclic <m> <expected_sum> <expected_l2>
 #include <cmath>
 #include <cstdio>
@@ -89,9 +89,9 @@ int main(int argc, char** argv) {
     expect("dimension mismatch", [&] { pascal_btdma::BtdmaGpuPlan p; p.create(m, nsys, nrow);
                                        pascal_btdma::solve_noncyclic(p, a, b, c, d, m, nsys, nrow - 1); },
            std::invalid_argument(""));
-    expect("cyclic", [&] { pascal_btdma::BtdmaGpuPlan p; p.create(m, nsys, nrow);
-                           pascal_btdma::solve_cyclic(p, a, b, c, d, m, nsys, nrow); },
-           std::logic_error(""));
+    expect("cyclic with nrow = 2", [&] { pascal_btdma::BtdmaGpuPlan p; p.create(m, nsys, 2);
+                                         pascal_btdma::solve_cyclic(p, a, b, c, d, m, nsys, 2); },
+           std::invalid_argument(""));
     (void)hipFree(a);
     (void)hipFree(b);
     (void)hipFree(c);

@@ -71,11 +71,11 @@ def test_plan_create_rejects_bad_arguments(args, exc):
         pb.BtdmaGpuPlan().create(*args)
 
 
-def test_uncreated_plan_and_cyclic_errors():
+def test_uncreated_plan_errors():
     plan = pb.BtdmaGpuPlan()
     with pytest.raises(RuntimeError):
         pb.solve_noncyclic(plan, 1, 1, 1, 1, 3, 10, 10, stream=0)
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(RuntimeError):
         pb.solve_cyclic(plan, 1, 1, 1, 1, 3, 10, 10, stream=0)
 
 

@@ -1,0 +1,68 @@
+"""Periodic (cyclic) solve on the device vs the oracle (btdma_many_cycl and
+btdma_many_cyclic_mpi restatements, both pinned to dense periodic solves)."""
+import numpy as np
+import pytest
+
+import pascal_btdma_amd as pb
+from helpers import cpu_layout, layout_to_vec, nat_to_layout, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def periodic_system(rng, n, nsys, m):
+    A = rng.uniform(-1, 1, (n, nsys, m, m))
+    C = rng.uniform(-1, 1, (n, nsys, m, m))
+    B = rng.uniform(-1, 1, (n, nsys, m, m))
+    s = np.abs(A).sum(-1) + np.abs(C).sum(-1) + np.abs(B).sum(-1) + 1.0
+    B += s[..., None] * np.eye(m)
+    for q in range(n):
+        for k in range(nsys):
+            p = rng.permutation(m)
+            A[q, k], B[q, k], C[q, k] = A[q, k][p], B[q, k][p], C[q, k][p]
+    d = rng.uniform(-1, 1, (n, nsys, m))
+    return A, B, C, d
+
+
+def gpu_cyclic(cuda, A, B, C, d, layout, npart):
+    import torch
+    n, nsys, m = d.shape
+    lay = int(layout)
+    t = [torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d)]
+    with pb.BtdmaGpuPlan().create(m, nsys, n, npart, layout) as plan:
+        pb.solve_cyclic(plan, *t, m, nsys, n)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    return layout_to_vec(t[3].cpu().numpy(), lay, n, nsys, m)
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 5, 8, 10])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_cyclic_single_partition(cuda, oracle, m, layout):
+    rng = np.random.default_rng(40 + m)
+    A, B, C, d = periodic_system(rng, 17, 70, m)
+    x = gpu_cyclic(cuda, A, B, C, d, layout, 1)
+    xo, info = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    assert rel_err(x, xo).max() <= TOL
+
+
+@pytest.mark.parametrize("m,n,npart", [(2, 24, 2), (3, 96, 4), (5, 96, 8), (3, 100, 7), (5, 9, 3)])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_cyclic_two_stage(cuda, oracle, m, n, npart, layout):
+    rng = np.random.default_rng(90 + n + m)
+    A, B, C, d = periodic_system(rng, n, 67, m)
+    x = gpu_cyclic(cuda, A, B, C, d, layout, npart)
+    xo, info = oracle.solve_cyclic_two_stage(cpu_layout(A), cpu_layout(B), cpu_layout(C), d, npart)
+    assert info == 0
+    assert rel_err(x, xo).max() <= TOL
+    x1, _ = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert rel_err(x, x1).max() <= TOL
+
+
+def test_cyclic_three_rows(cuda, oracle):
+    rng = np.random.default_rng(3)
+    A, B, C, d = periodic_system(rng, 3, 64, 4)
+    x = gpu_cyclic(cuda, A, B, C, d, pb.Layout.TILED, 1)
+    xo, _ = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert rel_err(x, xo).max() <= TOL

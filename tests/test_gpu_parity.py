@@ -167,5 +167,8 @@ def test_api_errors_on_device(cuda):
             pb.solve_noncyclic(plan, t, t, t, v, 3, 64, 9)  # dimension mismatch
         with pytest.raises(ValueError):
             pb.solve_noncyclic(plan, t[:10], t, t, v, 3, 64, 8)  # too small
-        with pytest.raises(NotImplementedError):
-            pb.solve_cyclic(plan, t, t, t, v, 3, 64, 8)
+    with pb.BtdmaGpuPlan().create(3, 64, 2) as plan:  # a periodic solve needs nrow >= 3
+        t = torch.zeros(64 * 2 * 9, dtype=torch.float64, device=cuda)
+        v = torch.zeros(64 * 2 * 3, dtype=torch.float64, device=cuda)
+        with pytest.raises(ValueError):
+            pb.solve_cyclic(plan, t, t, t, v, 3, 64, 2)

@@ -213,3 +213,14 @@ def test_study_closed_form_scalar_tridiagonal():
     x = sla.solve_banded((1, 1), ab, np.ones(n))
     assert abs(x[0] - 0.4508066615170332) < 1e-15
     assert abs(x[n // 2] - 0.4) < 1e-15
+
+
+@pytest.mark.parametrize("m,n,npart", [(1, 12, 2), (2, 24, 4), (3, 20, 3), (5, 30, 5), (3, 100, 7)])
+def test_cyclic_two_stage_matches_dense_periodic(oracle, m, n, npart):
+    rng = np.random.default_rng(700 + m * n)
+    A, B, C, d = rand_system(rng, n, 3, m, cyclic=True)
+    x, info = oracle.solve_cyclic_two_stage(cpu_layout(A), cpu_layout(B), cpu_layout(C), d, npart)
+    assert info == 0
+    np.testing.assert_allclose(x, dense_x(oracle, A, B, C, d, cyclic=True), rtol=0, atol=1e-12)
+    x1, _ = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    np.testing.assert_allclose(x, x1, rtol=0, atol=1e-13)
