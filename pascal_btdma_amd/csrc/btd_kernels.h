@@ -562,10 +562,14 @@ __global__ __launch_bounds__(64) void k_f32r(const float* __restrict__ A, const 
 //   [d̂|Â|Ĉ]_q = R^{-1}[d_q - A_q d̂_{q-1} | -A_q Â_{q-1} | C_q],  R = B_q - A_q Ĉ_{q-1}
 // (rows 0 and 1: R = B_q and the right-hand side [d_q | A_q | C_q]), so that
 //   x_q + Â_q x_0' + Ĉ_q x_{q+1} = d̂_q   (x_0' = first row of the partition, q >= 1).
-// The reference then rewrites every row in a backward sweep; here the backward sweep only
-// reads the forward values and carries the composed coefficients of row 1 in registers,
-// which is all the closing step needs (stage 3 uses the forward values directly), so it
-// stores nothing. Reduced rows go to the plan's buffers (tiled, 2P rows per system):
+// The reference then rewrites every row in a backward sweep. Here the closing step's only
+// need from that sweep, row 1 expressed through x_0' and x_{N-1}
+//   x_1 = alpha - beta x_0' - gamma x_{N-1},
+// is composed during the forward sweep (alpha, beta, gamma start as row 1's d̂, Â, Ĉ and take
+// alpha -= gamma d̂_k, beta -= gamma Â_k, gamma = -gamma Ĉ_k for k = 2..N-2), with the three
+// kept in LDS ([element][lane], sp); stage 3 uses the forward values directly, so the
+// partition is read once and nothing is re-read. Reduced rows go to the plan's buffers
+// (tiled, 2P rows per system):
 //   row 2*part   : (Â_1, I, Ĉ_1, d̂_1) after the close, couples x_1 to x_0 and x_N
 //   row 2*part+1 : (Â_N, I, Ĉ_N, d̂_N) forward values, couples x_N to x_1 and x_{N+1}
 template <int M, int LAYOUT>
@@ -582,6 +586,11 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
     const int64_t r0 = part_first[part];
     const int N = part_len[part];
     const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    extern __shared__ double sp[];  // alpha (M), beta (M*M), gamma (M*M), [element][lane]
+    const int lane = threadIdx.x;
+    auto SA = [&](int i) -> double& { return sp[i * 64 + lane]; };
+    auto SB = [&](int i, int j) -> double& { return sp[(M + i * M + j) * 64 + lane]; };
+    auto SG = [&](int i, int j) -> double& { return sp[(M + M * M + i * M + j) * 64 + lane]; };
     bool ok = true;
     double Ah[M][M], Ch[M][M], dh[M];
     for (int q = 0; q < N; ++q) {
@@ -635,50 +644,55 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
             store_mat<M>(A, am, g, Ah);
             store_vec<M>(D, av, g, dh);
         }
+        if (q == 1) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                SA(i) = dh[i];
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    SB(i, j) = Ah[i][j];
+                    SG(i, j) = Ch[i][j];
+                }
+            }
+        } else if (q >= 2 && keep) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                double gi[M];
+#pragma unroll
+                for (int k = 0; k < M; ++k) gi[k] = SG(i, k);
+                double a = SA(i);
+#pragma unroll
+                for (int k = 0; k < M; ++k) a = fma(-gi[k], dh[k], a);
+                SA(i) = a;
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    double b = SB(i, j), c = 0.0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) {
+                        b = fma(-gi[k], Ah[k][j], b);
+                        c = fma(-gi[k], Ch[k][j], c);
+                    }
+                    SB(i, j) = b;
+                    SG(i, j) = c;
+                }
+            }
+        }
     }
     const Addr<LAYOUT_TILED> rm(sys, 0, 2 * npart, M * M), rv(sys, 0, 2 * npart, M);
     store_mat<M>(redA, rm, 2 * part + 1, Ah);
     store_mat<M>(redC, rm, 2 * part + 1, Ch);
     store_vec<M>(redD, rv, 2 * part + 1, dh);
-    // backward composition, q = N-3..1, read only: with (Ah, Ch, dh) the coefficients of
-    // row q+1 in terms of x_0' and x_N, row q becomes d̂_q - Ĉ_q dh, Â_q - Ĉ_q Ah, -Ĉ_q Ch.
-    load_mat_c<M>(A, am, r0 + N - 2, Ah);
-    load_mat_c<M>(C, am, r0 + N - 2, Ch);
-    load_vec<M>(D, av, r0 + N - 2, dh);
-    for (int q = N - 3; q >= 1; --q) {
-        const int64_t g = r0 + q;
-        double Cq[M][M], An[M][M], dn[M], Cn[M][M];
-        load_mat_c<M>(C, am, g, Cq);
-        load_mat_c<M>(A, am, g, An);
-        load_vec<M>(D, av, g, dn);
 #pragma unroll
-        for (int i = 0; i < M; ++i) {
+    for (int i = 0; i < M; ++i) {
+        dh[i] = SA(i);
 #pragma unroll
-            for (int k = 0; k < M; ++k) dn[i] = fma(-Cq[i][k], dh[k], dn[i]);
-#pragma unroll
-            for (int j = 0; j < M; ++j) {
-                double sa = An[i][j], sc = 0.0;
-#pragma unroll
-                for (int k = 0; k < M; ++k) {
-                    sa = fma(-Cq[i][k], Ah[k][j], sa);
-                    sc = fma(-Cq[i][k], Ch[k][j], sc);
-                }
-                An[i][j] = sa;
-                Cn[i][j] = sc;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            dh[i] = dn[i];
-#pragma unroll
-            for (int j = 0; j < M; ++j) {
-                Ah[i][j] = An[i][j];
-                Ch[i][j] = Cn[i][j];
-            }
+        for (int j = 0; j < M; ++j) {
+            Ah[i][j] = SB(i, j);
+            Ch[i][j] = SG(i, j);
         }
     }
-    // close, q = 0: R = I - Ĉ_0 Â_1; [d̂|Â|Ĉ]_0 = R^{-1}[d̂_0 - Ĉ_0 d̂_1 | Â_0 | -Ĉ_0 Ĉ_1]
-    // with (Ah, Ch, dh) row 1 composed over the whole partition.
+    // close, q = 0: R = I - Ĉ_0 beta; [d̂|Â|Ĉ]_0 = R^{-1}[d̂_0 - Ĉ_0 alpha | Â_0 | -Ĉ_0 gamma]
+    // with (dh, Ah, Ch) = (alpha, beta, gamma), row 1 composed over the whole partition.
     {
         double A0[M][M], C0[M][M], d0[M], R[M][M], X[M][M], T[M][M], t[M];
         load_mat_c<M>(A, am, r0, A0);
@@ -1082,11 +1096,18 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     const int ntiles = (nsys + 63) / 64;
     const dim3 block(64), gridp((unsigned)(ntiles * npart)), gridr((unsigned)ntiles);
     if (ev) (void)hipEventRecord(ev[0], s);
+    const size_t lds = sizeof(double) * (2 * M * M + M) * 64;
+    if (lds > 65536) {  // above the default dynamic-LDS limit (m >= 8)
+        (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_TILED>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_REF_GPU>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    }
     if (layout == LAYOUT_TILED)
-        hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED>), gridp, block, 0, s, A, B, C, D, nsys, nrow, pf, pl,
+        hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
                            npart, redA, redC, redD, flag);
     else
-        hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU>), gridp, block, 0, s, A, B, C, D, nsys, nrow, pf, pl,
+        hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
                            npart, redA, redC, redD, flag);
     if (ev) (void)hipEventRecord(ev[1], s);
     if (cyclic)
