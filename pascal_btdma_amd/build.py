@@ -69,6 +69,7 @@ def build(verbose: bool = False, resource_usage: bool = False) -> str:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_cpp_examples(verbose)
+    build_fortran_examples(verbose)
     return LIB
 
 
@@ -87,6 +88,36 @@ def build_cpp_examples(verbose: bool = False) -> None:
         exe = os.path.join(out_dir, os.path.splitext(f)[0])
         cmd = [HIPCC, "-O2", "-std=c++17", os.path.join(src_dir, f), "-I", os.path.join(root, "include"),
                "-L", HERE, "-lpascal_btdma_amd", "-Wl,-rpath,$ORIGIN/../../../pascal_btdma_amd", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {f} failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"[build] {exe}")
+
+
+def build_fortran_examples(verbose: bool = False) -> None:
+    """The Fortran binding module (include/mod_btdma_amd.f90) and the Fortran programs under
+    tests/fortran, compiled with the image's amdflang (skipped if it is absent)."""
+    root = os.path.dirname(HERE)
+    src_dir = os.path.join(root, "tests", "fortran")
+    fc = shutil.which("amdflang") or "/opt/rocm/lib/llvm/bin/amdflang"
+    if not os.path.isdir(src_dir) or not os.path.exists(fc):
+        return
+    out_dir = os.path.join(src_dir, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    mod_obj = os.path.join(out_dir, "mod_btdma_amd.o")
+    cmd = [fc, "-O2", "-c", os.path.join(root, "include", "mod_btdma_amd.f90"), "-module-dir", out_dir,
+           "-o", mod_obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building the Fortran module failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "lib")
+    for f in sorted(os.listdir(src_dir)):
+        if not f.endswith(".f90"):
+            continue
+        exe = os.path.join(out_dir, os.path.splitext(f)[0])
+        cmd = [fc, "-O2", os.path.join(src_dir, f), mod_obj, "-I", out_dir, "-L", HERE, "-lpascal_btdma_amd",
+               "-L", rocm_lib, "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../../pascal_btdma_amd", "-o", exe]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"building {f} failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
