@@ -329,17 +329,21 @@ def extra_f32r(dev, steps: int, fp64_ms: float) -> dict:
 
 
 def extra_adjoint(dev, steps: int) -> dict:
-    """Config 4: WENO-like m = 3, n = 256, 262144 systems through the autograd Function:
-    forward (with factors kept) and backward (adjoint kernel) times, raw kernel times."""
+    """Config 4: WENO-like m = 3, n = 256, 262144 systems through the autograd Functions:
+    forward (with factors kept) and backward (adjoint kernel) times, raw kernel times, in
+    FP64 and with FP32-stored blocks + FP64 refinement. The blocks are rounded to FP32
+    values first, so that both variants solve the same systems."""
     import torch
     import pascal_btdma_amd as pb
-    from pascal_btdma_amd.autograd import block_tridiagonal_solve
+    from pascal_btdma_amd.autograd import block_tridiagonal_solve, block_tridiagonal_solve_f32r
     m, n, nsys = 3, 256, 262144
     lay = pb.Layout.TILED
     a, b, c = (torch.empty(pb.layout_elems(lay, nsys, n, m * m), dtype=torch.float64, device=dev)
                for _ in range(3))
     d = torch.empty(pb.layout_elems(lay, nsys, n, m), dtype=torch.float64, device=dev)
     pb.generate(pb.GenKind.EULER, 4, m, nsys, n, lay, a, b, c, d)
+    for t_ in (a, b, c):
+        t_.copy_(t_.float().double())
     w = torch.rand_like(d) * 2 - 1
     plan = pb.BtdmaGpuPlan().create(m, nsys, n, 1, lay)
     for t_ in (a, b, c, d):
@@ -371,17 +375,46 @@ def extra_adjoint(dev, steps: int) -> dict:
         da, db, dc = (torch.empty_like(ad) for _ in range(3))
         kb = event_time(lambda: pb.solve_adjoint(plan, ad, cw, F, P, xw, w, lam, da, db, dc),
                         lambda: None, steps, s)
+        x64, lam64 = xw.clone(), lam.clone()
+        del cw, F, P, da, db, dc, ad
+    # FP32-stored blocks, 2 refinement sweeps each way
+    a32, b32, c32 = (t_.detach().float().requires_grad_(True) for t_ in (a, b, c))
+    for t_ in (a, b, c):
+        t_.grad = None
+    fwd32, bwd32 = [], []
+    for it in range(steps + 1):
+        for t_ in (a32, b32, c32, d):
+            t_.grad = None
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(s)
+        x = block_tridiagonal_solve_f32r(plan, a32, b32, c32, d, 2)
+        e1.record(s)
+        x.backward(w)
+        e2.record(s)
+        torch.cuda.synchronize()
+        if it > 0:
+            fwd32.append(e0.elapsed_time(e1))
+            bwd32.append(e1.elapsed_time(e2))
+        if it < steps:
+            del x
+    err_x = float(((x.detach() - x64).abs().max() / x64.abs().max()).item())
+    err_l = float(((d.grad - lam64).abs().max() / lam64.abs().max()).item())
+    del x
     plan.destroy()
     rows = nsys * n
     f_ms, b_ms = float(np.median(fwd)), float(np.median(bwd))
     kf_ms, kb_ms = float(np.median(kf)), float(np.median(kb))
+    f32_ms, b32_ms = float(np.median(fwd32)), float(np.median(bwd32))
     return {"workload": "WENO-like m=3, n=256, 262144 systems, FP64, autograd",
             "forward_ms": f_ms, "backward_ms": b_ms, "backward_over_forward": b_ms / f_ms,
             "kernel_forward_save_ms": kf_ms, "kernel_adjoint_ms": kb_ms,
             "kernel_backward_over_forward": kb_ms / kf_ms,
             "forward_block_rows_per_s": rows / (f_ms * 1e-3),
             "kernel_adjoint_alg_GBs": 8.0 * (6 * m * m + 5 * m) * rows / (kb_ms * 1e-3) / 1e9,
-            "fp32_refine_variant": "not built (the adjoint runs in FP64)"}
+            "fp32r_forward_ms": f32_ms, "fp32r_backward_ms": b32_ms,
+            "fp32r_backward_over_forward": b32_ms / f32_ms,
+            "fp32r_speedup_vs_fp64": (f_ms + b_ms) / (f32_ms + b32_ms),
+            "fp32r_max_rel_err_x_vs_fp64": err_x, "fp32r_max_rel_err_lambda_vs_fp64": err_l}
 
 
 def main() -> int:

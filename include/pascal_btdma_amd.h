@@ -133,6 +133,19 @@ btd_status btd_solve_noncyclic_f32r(btd_plan_t plan, const float *a, const float
                                     const double *d, double *x, float *work, int sweeps,
                                     void *stream);
 
+/* Adjoint of btd_solve_noncyclic_f32r (no counterpart in the reference): lambda solves the
+ * transposed system T^T lambda = w from the same FP32-stored blocks a, b, c (as given for
+ * T), by an FP32 solve plus `sweeps` refinement sweeps with FP64 residuals. Same workspace
+ * as btd_solve_noncyclic_f32r; npart must be 1. */
+btd_status btd_solve_transposed_f32r(btd_plan_t plan, const float *a, const float *b, const float *c,
+                                     const double *w, double *lambda, float *work, int sweeps,
+                                     void *stream);
+
+/* dA_q = -lambda_q x_{q-1}^T, dB_q = -lambda_q x_q^T, dC_q = -lambda_q x_{q+1}^T, computed in
+ * FP64 and stored as FP32 block arrays (the gradients of FP32-stored blocks). */
+btd_status btd_outer_grads_f32(btd_plan_t plan, const double *lambda, const double *x, float *da,
+                               float *db, float *dc, void *stream);
+
 /* New (the reference ignores LAPACK/elimination info): synchronises the plan's last
  * stream use, stores 1 in *singular if any block LU met a zero pivot since the last
  * check (0 otherwise), and clears the flag. */

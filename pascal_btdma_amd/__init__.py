@@ -22,7 +22,8 @@ __all__ = [
     "SingularError", "solve_noncyclic", "solve_noncyclic_profiled", "solve_cyclic",
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
     "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
-    "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "BtdmaBatch", "Graph",
+    "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "solve_transposed_f32r",
+    "outer_grads_f32", "BtdmaBatch", "Graph",
 ]
 
 kMaxBlockSize = 10
@@ -270,6 +271,37 @@ def solve_noncyclic_f32r(plan: BtdmaGpuPlan, a, b, c, d, x, work, sweeps: int = 
             _check_tensor(name, t, n, dt)
     _check(_capi.lib().btd_solve_noncyclic_f32r(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
                                                 _ptr(x), _ptr(work), int(sweeps), _stream(stream)))
+
+
+def solve_transposed_f32r(plan: BtdmaGpuPlan, a, b, c, w, lam, work, sweeps: int = 2,
+                          stream=None) -> None:
+    """lambda = T^{-T} w for the FP32-stored blocks a, b, c of T (as given for T): FP32
+    solve of the transposed system plus `sweeps` FP64-residual refinement sweeps."""
+    import torch
+    if plan.created:
+        nm, nv = plan.matrix_elems(), plan.vector_elems()
+        for name, t, n, dt in (("a", a, nm, torch.float32), ("b", b, nm, torch.float32),
+                               ("c", c, nm, torch.float32), ("w", w, nv, torch.float64),
+                               ("lam", lam, nv, torch.float64),
+                               ("work", work, f32r_work_elems(plan), torch.float32)):
+            _check_tensor(name, t, n, dt)
+    _check(_capi.lib().btd_solve_transposed_f32r(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(w),
+                                                 _ptr(lam), _ptr(work), int(sweeps),
+                                                 _stream(stream)))
+
+
+def outer_grads_f32(plan: BtdmaGpuPlan, lam, x, da, db, dc, stream=None) -> None:
+    """FP32 gradients dA_q = -lambda_q x_{q-1}^T, dB_q = -lambda_q x_q^T,
+    dC_q = -lambda_q x_{q+1}^T from FP64 lambda and x."""
+    import torch
+    if plan.created:
+        nm, nv = plan.matrix_elems(), plan.vector_elems()
+        for name, t, n, dt in (("lam", lam, nv, torch.float64), ("x", x, nv, torch.float64),
+                               ("da", da, nm, torch.float32), ("db", db, nm, torch.float32),
+                               ("dc", dc, nm, torch.float32)):
+            _check_tensor(name, t, n, dt)
+    _check(_capi.lib().btd_outer_grads_f32(plan._h, _ptr(lam), _ptr(x), _ptr(da), _ptr(db),
+                                           _ptr(dc), _stream(stream)))
 
 
 class BtdmaBatch:

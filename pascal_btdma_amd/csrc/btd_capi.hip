@@ -283,7 +283,37 @@ btd_status btd_solve_noncyclic_f32r(btd_plan_t plan, const float* a, const float
     float* wc = work;
     float* wv = work + btd::layout_elems(plan->layout, plan->nsys, plan->nrow, plan->m * plan->m);
     BTD_HIP(btd::launch_f32r(plan->m, plan->layout, a, b, c, d, x, wc, wv, plan->nsys, plan->nrow,
-                             sweeps, plan->d_flag, (hipStream_t)stream));
+                             sweeps, plan->d_flag, (hipStream_t)stream, false));
+    return BTD_OK;
+}
+
+btd_status btd_solve_transposed_f32r(btd_plan_t plan, const float* a, const float* b, const float* c,
+                                     const double* w, double* lambda, float* work, int sweeps,
+                                     void* stream) {
+    g_err.clear();
+    btd_status st = check_single_partition(plan, "btd_solve_transposed_f32r");
+    if (st != BTD_OK) return st;
+    if (!a || !b || !c || !w || !lambda || !work)
+        return fail(BTD_ERR_INVALID_ARG, "btd_solve_transposed_f32r: NULL array");
+    if (sweeps < 0 || sweeps > 8)
+        return fail(BTD_ERR_INVALID_ARG, "btd_solve_transposed_f32r: sweeps must be in [0, 8]");
+    plan->last_stream = (hipStream_t)stream;
+    float* wc = work;
+    float* wv = work + btd::layout_elems(plan->layout, plan->nsys, plan->nrow, plan->m * plan->m);
+    BTD_HIP(btd::launch_f32r(plan->m, plan->layout, a, b, c, w, lambda, wc, wv, plan->nsys, plan->nrow,
+                             sweeps, plan->d_flag, (hipStream_t)stream, true));
+    return BTD_OK;
+}
+
+btd_status btd_outer_grads_f32(btd_plan_t plan, const double* lambda, const double* x, float* da,
+                               float* db, float* dc, void* stream) {
+    g_err.clear();
+    btd_status st = check_single_partition(plan, "btd_outer_grads_f32");
+    if (st != BTD_OK) return st;
+    if (!lambda || !x || !da || !db || !dc)
+        return fail(BTD_ERR_INVALID_ARG, "btd_outer_grads_f32: NULL array");
+    BTD_HIP(btd::launch_outer_f32(plan->m, plan->layout, lambda, x, da, db, dc, plan->nsys, plan->nrow,
+                                  (hipStream_t)stream));
     return BTD_OK;
 }
 

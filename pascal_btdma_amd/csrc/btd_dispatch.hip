@@ -100,7 +100,9 @@ hipError_t launch_adjoint_1(int, const double*, const double*, const double*, co
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_1(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_1(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 hipError_t launch_plain_save_2(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -108,7 +110,9 @@ hipError_t launch_adjoint_2(int, const double*, const double*, const double*, co
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_2(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_2(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 hipError_t launch_plain_save_3(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -116,7 +120,9 @@ hipError_t launch_adjoint_3(int, const double*, const double*, const double*, co
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_3(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_3(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 hipError_t launch_plain_save_4(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -124,7 +130,9 @@ hipError_t launch_adjoint_4(int, const double*, const double*, const double*, co
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_4(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_4(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 hipError_t launch_plain_save_5(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -132,7 +140,9 @@ hipError_t launch_adjoint_5(int, const double*, const double*, const double*, co
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_5(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_5(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 hipError_t launch_plain_save_6(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -140,7 +150,9 @@ hipError_t launch_adjoint_6(int, const double*, const double*, const double*, co
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_6(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_6(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 hipError_t launch_plain_save_7(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -148,7 +160,9 @@ hipError_t launch_adjoint_7(int, const double*, const double*, const double*, co
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_7(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_7(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 hipError_t launch_plain_save_8(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -156,7 +170,9 @@ hipError_t launch_adjoint_8(int, const double*, const double*, const double*, co
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_8(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_8(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 hipError_t launch_plain_save_9(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -164,7 +180,9 @@ hipError_t launch_adjoint_9(int, const double*, const double*, const double*, co
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_9(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_9(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 hipError_t launch_plain_save_10(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -172,7 +190,9 @@ hipError_t launch_adjoint_10(int, const double*, const double*, const double*, c
                             const double*, const double*, double*, double*, double*, double*, int,
                             int, hipStream_t);
 hipError_t launch_f32r_10(int, const float*, const float*, const float*, const double*, double*,
-                         float*, float*, int, int, int, int*, hipStream_t);
+                         float*, float*, int, int, int, int*, hipStream_t, bool);
+hipError_t launch_outer_f32_10(int, const double*, const double*, float*, float*, float*, int, int,
+                              hipStream_t);
 
 // ---------------------------------------------------------------------------- convert --
 __global__ __launch_bounds__(256) void k_convert(int src_layout, int dst_layout, int E, int nsys,
@@ -246,8 +266,8 @@ hipError_t launch_adjoint(int m, int layout, const double* A, const double* Cp, 
 
 hipError_t launch_f32r(int m, int layout, const float* A, const float* B, const float* C,
                        const double* D, double* X, float* Wc, float* Wv, int nsys, int nrow,
-                       int sweeps, int* flag, hipStream_t s) {
-#define CALL(M) launch_f32r_##M(layout, A, B, C, D, X, Wc, Wv, nsys, nrow, sweeps, flag, s)
+                       int sweeps, int* flag, hipStream_t s, bool transposed) {
+#define CALL(M) launch_f32r_##M(layout, A, B, C, D, X, Wc, Wv, nsys, nrow, sweeps, flag, s, transposed)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }
@@ -255,6 +275,13 @@ hipError_t launch_f32r(int m, int layout, const float* A, const float* B, const 
 hipError_t launch_cyclic(int m, int layout, double* A, const double* B, double* C, double* D,
                          int nsys, int nrow, int* flag, hipStream_t s) {
 #define CALL(M) launch_cyclic_##M(layout, A, B, C, D, nsys, nrow, flag, s)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_outer_f32(int m, int layout, const double* Lam, const double* X, float* dA,
+                            float* dB, float* dC, int nsys, int nrow, hipStream_t s) {
+#define CALL(M) launch_outer_f32_##M(layout, Lam, X, dA, dB, dC, nsys, nrow, s)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }

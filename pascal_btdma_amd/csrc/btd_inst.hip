@@ -52,8 +52,13 @@ hipError_t BTD_NAME(launch_adjoint_)(int layout, const double* A, const double* 
 
 hipError_t BTD_NAME(launch_f32r_)(int layout, const float* A, const float* B, const float* C,
                                   const double* D, double* X, float* Wc, float* Wv, int nsys,
-                                  int nrow, int sweeps, int* flag, hipStream_t s) {
-    return launch_f32r_m<BTD_M>(layout, A, B, C, D, X, Wc, Wv, nsys, nrow, sweeps, flag, s);
+                                  int nrow, int sweeps, int* flag, hipStream_t s, bool transposed) {
+    return launch_f32r_m<BTD_M>(layout, A, B, C, D, X, Wc, Wv, nsys, nrow, sweeps, flag, s, transposed);
+}
+
+hipError_t BTD_NAME(launch_outer_f32_)(int layout, const double* Lam, const double* X, float* dA,
+                                       float* dB, float* dC, int nsys, int nrow, hipStream_t s) {
+    return launch_outer_f32_m<BTD_M>(layout, Lam, X, dA, dB, dC, nsys, nrow, s);
 }
 
 }  // namespace btd

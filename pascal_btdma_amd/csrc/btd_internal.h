@@ -24,7 +24,10 @@ hipError_t launch_adjoint(int m, int layout, const double* A, const double* Cp, 
 
 hipError_t launch_f32r(int m, int layout, const float* A, const float* B, const float* C,
                        const double* D, double* X, float* Wc, float* Wv, int nsys, int nrow,
-                       int sweeps, int* flag, hipStream_t s);
+                       int sweeps, int* flag, hipStream_t s, bool transposed);
+
+hipError_t launch_outer_f32(int m, int layout, const double* Lam, const double* X, float* dA,
+                            float* dB, float* dC, int nsys, int nrow, hipStream_t s);
 
 hipError_t launch_two_stage(int m, int layout, double* A, const double* B, double* C, double* D,
                             int nsys, int nrow, const int* part_first, const int* part_len,

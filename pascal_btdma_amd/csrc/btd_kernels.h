@@ -420,7 +420,10 @@ __device__ __forceinline__ void lu_solve_f(const float (&a)[M][M], const int (&p
     }
 }
 
-template <int M, int LAYOUT>
+// TR = true solves the transposed system T^T x = d from the same stored blocks: its row q
+// has sub-diagonal C_{q-1}^T, diagonal B_q^T and super-diagonal A_{q+1}^T (the adjoint of the
+// FP32-stored solve).
+template <int M, int LAYOUT, bool TR>
 __global__ __launch_bounds__(64) void k_f32r(const float* __restrict__ A, const float* __restrict__ B,
                                              const float* __restrict__ C,
                                              const double* __restrict__ D, double* X,
@@ -442,24 +445,26 @@ __global__ __launch_bounds__(64) void k_f32r(const float* __restrict__ A, const 
         }
         for (int64_t q = 0; q < nrow; ++q) {
             float R[M][M], a[M][M], v[M];
+            const bool has_next = q + 1 < nrow;
+            // element (i, j) of this row's sub-diagonal, diagonal and super-diagonal block
             const float* rb = B + am.at(q, 0);
-            const float* ra = A + am.at(q, 0);
+            const float* ra = TR ? C + am.at(q > 0 ? q - 1 : 0, 0) : A + am.at(q, 0);
+            const float* rc = TR ? A + am.at(has_next ? q + 1 : q, 0) : C + am.at(q, 0);
 #pragma unroll
             for (int j = 0; j < M; ++j)
 #pragma unroll
                 for (int i = 0; i < M; ++i) {
-                    R[i][j] = rb[(i + M * j) * es];
-                    a[i][j] = q > 0 ? ra[(i + M * j) * es] : 0.0f;
+                    const int e = TR ? (j + M * i) : (i + M * j);
+                    R[i][j] = rb[e * es];
+                    a[i][j] = q > 0 ? ra[e * es] : 0.0f;
                 }
             double r[M];
             load_vec<M>(D, av, q, r);
             float cq[M][M];
-            const float* rc = C + am.at(q, 0);
-            const bool has_next = q + 1 < nrow;
 #pragma unroll
             for (int j = 0; j < M; ++j)
 #pragma unroll
-                for (int i = 0; i < M; ++i) cq[i][j] = has_next ? rc[(i + M * j) * es] : 0.0f;
+                for (int i = 0; i < M; ++i) cq[i][j] = has_next ? rc[(TR ? (j + M * i) : (i + M * j)) * es] : 0.0f;
             if (sweep > 0) {
                 double xp[M];
                 if (has_next) {
@@ -776,6 +781,49 @@ __global__ __launch_bounds__(64) void k_reduced(const double* __restrict__ redA,
     if (!ok) atomicOr(flag, 1);
 }
 
+// Banded outer-product gradients of the FP32-stored solve, computed in FP64 and stored in
+// FP32: dA_q = -lambda_q x_{q-1}^T, dB_q = -lambda_q x_q^T, dC_q = -lambda_q x_{q+1}^T.
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_outer_f32(const double* __restrict__ Lam,
+                                                  const double* __restrict__ X,
+                                                  float* __restrict__ dA, float* __restrict__ dB,
+                                                  float* __restrict__ dC, int nsys, int nrow) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    const int64_t es = am.estride();
+    double xm[M], xc[M], xp[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) xm[i] = 0.0;
+    load_vec<M>(X, av, 0, xc);
+    for (int64_t q = 0; q < nrow; ++q) {
+        double l[M];
+        load_vec<M>(Lam, av, q, l);
+        if (q + 1 < nrow) {
+            load_vec<M>(X, av, q + 1, xp);
+        } else {
+#pragma unroll
+            for (int i = 0; i < M; ++i) xp[i] = 0.0;
+        }
+        float* ra = dA + am.at(q, 0);
+        float* rb = dB + am.at(q, 0);
+        float* rc = dC + am.at(q, 0);
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                ra[(i + M * j) * es] = (float)(-l[i] * xm[j]);
+                rb[(i + M * j) * es] = (float)(-l[i] * xc[j]);
+                rc[(i + M * j) * es] = (float)(-l[i] * xp[j]);
+            }
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            xm[i] = xc[i];
+            xc[i] = xp[i];
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------- cyclic --
 // Periodic block Thomas, btdma_many_cycl (mod_btdma_cpu.f90:80-183; GPU
 // mod_btdma_gpu_v2.f90:352-616): row 0 couples to x_{n-1} through A_0 and row n-1 to x_0
@@ -1066,14 +1114,35 @@ hipError_t launch_adjoint_m(int layout, const double* A, const double* Cp, const
 template <int M>
 hipError_t launch_f32r_m(int layout, const float* A, const float* B, const float* C,
                          const double* D, double* X, float* Wc, float* Wv, int nsys, int nrow,
-                         int sweeps, int* flag, hipStream_t s) {
+                         int sweeps, int* flag, hipStream_t s, bool transposed) {
+    const dim3 grid((nsys + 63) / 64), block(64);
+    if (layout == LAYOUT_TILED) {
+        if (transposed)
+            hipLaunchKernelGGL((k_f32r<M, LAYOUT_TILED, true>), grid, block, 0, s, A, B, C, D, X, Wc, Wv,
+                               nsys, nrow, sweeps, flag);
+        else
+            hipLaunchKernelGGL((k_f32r<M, LAYOUT_TILED, false>), grid, block, 0, s, A, B, C, D, X, Wc, Wv,
+                               nsys, nrow, sweeps, flag);
+    } else {
+        if (transposed)
+            hipLaunchKernelGGL((k_f32r<M, LAYOUT_REF_GPU, true>), grid, block, 0, s, A, B, C, D, X, Wc,
+                               Wv, nsys, nrow, sweeps, flag);
+        else
+            hipLaunchKernelGGL((k_f32r<M, LAYOUT_REF_GPU, false>), grid, block, 0, s, A, B, C, D, X, Wc,
+                               Wv, nsys, nrow, sweeps, flag);
+    }
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_outer_f32_m(int layout, const double* Lam, const double* X, float* dA, float* dB,
+                              float* dC, int nsys, int nrow, hipStream_t s) {
     const dim3 grid((nsys + 63) / 64), block(64);
     if (layout == LAYOUT_TILED)
-        hipLaunchKernelGGL((k_f32r<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, X, Wc, Wv, nsys,
-                           nrow, sweeps, flag);
+        hipLaunchKernelGGL((k_outer_f32<M, LAYOUT_TILED>), grid, block, 0, s, Lam, X, dA, dB, dC, nsys, nrow);
     else
-        hipLaunchKernelGGL((k_f32r<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, X, Wc, Wv, nsys,
-                           nrow, sweeps, flag);
+        hipLaunchKernelGGL((k_outer_f32<M, LAYOUT_REF_GPU>), grid, block, 0, s, Lam, X, dA, dB, dC, nsys,
+                           nrow);
     return hipGetLastError();
 }
 

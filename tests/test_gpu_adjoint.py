@@ -119,3 +119,98 @@ def test_autograd_function_and_finite_differences(cuda, oracle):
     print(f"finite-difference max rel err over 64 entries: {worst:.3e}")
     assert worst <= 1e-6
     plan.destroy()
+
+
+@pytest.mark.parametrize("m", [2, 3, 5])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_f32r_transposed_solve_and_fp32_gradients(cuda, oracle, m, layout):
+    """FP32-stored blocks: lambda = T^{-T} w by the transposed FP32 solve plus refinement
+    sweeps against the oracle's FP64 transposed solve of the same FP32-rounded blocks
+    (<= 1e-12 after 2 sweeps on these well-conditioned rows), and the FP32 outer-product
+    gradients against -lambda x^T rounded to FP32 (one FP32 rounding: <= 2^-24 relative)."""
+    import torch
+    nsys, n = 100, 64
+    A, B, C, d = pb.generate_host(pb.GenKind.EULER, 80 + m, m, nsys, n)
+    A, B, C = (X.astype(np.float32).astype(np.float64) for X in (A, B, C))
+    w = np.random.default_rng(10 + m).uniform(-1, 1, d.shape)
+    lay = int(layout)
+    f32 = lambda X: torch.from_numpy(nat_to_layout(X, lay).astype(np.float32)).to(cuda)
+    f64 = lambda X: torch.from_numpy(nat_to_layout(X, lay)).to(cuda)
+    ta, tb, tc, tw, td = f32(A), f32(B), f32(C), f64(w), f64(d)
+    x, lo, dAo, dBo, dCo = oracle_grads(oracle, A, B, C, d, w)
+    errs = []
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
+        work = torch.empty(pb.f32r_work_elems(plan), dtype=torch.float32, device=cuda)
+        for sweeps in range(4):
+            lam = torch.zeros_like(tw)
+            pb.solve_transposed_f32r(plan, ta, tb, tc, tw, lam, work, sweeps)
+            torch.cuda.synchronize()
+            assert not plan.check()
+            errs.append(rel_err(layout_to_vec(lam.cpu().numpy(), lay, n, nsys, m), lo).max())
+        tx = f64(x)
+        da, db, dc = (torch.empty_like(ta) for _ in range(3))
+        pb.outer_grads_f32(plan, lam, tx, da, db, dc)
+        torch.cuda.synchronize()
+    print(f"m={m} {layout.name} transposed f32r err by sweeps: {['%.2e' % e for e in errs]}")
+    assert 1e-9 < errs[0] < 1e-4
+    assert errs[2] <= 1e-12 and errs[3] <= 1e-12
+    for got, ref in ((da, dAo), (db, dBo), (dc, dCo)):
+        g = layout_to_mat(got.cpu().numpy().astype(np.float64), lay, n, nsys, m)
+        np.testing.assert_allclose(g, ref, rtol=2.0 ** -23, atol=1e-30)
+    assert np.all(layout_to_mat(da.cpu().numpy(), lay, n, nsys, m)[0] == 0)
+    assert np.all(layout_to_mat(dc.cpu().numpy(), lay, n, nsys, m)[-1] == 0)
+
+
+def test_f32r_autograd_function_and_finite_differences(cuda, oracle):
+    """block_tridiagonal_solve_f32r: d.grad against the oracle's transposed solve of the
+    FP32-rounded blocks (<= 1e-12 with 3 sweeps), block gradients against -lambda x^T in
+    FP32, and central differences (h = 2^-10 on the blocks,
+    1e-6 on d) of the FP64 oracle solve on 32 random entries (tolerance 1e-5: the
+    difference quotient's own truncation and rounding)."""
+    import torch
+    from pascal_btdma_amd.autograd import block_tridiagonal_solve_f32r
+    m, nsys, n = 3, 64, 256
+    lay = int(pb.Layout.TILED)
+    A, B, C, d = pb.generate_host(pb.GenKind.EULER, 45, m, nsys, n)
+    A, B, C = (X.astype(np.float32).astype(np.float64) for X in (A, B, C))
+    w = np.random.default_rng(4).uniform(-1, 1, d.shape)
+    f32 = lambda X: torch.from_numpy(nat_to_layout(X, lay).astype(np.float32)).to(cuda).requires_grad_()
+    ta, tb, tc = f32(A), f32(B), f32(C)
+    td = torch.from_numpy(nat_to_layout(d, lay)).to(cuda).requires_grad_()
+    tw = torch.from_numpy(nat_to_layout(w, lay)).to(cuda)
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, pb.Layout.TILED) as plan:
+        x = block_tridiagonal_solve_f32r(plan, ta, tb, tc, td, 3)
+        (x * tw).sum().backward()
+        torch.cuda.synchronize()
+    x_, lo, dAo, dBo, dCo = oracle_grads(oracle, A, B, C, d, w)
+    assert rel_err(layout_to_vec(x.detach().cpu().numpy(), lay, n, nsys, m), x_).max() <= 1e-12
+    assert rel_err(layout_to_vec(td.grad.cpu().numpy(), lay, n, nsys, m), lo).max() <= 1e-12
+    assert ta.grad.dtype == torch.float32 and tb.grad.dtype == torch.float32
+    grads = {"A": layout_to_mat(ta.grad.cpu().numpy().astype(np.float64), lay, n, nsys, m),
+             "B": layout_to_mat(tb.grad.cpu().numpy().astype(np.float64), lay, n, nsys, m),
+             "C": layout_to_mat(tc.grad.cpu().numpy().astype(np.float64), lay, n, nsys, m),
+             "d": layout_to_vec(td.grad.cpu().numpy(), lay, n, nsys, m)}
+    for name, ref in (("A", dAo), ("B", dBo), ("C", dCo)):
+        np.testing.assert_allclose(grads[name], ref, rtol=2.0 ** -23, atol=1e-30)
+    rng = np.random.default_rng(8)
+    base = {"A": A, "B": B, "C": C, "d": d}
+    worst = 0.0
+    for k in range(32):
+        name = ["A", "B", "C", "d"][k % 4]
+        s = int(rng.integers(nsys))
+        q = int(rng.integers(1 if name == "A" else 0, n - 1 if name == "C" else n))
+        idx = (q, s, int(rng.integers(m))) + (() if name == "d" else (int(rng.integers(m)),))
+        h = 1e-6 if name == "d" else 2.0 ** -10
+        vals = []
+        for sign in (1, -1):
+            arrs = {kk: v[:, s:s + 1].copy() for kk, v in base.items()}
+            arrs[name][(idx[0], 0) + idx[2:]] += sign * h
+            xs, info = oracle.solve_plain(cpu_layout(arrs["A"]), cpu_layout(arrs["B"]),
+                                          cpu_layout(arrs["C"]), arrs["d"])
+            assert info == 0
+            vals.append(float((xs[:, 0] * w[:, s]).sum()))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        g = grads[name][idx]
+        worst = max(worst, abs(fd - g) / max(1.0, abs(g)))
+    print(f"f32r finite-difference max rel err over 32 entries: {worst:.3e}")
+    assert worst <= 1e-5
