@@ -105,10 +105,25 @@ def cpu_baseline(w: dict, target_s: float = 12.0) -> dict:
         total_t += run(arrs)
         total_rows += S * n
         reps += 1
-    return {"value": total_rows / total_t, "unit": "block-rows/s", "cores": pyoracle.last_threads(),
+    cores = pyoracle.last_threads()
+    # one thread, on a slice sized for about a quarter of target_s
+    S1 = max(64, min(S, int(S / cores * 0.25 * target_s / max(total_t / reps, 1e-3))))
+    pyoracle.set_threads(1)
+    t1 = run([x[:, :S1].copy() for x in arrs])
+    pyoracle.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": total_rows / total_t, "unit": "block-rows/s", "cores": cores,
             "kind": "port",
             "sample": f"first {S} systems of the same workload ({S * n} block rows), solved {reps}x "
-                      f"in {total_t:.1f} s with the C oracle, OpenMP over systems"}
+                      f"in {total_t:.1f} s with the C oracle, OpenMP over systems",
+            "single_thread_value": S1 * n / t1,
+            "single_thread_sample": f"first {S1} systems, one thread, {t1:.1f} s",
+            "cpu_model": model, "nproc": os.cpu_count()}
 
 
 # ------------------------------------------------------------------------------ extras --

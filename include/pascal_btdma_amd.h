@@ -90,7 +90,9 @@ btd_status btd_solve_noncyclic(btd_plan_t plan, double *a, const double *b, doub
 
 /* Replaces solve_noncyclic_profiled (pascal_btdma_cuda.hpp:147-156, pascal_btdma_cuda.cu:1047-1061)
  * and btdma_many_mpi_gpu_v2_profiled (mod_btdma_gpu_v2.f90:169-198). Synchronises the
- * stream, fills *timings (may be NULL) and returns BTD_ERR_SINGULAR on a zero pivot. */
+ * stream, fills *timings (may be NULL) and returns BTD_ERR_SINGULAR if this solve met a
+ * zero pivot (earlier unchecked zero pivots of the plan are dropped first); the record then
+ * stays for btd_plan_info. */
 btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double *a, const double *b, double *c,
                                         double *d, int m, int nsys, int nrow, btd_timings *timings,
                                         void *stream);
@@ -148,8 +150,14 @@ btd_status btd_outer_grads_f32(btd_plan_t plan, const double *lambda, const doub
 
 /* New (the reference ignores LAPACK/elimination info): synchronises the plan's last
  * stream use, stores 1 in *singular if any block LU met a zero pivot since the last
- * check (0 otherwise), and clears the flag. */
+ * check (0 otherwise), and clears the flag and the per-system record. */
 btd_status btd_plan_check(btd_plan_t plan, int *singular);
+
+/* New, the per-system form of btd_plan_check (the reference's dgesv `info`, which it
+ * ignores, mod_btdma_cpu.f90:38): synchronises like btd_plan_check and fills the host array
+ * info[nsys] with 1 for every system whose solve met a zero pivot since the last check
+ * (system index in the caller's numbering), 0 for the others; then clears the record. */
+btd_status btd_plan_info(btd_plan_t plan, int *info);
 
 /* Thread-local message for the last error on this thread ("" if none). */
 const char *btd_last_error(void);

@@ -162,6 +162,14 @@ class BtdmaGpuPlan:
         _check(_capi.lib().btd_plan_check(self._h, ctypes.byref(s)))
         return bool(s.value)
 
+    def info(self):
+        """Synchronise; int32 numpy array of nsys entries, 1 for each system whose solve met
+        a zero pivot since the last check, else 0 (clears the record like check())."""
+        import numpy as np
+        out = np.zeros(self.nsys, dtype=np.int32)
+        _check(_capi.lib().btd_plan_info(self._h, out.ctypes.data))
+        return out
+
     def matrix_elems(self) -> int:
         return layout_elems(self.layout, self.nsys, self.nrow, self.m * self.m)
 

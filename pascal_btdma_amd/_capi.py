@@ -39,6 +39,7 @@ SIGNATURES = {
     "btd_solve_transposed_f32r": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp]),
     "btd_outer_grads_f32": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "btd_plan_check": (_i, [_vp, ctypes.POINTER(_i)]),
+    "btd_plan_info": (_i, [_vp, _vp]),
     "btd_last_error": (ctypes.c_char_p, []),
     "btd_matrix_index": (ctypes.c_size_t, [_i, _i, _i, _i, _i, _i, _i]),
     "btd_vector_index": (ctypes.c_size_t, [_i, _i, _i, _i, _i]),

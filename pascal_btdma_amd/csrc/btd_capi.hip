@@ -45,7 +45,7 @@ struct btd_plan_s {
     double* redA = nullptr;       // reduced system (tiled, 2*npart rows); B = I is implicit
     double* redC = nullptr;
     double* redD = nullptr;
-    int* d_flag = nullptr;  // zero-pivot flag
+    int* d_flag = nullptr;  // zero-pivot flag (word 0), then one word per system
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     hipStream_t last_stream = nullptr;
 };
@@ -111,8 +111,8 @@ btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npar
     p->npart = npart;
     p->layout = layout;
     hipError_t e = hipGetDevice(&p->device);
-    if (e == hipSuccess) e = hipMalloc(&p->d_flag, sizeof(int));
-    if (e == hipSuccess) e = hipMemset(p->d_flag, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&p->d_flag, sizeof(int) * (1 + (size_t)nsys));
+    if (e == hipSuccess) e = hipMemset(p->d_flag, 0, sizeof(int) * (1 + (size_t)nsys));
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[i]);
     if (e == hipSuccess && npart > 1) {
         p->first.resize(npart);
@@ -194,15 +194,35 @@ static double ev_seconds(hipEvent_t a, hipEvent_t b) {
     return ms * 1e-3;
 }
 
-btd_status btd_plan_check(btd_plan_t plan, int* singular) {
-    g_err.clear();
-    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_plan_check: uncreated plan");
+// Reads the zero-pivot flag after the plan's last stream has drained; when it is set, copies
+// the per-system words to `info` (if given) and clears flag and words.
+static btd_status read_status(btd_plan_t plan, int* singular, int* info) {
     BTD_HIP(hipStreamSynchronize(plan->last_stream));
     int h = 0;
     BTD_HIP(hipMemcpy(&h, plan->d_flag, sizeof(int), hipMemcpyDeviceToHost));
-    BTD_HIP(hipMemset(plan->d_flag, 0, sizeof(int)));
+    if (h) {
+        if (info)
+            BTD_HIP(hipMemcpy(info, plan->d_flag + 1, sizeof(int) * (size_t)plan->nsys,
+                              hipMemcpyDeviceToHost));
+        BTD_HIP(hipMemset(plan->d_flag, 0, sizeof(int) * (1 + (size_t)plan->nsys)));
+    } else if (info) {
+        std::memset(info, 0, sizeof(int) * (size_t)plan->nsys);
+    }
     if (singular) *singular = h ? 1 : 0;
     return BTD_OK;
+}
+
+btd_status btd_plan_check(btd_plan_t plan, int* singular) {
+    g_err.clear();
+    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_plan_check: uncreated plan");
+    return read_status(plan, singular, nullptr);
+}
+
+btd_status btd_plan_info(btd_plan_t plan, int* info) {
+    g_err.clear();
+    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_plan_info: uncreated plan");
+    if (!info) return fail(BTD_ERR_INVALID_ARG, "btd_plan_info: NULL info");
+    return read_status(plan, nullptr, info);
 }
 
 btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double* a, const double* b, double* c,
@@ -212,6 +232,8 @@ btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double* a, const double
     btd_status st = check_solve(plan, a, b, c, d, m, nsys, nrow);
     if (st != BTD_OK) return st;
     hipStream_t s = (hipStream_t)stream;
+    // this call reports its own solve only: earlier unchecked zero pivots are dropped
+    BTD_HIP(hipMemsetAsync(plan->d_flag, 0, sizeof(int) * (1 + (size_t)plan->nsys), s));
     st = solve_impl(plan, a, b, c, d, s, true);
     if (st != BTD_OK) return st;
     const int last = plan->npart == 1 ? 1 : 3;
@@ -228,9 +250,9 @@ btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double* a, const double
         t->total = ev_seconds(plan->ev[0], plan->ev[last]);
     }
     int sing = 0;
-    st = btd_plan_check(plan, &sing);
-    if (st != BTD_OK) return st;
-    if (sing) return fail(BTD_ERR_SINGULAR, "a block LU met a zero pivot");
+    BTD_HIP(hipMemcpy(&sing, plan->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (sing)  // the flag and per-system words stay set for btd_plan_info / btd_plan_check
+        return fail(BTD_ERR_SINGULAR, "a block LU met a zero pivot (btd_plan_info names the systems)");
     return BTD_OK;
 }
 

@@ -192,6 +192,13 @@ __device__ __forceinline__ void unpack_piv(uint64_t w, int (&piv)[M]) {
     for (int k = 0; k < M; ++k) piv[k] = (int)((w >> (4 * k)) & 15u);
 }
 
+// A zero pivot in system `sys`: raise the plan's flag (word 0) and the system's own word
+// (1 + sys). Words only go from 0 to 1, so concurrent writers of one system agree.
+__device__ __forceinline__ void report_singular(int* flag, int64_t sys) {
+    atomicOr(flag, 1);
+    flag[1 + sys] = 1;
+}
+
 template <int M, int LAYOUT, bool SAVE>
 __device__ __forceinline__ void plain_solve(const double* __restrict__ A,
                                             const double* __restrict__ B, double* C, double* D,
@@ -246,7 +253,7 @@ __device__ __forceinline__ void plain_solve(const double* __restrict__ A,
         for (int i = 0; i < M; ++i) dp[i] = x[i];
         store_vec<M>(D, av, q, x);
     }
-    if (!ok) atomicOr(flag, 1);
+    if (!ok) report_singular(flag, sys);
 }
 
 template <int M, int LAYOUT>
@@ -558,7 +565,7 @@ __global__ __launch_bounds__(64) void k_f32r(const float* __restrict__ A, const 
             store_vec<M>(X, av, q, x);
         }
     }
-    if (!ok) atomicOr(flag, 1);
+    if (!ok) report_singular(flag, sys);
 }
 
 // ------------------------------------------------------------------------------ modi --
@@ -725,7 +732,7 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
         store_mat<M>(redC, rm, 2 * part, X);
         store_vec<M>(redD, rv, 2 * part, d0);
     }
-    if (!ok) atomicOr(flag, 1);
+    if (!ok) report_singular(flag, sys);
 }
 
 // ---------------------------------------------------------------------------- reduced --
@@ -778,7 +785,7 @@ __global__ __launch_bounds__(64) void k_reduced(const double* __restrict__ redA,
         for (int i = 0; i < M; ++i) dp[i] = x[i];
         store_vec<M>(redD, av, q, x);
     }
-    if (!ok) atomicOr(flag, 1);
+    if (!ok) report_singular(flag, sys);
 }
 
 // Banded outer-product gradients of the FP32-stored solve, computed in FP64 and stored in
@@ -991,7 +998,7 @@ __global__ __launch_bounds__(64) void k_cyclic(double* A, const double* __restri
     const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (sys >= nsys) return;
     const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
-    if (!cyclic_solve<M, LAYOUT, false>(A, B, C, D, am, av, nrow)) atomicOr(flag, 1);
+    if (!cyclic_solve<M, LAYOUT, false>(A, B, C, D, am, av, nrow)) report_singular(flag, sys);
 }
 
 // Periodic reduced system (2P rows, B = I) of the two-stage cyclic solve
@@ -1002,7 +1009,8 @@ __global__ __launch_bounds__(64) void k_reduced_cyclic(double* redA, double* red
     const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (sys >= nsys) return;
     const Addr<LAYOUT_TILED> am(sys, 0, nrr, M * M), av(sys, 0, nrr, M);
-    if (!cyclic_solve<M, LAYOUT_TILED, true>(redA, nullptr, redC, redD, am, av, nrr)) atomicOr(flag, 1);
+    if (!cyclic_solve<M, LAYOUT_TILED, true>(redA, nullptr, redC, redD, am, av, nrr))
+        report_singular(flag, sys);
 }
 
 // ----------------------------------------------------------------------------- update --

@@ -150,12 +150,32 @@ def test_singular_block_is_reported(cuda):
         with pytest.raises(pb.SingularError):
             pb.solve_noncyclic_profiled(plan, ta, tb, tc, td, m, nsys, n)
         ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d))
+        assert plan.info().nonzero()[0].tolist() == [17]  # the failing profiled solve's record
         pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
         assert plan.check()
         assert not plan.check()  # the check clears the flag
+        assert not plan.info().any()
     # the other systems are unaffected
     x = layout_to_vec(td.cpu().numpy(), lay, n, nsys, m)
     assert np.isfinite(x[:, :17]).all() and np.isfinite(x[:, 18:]).all()
+
+
+@pytest.mark.parametrize("npart,cyclic", [(1, False), (3, False), (1, True), (3, True)])
+def test_per_system_info(cuda, npart, cyclic):
+    """A zero block row in two systems (one of them in a later partition) is named per
+    system by btd_plan_info, for the plain, two-stage and periodic solves."""
+    import torch
+    m, nsys, n = 3, 130, 12
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 6, m, nsys, n)
+    for s_, q in ((5, 1), (129, 10)):
+        A[q, s_] = B[q, s_] = C[q, s_] = 0.0
+    lay = int(pb.Layout.TILED)
+    ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d))
+    with pb.BtdmaGpuPlan().create(m, nsys, n, npart, pb.Layout.TILED) as plan:
+        (pb.solve_cyclic if cyclic else pb.solve_noncyclic)(plan, ta, tb, tc, td, m, nsys, n)
+        info = plan.info()
+        assert info.nonzero()[0].tolist() == [5, 129]
+        assert not plan.check()
 
 
 def test_api_errors_on_device(cuda):
