@@ -148,6 +148,35 @@ btd_status btd_solve_transposed_f32r(btd_plan_t plan, const float *a, const floa
 btd_status btd_outer_grads_f32(btd_plan_t plan, const double *lambda, const double *x, float *da,
                                float *db, float *dc, void *stream);
 
+/* ---- euler3d ADI sweeps: assembly straight into the solve's layout ----------------------
+ * Fields are Fortran-ordered device arrays: U(0:n1+1, 0:n2+1, 0:n3+1, 5), the conserved
+ * variables with their one-cell halo already filled; fields without halo are
+ * F(n1, n2, n3, m). The sweep along `axis` (0 = x, 1 = y, 2 = z) is a batch of nsys systems
+ * of nrow block rows numbered as the reference numbers them after its transposes:
+ *   x: nrow = n1, nsys = n2*n3, sys = j + n2*k;   y: nrow = n2, nsys = n1*n3, sys = i + n1*k;
+ *   z: nrow = n3, nsys = n1*n2, sys = i + n1*j     (0-based interior i, j, k).
+ * The plan must have that nrow and nsys (and m = 5 for the assembly). */
+
+/* Replaces build_matrix_{x,y,z}_kernel (euler3d_mpi_gpu.f90:803-929) followed by
+ * transpose_{jk,ik}_kernel / transpose_{jk,ik}_5d_kernel (:931-957, :976-1002): writes the
+ * sweep's blocks a, b, c (A = -cc J(U_prev) - cd I, B = (1 + 2cd) I, C = cc J(U_next) - cd I,
+ * cc = dt/(4h), cd = eps*dt/(2h^2), J the flux Jacobian along the axis, compute_Amat/Bmat/Cmat
+ * :749-801) and, if rhs is given, d = rhs reordered, in the plan's layout. rhs and d are
+ * both NULL or both given. Asynchronous. */
+btd_status btd_euler_assemble(btd_plan_t plan, int axis, const double *U, const double *rhs, int n1,
+                              int n2, int n3, double h, double dt, double eps, double *a, double *b,
+                              double *c, double *d, void *stream);
+
+/* Field F(n1, n2, n3, m) -> the sweep's vector d in the plan's layout (transpose_jk_kernel /
+ * transpose_ik_kernel, :931-943, :976-988, for callers that assemble their own blocks). */
+btd_status btd_sweep_gather(btd_plan_t plan, int axis, const double *field, int n1, int n2, int n3,
+                            double *d, void *stream);
+
+/* The sweep's vector d -> field F(n1, n2, n3, m) (transpose_kj_kernel :959-974,
+ * transpose_ki_kernel :1004-1017). */
+btd_status btd_sweep_scatter(btd_plan_t plan, int axis, const double *d, int n1, int n2, int n3,
+                             double *field, void *stream);
+
 /* New (the reference ignores LAPACK/elimination info): synchronises the plan's last
  * stream use, stores 1 in *singular if any block LU met a zero pivot since the last
  * check (0 otherwise), and clears the flag and the per-system record. */

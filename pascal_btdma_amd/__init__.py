@@ -23,7 +23,7 @@ __all__ = [
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
     "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
     "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "solve_transposed_f32r",
-    "outer_grads_f32", "BtdmaBatch", "Graph",
+    "outer_grads_f32", "euler_assemble", "sweep_gather", "sweep_scatter", "sweep_shape", "BtdmaBatch", "Graph",
 ]
 
 kMaxBlockSize = 10
@@ -310,6 +310,60 @@ def outer_grads_f32(plan: BtdmaGpuPlan, lam, x, da, db, dc, stream=None) -> None
             _check_tensor(name, t, n, dt)
     _check(_capi.lib().btd_outer_grads_f32(plan._h, _ptr(lam), _ptr(x), _ptr(da), _ptr(db),
                                            _ptr(dc), _stream(stream)))
+
+
+def sweep_shape(axis: int, n1: int, n2: int, n3: int):
+    """(nrow, nsys) of the sweep along axis 0 (x), 1 (y) or 2 (z) of an n1 x n2 x n3 grid."""
+    return ((n1, n2 * n3), (n2, n1 * n3), (n3, n1 * n2))[int(axis)]
+
+
+def _check_fields(n1, n2, n3, m, **fields):
+    import torch
+    sizes = {"U": 5 * (n1 + 2) * (n2 + 2) * (n3 + 2)}
+    for name, t in fields.items():
+        if t is None:
+            continue
+        _check_tensor(name, t, sizes.get(name, m * n1 * n2 * n3), torch.float64)
+
+
+def euler_assemble(plan: BtdmaGpuPlan, axis: int, U, rhs, n1: int, n2: int, n3: int, h: float,
+                   dt: float, eps: float, a, b, c, d=None, stream=None) -> None:
+    """Blocks (and, with rhs, the right-hand side) of the euler3d ADI sweep along `axis`
+    from the haloed conserved field U(0:n1+1, 0:n2+1, 0:n3+1, 5) (Fortran order, flat float64
+    device tensor), written in the plan's layout; replaces the reference's build_matrix
+    kernel plus its transposes."""
+    import torch
+    if plan.created:
+        nm, nv = plan.matrix_elems(), plan.vector_elems()
+        _check_fields(n1, n2, n3, 5, U=U, rhs=rhs)
+        for name, t, n in (("a", a, nm), ("b", b, nm), ("c", c, nm), ("d", d, nv)):
+            if t is not None:
+                _check_tensor(name, t, n, torch.float64)
+    _check(_capi.lib().btd_euler_assemble(plan._h, int(axis), _ptr(U), _ptr(rhs), int(n1), int(n2),
+                                          int(n3), float(h), float(dt), float(eps), _ptr(a), _ptr(b),
+                                          _ptr(c), _ptr(d), _stream(stream)))
+
+
+def sweep_gather(plan: BtdmaGpuPlan, axis: int, field, n1: int, n2: int, n3: int, d,
+                 stream=None) -> None:
+    """Field F(n1, n2, n3, m) -> the sweep's vector d in the plan's layout."""
+    import torch
+    if plan.created:
+        _check_fields(n1, n2, n3, plan.m, field=field)
+        _check_tensor("d", d, plan.vector_elems(), torch.float64)
+    _check(_capi.lib().btd_sweep_gather(plan._h, int(axis), _ptr(field), int(n1), int(n2), int(n3),
+                                        _ptr(d), _stream(stream)))
+
+
+def sweep_scatter(plan: BtdmaGpuPlan, axis: int, d, n1: int, n2: int, n3: int, field,
+                  stream=None) -> None:
+    """The sweep's vector d -> field F(n1, n2, n3, m)."""
+    import torch
+    if plan.created:
+        _check_fields(n1, n2, n3, plan.m, field=field)
+        _check_tensor("d", d, plan.vector_elems(), torch.float64)
+    _check(_capi.lib().btd_sweep_scatter(plan._h, int(axis), _ptr(d), int(n1), int(n2), int(n3),
+                                         _ptr(field), _stream(stream)))
 
 
 class BtdmaBatch:

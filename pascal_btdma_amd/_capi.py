@@ -18,6 +18,7 @@ _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _u64 = ctypes.c_uint64
 _dp = ctypes.c_void_p  # device or host double* passed as an address
+_d = ctypes.c_double
 
 
 class BtdTimings(ctypes.Structure):
@@ -38,6 +39,9 @@ SIGNATURES = {
     "btd_solve_noncyclic_f32r": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp]),
     "btd_solve_transposed_f32r": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp]),
     "btd_outer_grads_f32": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "btd_euler_assemble": (_i, [_vp, _i, _dp, _dp, _i, _i, _i, _d, _d, _d, _dp, _dp, _dp, _dp, _vp]),
+    "btd_sweep_gather": (_i, [_vp, _i, _dp, _i, _i, _i, _dp, _vp]),
+    "btd_sweep_scatter": (_i, [_vp, _i, _dp, _i, _i, _i, _dp, _vp]),
     "btd_plan_check": (_i, [_vp, ctypes.POINTER(_i)]),
     "btd_plan_info": (_i, [_vp, _vp]),
     "btd_last_error": (ctypes.c_char_p, []),

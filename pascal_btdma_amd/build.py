@@ -35,6 +35,7 @@ UNITS = [("btd_inst.hip", [f"-DBTD_M={m}"], f"btd_inst_m{m}.o") for m in range(1
     ("btd_capi.hip", [], "btd_capi.o"),
     ("btd_batch.hip", [], "btd_batch.o"),
     ("btd_batch_api.hip", [], "btd_batch_api.o"),
+    ("btd_euler.hip", [], "btd_euler.o"),
 ]
 
 
@@ -53,7 +54,9 @@ def _compile(unit, extra: list[str], verbose: bool) -> tuple[str, float, str]:
 def build(verbose: bool = False, resource_usage: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
-    units = [UNITS[-2]] + list(reversed(UNITS[:10])) + UNITS[10:-2] + UNITS[-1:]  # slowest first
+    # slowest first: the batch kernels, then the per-m solve kernels
+    units = [u for u in UNITS if u[0] == "btd_batch.hip"] + list(reversed(UNITS[:10])) + \
+        [u for u in UNITS[10:] if u[0] != "btd_batch.hip"]
     with cf.ThreadPoolExecutor(max_workers=int(os.environ.get("BTD_BUILD_JOBS", "8"))) as ex:
         results = list(ex.map(lambda u: _compile(u, extra, verbose or resource_usage), units))
     objs = [r[0] for r in results]

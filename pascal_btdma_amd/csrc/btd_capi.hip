@@ -212,6 +212,63 @@ static btd_status read_status(btd_plan_t plan, int* singular, int* info) {
     return BTD_OK;
 }
 
+// ------------------------------------------------------------------ euler3d sweeps --
+static btd_status check_sweep(btd_plan_t plan, int axis, int n1, int n2, int n3, const char* what) {
+    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, std::string(what) + ": uncreated plan");
+    if (axis < 0 || axis > 2) return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": axis must be 0, 1 or 2");
+    if (n1 < 1 || n2 < 1 || n3 < 1) return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": grid sizes must be >= 1");
+    const int64_t nrow = axis == 0 ? n1 : (axis == 1 ? n2 : n3);
+    const int64_t nsys = axis == 0 ? (int64_t)n2 * n3 : (axis == 1 ? (int64_t)n1 * n3 : (int64_t)n1 * n2);
+    if (nrow != plan->nrow || nsys != plan->nsys)
+        return fail(BTD_ERR_DIM_MISMATCH,
+                    std::string(what) + ": the plan's nrow/nsys do not match the sweep's grid line "
+                                        "length/count");
+    return BTD_OK;
+}
+
+btd_status btd_euler_assemble(btd_plan_t plan, int axis, const double* U, const double* rhs, int n1,
+                              int n2, int n3, double h, double dt, double eps, double* a, double* b,
+                              double* c, double* d, void* stream) {
+    g_err.clear();
+    btd_status st = check_sweep(plan, axis, n1, n2, n3, "btd_euler_assemble");
+    if (st != BTD_OK) return st;
+    if (plan->m != 5) return fail(BTD_ERR_INVALID_ARG, "btd_euler_assemble: the Euler system has m = 5");
+    if (!U || !a || !b || !c) return fail(BTD_ERR_INVALID_ARG, "btd_euler_assemble: NULL array");
+    if ((rhs == nullptr) != (d == nullptr))
+        return fail(BTD_ERR_INVALID_ARG, "btd_euler_assemble: rhs and d must both be given or both be NULL");
+    if (!(h > 0.0)) return fail(BTD_ERR_INVALID_ARG, "btd_euler_assemble: h must be > 0");
+    const double cc = dt / (4.0 * h);
+    const double cd = eps * dt / (2.0 * (h * h));
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_euler_assemble(plan->layout, axis, U, rhs, n1, n2, n3, cc, cd, a, b, c, d,
+                                       plan->nsys, plan->nrow, (hipStream_t)stream));
+    return BTD_OK;
+}
+
+static btd_status sweep_move(btd_plan_t plan, int axis, double* field, double* d, int n1, int n2,
+                             int n3, bool scatter, void* stream, const char* what) {
+    g_err.clear();
+    btd_status st = check_sweep(plan, axis, n1, n2, n3, what);
+    if (st != BTD_OK) return st;
+    if (!field || !d) return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": NULL array");
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_sweep_move(plan->layout, axis, field, d, n1, n2, n3, plan->m, plan->nsys,
+                                   plan->nrow, scatter, (hipStream_t)stream));
+    return BTD_OK;
+}
+
+btd_status btd_sweep_gather(btd_plan_t plan, int axis, const double* field, int n1, int n2, int n3,
+                            double* d, void* stream) {
+    return sweep_move(plan, axis, const_cast<double*>(field), d, n1, n2, n3, false, stream,
+                      "btd_sweep_gather");
+}
+
+btd_status btd_sweep_scatter(btd_plan_t plan, int axis, const double* d, int n1, int n2, int n3,
+                             double* field, void* stream) {
+    return sweep_move(plan, axis, field, const_cast<double*>(d), n1, n2, n3, true, stream,
+                      "btd_sweep_scatter");
+}
+
 btd_status btd_plan_check(btd_plan_t plan, int* singular) {
     g_err.clear();
     if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_plan_check: uncreated plan");

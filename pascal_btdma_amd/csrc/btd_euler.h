@@ -1,0 +1,192 @@
+// Sweep assembly for the euler3d ADI step (Fortran_Original/examples/euler3d/
+// euler3d_mpi_gpu.f90): the block system of one sweep is built from the conserved field
+// straight into the solve's layout, with the system numbering the reference produces by
+// transposing (x sweep: sys = j + n2 k, rows i; y sweep: sys = i + n1 k, rows j; z sweep:
+// sys = i + n1 j, rows k). This replaces build_matrix_{x,y,z}_kernel (:803-929) plus the
+// transposes of dU, trA, trB, trC before the solve (:931-1002) and of dU after it
+// (:959-974, :1004-1017).
+//
+// Fields are Fortran-ordered: U(0:n1+1, 0:n2+1, 0:n3+1, 5) with a one-cell halo,
+// F(n1, n2, n3, m) without. One lane owns one system and walks a chunk of its rows, keeping
+// the primitive variables of the previous, current and next cell in registers, so each row
+// reads one new cell of U.
+#pragma once
+#include "btd_kernels.h"
+
+namespace btd {
+
+constexpr double kGamma = 1.4;  // euler3d_mpi_gpu.f90:15
+
+struct Prim {
+    double rho, u, v, w, E;
+};
+
+// The strides of the three grid axes (and of the variable index) in the haloed field.
+struct Grid {
+    int64_t n1, n2, n3;
+    __device__ __forceinline__ int64_t sj() const { return n1 + 2; }
+    __device__ __forceinline__ int64_t sk() const { return (n1 + 2) * (n2 + 2); }
+    __device__ __forceinline__ int64_t sv() const { return (n1 + 2) * (n2 + 2) * (n3 + 2); }
+};
+
+// (i, j, k) (0-based interior indices) of system `sys`, row `row` of the sweep along AXIS.
+template <int AXIS>
+__device__ __forceinline__ void sweep_cell(const Grid& g, int64_t sys, int64_t row, int64_t& i,
+                                           int64_t& j, int64_t& k) {
+    if constexpr (AXIS == 0) {
+        i = row;
+        j = sys % g.n2;
+        k = sys / g.n2;
+    } else if constexpr (AXIS == 1) {
+        j = row;
+        i = sys % g.n1;
+        k = sys / g.n1;
+    } else {
+        k = row;
+        i = sys % g.n1;
+        j = sys / g.n1;
+    }
+}
+
+__device__ __forceinline__ Prim load_prim(const double* __restrict__ U, int64_t off, int64_t sv) {
+#pragma clang fp contract(off)
+    Prim p;
+    p.rho = U[off];
+    p.u = U[off + sv] / p.rho;
+    p.v = U[off + 2 * sv] / p.rho;
+    p.w = U[off + 3 * sv] / p.rho;
+    p.E = U[off + 4 * sv] / p.rho;
+    return p;
+}
+
+// Flux Jacobian along AXIS at one cell: compute_Amat (:749-765), compute_Bmat (:767-783),
+// compute_Cmat (:785-801), J[iv][jv], the reference's operation order, no FMA contraction.
+template <int AXIS>
+__device__ __forceinline__ void euler_jacobian(const Prim& p, double (&J)[5][5]) {
+#pragma clang fp contract(off)
+    const double uu = p.u, vv = p.v, ww = p.w, E = p.E;
+    const double q2 = uu * uu + vv * vv + ww * ww;
+    const double H = E + (kGamma - 1.0) * (E - 0.5 * q2);
+    const double c1 = kGamma - 1.0, c2 = 0.5 * c1;
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) J[i][j] = 0.0;
+    if constexpr (AXIS == 0) {
+        J[0][1] = 1.0;
+        J[1][0] = c2 * q2 - uu * uu; J[1][1] = (3.0 - kGamma) * uu;
+        J[1][2] = -c1 * vv; J[1][3] = -c1 * ww; J[1][4] = c1;
+        J[2][0] = -uu * vv; J[2][1] = vv; J[2][2] = uu;
+        J[3][0] = -uu * ww; J[3][1] = ww; J[3][3] = uu;
+        J[4][0] = (c2 * q2 - H) * uu; J[4][1] = H - c1 * (uu * uu);
+        J[4][2] = -c1 * uu * vv; J[4][3] = -c1 * uu * ww; J[4][4] = kGamma * uu;
+    } else if constexpr (AXIS == 1) {
+        J[0][2] = 1.0;
+        J[1][0] = -uu * vv; J[1][1] = vv; J[1][2] = uu;
+        J[2][0] = c2 * q2 - vv * vv; J[2][1] = -c1 * uu;
+        J[2][2] = (3.0 - kGamma) * vv; J[2][3] = -c1 * ww; J[2][4] = c1;
+        J[3][0] = -vv * ww; J[3][2] = ww; J[3][3] = vv;
+        J[4][0] = (c2 * q2 - H) * vv; J[4][1] = -c1 * uu * vv;
+        J[4][2] = H - c1 * (vv * vv); J[4][3] = -c1 * vv * ww; J[4][4] = kGamma * vv;
+    } else {
+        J[0][3] = 1.0;
+        J[1][0] = -uu * ww; J[1][1] = ww; J[1][3] = uu;
+        J[2][0] = -vv * ww; J[2][2] = ww; J[2][3] = vv;
+        J[3][0] = c2 * q2 - ww * ww; J[3][1] = -c1 * uu; J[3][2] = -c1 * vv;
+        J[3][3] = (3.0 - kGamma) * ww; J[3][4] = c1;
+        J[4][0] = (c2 * q2 - H) * ww; J[4][1] = -c1 * uu * ww;
+        J[4][2] = -c1 * vv * ww; J[4][3] = H - c1 * (ww * ww); J[4][4] = kGamma * ww;
+    }
+}
+
+// Row blocks of the sweep from the Jacobians of the previous and next cell:
+//   A = -cc J_prev - cd I,  B = (1 + 2 cd) I,  C = cc J_next - cd I  (build_matrix_*, :836-848).
+__device__ __forceinline__ void euler_blocks(const double (&Jl)[5][5], const double (&Jr)[5][5],
+                                             double cc, double cd, double (&A)[5][5],
+                                             double (&B)[5][5], double (&C)[5][5]) {
+#pragma clang fp contract(off)
+    const double bd = 1.0 + 2.0 * cd;
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const double a = -cc * Jl[i][j], c = cc * Jr[i][j];
+            A[i][j] = i == j ? a - cd : a;
+            B[i][j] = i == j ? bd : 0.0;
+            C[i][j] = i == j ? c - cd : c;
+        }
+}
+
+// Assembly: rows [r0, r0 + R) of 64 systems per block (blockIdx.y = row chunk). Writes A, B,
+// C (and, when F is given, d gathered from the field F) in the solve's LAYOUT.
+template <int LAYOUT, int AXIS>
+__global__ __launch_bounds__(64) void k_euler_assemble(const double* __restrict__ U,
+                                                       const double* __restrict__ F, Grid g,
+                                                       double cc, double cd, double* __restrict__ A,
+                                                       double* __restrict__ B, double* __restrict__ C,
+                                                       double* __restrict__ D, int64_t nsys,
+                                                       int64_t nrow, int R) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const int64_t r0 = (int64_t)blockIdx.y * R;
+    const int64_t r1 = r0 + R < nrow ? r0 + R : nrow;
+    const Addr<LAYOUT> am(sys, nsys, nrow, 25), av(sys, nsys, nrow, 5);
+    const int64_t sv = g.sv();
+    const int64_t step = AXIS == 0 ? 1 : (AXIS == 1 ? g.sj() : g.sk());
+    int64_t i, j, k;
+    sweep_cell<AXIS>(g, sys, r0, i, j, k);
+    int64_t off = (i + 1) + (j + 1) * g.sj() + (k + 1) * g.sk();  // cell of row r0
+    const int64_t fstep = AXIS == 0 ? 1 : (AXIS == 1 ? g.n1 : g.n1 * g.n2);
+    const int64_t fsv = g.n1 * g.n2 * g.n3;
+    int64_t foff = i + g.n1 * (j + g.n2 * k);
+    Prim prev = load_prim(U, off - step, sv), cur = load_prim(U, off, sv);
+    for (int64_t r = r0; r < r1; ++r) {
+        const Prim next = load_prim(U, off + step, sv);
+        double Jl[5][5], Jr[5][5], Ab[5][5], Bb[5][5], Cb[5][5];
+        euler_jacobian<AXIS>(prev, Jl);
+        euler_jacobian<AXIS>(next, Jr);
+        euler_blocks(Jl, Jr, cc, cd, Ab, Bb, Cb);
+        store_mat<5>(A, am, r, Ab);
+        store_mat<5>(B, am, r, Bb);
+        store_mat<5>(C, am, r, Cb);
+        if (F) {
+            double v[5];
+#pragma unroll
+            for (int e = 0; e < 5; ++e) v[e] = F[foff + e * fsv];
+            store_vec<5>(D, av, r, v);
+        }
+        prev = cur;
+        cur = next;
+        off += step;
+        foff += fstep;
+    }
+}
+
+// Field F(n1, n2, n3, m) <-> sweep vector in LAYOUT (SCATTER = false: gather into D,
+// true: scatter D into F). Rows [r0, r0 + R) of 64 systems per block.
+template <int LAYOUT, int AXIS, bool SCATTER>
+__global__ __launch_bounds__(64) void k_sweep_move(double* __restrict__ F, double* __restrict__ D,
+                                                   Grid g, int m, int64_t nsys, int64_t nrow,
+                                                   int R) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const int64_t r0 = (int64_t)blockIdx.y * R;
+    const int64_t r1 = r0 + R < nrow ? r0 + R : nrow;
+    const Addr<LAYOUT> av(sys, nsys, nrow, m);
+    int64_t i, j, k;
+    sweep_cell<AXIS>(g, sys, r0, i, j, k);
+    const int64_t fstep = AXIS == 0 ? 1 : (AXIS == 1 ? g.n1 : g.n1 * g.n2);
+    const int64_t fsv = g.n1 * g.n2 * g.n3;
+    int64_t foff = i + g.n1 * (j + g.n2 * k);
+    for (int64_t r = r0; r < r1; ++r) {
+        for (int e = 0; e < m; ++e) {
+            if constexpr (SCATTER)
+                F[foff + e * fsv] = D[av.at(r, e)];
+            else
+                D[av.at(r, e)] = F[foff + e * fsv];
+        }
+        foff += fstep;
+    }
+}
+
+}  // namespace btd

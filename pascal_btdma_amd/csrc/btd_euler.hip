@@ -1,0 +1,66 @@
+// Launchers of the euler3d sweep assembly and of the field <-> sweep moves.
+#include "btd_euler.h"
+
+namespace btd {
+
+namespace {
+constexpr int kRowsPerBlock = 32;
+
+template <int LAYOUT, int AXIS>
+hipError_t assemble_la(const double* U, const double* F, const Grid& g, double cc, double cd,
+                       double* A, double* B, double* C, double* D, int64_t nsys, int64_t nrow,
+                       hipStream_t s) {
+    const dim3 grid((unsigned)((nsys + 63) / 64), (unsigned)((nrow + kRowsPerBlock - 1) / kRowsPerBlock));
+    hipLaunchKernelGGL((k_euler_assemble<LAYOUT, AXIS>), grid, dim3(64), 0, s, U, F, g, cc, cd, A, B, C,
+                       D, nsys, nrow, kRowsPerBlock);
+    return hipGetLastError();
+}
+
+template <int LAYOUT, int AXIS>
+hipError_t move_la(double* F, double* D, const Grid& g, int m, int64_t nsys, int64_t nrow,
+                   bool scatter, hipStream_t s) {
+    const dim3 grid((unsigned)((nsys + 63) / 64), (unsigned)((nrow + kRowsPerBlock - 1) / kRowsPerBlock));
+    if (scatter)
+        hipLaunchKernelGGL((k_sweep_move<LAYOUT, AXIS, true>), grid, dim3(64), 0, s, F, D, g, m, nsys,
+                           nrow, kRowsPerBlock);
+    else
+        hipLaunchKernelGGL((k_sweep_move<LAYOUT, AXIS, false>), grid, dim3(64), 0, s, F, D, g, m, nsys,
+                           nrow, kRowsPerBlock);
+    return hipGetLastError();
+}
+
+template <int LAYOUT>
+hipError_t assemble_l(int axis, const double* U, const double* F, const Grid& g, double cc,
+                      double cd, double* A, double* B, double* C, double* D, int64_t nsys,
+                      int64_t nrow, hipStream_t s) {
+    if (axis == 0) return assemble_la<LAYOUT, 0>(U, F, g, cc, cd, A, B, C, D, nsys, nrow, s);
+    if (axis == 1) return assemble_la<LAYOUT, 1>(U, F, g, cc, cd, A, B, C, D, nsys, nrow, s);
+    return assemble_la<LAYOUT, 2>(U, F, g, cc, cd, A, B, C, D, nsys, nrow, s);
+}
+
+template <int LAYOUT>
+hipError_t move_l(int axis, double* F, double* D, const Grid& g, int m, int64_t nsys, int64_t nrow,
+                  bool scatter, hipStream_t s) {
+    if (axis == 0) return move_la<LAYOUT, 0>(F, D, g, m, nsys, nrow, scatter, s);
+    if (axis == 1) return move_la<LAYOUT, 1>(F, D, g, m, nsys, nrow, scatter, s);
+    return move_la<LAYOUT, 2>(F, D, g, m, nsys, nrow, scatter, s);
+}
+}  // namespace
+
+hipError_t launch_euler_assemble(int layout, int axis, const double* U, const double* F, int n1,
+                                 int n2, int n3, double cc, double cd, double* A, double* B,
+                                 double* C, double* D, int nsys, int nrow, hipStream_t s) {
+    const Grid g{n1, n2, n3};
+    if (layout == LAYOUT_TILED)
+        return assemble_l<LAYOUT_TILED>(axis, U, F, g, cc, cd, A, B, C, D, nsys, nrow, s);
+    return assemble_l<LAYOUT_REF_GPU>(axis, U, F, g, cc, cd, A, B, C, D, nsys, nrow, s);
+}
+
+hipError_t launch_sweep_move(int layout, int axis, double* F, double* D, int n1, int n2, int n3,
+                             int m, int nsys, int nrow, bool scatter, hipStream_t s) {
+    const Grid g{n1, n2, n3};
+    if (layout == LAYOUT_TILED) return move_l<LAYOUT_TILED>(axis, F, D, g, m, nsys, nrow, scatter, s);
+    return move_l<LAYOUT_REF_GPU>(axis, F, D, g, m, nsys, nrow, scatter, s);
+}
+
+}  // namespace btd

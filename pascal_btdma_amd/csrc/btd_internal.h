@@ -26,6 +26,13 @@ hipError_t launch_f32r(int m, int layout, const float* A, const float* B, const 
                        const double* D, double* X, float* Wc, float* Wv, int nsys, int nrow,
                        int sweeps, int* flag, hipStream_t s, bool transposed);
 
+hipError_t launch_euler_assemble(int layout, int axis, const double* U, const double* F, int n1,
+                                 int n2, int n3, double cc, double cd, double* A, double* B,
+                                 double* C, double* D, int nsys, int nrow, hipStream_t s);
+
+hipError_t launch_sweep_move(int layout, int axis, double* F, double* D, int n1, int n2, int n3,
+                             int m, int nsys, int nrow, bool scatter, hipStream_t s);
+
 hipError_t launch_outer_f32(int m, int layout, const double* Lam, const double* X, float* dA,
                             float* dB, float* dC, int nsys, int nrow, hipStream_t s);
 

@@ -45,6 +45,12 @@ def layout_to_vec(flat: np.ndarray, layout: int, n: int, nsys: int, m: int) -> n
     return Y[:, :nsys].copy()
 
 
+def layout_to_mat(flat: np.ndarray, layout: int, n: int, nsys: int, m: int) -> np.ndarray:
+    """Inverse of nat_to_layout for blocks: returns (n, nsys, m, m) natural form."""
+    v = layout_to_vec(flat, layout, n, nsys, m * m)  # (n, nsys, E) with e = i + m*j
+    return np.swapaxes(v.reshape(n, nsys, m, m), -1, -2).copy()
+
+
 def cpu_layout(X: np.ndarray) -> np.ndarray:
     """natural -> reference CPU layout arrays for the oracle."""
     return np.ascontiguousarray(np.swapaxes(X, -1, -2)) if X.ndim == 4 else np.ascontiguousarray(X)

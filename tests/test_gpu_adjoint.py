@@ -9,15 +9,9 @@ import numpy as np
 import pytest
 
 import pascal_btdma_amd as pb
-from helpers import cpu_layout, layout_to_vec, nat_to_layout, rel_err
+from helpers import cpu_layout, layout_to_mat, layout_to_vec, nat_to_layout, rel_err
 
 pytestmark = pytest.mark.gpu
-
-
-def layout_to_mat(flat, layout, n, nsys, m):
-    """Inverse of nat_to_layout for blocks: (n, nsys, m, m) natural form."""
-    v = layout_to_vec(flat, layout, n, nsys, m * m)  # (n, nsys, E) with e = i + m*j
-    return np.swapaxes(v.reshape(n, nsys, m, m), -1, -2).copy()
 
 
 def oracle_grads(oracle, A, B, C, d, w):
