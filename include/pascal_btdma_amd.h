@@ -167,6 +167,18 @@ btd_status btd_euler_assemble(btd_plan_t plan, int axis, const double *U, const 
                               int n2, int n3, double h, double dt, double eps, double *a, double *b,
                               double *c, double *d, void *stream);
 
+/* Matrix-free periodic sweep (no reference counterpart; it does what build_matrix_*,
+ * the transposes, btdma_many_cycl_mpi_gpu_v2 and the transpose back do at
+ * euler3d_mpi_gpu.f90:1260-1325 for one sweep, single partition): solves the sweep's periodic
+ * system with the blocks of btd_euler_assemble computed inside the solve, and writes the
+ * solution into the field out(n1, n2, n3, 5); rhs(n1, n2, n3, 5) is the right-hand side
+ * field, and out may be rhs itself (the reference overwrites dU). npart must be 1 and
+ * nrow >= 3. work: device scratch of 2*btd_layout_elems(layout, nsys, nrow, 25) +
+ * btd_layout_elems(layout, nsys, nrow, 5) doubles. Zero pivots are recorded in the plan. */
+btd_status btd_euler_sweep_cyclic(btd_plan_t plan, int axis, const double *U, const double *rhs,
+                                  int n1, int n2, int n3, double h, double dt, double eps,
+                                  double *work, double *out, void *stream);
+
 /* Field F(n1, n2, n3, m) -> the sweep's vector d in the plan's layout (transpose_jk_kernel /
  * transpose_ik_kernel, :931-943, :976-988, for callers that assemble their own blocks). */
 btd_status btd_sweep_gather(btd_plan_t plan, int axis, const double *field, int n1, int n2, int n3,

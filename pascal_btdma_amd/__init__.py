@@ -23,7 +23,8 @@ __all__ = [
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
     "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
     "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "solve_transposed_f32r",
-    "outer_grads_f32", "euler_assemble", "sweep_gather", "sweep_scatter", "sweep_shape", "BtdmaBatch", "Graph",
+    "outer_grads_f32", "euler_assemble", "euler_sweep_cyclic",
+    "euler_sweep_work_elems", "sweep_gather", "sweep_scatter", "sweep_shape", "BtdmaBatch", "Graph",
 ]
 
 kMaxBlockSize = 10
@@ -342,6 +343,25 @@ def euler_assemble(plan: BtdmaGpuPlan, axis: int, U, rhs, n1: int, n2: int, n3: 
     _check(_capi.lib().btd_euler_assemble(plan._h, int(axis), _ptr(U), _ptr(rhs), int(n1), int(n2),
                                           int(n3), float(h), float(dt), float(eps), _ptr(a), _ptr(b),
                                           _ptr(c), _ptr(d), _stream(stream)))
+
+
+def euler_sweep_work_elems(plan: BtdmaGpuPlan) -> int:
+    """Doubles of device scratch btd_euler_sweep_cyclic needs for this plan."""
+    return 2 * plan.matrix_elems() + plan.vector_elems()
+
+
+def euler_sweep_cyclic(plan: BtdmaGpuPlan, axis: int, U, rhs, n1: int, n2: int, n3: int, h: float,
+                       dt: float, eps: float, work, out, stream=None) -> None:
+    """Matrix-free periodic euler3d sweep along `axis`: the blocks btd_euler_assemble would
+    write are computed inside the solve; the solution goes into the field `out`
+    (n1, n2, n3, 5), which may be `rhs` itself."""
+    import torch
+    if plan.created:
+        _check_fields(n1, n2, n3, 5, U=U, rhs=rhs, out=out)
+        _check_tensor("work", work, euler_sweep_work_elems(plan), torch.float64)
+    _check(_capi.lib().btd_euler_sweep_cyclic(plan._h, int(axis), _ptr(U), _ptr(rhs), int(n1),
+                                              int(n2), int(n3), float(h), float(dt), float(eps),
+                                              _ptr(work), _ptr(out), _stream(stream)))
 
 
 def sweep_gather(plan: BtdmaGpuPlan, axis: int, field, n1: int, n2: int, n3: int, d,

@@ -245,6 +245,26 @@ btd_status btd_euler_assemble(btd_plan_t plan, int axis, const double* U, const 
     return BTD_OK;
 }
 
+btd_status btd_euler_sweep_cyclic(btd_plan_t plan, int axis, const double* U, const double* rhs,
+                                  int n1, int n2, int n3, double h, double dt, double eps,
+                                  double* work, double* out, void* stream) {
+    g_err.clear();
+    btd_status st = check_sweep(plan, axis, n1, n2, n3, "btd_euler_sweep_cyclic");
+    if (st != BTD_OK) return st;
+    if (plan->m != 5) return fail(BTD_ERR_INVALID_ARG, "btd_euler_sweep_cyclic: the Euler system has m = 5");
+    if (plan->npart != 1)
+        return fail(BTD_ERR_INVALID_ARG, "btd_euler_sweep_cyclic: needs a plan with npart == 1");
+    if (plan->nrow < 3) return fail(BTD_ERR_INVALID_ARG, "btd_euler_sweep_cyclic: needs nrow >= 3");
+    if (!U || !rhs || !work || !out) return fail(BTD_ERR_INVALID_ARG, "btd_euler_sweep_cyclic: NULL array");
+    if (!(h > 0.0)) return fail(BTD_ERR_INVALID_ARG, "btd_euler_sweep_cyclic: h must be > 0");
+    const double cc = dt / (4.0 * h);
+    const double cd = eps * dt / (2.0 * (h * h));
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_euler_sweep_cyclic(plan->layout, axis, U, rhs, n1, n2, n3, cc, cd, work, out,
+                                           plan->nsys, plan->nrow, plan->d_flag, (hipStream_t)stream));
+    return BTD_OK;
+}
+
 static btd_status sweep_move(btd_plan_t plan, int axis, double* field, double* d, int n1, int n2,
                              int n3, bool scatter, void* stream, const char* what) {
     g_err.clear();

@@ -162,6 +162,210 @@ __global__ __launch_bounds__(64) void k_euler_assemble(const double* __restrict_
     }
 }
 
+// Matrix-free periodic sweep: the solve of btdma_many_cycl (mod_btdma_cpu.f90:80-183;
+// mod_btdma_gpu_v2.f90:352-616) on the sweep's system, with every block computed from U where
+// the solve needs it, so A, B, C are never stored. Row 0 couples to row n-1 (periodic halo);
+// x_0 is kept as a separate unknown and rows 1..n-1 are eliminated with the fill-in column E
+// (E_1 = -A_1, E_{n-1} = -C_{n-1}):
+//   forward, q = 1..n-1:  R = B_q - A_q C'_{q-1},
+//                         [d'_q | E'_q | C'_q] = R^{-1} [rhs_q - A_q d'_{q-1} | X_q - A_q E'_{q-1} | C_q]
+//   so that x_q = d'_q + E'_q x_0 - C'_q x_{q+1} (C'_{n-1} = 0). Row 1 expressed through x_0
+//   alone, x_1 = alpha + beta x_0, is composed during the same sweep (alpha, beta, gamma start as
+//   d'_1, E'_1, -C'_1 and take alpha += gamma d'_q, beta += gamma E'_q, gamma = -gamma C'_q), with
+//   the three kept in LDS. Then row 0:
+//   (B_0 + A_0 E'_{n-1} + C_0 beta) x_0 = rhs_0 - A_0 d'_{n-1} - C_0 alpha,
+//   and one backward sweep x_q = d'_q + E'_q x_0 - C'_q x_{q+1} writes x into the field F.
+// C', E', d' of rows 1..n-2 go to the workspace (Wc, We, Wd, the plan's layout); each lane reads
+// its own field cells before writing them, so F may be the rhs field itself. Needs n >= 3.
+template <int AXIS>
+__device__ __forceinline__ void euler_row_blocks(const Prim& prev, const Prim& next, double cc,
+                                                 double cd, double (&A)[5][5], double (&C)[5][5]) {
+    double Jl[5][5], Jr[5][5], Bb[5][5];
+    euler_jacobian<AXIS>(prev, Jl);
+    euler_jacobian<AXIS>(next, Jr);
+    euler_blocks(Jl, Jr, cc, cd, A, Bb, C);
+}
+
+template <int LAYOUT, int AXIS>
+__global__ __launch_bounds__(64) void k_euler_sweep_cyclic(const double* __restrict__ U,
+                                                           const double* rhs, Grid g, double cc,
+                                                           double cd, double* __restrict__ Wc,
+                                                           double* __restrict__ We,
+                                                           double* __restrict__ Wd, double* F,
+                                                           int64_t nsys, int64_t nrow,
+                                                           int* __restrict__ flag) {
+    constexpr int M = 5;
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    __shared__ double sp[(M + 2 * M * M) * 64];  // alpha, beta, gamma: [element][lane]
+    const int lane = threadIdx.x;
+    auto SA = [&](int i) -> double& { return sp[i * 64 + lane]; };
+    auto SB = [&](int i, int j) -> double& { return sp[(M + i * M + j) * 64 + lane]; };
+    auto SG = [&](int i, int j) -> double& { return sp[(M + M * M + i * M + j) * 64 + lane]; };
+    const double bd = 1.0 + 2.0 * cd;
+    const int64_t sv = g.sv();
+    const int64_t step = AXIS == 0 ? 1 : (AXIS == 1 ? g.sj() : g.sk());
+    const int64_t fstep = AXIS == 0 ? 1 : (AXIS == 1 ? g.n1 : g.n1 * g.n2);
+    const int64_t fsv = g.n1 * g.n2 * g.n3;
+    int64_t i0, j0, k0;
+    sweep_cell<AXIS>(g, sys, 0, i0, j0, k0);
+    const int64_t off0 = (i0 + 1) + (j0 + 1) * g.sj() + (k0 + 1) * g.sk();  // cell of row 0
+    const int64_t foff0 = i0 + g.n1 * (j0 + g.n2 * k0);
+    bool ok = true;
+    double Cp[M][M], Ep[M][M], dp[M];
+    Prim prev = load_prim(U, off0, sv), cur = load_prim(U, off0 + step, sv);  // cells 0, 1
+    for (int64_t q = 1; q < nrow; ++q) {
+        const bool last = q == nrow - 1;
+        const Prim next = load_prim(U, off0 + (q + 1) * step, sv);
+        double A[M][M], C[M][M], R[M][M], X[M][M], v[M];
+        euler_row_blocks<AXIS>(prev, next, cc, cd, A, C);
+#pragma unroll
+        for (int e = 0; e < M; ++e) v[e] = rhs[foff0 + q * fstep + e * fsv];
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                R[i][j] = i == j ? bd : 0.0;
+                X[i][j] = q == 1 ? -A[i][j] : (last ? -C[i][j] : 0.0);
+            }
+        if (q > 1) {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    const double a = A[i][k];
+                    v[i] = fma(-a, dp[k], v[i]);
+#pragma unroll
+                    for (int j = 0; j < M; ++j) {
+                        R[i][j] = fma(-a, Cp[k][j], R[i][j]);
+                        X[i][j] = fma(-a, Ep[k][j], X[i][j]);
+                    }
+                }
+        }
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, v);
+        solve_rows<M>(R, piv, X);
+        if (!last) {
+            solve_rows<M>(R, piv, C);
+        } else {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) C[i][j] = 0.0;
+        }
+        // x_1 = alpha + beta x_0 + gamma x_{q+1}
+        if (q == 1) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                SA(i) = v[i];
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    SB(i, j) = X[i][j];
+                    SG(i, j) = -C[i][j];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                double gi[M];
+#pragma unroll
+                for (int k = 0; k < M; ++k) gi[k] = SG(i, k);
+                double a = SA(i);
+#pragma unroll
+                for (int k = 0; k < M; ++k) a = fma(gi[k], v[k], a);
+                SA(i) = a;
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    double b = SB(i, j), c = 0.0;
+#pragma unroll
+                    for (int k = 0; k < M; ++k) {
+                        b = fma(gi[k], X[k][j], b);
+                        c = fma(-gi[k], C[k][j], c);
+                    }
+                    SB(i, j) = b;
+                    SG(i, j) = c;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dp[i] = v[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                Cp[i][j] = C[i][j];
+                Ep[i][j] = X[i][j];
+            }
+        }
+        if (!last) {
+            store_mat<M>(Wc, am, q, Cp);
+            store_mat<M>(We, am, q, Ep);
+            store_vec<M>(Wd, av, q, dp);
+        }
+        prev = cur;
+        cur = next;
+    }
+    // row 0: (Ep, dp) hold row n-1
+    double x0[M];
+    {
+        double A0[M][M], C0[M][M], R[M][M];
+        const Prim pl = load_prim(U, off0 - step, sv), pr = load_prim(U, off0 + step, sv);
+        euler_row_blocks<AXIS>(pl, pr, cc, cd, A0, C0);
+#pragma unroll
+        for (int e = 0; e < M; ++e) x0[e] = rhs[foff0 + e * fsv];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+#pragma unroll
+            for (int k = 0; k < M; ++k)
+                x0[i] = fma(-A0[i][k], dp[k], fma(-C0[i][k], SA(k), x0[i]));
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double s = i == j ? bd : 0.0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) s = fma(A0[i][k], Ep[k][j], fma(C0[i][k], SB(k, j), s));
+                R[i][j] = s;
+            }
+        }
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, x0);
+    }
+    // backward: x_{n-1} = d'_{n-1} + E'_{n-1} x_0, then x_q = d'_q + E'_q x_0 - C'_q x_{q+1}
+    double xn[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        double s = dp[i];
+#pragma unroll
+        for (int k = 0; k < M; ++k) s = fma(Ep[i][k], x0[k], s);
+        xn[i] = s;
+    }
+#pragma unroll
+    for (int e = 0; e < M; ++e) F[foff0 + (nrow - 1) * fstep + e * fsv] = xn[e];
+    for (int64_t q = nrow - 2; q >= 1; --q) {
+        double c[M][M], e_[M][M], d[M];
+        load_mat_c<M>(Wc, am, q, c);
+        load_mat_c<M>(We, am, q, e_);
+        load_vec<M>(Wd, av, q, d);
+        double x[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            double s = d[i];
+#pragma unroll
+            for (int k = 0; k < M; ++k) s = fma(e_[i][k], x0[k], fma(-c[i][k], xn[k], s));
+            x[i] = s;
+        }
+#pragma unroll
+        for (int e = 0; e < M; ++e) {
+            F[foff0 + q * fstep + e * fsv] = x[e];
+            xn[e] = x[e];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < M; ++e) F[foff0 + e * fsv] = x0[e];
+    if (!ok) report_singular(flag, sys);
+}
+
 // Field F(n1, n2, n3, m) <-> sweep vector in LAYOUT (SCATTER = false: gather into D,
 // true: scatter D into F). Rows [r0, r0 + R) of 64 systems per block.
 template <int LAYOUT, int AXIS, bool SCATTER>

@@ -56,6 +56,39 @@ hipError_t launch_euler_assemble(int layout, int axis, const double* U, const do
     return assemble_l<LAYOUT_REF_GPU>(axis, U, F, g, cc, cd, A, B, C, D, nsys, nrow, s);
 }
 
+namespace {
+template <int LAYOUT, int AXIS>
+hipError_t sweep_cyclic_la(const double* U, const double* rhs, const Grid& g, double cc, double cd,
+                           double* Wc, double* We, double* Wd, double* F, int64_t nsys,
+                           int64_t nrow, int* flag, hipStream_t s) {
+    hipLaunchKernelGGL((k_euler_sweep_cyclic<LAYOUT, AXIS>), dim3((unsigned)((nsys + 63) / 64)), dim3(64),
+                       0, s, U, rhs, g, cc, cd, Wc, We, Wd, F, nsys, nrow, flag);
+    return hipGetLastError();
+}
+
+template <int LAYOUT>
+hipError_t sweep_cyclic_l(int axis, const double* U, const double* rhs, const Grid& g, double cc,
+                          double cd, double* Wc, double* We, double* Wd, double* F, int64_t nsys,
+                          int64_t nrow, int* flag, hipStream_t s) {
+    if (axis == 0) return sweep_cyclic_la<LAYOUT, 0>(U, rhs, g, cc, cd, Wc, We, Wd, F, nsys, nrow, flag, s);
+    if (axis == 1) return sweep_cyclic_la<LAYOUT, 1>(U, rhs, g, cc, cd, Wc, We, Wd, F, nsys, nrow, flag, s);
+    return sweep_cyclic_la<LAYOUT, 2>(U, rhs, g, cc, cd, Wc, We, Wd, F, nsys, nrow, flag, s);
+}
+}  // namespace
+
+hipError_t launch_euler_sweep_cyclic(int layout, int axis, const double* U, const double* rhs,
+                                     int n1, int n2, int n3, double cc, double cd, double* work,
+                                     double* F, int nsys, int nrow, int* flag, hipStream_t s) {
+    const Grid g{n1, n2, n3};
+    const int64_t nm = layout_elems(layout, nsys, nrow, 25);
+    double* Wc = work;
+    double* We = work + nm;
+    double* Wd = work + 2 * nm;
+    if (layout == LAYOUT_TILED)
+        return sweep_cyclic_l<LAYOUT_TILED>(axis, U, rhs, g, cc, cd, Wc, We, Wd, F, nsys, nrow, flag, s);
+    return sweep_cyclic_l<LAYOUT_REF_GPU>(axis, U, rhs, g, cc, cd, Wc, We, Wd, F, nsys, nrow, flag, s);
+}
+
 hipError_t launch_sweep_move(int layout, int axis, double* F, double* D, int n1, int n2, int n3,
                              int m, int nsys, int nrow, bool scatter, hipStream_t s) {
     const Grid g{n1, n2, n3};

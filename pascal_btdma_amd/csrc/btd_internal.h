@@ -30,6 +30,10 @@ hipError_t launch_euler_assemble(int layout, int axis, const double* U, const do
                                  int n2, int n3, double cc, double cd, double* A, double* B,
                                  double* C, double* D, int nsys, int nrow, hipStream_t s);
 
+hipError_t launch_euler_sweep_cyclic(int layout, int axis, const double* U, const double* rhs,
+                                     int n1, int n2, int n3, double cc, double cd, double* work,
+                                     double* F, int nsys, int nrow, int* flag, hipStream_t s);
+
 hipError_t launch_sweep_move(int layout, int axis, double* F, double* D, int n1, int n2, int n3,
                              int m, int nsys, int nrow, bool scatter, hipStream_t s);
 

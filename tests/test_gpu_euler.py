@@ -115,3 +115,44 @@ def test_sweep_argument_errors(cuda):
             pb.euler_assemble(plan, 3, U, None, n1, n2, n3, H, DT, EPS, t, t, t)
         pb.euler_assemble(plan, 0, U.fill_(1.0), None, n1, n2, n3, H, DT, EPS, t, t, t)
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("grid", [(12, 10, 9), (3, 70, 4), (5, 3, 66)], ids=lambda g: "x".join(map(str, g)))
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_matrix_free_sweep_matches_oracle(cuda, oracle, grid, axis, layout):
+    """btd_euler_sweep_cyclic (blocks computed inside the periodic solve, solution written to
+    the field) against the oracle's assembly and periodic solve, <= 1e-12; written in place
+    over the rhs field, as the reference overwrites dU. Includes nrow = 3."""
+    import torch
+    n1, n2, n3 = grid
+    U = ea.vortex_field(n1, n2, n3, seed=20 + axis)
+    rhs = np.random.default_rng(30 + axis).standard_normal((5, n3, n2, n1))
+    nrow, nsys = pb.sweep_shape(axis, n1, n2, n3)
+    if nrow < 3:
+        pytest.skip("periodic sweep needs nrow >= 3")
+    with pb.BtdmaGpuPlan().create(5, nsys, nrow, 1, layout) as plan:
+        work = torch.full((pb.euler_sweep_work_elems(plan),), np.nan, dtype=torch.float64, device=cuda)
+        f = upload(cuda, rhs)
+        pb.euler_sweep_cyclic(plan, axis, upload(cuda, U), f, n1, n2, n3, H, DT, EPS, work, f)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    A, B, C, dref = ea.assemble(U, rhs, axis, H, DT, EPS)
+    x, info = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), dref)
+    assert info == 0
+    xs = ea.to_sweep(np.moveaxis(f.cpu().numpy().reshape(5, n3, n2, n1), 0, -1), axis)
+    assert rel_err(xs, x).max() <= 1e-12
+
+
+def test_matrix_free_sweep_argument_errors(cuda):
+    import torch
+    n1, n2, n3 = 6, 5, 4
+    U = torch.ones(5 * 8 * 7 * 6, dtype=torch.float64, device=cuda)
+    f = torch.zeros(5 * n1 * n2 * n3, dtype=torch.float64, device=cuda)
+    with pb.BtdmaGpuPlan().create(5, n2 * n3, n1, 2, pb.Layout.TILED) as plan:  # npart != 1
+        w = torch.zeros(pb.euler_sweep_work_elems(plan), dtype=torch.float64, device=cuda)
+        with pytest.raises(ValueError):
+            pb.euler_sweep_cyclic(plan, 0, U, f, n1, n2, n3, H, DT, EPS, w, f)
+    with pb.BtdmaGpuPlan().create(5, n2 * n3, n1, 1, pb.Layout.TILED) as plan:
+        with pytest.raises(ValueError):  # workspace too small
+            pb.euler_sweep_cyclic(plan, 0, U, f, n1, n2, n3, H, DT, EPS, f, f)
