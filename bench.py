@@ -126,6 +126,39 @@ def cpu_baseline(w: dict, target_s: float = 12.0) -> dict:
             "cpu_model": model, "nproc": os.cpu_count()}
 
 
+class ParityChecks:
+    """Device results recorded during the GPU part, to be checked against the oracle in the
+    CPU leg: each entry holds a function returning the natural-form inputs (from the
+    product's host generator), the device solution x (n, S, m) for those S systems, and a
+    sink that receives the per-system relative errors."""
+
+    def __init__(self):
+        self.items = []
+
+    def add(self, inputs, x, sink):
+        self.items.append((inputs, np.asarray(x), sink))
+
+
+def max_sink(out: dict, key: str):
+    """A sink that keeps the largest per-system error seen under out[key]."""
+    def sink(e):
+        out[key] = max(out[key] or 0.0, float(np.max(e)))
+    return sink
+
+
+def cpu_leg(w: dict, target_s: float, checks: ParityChecks, baseline: bool) -> dict | None:
+    """The CPU leg of the benchmark: times the oracle on a bounded sample of the headline
+    workload (unless baseline is False) and checks every recorded device result against the
+    oracle's solve of the same inputs."""
+    from oracle import pyoracle
+    for inputs, x, sink in checks.items:
+        A, B, C, dd = inputs()
+        xo, info = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
+                                        pyoracle.to_cpu_layout(C), dd)
+        sink(rel_err_sys(x, xo))
+    return cpu_baseline(w, target_s) if baseline else None
+
+
 # ------------------------------------------------------------------------------ extras --
 
 def extra_study(dev) -> dict:
@@ -162,11 +195,11 @@ def extra_study(dev) -> dict:
     return out
 
 
-def extra_config0(dev) -> dict:
-    """Config 0: 1024 systems, m = 3, n = 64 (the CPU-runnable case), FP64 plain."""
+def extra_config0(dev, checks: ParityChecks) -> dict:
+    """Config 0: 1024 systems, m = 3, n = 64 (the CPU-runnable case), FP64 plain; all
+    systems are checked against the oracle in the CPU leg."""
     import torch
     import pascal_btdma_amd as pb
-    from oracle import pyoracle
     w = WORKLOADS["ddperm_m3"]
     m, n, nsys = w["m"], w["n"], w["nsys"]
     lay = pb.Layout.TILED
@@ -184,22 +217,21 @@ def extra_config0(dev) -> dict:
         t = event_time(lambda: pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n), restore, 10,
                        torch.cuda.current_stream())
     x = tiled_vec_subset(d.cpu().numpy(), n, m, list(range(nsys // 64)))
-    A, B, C, dd = pb.generate_host(w["kind"], w["seed"], m, nsys, n)
-    xo, _ = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
-                                 pyoracle.to_cpu_layout(C), dd)
     ms = float(np.median(t))
-    return {"workload": w["name"], "ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3),
-            "max_rel_err_vs_oracle_all_systems": float(rel_err_sys(x, xo).max())}
+    out = {"workload": w["name"], "ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3),
+           "max_rel_err_vs_oracle_all_systems": None}
+    checks.add(lambda: pb.generate_host(w["kind"], w["seed"], m, nsys, n), x,
+               max_sink(out, "max_rel_err_vs_oracle_all_systems"))
+    return out
 
 
-def extra_batch(dev, steps: int) -> dict:
+def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
     """Config 2: 20000 systems with m uniform in {2,3,5,8,10}, n log-uniform in [32,1024],
     plus 512 long systems (m = 5, n = 65536) split into P partitions, all replayed as one
     captured graph (the batch's persistent kernel and the long systems' three kernels on a
     forked branch). The autotuner picks the batch placement/grid and P in {8,16,32}."""
     import torch
     import pascal_btdma_amd as pb
-    from oracle import pyoracle
     from pascal_btdma_amd.tuner import tune_batch, tune_partitions
     rng = np.random.default_rng(2026)
     nb = 20000
@@ -268,18 +300,7 @@ def extra_batch(dev, steps: int) -> dict:
     g.launch(s1)
     s1.synchronize()
     hd = d.cpu().numpy()
-    worst = 0.0
-    for s in range(0, nb, 997):
-        A, B, C, dd = pb.generate_host(pb.GenKind.DDPERM, 7, int(m[s]), nb, int(n[s]), s, 1)
-        xo, _ = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
-                                     pyoracle.to_cpu_layout(C), dd)
-        x = bt.extract(hd, s)
-        worst = max(worst, float(np.abs(x - xo[:, 0]).max() / np.abs(xo[:, 0]).max()))
-    A, B, C, dd = pb.generate_host(pb.GenKind.DDPERM, 8, lm, lnsys, ln, 0, 2)
-    xo, _ = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
-                                 pyoracle.to_cpu_layout(C), dd)
     xl = tiled_vec_subset(ld[: ln * lm * 64].cpu().numpy(), ln, lm, [0])[:, :2]
-    worst_long = float(rel_err_sys(xl, xo).max())
     g.destroy()
     plan.destroy()
     rows_b = int(n.sum())
@@ -287,7 +308,7 @@ def extra_batch(dev, steps: int) -> dict:
     bytes_b = float(sum(8.0 * (3 * mm * mm + 2 * mm) * nn for mm, nn in zip(m.tolist(), n.tolist())))
     bytes_l = algorithmic_bytes(lm, rows_l)
     ms = float(np.median(t))
-    return {"workload": "20000 systems m in {2,3,5,8,10}, n log-uniform [32,1024] + 512 systems m=5 "
+    out = {"workload": "20000 systems m in {2,3,5,8,10}, n log-uniform [32,1024] + 512 systems m=5 "
                         "n=65536 in P partitions, one graph launch",
             "ms": ms, "block_rows": rows_b + rows_l,
             "block_rows_per_s": (rows_b + rows_l) / (ms * 1e-3),
@@ -297,16 +318,22 @@ def extra_batch(dev, steps: int) -> dict:
             "batch_only_ms": float(np.median(tb_only)), "long_only_ms": float(np.median(tl_only)),
             "long_only_GBs": bytes_l / (np.median(tl_only) * 1e-3) / 1e9,
             "tuned_batch": tb, "tuned_partitions": tp, "batch_tiles": bt.ntiles,
-            "max_rel_err_vs_oracle_sampled": max(worst, worst_long)}
+            "max_rel_err_vs_oracle_sampled": None,
+            "parity_sample": "every 997th batch system and the first 2 long systems"}
+    sink = max_sink(out, "max_rel_err_vs_oracle_sampled")
+    for s in range(0, nb, 997):
+        checks.add(lambda s=s: pb.generate_host(pb.GenKind.DDPERM, 7, int(m[s]), nb, int(n[s]), s, 1),
+                   bt.extract(hd, s)[:, None, :], sink)
+    checks.add(lambda: pb.generate_host(pb.GenKind.DDPERM, 8, lm, lnsys, ln, 0, 2), xl, sink)
+    return out
 
 
-def extra_f32r(dev, steps: int, fp64_ms: float) -> dict:
+def extra_f32r(dev, steps: int, fp64_ms: float, checks: ParityChecks) -> dict:
     """Config 3: m = 5, n = 512, 131072 systems, blocks with cond log-uniform [1e1, 1e6]
     stored in FP32; FP32 solve + s FP64-residual refinement sweeps, s = 0..3, vs the FP64
     plain kernel's time on config 1, with the error vs the oracle on the same FP32 inputs."""
     import torch
     import pascal_btdma_amd as pb
-    from oracle import pyoracle
     m, n, nsys = 5, 512, 131072
     lay = pb.Layout.TILED
     a, b, c = (torch.empty(pb.layout_elems(lay, nsys, n, m * m), dtype=torch.float32, device=dev)
@@ -316,9 +343,15 @@ def extra_f32r(dev, steps: int, fp64_ms: float) -> dict:
     x = torch.empty_like(d)
     out = {"workload": "cond-controlled B (1e1..1e6), m=5, n=512, 131072 systems, FP32 blocks",
            "fp64_plain_ms_config1": fp64_ms}
-    A, B, C, dd = pb.generate_host(pb.GenKind.COND, 3, m, nsys, n, 0, 128)
-    xo, _ = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
-                                 pyoracle.to_cpu_layout(C), dd)
+    sample = lambda: pb.generate_host(pb.GenKind.COND, 3, m, nsys, n, 0, 128)
+
+    def error_sink(r):
+        def sink(err):
+            r["max_rel_err_vs_oracle"] = float(err.max())
+            r["median_rel_err_vs_oracle"] = float(np.median(err))
+            r["frac_systems_le_1e-12"] = float((err <= 1e-12).mean())
+        return sink
+
     with pb.BtdmaGpuPlan().create(m, nsys, n, 1, lay) as plan:
         work = torch.empty(pb.f32r_work_elems(plan), dtype=torch.float32, device=dev)
         for sweeps in (0, 1, 2, 3):
@@ -326,19 +359,17 @@ def extra_f32r(dev, steps: int, fp64_ms: float) -> dict:
             t = event_time(lambda: pb.solve_noncyclic_f32r(plan, a, b, c, d, x, work, sweeps),
                            lambda: None, steps, torch.cuda.current_stream())
             xs = tiled_vec_subset(x[: 2 * n * m * 64].cpu().numpy(), n, m, [0, 1])
-            err = rel_err_sys(xs, xo)
             ms = float(np.median(t))
-            out[f"sweeps{sweeps}"] = {"ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3),
-                                      "speedup_vs_fp64_plain": fp64_ms / ms,
-                                      "max_rel_err_vs_oracle": float(err.max()),
-                                      "median_rel_err_vs_oracle": float(np.median(err)),
-                                      "frac_systems_le_1e-12": float((err <= 1e-12).mean())}
+            r = out[f"sweeps{sweeps}"] = {"ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3),
+                                          "speedup_vs_fp64_plain": fp64_ms / ms}
+            checks.add(sample, xs, error_sink(r))
         # the FP64 plain kernel on the same (FP32-rounded) inputs: the two-FP64-solver floor
         a64, b64, c64 = a.double(), b.double(), c.double()
         d64 = d.clone()
         pb.solve_noncyclic(plan, a64, b64, c64, d64, m, nsys, n)
         xs = tiled_vec_subset(d64[: 2 * n * m * 64].cpu().numpy(), n, m, [0, 1])
-        out["fp64_plain_same_inputs_max_rel_err"] = float(rel_err_sys(xs, xo).max())
+        out["fp64_plain_same_inputs_max_rel_err"] = None
+        checks.add(sample, xs, max_sink(out, "fp64_plain_same_inputs_max_rel_err"))
     out["error_sample"] = "first 128 systems vs the oracle on the same FP32-rounded blocks"
     return out
 
@@ -432,6 +463,103 @@ def extra_adjoint(dev, steps: int) -> dict:
             "fp32r_max_rel_err_x_vs_fp64": err_x, "fp32r_max_rel_err_lambda_vs_fp64": err_l}
 
 
+def euler_state(dev, n1: int, n2: int, n3: int):
+    """A smooth periodic conserved state on an n1 x n2 x n3 grid with its one-cell periodic
+    halo, as a flat float64 device tensor in Fortran order U(0:n1+1, 0:n2+1, 0:n3+1, 5)."""
+    import torch
+    two_pi, gamma = 2 * np.pi, 1.4
+    ax = [(torch.arange(n, dtype=torch.float64, device=dev) + 0.5) / n for n in (n3, n2, n1)]
+    Z, Y, X = torch.meshgrid(*ax, indexing="ij")
+    rho = 1.0 + 0.2 * torch.sin(two_pi * X) * torch.cos(two_pi * Y)
+    u, v, w = 0.3 * torch.sin(two_pi * Y), 0.3 * torch.cos(two_pi * Z), 0.2 * torch.sin(two_pi * X)
+    p = 1.0 + 0.1 * torch.cos(two_pi * (X + Y + Z))
+    E = p / ((gamma - 1.0) * rho) + 0.5 * (u * u + v * v + w * w)
+    inner = torch.stack([rho, rho * u, rho * v, rho * w, rho * E])  # (5, n3, n2, n1)
+    for dim in (1, 2, 3):  # periodic halo
+        inner = torch.cat([inner.narrow(dim, inner.shape[dim] - 1, 1), inner, inner.narrow(dim, 0, 1)], dim)
+    return inner.contiguous().reshape(-1)
+
+
+def extra_euler(dev, steps: int, n: int = 256) -> dict:
+    """The euler3d ADI step's three periodic sweeps (z, y, x) on an n^3 grid, m = 5: the
+    assembled path (btd_euler_assemble -> btd_solve_cyclic -> btd_sweep_scatter) and the
+    matrix-free sweep (btd_euler_sweep_cyclic), per sweep, with the agreement of the two; plus
+    the device time of the transposes the reference runs around the y and x sweeps
+    (euler3d_mpi_gpu.f90:1287-1325: dU, trA, trB, trC in and dU out), timed as torch copies of
+    the same arrays, which the assembly makes unnecessary."""
+    import torch
+    import pascal_btdma_amd as pb
+    n1 = n2 = n3 = n
+    h, dt, eps = 1.0 / n, 0.5 / n, 0.01
+    lay = pb.Layout.TILED
+    s = torch.cuda.current_stream()
+    U = euler_state(dev, n1, n2, n3)
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    rhs = torch.randn(5 * n1 * n2 * n3, dtype=torch.float64, device=dev, generator=g)
+    out = {"workload": f"euler3d ADI step, {n}^3 grid, m=5, periodic sweeps, FP64",
+           "cells": n1 * n2 * n3}
+    tot = {"assembled": 0.0, "matrix_free": 0.0, "reference_transposes": 0.0}
+    for axis, name in ((2, "z"), (1, "y"), (0, "x")):
+        nrow, nsys = pb.sweep_shape(axis, n1, n2, n3)
+        with pb.BtdmaGpuPlan().create(5, nsys, nrow, 1, lay) as plan:
+            a, b, c = (torch.empty(plan.matrix_elems(), dtype=torch.float64, device=dev) for _ in range(3))
+            d = torch.empty(plan.vector_elems(), dtype=torch.float64, device=dev)
+            x1 = torch.empty_like(rhs)
+            t_as = event_time(lambda: pb.euler_assemble(plan, axis, U, rhs, n1, n2, n3, h, dt, eps, a, b, c, d),
+                              lambda: None, steps, s)
+            t_sv = event_time(lambda: pb.solve_cyclic(plan, a, b, c, d, 5, nsys, nrow),
+                              lambda: pb.euler_assemble(plan, axis, U, rhs, n1, n2, n3, h, dt, eps,
+                                                        a, b, c, d), steps, s)
+            t_sc = event_time(lambda: pb.sweep_scatter(plan, axis, d, n1, n2, n3, x1), lambda: None, steps, s)
+            del a, b, c
+            work = torch.empty(pb.euler_sweep_work_elems(plan), dtype=torch.float64, device=dev)
+            x2 = torch.empty_like(rhs)
+            t_mf = event_time(lambda: pb.euler_sweep_cyclic(plan, axis, U, rhs, n1, n2, n3, h, dt, eps,
+                                                            work, x2), lambda: None, steps, s)
+            torch.cuda.synchronize()
+            assert not plan.check()
+            diff = float(((x1 - x2).abs().max() / x1.abs().max()).item())
+            del work, d
+        # the reference's transposes for this sweep (none for z)
+        t_tr = [0.0]
+        if axis != 2:
+            blk = torch.empty((25, n3, n2, n1), dtype=torch.float64, device=dev)
+            vec = rhs.view(5, n3, n2, n1)
+            perm = (0, 2, 1, 3) if axis == 1 else (0, 3, 1, 2)
+            back = (0, 2, 1, 3) if axis == 1 else (0, 2, 3, 1)
+
+            def transposes():
+                for _ in range(3):
+                    blk.permute(perm).contiguous()
+                vv = vec.permute(perm).contiguous()
+                vv.permute(back).contiguous()
+            t_tr = event_time(transposes, lambda: None, steps, s)
+            del blk
+        torch.cuda.empty_cache()
+        r = {"assemble_ms": float(np.median(t_as)), "solve_cyclic_ms": float(np.median(t_sv)),
+             "scatter_ms": float(np.median(t_sc)), "matrix_free_ms": float(np.median(t_mf)),
+             "reference_transposes_ms": float(np.median(t_tr)),
+             "matrix_free_vs_assembled_max_rel_diff": diff}
+        r["assembled_ms"] = r["assemble_ms"] + r["solve_cyclic_ms"] + r["scatter_ms"]
+        r["matrix_free_speedup_vs_assembled"] = r["assembled_ms"] / r["matrix_free_ms"]
+        r["matrix_free_GBs_traffic_model"] = 8.0 * 125 * nsys * nrow / (r["matrix_free_ms"] * 1e-3) / 1e9
+        out[f"sweep_{name}"] = r
+        tot["assembled"] += r["assembled_ms"]
+        tot["matrix_free"] += r["matrix_free_ms"]
+        tot["reference_transposes"] += r["reference_transposes_ms"]
+    out["adi_step_assembled_ms"] = tot["assembled"]
+    out["adi_step_matrix_free_ms"] = tot["matrix_free"]
+    out["adi_step_reference_style_estimate_ms"] = tot["assembled"] + tot["reference_transposes"]
+    out["matrix_free_speedup_vs_reference_style_estimate"] = \
+        out["adi_step_reference_style_estimate_ms"] / tot["matrix_free"]
+    out["cells_per_s_matrix_free_adi_step"] = out["cells"] / (tot["matrix_free"] * 1e-3)
+    out["traffic_model"] = ("matrix-free sweep: 125 doubles per block row (U 5, rhs 5, C'/E'/d' "
+                            "55 written and 55 read, x 5); assembled: assembly 90 + periodic solve "
+                            "255 + scatter 10")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,7 +569,8 @@ def main() -> int:
     ap.add_argument("--layout", default="tiled", choices=["tiled", "ref_gpu"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
-    ap.add_argument("--only", default="", help="comma list of extras to run (study,config0,batch,f32r,adjoint)")
+    ap.add_argument("--only", default="",
+                    help="comma list of extras to run (study,config0,batch,f32r,adjoint,euler)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="only copy one buffer of this many GiB to another (counter calibration)")
@@ -465,8 +594,8 @@ def main() -> int:
     import torch
 
     import pascal_btdma_amd as pb
-    from oracle import pyoracle
 
+    checks = ParityChecks()
     w = WORKLOADS[args.workload]
     m, n, nsys = w["m"], w["n"], w["nsys"]
     lay = pb.Layout.TILED if args.layout == "tiled" else pb.Layout.REF_GPU
@@ -517,14 +646,11 @@ def main() -> int:
         full = d.view(m, n, nsys)
         sys_ids = [s for t in tiles for s in range(t * 64, min(nsys, t * 64 + 64))]
         x = full[:, :, sys_ids].permute(1, 2, 0).cpu().numpy()
-    errs = []
+    parity = {"max_rel_err_vs_oracle": None}
     for lo in range(0, len(sys_ids), 64):
         s0 = sys_ids[lo]
-        A, B, C, dd = pb.generate_host(w["kind"], w["seed"], m, nsys, n, s0, 64)
-        xo, info = pyoracle.solve_plain(pyoracle.to_cpu_layout(A), pyoracle.to_cpu_layout(B),
-                                        pyoracle.to_cpu_layout(C), dd)
-        errs.append(rel_err_sys(x[:, lo:lo + 64], xo).max())
-    max_err = float(max(errs))
+        checks.add(lambda s0=s0: pb.generate_host(w["kind"], w["seed"], m, nsys, n, s0, 64),
+                   x[:, lo:lo + 64], max_sink(parity, "max_rel_err_vs_oracle"))
     plan.destroy()
     del a, b, c, d, c0, d0
     torch.cuda.empty_cache()
@@ -539,10 +665,11 @@ def main() -> int:
     if not args.no_extras:
         only = set(filter(None, args.only.split(",")))
         jobs = [("study", lambda: extra_study(dev)),
-                ("config0", lambda: extra_config0(dev)),
-                ("batch", lambda: extra_batch(dev, 3)),
-                ("f32r", lambda: extra_f32r(dev, 3, ms)),
-                ("adjoint", lambda: extra_adjoint(dev, 3))]
+                ("config0", lambda: extra_config0(dev, checks)),
+                ("batch", lambda: extra_batch(dev, 3, checks)),
+                ("f32r", lambda: extra_f32r(dev, 3, ms, checks)),
+                ("adjoint", lambda: extra_adjoint(dev, 3)),
+                ("euler", lambda: extra_euler(dev, 3))]
         for name, job in jobs:
             if only and name not in only:
                 continue
@@ -557,7 +684,8 @@ def main() -> int:
             print(f"[bench] extra {name} done in {extras[name]['bench_seconds']} s", file=sys.stderr,
                   flush=True)
 
-    cpu = None if args.no_cpu else cpu_baseline(w, args.cpu_seconds)
+    # the CPU leg: the oracle timed on the host and every recorded device result checked
+    cpu = cpu_leg(w, args.cpu_seconds, checks, baseline=not args.no_cpu)
     line = {
         "metric": METRIC, "value": value, "unit": "block-rows/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -568,7 +696,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
-        "max_rel_err_vs_oracle": max_err, "parity_systems": len(sys_ids),
+        "max_rel_err_vs_oracle": parity["max_rel_err_vs_oracle"], "parity_systems": len(sys_ids),
         "singular": singular, "ms_min": float(times.min()), "ms_max": float(times.max()),
         "wall_s_timed_region_incl_restore": wall,
         "measured_copy_GBs": copy_gbs,
