@@ -174,7 +174,8 @@ btd_status btd_euler_assemble(btd_plan_t plan, int axis, const double *U, const 
  * solution into the field out(n1, n2, n3, 5); rhs(n1, n2, n3, 5) is the right-hand side
  * field, and out may be rhs itself (the reference overwrites dU). npart must be 1 and
  * nrow >= 3. work: device scratch of 2*btd_layout_elems(layout, nsys, nrow, 25) +
- * btd_layout_elems(layout, nsys, nrow, 5) doubles. Zero pivots are recorded in the plan. */
+ * btd_layout_elems(layout, nsys, nrow, 5) + btd_layout_elems(layout, nsys, nrow + 2, 5)
+ * doubles. Zero pivots are recorded in the plan. */
 btd_status btd_euler_sweep_cyclic(btd_plan_t plan, int axis, const double *U, const double *rhs,
                                   int n1, int n2, int n3, double h, double dt, double eps,
                                   double *work, double *out, void *stream);

@@ -347,7 +347,8 @@ def euler_assemble(plan: BtdmaGpuPlan, axis: int, U, rhs, n1: int, n2: int, n3: 
 
 def euler_sweep_work_elems(plan: BtdmaGpuPlan) -> int:
     """Doubles of device scratch btd_euler_sweep_cyclic needs for this plan."""
-    return 2 * plan.matrix_elems() + plan.vector_elems()
+    return (2 * plan.matrix_elems() + plan.vector_elems()
+            + layout_elems(plan.layout, plan.nsys, plan.nrow + 2, plan.m))
 
 
 def euler_sweep_cyclic(plan: BtdmaGpuPlan, axis: int, U, rhs, n1: int, n2: int, n3: int, h: float,

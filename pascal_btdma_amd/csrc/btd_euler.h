@@ -177,6 +177,9 @@ __global__ __launch_bounds__(64) void k_euler_assemble(const double* __restrict_
 //   and one backward sweep x_q = d'_q + E'_q x_0 - C'_q x_{q+1} writes x into the field F.
 // C', E', d' of rows 1..n-2 go to the workspace (Wc, We, Wd, the plan's layout); each lane reads
 // its own field cells before writing them, so F may be the rhs field itself. Needs n >= 3.
+// LINES = true reads U and rhs from the sweep's lines (U with its two halo rows, rows 0..n+1, and
+// rhs rows 0..n-1, both in LAYOUT, E = 5; filled by k_xlines) and writes x to the rhs lines: a
+// lane reads each row of rhs / d' before it writes that row, so rhs, Wd and F may be one buffer.
 template <int AXIS>
 __device__ __forceinline__ void euler_row_blocks(const Prim& prev, const Prim& next, double cc,
                                                  double cd, double (&A)[5][5], double (&C)[5][5]) {
@@ -186,18 +189,18 @@ __device__ __forceinline__ void euler_row_blocks(const Prim& prev, const Prim& n
     euler_blocks(Jl, Jr, cc, cd, A, Bb, C);
 }
 
-template <int LAYOUT, int AXIS>
+template <int LAYOUT, int AXIS, bool LINES>
 __global__ __launch_bounds__(64) void k_euler_sweep_cyclic(const double* __restrict__ U,
                                                            const double* rhs, Grid g, double cc,
                                                            double cd, double* __restrict__ Wc,
-                                                           double* __restrict__ We,
-                                                           double* __restrict__ Wd, double* F,
-                                                           int64_t nsys, int64_t nrow,
+                                                           double* __restrict__ We, double* Wd,
+                                                           double* F, int64_t nsys, int64_t nrow,
                                                            int* __restrict__ flag) {
     constexpr int M = 5;
     const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (sys >= nsys) return;
     const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    const Addr<LAYOUT> au(sys, nsys, nrow + 2, M);  // LINES: U's line, rows 0..nrow+1
     __shared__ double sp[(M + 2 * M * M) * 64];  // alpha, beta, gamma: [element][lane]
     const int lane = threadIdx.x;
     auto SA = [&](int i) -> double& { return sp[i * 64 + lane]; };
@@ -212,16 +215,25 @@ __global__ __launch_bounds__(64) void k_euler_sweep_cyclic(const double* __restr
     sweep_cell<AXIS>(g, sys, 0, i0, j0, k0);
     const int64_t off0 = (i0 + 1) + (j0 + 1) * g.sj() + (k0 + 1) * g.sk();  // cell of row 0
     const int64_t foff0 = i0 + g.n1 * (j0 + g.n2 * k0);
+    // the cell of row q (q = -1..nrow) and element e of row q of rhs / x
+    auto prim = [&](int64_t q) -> Prim {
+        if constexpr (LINES) return load_prim(U, au.at(q + 1, 0), au.estride());
+        return load_prim(U, off0 + q * step, sv);
+    };
+    auto fo = [&](int64_t q, int e) -> int64_t {
+        if constexpr (LINES) return av.at(q, e);
+        return foff0 + q * fstep + e * fsv;
+    };
     bool ok = true;
     double Cp[M][M], Ep[M][M], dp[M];
-    Prim prev = load_prim(U, off0, sv), cur = load_prim(U, off0 + step, sv);  // cells 0, 1
+    Prim prev = prim(0), cur = prim(1);
     for (int64_t q = 1; q < nrow; ++q) {
         const bool last = q == nrow - 1;
-        const Prim next = load_prim(U, off0 + (q + 1) * step, sv);
+        const Prim next = prim(q + 1);
         double A[M][M], C[M][M], R[M][M], X[M][M], v[M];
         euler_row_blocks<AXIS>(prev, next, cc, cd, A, C);
 #pragma unroll
-        for (int e = 0; e < M; ++e) v[e] = rhs[foff0 + q * fstep + e * fsv];
+        for (int e = 0; e < M; ++e) v[e] = rhs[fo(q, e)];
 #pragma unroll
         for (int i = 0; i < M; ++i)
 #pragma unroll
@@ -310,10 +322,10 @@ __global__ __launch_bounds__(64) void k_euler_sweep_cyclic(const double* __restr
     double x0[M];
     {
         double A0[M][M], C0[M][M], R[M][M];
-        const Prim pl = load_prim(U, off0 - step, sv), pr = load_prim(U, off0 + step, sv);
+        const Prim pl = prim(-1), pr = prim(1);
         euler_row_blocks<AXIS>(pl, pr, cc, cd, A0, C0);
 #pragma unroll
-        for (int e = 0; e < M; ++e) x0[e] = rhs[foff0 + e * fsv];
+        for (int e = 0; e < M; ++e) x0[e] = rhs[fo(0, e)];
 #pragma unroll
         for (int i = 0; i < M; ++i) {
 #pragma unroll
@@ -341,7 +353,7 @@ __global__ __launch_bounds__(64) void k_euler_sweep_cyclic(const double* __restr
         xn[i] = s;
     }
 #pragma unroll
-    for (int e = 0; e < M; ++e) F[foff0 + (nrow - 1) * fstep + e * fsv] = xn[e];
+    for (int e = 0; e < M; ++e) F[fo(nrow - 1, e)] = xn[e];
     for (int64_t q = nrow - 2; q >= 1; --q) {
         double c[M][M], e_[M][M], d[M];
         load_mat_c<M>(Wc, am, q, c);
@@ -357,13 +369,77 @@ __global__ __launch_bounds__(64) void k_euler_sweep_cyclic(const double* __restr
         }
 #pragma unroll
         for (int e = 0; e < M; ++e) {
-            F[foff0 + q * fstep + e * fsv] = x[e];
+            F[fo(q, e)] = x[e];
             xn[e] = x[e];
         }
     }
 #pragma unroll
-    for (int e = 0; e < M; ++e) F[foff0 + e * fsv] = x0[e];
+    for (int e = 0; e < M; ++e) F[fo(0, e)] = x0[e];
     if (!ok) report_singular(flag, sys);
+}
+
+// x-line transposes. Along x the lines are the field's contiguous direction, so lanes that own
+// consecutive x lines touch a different cache line each and refetch it after eviction. These
+// kernels move a field to (TO_LINES) or from the x sweep's order in LAYOUT through 64 x 64 LDS
+// tiles: a 256-thread block owns 64 lines (one tile of systems) and 64 consecutive x positions,
+// reads each line's 64 doubles as one contiguous run and writes each row's 64 lanes as one
+// contiguous run. Field geometry: element (x, line, v) of the field is at
+//   x + hx + sj * (j + h) + sk * (k + h) + sv * v,   line = j + n2 * k,
+// with halo offsets hx (x direction) and h (j, k); the lines hold L positions per line.
+struct XField {
+    int64_t n2, sj, sk, sv;
+    int h, hx;
+};
+
+template <int LAYOUT, bool TO_LINES>
+__global__ __launch_bounds__(256) void k_xlines(double* __restrict__ F, double* __restrict__ Ln,
+                                                XField f, int nv, int64_t nsys, int64_t L) {
+    __shared__ double tile[64][65];
+    const int t = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t sys0 = (int64_t)blockIdx.x * 64;
+    const int64_t x0 = (int64_t)blockIdx.y * 64;
+    for (int v = 0; v < nv; ++v) {
+        if constexpr (TO_LINES) {
+            // read: lane t = x position, 16 lines per wave
+            for (int l = w; l < 64; l += 4) {
+                const int64_t sys = sys0 + l, x = x0 + t;
+                double val = 0.0;
+                if (sys < nsys && x < L) {
+                    const int64_t j = sys % f.n2, k = sys / f.n2;
+                    val = F[x + f.hx + f.sj * (j + f.h) + f.sk * (k + f.h) + f.sv * v];
+                }
+                tile[t][l] = val;
+            }
+            __syncthreads();
+            // write: lane t = line, 16 x positions per wave
+            for (int xx = w; xx < 64; xx += 4) {
+                const int64_t x = x0 + xx, sys = sys0 + t;
+                if (x < L && sys < nsys) {
+                    const Addr<LAYOUT> a(sys, nsys, L, nv);
+                    Ln[a.at(x, v)] = tile[xx][t];
+                }
+            }
+        } else {
+            for (int xx = w; xx < 64; xx += 4) {
+                const int64_t x = x0 + xx, sys = sys0 + t;
+                double val = 0.0;
+                if (x < L && sys < nsys) {
+                    const Addr<LAYOUT> a(sys, nsys, L, nv);
+                    val = Ln[a.at(x, v)];
+                }
+                tile[xx][t] = val;
+            }
+            __syncthreads();
+            for (int l = w; l < 64; l += 4) {
+                const int64_t sys = sys0 + l, x = x0 + t;
+                if (sys < nsys && x < L) {
+                    const int64_t j = sys % f.n2, k = sys / f.n2;
+                    F[x + f.hx + f.sj * (j + f.h) + f.sk * (k + f.h) + f.sv * v] = tile[t][l];
+                }
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // Field F(n1, n2, n3, m) <-> sweep vector in LAYOUT (SCATTER = false: gather into D,

@@ -45,22 +45,26 @@ def test_assembly_is_bit_exact(cuda, grid, axis, layout):
     np.testing.assert_array_equal(layout_to_vec(d.cpu().numpy(), lay, nrow, nsys, 5), dref)
 
 
+@pytest.mark.parametrize("grid", [(9, 70, 5), (130, 9, 8)], ids=lambda g: "x".join(map(str, g)))
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("m", [3, 5])
-def test_gather_scatter(cuda, axis, m):
+def test_gather_scatter(cuda, grid, axis, m):
+    """Along x the move goes through the LDS-tiled line transpose (several x tiles and system
+    tiles in the second grid)."""
     import torch
-    n1, n2, n3 = 9, 70, 5
+    n1, n2, n3 = grid
     F = np.random.default_rng(axis).standard_normal((m, n3, n2, n1))
     nrow, nsys = pb.sweep_shape(axis, n1, n2, n3)
-    with pb.BtdmaGpuPlan().create(m, nsys, nrow, 1, pb.Layout.TILED) as plan:
-        d = torch.zeros(plan.vector_elems(), dtype=torch.float64, device=cuda)
-        back = torch.zeros(F.size, dtype=torch.float64, device=cuda)
-        pb.sweep_gather(plan, axis, upload(cuda, F), n1, n2, n3, d)
-        pb.sweep_scatter(plan, axis, d, n1, n2, n3, back)
-        torch.cuda.synchronize()
-    np.testing.assert_array_equal(layout_to_vec(d.cpu().numpy(), 2, nrow, nsys, m),
-                                  ea.to_sweep(np.moveaxis(F, 0, -1), axis))
-    np.testing.assert_array_equal(back.cpu().numpy().reshape(F.shape), F)
+    for layout in (pb.Layout.TILED, pb.Layout.REF_GPU):
+        with pb.BtdmaGpuPlan().create(m, nsys, nrow, 1, layout) as plan:
+            d = torch.zeros(plan.vector_elems(), dtype=torch.float64, device=cuda)
+            back = torch.zeros(F.size, dtype=torch.float64, device=cuda)
+            pb.sweep_gather(plan, axis, upload(cuda, F), n1, n2, n3, d)
+            pb.sweep_scatter(plan, axis, d, n1, n2, n3, back)
+            torch.cuda.synchronize()
+        np.testing.assert_array_equal(layout_to_vec(d.cpu().numpy(), int(layout), nrow, nsys, m),
+                                      ea.to_sweep(np.moveaxis(F, 0, -1), axis))
+        np.testing.assert_array_equal(back.cpu().numpy().reshape(F.shape), F)
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
@@ -117,7 +121,8 @@ def test_sweep_argument_errors(cuda):
         torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("grid", [(12, 10, 9), (3, 70, 4), (5, 3, 66)], ids=lambda g: "x".join(map(str, g)))
+@pytest.mark.parametrize("grid", [(12, 10, 9), (3, 70, 4), (5, 3, 66), (130, 9, 8)],
+                         ids=lambda g: "x".join(map(str, g)))
 @pytest.mark.parametrize("axis", [0, 1, 2])
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
 def test_matrix_free_sweep_matches_oracle(cuda, oracle, grid, axis, layout):
