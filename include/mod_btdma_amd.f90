@@ -47,10 +47,30 @@ module mod_btdma_amd
         type(c_ptr) function btd_last_error() bind(C)
             import :: c_ptr
         end function
+        integer(c_int) function btd_euler_assemble(plan, axis, U, rhs, n1, n2, n3, h, dt, eps, a, b, c, d, &
+                                                   stream) bind(C)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: plan, U, rhs, a, b, c, d, stream
+            integer(c_int), value :: axis, n1, n2, n3
+            real(c_double), value :: h, dt, eps
+        end function
+        integer(c_int) function btd_sweep_scatter(plan, axis, d, n1, n2, n3, field, stream) bind(C)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: plan, d, field, stream
+            integer(c_int), value :: axis, n1, n2, n3
+        end function
+        integer(c_int) function btd_euler_sweep_cyclic(plan, axis, U, rhs, n1, n2, n3, h, dt, eps, work, out, &
+                                                       stream) bind(C)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: plan, U, rhs, work, out, stream
+            integer(c_int), value :: axis, n1, n2, n3
+            real(c_double), value :: h, dt, eps
+        end function
     end interface
 
     public :: btdma_makeplan_amd, btdma_many_amd, btdma_many_amd_profiled
     public :: btdma_many_cycl_amd, btdma_cleanplan_amd
+    public :: btdma_euler_assemble_amd, btdma_sweep_scatter_amd, btdma_euler_sweep_cyclic_amd
 
 contains
 
@@ -91,6 +111,38 @@ contains
         type(c_ptr), intent(in) :: A, B, C, D, plan
         integer, intent(in) :: m, nsys, nrow_sub
         call check(btd_solve_cyclic(plan, A, B, C, D, m, nsys, nrow_sub, c_null_ptr), 'btdma_many_cycl_amd')
+    end subroutine
+
+    ! euler3d sweeps (axis 0 = x, 1 = y, 2 = z; the plan is the sweep's: nsys = the product of the other
+    ! two grid sizes, nrow = the axis' size). U(0:n1+1,0:n2+1,0:n3+1,5) with its halo filled and
+    ! dU(n1,n2,n3,5) are device fields in the reference's order.
+    ! build_matrix_{x,y,z}_kernel plus the transposes (euler3d_mpi_gpu.f90:803-1002): the sweep's
+    ! A, B, C and D = dU reordered, in the plan's layout
+    subroutine btdma_euler_assemble_amd(plan, axis, U, dU, n1, n2, n3, h, dt, eps, A, B, C, D)
+        type(c_ptr), intent(in) :: plan, U, dU, A, B, C, D
+        integer, intent(in) :: axis, n1, n2, n3
+        real(c_double), intent(in) :: h, dt, eps
+        call check(btd_euler_assemble(plan, axis, U, dU, n1, n2, n3, h, dt, eps, A, B, C, D, c_null_ptr), &
+                   'btdma_euler_assemble_amd')
+    end subroutine
+
+    ! transpose_kj_kernel / transpose_ki_kernel (euler3d_mpi_gpu.f90:959-974, 1004-1017): the sweep's
+    ! solution D back into the field dU
+    subroutine btdma_sweep_scatter_amd(plan, axis, D, n1, n2, n3, dU)
+        type(c_ptr), intent(in) :: plan, D, dU
+        integer, intent(in) :: axis, n1, n2, n3
+        call check(btd_sweep_scatter(plan, axis, D, n1, n2, n3, dU, c_null_ptr), 'btdma_sweep_scatter_amd')
+    end subroutine
+
+    ! the whole periodic sweep without storing A, B, C; the solution replaces dU when out = dU.
+    ! work: device scratch of 2*nm + nv + nu doubles (nm = nsys*nrow*25, nv = nsys*nrow*5,
+    ! nu = nsys*(nrow+2)*5 in the reference layout)
+    subroutine btdma_euler_sweep_cyclic_amd(plan, axis, U, dU, n1, n2, n3, h, dt, eps, work, out)
+        type(c_ptr), intent(in) :: plan, U, dU, work, out
+        integer, intent(in) :: axis, n1, n2, n3
+        real(c_double), intent(in) :: h, dt, eps
+        call check(btd_euler_sweep_cyclic(plan, axis, U, dU, n1, n2, n3, h, dt, eps, work, out, c_null_ptr), &
+                   'btdma_euler_sweep_cyclic_amd')
     end subroutine
 
     ! btdma_cleanplan_gpu_v2(plan) (mod_btdma_gpu_v2.f90:111)
