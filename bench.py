@@ -363,6 +363,36 @@ def extra_f32r(dev, steps: int, fp64_ms: float, checks: ParityChecks) -> dict:
             r = out[f"sweeps{sweeps}"] = {"ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3),
                                           "speedup_vs_fp64_plain": fp64_ms / ms}
             checks.add(sample, xs, error_sink(r))
+        # FP32 storage with FP64 arithmetic: no refinement, the FP64 solve of the stored values
+        work64 = torch.empty(plan.matrix_elems(), dtype=torch.float64, device=dev)
+        xs_ = torch.empty_like(d)
+        t = event_time(lambda: pb.solve_noncyclic_f32s(plan, a, b, c, xs_, work64),
+                       lambda: xs_.copy_(d), steps, torch.cuda.current_stream())
+        ms = float(np.median(t))
+        r = out["fp32_storage_fp64_arithmetic"] = {
+            "ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3), "speedup_vs_fp64_plain": fp64_ms / ms,
+            "bytes_per_row_model": 28 * m * m + 32 * m}
+        xs = tiled_vec_subset(xs_[: 2 * n * m * 64].cpu().numpy(), n, m, [0, 1])
+        checks.add(sample, xs, error_sink(r))
+        del work64, xs_
+        # the same mode on the headline's well-conditioned Euler-like blocks rounded to FP32
+        pb.generate_f32(pb.GenKind.EULER, 1, m, nsys, n, lay, a, b, c, d)
+        work64 = torch.empty(plan.matrix_elems(), dtype=torch.float64, device=dev)
+        xs_ = torch.empty_like(d)
+        t = event_time(lambda: pb.solve_noncyclic_f32s(plan, a, b, c, xs_, work64),
+                       lambda: xs_.copy_(d), steps, torch.cuda.current_stream())
+        ms = float(np.median(t))
+        r = out["fp32_storage_fp64_arithmetic_euler_blocks"] = {
+            "workload": "the headline's Euler-like m=5, n=512, 131072 systems with A, B, C rounded to FP32",
+            "ms": ms, "speedup_vs_fp64_plain": fp64_ms / ms}
+
+        def euler32():
+            A_, B_, C_, d_ = pb.generate_host(pb.GenKind.EULER, 1, m, nsys, n, 0, 128)
+            return (*(X.astype(np.float32).astype(np.float64) for X in (A_, B_, C_)), d_)
+        xs = tiled_vec_subset(xs_[: 2 * n * m * 64].cpu().numpy(), n, m, [0, 1])
+        checks.add(euler32, xs, error_sink(r))
+        del work64, xs_
+        pb.generate_f32(pb.GenKind.COND, 3, m, nsys, n, lay, a, b, c, d)
         # the FP64 plain kernel on the same (FP32-rounded) inputs: the two-FP64-solver floor
         a64, b64, c64 = a.double(), b.double(), c.double()
         d64 = d.clone()

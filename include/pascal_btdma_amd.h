@@ -135,6 +135,14 @@ btd_status btd_solve_noncyclic_f32r(btd_plan_t plan, const float *a, const float
                                     const double *d, double *x, float *work, int sweeps,
                                     void *stream);
 
+/* FP32-stored blocks, FP64 arithmetic (no counterpart in the reference): the FP64 solve of the
+ * blocks' values, bit for bit what btd_solve_noncyclic computes on FP64 copies of a, b, c,
+ * reading 4 bytes per block element. a, b, c are only read; d is in place (x returned in d);
+ * work: btd_layout_elems(layout, nsys, nrow, m*m) doubles of device scratch (holds C'). npart
+ * must be 1. */
+btd_status btd_solve_noncyclic_f32s(btd_plan_t plan, const float *a, const float *b, const float *c,
+                                    double *d, double *work, void *stream);
+
 /* Adjoint of btd_solve_noncyclic_f32r (no counterpart in the reference): lambda solves the
  * transposed system T^T lambda = w from the same FP32-stored blocks a, b, c (as given for
  * T), by an FP32 solve plus `sweeps` refinement sweeps with FP64 residuals. Same workspace

@@ -22,7 +22,7 @@ __all__ = [
     "SingularError", "solve_noncyclic", "solve_noncyclic_profiled", "solve_cyclic",
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
     "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
-    "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "solve_transposed_f32r",
+    "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "solve_transposed_f32r", "solve_noncyclic_f32s",
     "outer_grads_f32", "euler_assemble", "euler_sweep_cyclic",
     "euler_sweep_work_elems", "sweep_gather", "sweep_scatter", "sweep_shape", "BtdmaBatch", "Graph",
 ]
@@ -280,6 +280,21 @@ def solve_noncyclic_f32r(plan: BtdmaGpuPlan, a, b, c, d, x, work, sweeps: int = 
             _check_tensor(name, t, n, dt)
     _check(_capi.lib().btd_solve_noncyclic_f32r(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
                                                 _ptr(x), _ptr(work), int(sweeps), _stream(stream)))
+
+
+def solve_noncyclic_f32s(plan: BtdmaGpuPlan, a, b, c, d, work, stream=None) -> None:
+    """FP32-stored blocks (a, b, c float32, only read), FP64 arithmetic: d (float64) is
+    replaced by x, the FP64 solve of the blocks' values; work holds C' (plan.matrix_elems()
+    float64)."""
+    import torch
+    if plan.created:
+        nm, nv = plan.matrix_elems(), plan.vector_elems()
+        for name, t, n, dt in (("a", a, nm, torch.float32), ("b", b, nm, torch.float32),
+                               ("c", c, nm, torch.float32), ("d", d, nv, torch.float64),
+                               ("work", work, nm, torch.float64)):
+            _check_tensor(name, t, n, dt)
+    _check(_capi.lib().btd_solve_noncyclic_f32s(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
+                                                _ptr(work), _stream(stream)))
 
 
 def solve_transposed_f32r(plan: BtdmaGpuPlan, a, b, c, w, lam, work, sweeps: int = 2,

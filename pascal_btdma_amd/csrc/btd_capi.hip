@@ -386,6 +386,19 @@ btd_status btd_solve_noncyclic_f32r(btd_plan_t plan, const float* a, const float
     return BTD_OK;
 }
 
+btd_status btd_solve_noncyclic_f32s(btd_plan_t plan, const float* a, const float* b, const float* c,
+                                    double* d, double* work, void* stream) {
+    g_err.clear();
+    btd_status st = check_single_partition(plan, "btd_solve_noncyclic_f32s");
+    if (st != BTD_OK) return st;
+    if (!a || !b || !c || !d || !work)
+        return fail(BTD_ERR_INVALID_ARG, "btd_solve_noncyclic_f32s: NULL array");
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_plain_f32s(plan->m, plan->layout, a, b, c, work, d, plan->nsys, plan->nrow,
+                                   plan->d_flag, (hipStream_t)stream));
+    return BTD_OK;
+}
+
 btd_status btd_solve_transposed_f32r(btd_plan_t plan, const float* a, const float* b, const float* c,
                                      const double* w, double* lambda, float* work, int sweeps,
                                      void* stream) {

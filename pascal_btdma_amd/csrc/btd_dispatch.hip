@@ -103,6 +103,8 @@ hipError_t launch_f32r_1(int, const float*, const float*, const float*, const do
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_1(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_1(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 hipError_t launch_plain_save_2(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -113,6 +115,8 @@ hipError_t launch_f32r_2(int, const float*, const float*, const float*, const do
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_2(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_2(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 hipError_t launch_plain_save_3(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -123,6 +127,8 @@ hipError_t launch_f32r_3(int, const float*, const float*, const float*, const do
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_3(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_3(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 hipError_t launch_plain_save_4(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -133,6 +139,8 @@ hipError_t launch_f32r_4(int, const float*, const float*, const float*, const do
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_4(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_4(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 hipError_t launch_plain_save_5(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -143,6 +151,8 @@ hipError_t launch_f32r_5(int, const float*, const float*, const float*, const do
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_5(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_5(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 hipError_t launch_plain_save_6(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -153,6 +163,8 @@ hipError_t launch_f32r_6(int, const float*, const float*, const float*, const do
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_6(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_6(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 hipError_t launch_plain_save_7(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -163,6 +175,8 @@ hipError_t launch_f32r_7(int, const float*, const float*, const float*, const do
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_7(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_7(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 hipError_t launch_plain_save_8(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -173,6 +187,8 @@ hipError_t launch_f32r_8(int, const float*, const float*, const float*, const do
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_8(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_8(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 hipError_t launch_plain_save_9(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -183,6 +199,8 @@ hipError_t launch_f32r_9(int, const float*, const float*, const float*, const do
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_9(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_9(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 hipError_t launch_plain_save_10(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -193,6 +211,8 @@ hipError_t launch_f32r_10(int, const float*, const float*, const float*, const d
                          float*, float*, int, int, int, int*, hipStream_t, bool);
 hipError_t launch_outer_f32_10(int, const double*, const double*, float*, float*, float*, int, int,
                               hipStream_t);
+hipError_t launch_plain_f32s_10(int, const float*, const float*, const float*, double*, double*, int,
+                               int, int*, hipStream_t);
 
 // ---------------------------------------------------------------------------- convert --
 __global__ __launch_bounds__(256) void k_convert(int src_layout, int dst_layout, int E, int nsys,
@@ -275,6 +295,13 @@ hipError_t launch_f32r(int m, int layout, const float* A, const float* B, const 
 hipError_t launch_cyclic(int m, int layout, double* A, const double* B, double* C, double* D,
                          int nsys, int nrow, int* flag, hipStream_t s) {
 #define CALL(M) launch_cyclic_##M(layout, A, B, C, D, nsys, nrow, flag, s)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_plain_f32s(int m, int layout, const float* A, const float* B, const float* C,
+                             double* Cw, double* D, int nsys, int nrow, int* flag, hipStream_t s) {
+#define CALL(M) launch_plain_f32s_##M(layout, A, B, C, Cw, D, nsys, nrow, flag, s)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }

@@ -37,6 +37,9 @@ hipError_t launch_euler_sweep_cyclic(int layout, int axis, const double* U, cons
 hipError_t launch_sweep_move(int layout, int axis, double* F, double* D, int n1, int n2, int n3,
                              int m, int nsys, int nrow, bool scatter, hipStream_t s);
 
+hipError_t launch_plain_f32s(int m, int layout, const float* A, const float* B, const float* C,
+                             double* Cw, double* D, int nsys, int nrow, int* flag, hipStream_t s);
+
 hipError_t launch_outer_f32(int m, int layout, const double* Lam, const double* X, float* dA,
                             float* dB, float* dC, int nsys, int nrow, hipStream_t s);
 

@@ -46,10 +46,11 @@ struct Addr {
 };
 
 // ----------------------------------------------------------------------- row helpers --
-template <int M, int LAYOUT>
-__device__ __forceinline__ void load_mat(const double* __restrict__ p, const Addr<LAYOUT>& a,
+// Block loads take FP64 or FP32 storage (T) and return FP64 values.
+template <int M, int LAYOUT, typename T>
+__device__ __forceinline__ void load_mat(const T* __restrict__ p, const Addr<LAYOUT>& a,
                                          int64_t row, double (&X)[M][M]) {
-    const double* r = p + a.at(row, 0);
+    const T* r = p + a.at(row, 0);
     const int64_t es = a.estride();
 #pragma unroll
     for (int j = 0; j < M; ++j)
@@ -57,10 +58,10 @@ __device__ __forceinline__ void load_mat(const double* __restrict__ p, const Add
         for (int i = 0; i < M; ++i) X[i][j] = __builtin_nontemporal_load(r + (i + M * j) * es);
 }
 
-template <int M, int LAYOUT>
-__device__ __forceinline__ void load_col(const double* p, const Addr<LAYOUT>& a, int64_t row,
+template <int M, int LAYOUT, typename T>
+__device__ __forceinline__ void load_col(const T* p, const Addr<LAYOUT>& a, int64_t row,
                                          int j, double (&x)[M]) {
-    const double* r = p + a.at(row, 0);
+    const T* r = p + a.at(row, 0);
     const int64_t es = a.estride();
 #pragma unroll
     for (int i = 0; i < M; ++i) x[i] = r[(i + M * j) * es];
@@ -94,8 +95,8 @@ __device__ __forceinline__ void store_vec(double* p, const Addr<LAYOUT>& a, int6
 }
 
 // R <- R - A*Y and v <- v - A*y, A streamed by columns.
-template <int M, int LAYOUT>
-__device__ __forceinline__ void gemm_sub(const double* __restrict__ Ap, const Addr<LAYOUT>& am,
+template <int M, int LAYOUT, typename T>
+__device__ __forceinline__ void gemm_sub(const T* __restrict__ Ap, const Addr<LAYOUT>& am,
                                          int64_t row, const double (&Y)[M][M], const double (&y)[M],
                                          double (&R)[M][M], double (&v)[M]) {
 #pragma unroll
@@ -199,9 +200,11 @@ __device__ __forceinline__ void report_singular(int* flag, int64_t sys) {
     flag[1 + sys] = 1;
 }
 
-template <int M, int LAYOUT, bool SAVE>
-__device__ __forceinline__ void plain_solve(const double* __restrict__ A,
-                                            const double* __restrict__ B, double* C, double* D,
+// Blocks stored as TB (double, or float for the FP32-storage solve, which computes in FP64);
+// C'_q goes to Cw (C itself for the in-place FP64 solve).
+template <int M, int LAYOUT, bool SAVE, typename TB>
+__device__ __forceinline__ void plain_solve(const TB* __restrict__ A, const TB* __restrict__ B,
+                                            const TB* C, double* Cw, double* D,
                                             double* __restrict__ F, uint64_t* __restrict__ PV,
                                             int nsys, int nrow, int* __restrict__ flag) {
     const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
@@ -236,7 +239,7 @@ __device__ __forceinline__ void plain_solve(const double* __restrict__ A,
                 lu_solve<M>(R, piv, c);
 #pragma unroll
                 for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
-                store_col<M>(C, am, q, j, c);
+                store_col<M>(Cw, am, q, j, c);
             }
         }
     }
@@ -244,7 +247,7 @@ __device__ __forceinline__ void plain_solve(const double* __restrict__ A,
     for (int64_t q = nrow - 2; q >= 0; --q) {
         double x[M], c[M][M];
         load_vec<M>(D, av, q, x);
-        load_mat<M>(C, am, q, c);
+        load_mat<M>(static_cast<const double*>(Cw), am, q, c);
 #pragma unroll
         for (int i = 0; i < M; ++i)
 #pragma unroll
@@ -260,7 +263,18 @@ template <int M, int LAYOUT>
 __global__ __launch_bounds__(64) void k_plain(const double* __restrict__ A,
                                               const double* __restrict__ B, double* C, double* D,
                                               int nsys, int nrow, int* __restrict__ flag) {
-    plain_solve<M, LAYOUT, false>(A, B, C, D, nullptr, nullptr, nsys, nrow, flag);
+    plain_solve<M, LAYOUT, false, double>(A, B, C, C, D, nullptr, nullptr, nsys, nrow, flag);
+}
+
+// FP32-stored blocks, FP64 arithmetic: the FP64 solve of the blocks' values (bit for bit what
+// k_plain computes on their FP64 copies), reading 4 bytes per block element; C' (FP64) goes to
+// the workspace Cw, x replaces D.
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_plain_f32s(const float* __restrict__ A,
+                                                   const float* __restrict__ B, const float* C,
+                                                   double* Cw, double* D, int nsys, int nrow,
+                                                   int* __restrict__ flag) {
+    plain_solve<M, LAYOUT, false, float>(A, B, C, Cw, D, nullptr, nullptr, nsys, nrow, flag);
 }
 
 template <int M, int LAYOUT>
@@ -269,7 +283,7 @@ __global__ __launch_bounds__(64) void k_plain_save(const double* __restrict__ A,
                                                    double* D, double* __restrict__ F,
                                                    uint64_t* __restrict__ PV, int nsys, int nrow,
                                                    int* __restrict__ flag) {
-    plain_solve<M, LAYOUT, true>(A, B, C, D, F, PV, nsys, nrow, flag);
+    plain_solve<M, LAYOUT, true, double>(A, B, C, C, D, F, PV, nsys, nrow, flag);
 }
 
 // --------------------------------------------------------------------------- adjoint --
@@ -1116,6 +1130,18 @@ hipError_t launch_adjoint_m(int layout, const double* A, const double* Cp, const
     else
         hipLaunchKernelGGL((k_adjoint<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, Cp, F, PV, X, W, Lam,
                            dA, dB, dC, nsys, nrow);
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_plain_f32s_m(int layout, const float* A, const float* B, const float* C, double* Cw,
+                               double* D, int nsys, int nrow, int* flag, hipStream_t s) {
+    const dim3 grid((nsys + 63) / 64), block(64);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_plain_f32s<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, Cw, D, nsys, nrow, flag);
+    else
+        hipLaunchKernelGGL((k_plain_f32s<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, Cw, D, nsys, nrow,
+                           flag);
     return hipGetLastError();
 }
 

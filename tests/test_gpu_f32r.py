@@ -78,3 +78,65 @@ def test_f32r_rejects_bad_arguments(cuda):
             pb._check(pb._capi.lib().btd_solve_noncyclic_f32r(plan._h, t.data_ptr(), t.data_ptr(),
                                                               t.data_ptr(), t.data_ptr(), t.data_ptr(),
                                                               t.data_ptr(), 9, None))
+
+
+@pytest.mark.parametrize("m", list(range(1, 11)))
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_fp32_storage_is_the_fp64_solve_of_the_stored_values(cuda, m, layout):
+    """btd_solve_noncyclic_f32s (FP32 blocks, FP64 arithmetic) equals, bit for bit, the FP64
+    solve btd_solve_noncyclic of the same blocks converted to FP64; the blocks are left as
+    they were."""
+    import torch
+    nsys, n = 70, 13
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 300 + m, m, nsys, n)
+    A, B, C = fp32_round(A, B, C)
+    lay = int(layout)
+    f32 = lambda X: torch.from_numpy(nat_to_layout(X, lay).astype(np.float32)).to(cuda)
+    ta, tb, tc = f32(A), f32(B), f32(C)
+    td = torch.from_numpy(nat_to_layout(d, lay)).to(cuda)
+    keep = [t.clone() for t in (ta, tb, tc)]
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
+        work = torch.empty(plan.matrix_elems(), dtype=torch.float64, device=cuda)
+        x32 = td.clone()
+        pb.solve_noncyclic_f32s(plan, ta, tb, tc, x32, work)
+        a64, b64, c64, x64 = ta.double(), tb.double(), tc.double(), td.clone()
+        pb.solve_noncyclic(plan, a64, b64, c64, x64, m, nsys, n)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    assert torch.equal(x32, x64)
+    for t, k in zip((ta, tb, tc), keep):
+        assert torch.equal(t, k)
+
+
+def test_fp32_storage_on_conditioned_blocks(cuda, oracle):
+    """cond(B_q) log-uniform in [1e1, 1e6]: the FP32-storage solve needs no refinement; it is the
+    FP64 solve of the stored values, within the FP64 floor of the oracle on those values."""
+    import torch
+    m, nsys, n = 5, 128, 64
+    A, B, C, d = pb.generate_host(pb.GenKind.COND, 9, m, nsys, n)
+    xo, info = oracle.solve_plain(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    x64, _ = gpu_solve(A, B, C, d, int(pb.Layout.TILED))
+    lay = int(pb.Layout.TILED)
+    f32 = lambda X: torch.from_numpy(nat_to_layout(X, lay).astype(np.float32)).to(cuda)
+    ta, tb, tc = f32(A), f32(B), f32(C)
+    td = torch.from_numpy(nat_to_layout(d, lay)).to(cuda)
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, pb.Layout.TILED) as plan:
+        work = torch.empty(plan.matrix_elems(), dtype=torch.float64, device=cuda)
+        pb.solve_noncyclic_f32s(plan, ta, tb, tc, td, work)
+        torch.cuda.synchronize()
+    x = layout_to_vec(td.cpu().numpy(), lay, n, nsys, m)
+    np.testing.assert_array_equal(x, x64)
+    err = rel_err(x, xo)
+    print(f"FP32 storage, FP64 arithmetic, cond up to 1e6: max rel err vs oracle {err.max():.2e}")
+    assert err.max() <= 1e-9  # the two FP64 solvers' floor at cond 1e6 (6.5e-11 measured)
+
+
+def test_fp32_storage_rejects_partitions(cuda):
+    import torch
+    t = torch.zeros(10, dtype=torch.float32, device=cuda)
+    with pb.BtdmaGpuPlan().create(3, 64, 8, 2, pb.Layout.TILED) as plan:
+        with pytest.raises(ValueError):
+            pb._check(pb._capi.lib().btd_solve_noncyclic_f32s(plan._h, t.data_ptr(), t.data_ptr(),
+                                                              t.data_ptr(), t.data_ptr(), t.data_ptr(),
+                                                              None))
