@@ -188,7 +188,17 @@ def extra_study(dev) -> dict:
                 t = event_time(lambda: pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n), restore, 5, s)
                 res[P] = float(np.median(t))
         best = min(res, key=res.get)
+        with pb.BtdmaGpuPlan().create(m, nsys, n, 0, lay) as plan:
+            def restore():
+                a.copy_(a0)
+                c.copy_(c0)
+                d.copy_(d0)
+            t = event_time(lambda: pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n), restore, 5,
+                           torch.cuda.current_stream())
+            auto_np, ms_auto = plan.npart, float(np.median(t))
         out[f"m{m}"] = {"ms_npart1": res[1], "best_npart": best, "ms_best": res[best],
+                        "auto_npart": auto_np, "ms_auto_npart": ms_auto,
+                        "speedup_auto_npart": h200 / ms_auto,
                         "h200_ms_published": h200, "speedup_npart1": h200 / res[1],
                         "speedup_best": h200 / res[best],
                         "block_rows_per_s_best": nsys * n / (res[best] * 1e-3)}
@@ -218,7 +228,19 @@ def extra_config0(dev, checks: ParityChecks) -> dict:
                        torch.cuda.current_stream())
     x = tiled_vec_subset(d.cpu().numpy(), n, m, list(range(nsys // 64)))
     ms = float(np.median(t))
+    # the same batch with the partition count chosen by the library (npart = 0)
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 0, lay) as plan:
+        a0 = a.clone()
+
+        def restore_a():
+            a.copy_(a0)
+            c.copy_(c0)
+            d.copy_(d0)
+        ta = event_time(lambda: pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n), restore_a, 10,
+                        torch.cuda.current_stream())
+        auto_np = plan.npart
     out = {"workload": w["name"], "ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3),
+           "auto_npart": auto_np, "ms_auto_npart": float(np.median(ta)),
            "max_rel_err_vs_oracle_all_systems": None}
     checks.add(lambda: pb.generate_host(w["kind"], w["seed"], m, nsys, n), x,
                max_sink(out, "max_rel_err_vs_oracle_all_systems"))

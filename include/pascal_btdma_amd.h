@@ -72,10 +72,20 @@ typedef struct btd_timings {
  * pascal_btdma_cuda.cu:956-1005) and btdma_makeplan_gpu_v2(plan, m, nsys, nrow_sub, comm)
  * (mod_btdma_gpu_v2.f90:70-109). `npart` takes the place of the communicator size:
  * npart = 1 is the single-partition solve; npart > 1 splits every system's rows with
- * btd_partition_1d and needs nrow >= 3*npart. layout is BTD_LAYOUT_REF_GPU or
- * BTD_LAYOUT_TILED. 1 <= m <= BTD_MAX_M, nsys >= 1, nrow >= 2. Allocates device scratch
+ * btd_partition_1d and needs nrow >= 3*npart; npart = 0 lets the library choose
+ * (btd_auto_npart; read the choice back with btd_plan_npart). layout is BTD_LAYOUT_REF_GPU
+ * or BTD_LAYOUT_TILED. 1 <= m <= BTD_MAX_M, nsys >= 1, nrow >= 2. Allocates device scratch
  * on the current device. */
 btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npart, int layout);
+
+/* New: the partition count a plan created with npart = 0 gets on one GPU, where partitions are
+ * an occupancy choice rather than a rank count: 1 (the single-kernel solve) for nsys >= 32768,
+ * otherwise about 65536 / nsys (one wave per SIMD), at most nrow / 8, and 1 when that is < 2.
+ * Pure function (no device use). */
+int btd_auto_npart(int m, int nsys, int nrow);
+
+/* The plan's partition count (the one chosen when it was created with npart = 0). */
+btd_status btd_plan_npart(btd_plan_t plan, int *npart);
 
 /* Replaces BtdmaGpuPlan::destroy (pascal_btdma_cuda.hpp:134, pascal_btdma_cuda.cu:1007-1033)
  * and btdma_cleanplan_gpu_v2 (mod_btdma_gpu_v2.f90:111-129). NULL is accepted. */

@@ -146,6 +146,10 @@ class BtdmaGpuPlan:
         _check(_capi.lib().btd_plan_create(ctypes.byref(h), int(m), int(nsys), int(nrow),
                                            int(npart), int(layout)))
         self._h = h
+        if int(npart) == 0:  # chosen by the library
+            p = ctypes.c_int(0)
+            _check(_capi.lib().btd_plan_npart(h, ctypes.byref(p)))
+            npart = p.value
         self.m, self.nsys, self.nrow, self.npart = int(m), int(nsys), int(nrow), int(npart)
         self.layout = Layout(layout)
         self.created = True

@@ -45,6 +45,8 @@ SIGNATURES = {
     "btd_sweep_gather": (_i, [_vp, _i, _dp, _i, _i, _i, _dp, _vp]),
     "btd_sweep_scatter": (_i, [_vp, _i, _dp, _i, _i, _i, _dp, _vp]),
     "btd_plan_check": (_i, [_vp, ctypes.POINTER(_i)]),
+    "btd_auto_npart": (_i, [_i, _i, _i]),
+    "btd_plan_npart": (_i, [_vp, ctypes.POINTER(_i)]),
     "btd_plan_info": (_i, [_vp, _vp]),
     "btd_last_error": (ctypes.c_char_p, []),
     "btd_matrix_index": (ctypes.c_size_t, [_i, _i, _i, _i, _i, _i, _i]),

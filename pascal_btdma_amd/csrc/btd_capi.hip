@@ -87,6 +87,21 @@ size_t btd_layout_elems(int layout, int nsys, int nrow, int elems_per_block) {
     return (size_t)btd::layout_elems(layout, nsys, nrow, elems_per_block);
 }
 
+int btd_auto_npart(int m, int nsys, int nrow) {
+    // One GPU: partitions are an occupancy choice. A batch of at least half a wave per SIMD
+    // (32768 systems) is solved by the single-kernel solve; a smaller one gets enough
+    // partitions for about one wave per SIMD (65536 lanes), keeping >= 8 rows per partition
+    // so that the reduced system of 2P rows stays small against the partition work.
+    (void)m;
+    constexpr int64_t kLanes = 65536;
+    if (nsys < 1 || nrow < 2) return 1;
+    if ((int64_t)nsys >= kLanes / 2) return 1;
+    const int64_t want = (kLanes + nsys - 1) / nsys;
+    const int64_t most = nrow / 8;
+    const int64_t p = want < most ? want : most;
+    return p < 2 ? 1 : (int)p;
+}
+
 btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npart, int layout) {
     g_err.clear();
     if (!plan) return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: plan pointer is NULL");
@@ -95,7 +110,9 @@ btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npar
         return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: requires 1 <= m <= 10");
     if (nsys < 1 || nrow < 2)
         return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: requires nsys > 0 and nrow >= 2");
-    if (npart < 1) return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: requires npart >= 1");
+    if (npart < 0)
+        return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: requires npart >= 1 (or 0: chosen by the library)");
+    if (npart == 0) npart = btd_auto_npart(m, nsys, nrow);
     if (npart > 1 && nrow < 3 * npart)
         return fail(BTD_ERR_INVALID_ARG,
                     "btd_plan_create: the two-stage solve needs at least 3 rows per partition "
@@ -140,6 +157,14 @@ btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npar
     }
     p->created = true;
     *plan = p;
+    return BTD_OK;
+}
+
+btd_status btd_plan_npart(btd_plan_t plan, int* npart) {
+    g_err.clear();
+    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_plan_npart: uncreated plan");
+    if (!npart) return fail(BTD_ERR_INVALID_ARG, "btd_plan_npart: NULL npart");
+    *npart = plan->npart;
     return BTD_OK;
 }
 

@@ -14,10 +14,12 @@ inputs, runs `iterations` profiled solves and writes
   of the solution and x at system 0, element 0 of rows 0, n3/2 and n3-1.
 
 `implementation` is "hip-mi355x" and `mpi_mode` "in-hbm", so the reference's analysis keeps
-these rows apart from its cuda-cxx / fortran-original rows.
+these rows apart from its cuda-cxx / fortran-original rows. With --npart auto every case runs
+with the partition count the library chooses for one GPU (btd_auto_npart) instead of nranks;
+those rows keep the case's nranks and carry mpi_mode "in-hbm-np<P>" and the partitions' row counts.
 
     python -m pascal_btdma_amd.study --profile-out P.csv --signature-out S.csv [--iterations 10]
-        [--case nranks,n1,n2,n3,m ...]
+        [--case nranks,n1,n2,n3,m ...] [--npart auto]
 """
 from __future__ import annotations
 
@@ -55,7 +57,7 @@ def _fmt(x: float, digits: int) -> str:
     return f"{x:.{digits}g}"
 
 
-def run_case(case, iterations: int = 10, device=None):
+def run_case(case, iterations: int = 10, device=None, auto_npart: bool = False):
     """Profile one case; returns (timing rows, signature row) as lists of CSV strings."""
     import torch
     import pascal_btdma_amd as pb
@@ -68,11 +70,13 @@ def run_case(case, iterations: int = 10, device=None):
     d = torch.empty(nv, dtype=torch.float64, device=dev)
     pb.generate(pb.GenKind.DIAG, 0, m, nsys, nrow, lay, a, b, c, d)
     a0, c0, d0 = a.clone(), c.clone(), d.clone()
-    sizes = [pb.partition_1d(0, nrow - 1, nranks, r) for r in range(nranks)]
+    npart = pb._capi.lib().btd_auto_npart(m, nsys, nrow) if auto_npart else nranks
+    mode = f"{MPI_MODE}-np{npart}" if auto_npart else MPI_MODE
+    sizes = [pb.partition_1d(0, nrow - 1, npart, r) for r in range(npart)]
     rows = [last - first + 1 for first, last in sizes]
     head = f"btdma,noncyclic,{IMPLEMENTATION},{nranks},{n1},{n2},{n3},{m},{nsys},{min(rows)},{max(rows)}"
     timing, signature = [], None
-    with pb.BtdmaGpuPlan().create(m, nsys, nrow, nranks, lay) as plan:
+    with pb.BtdmaGpuPlan().create(m, nsys, nrow, npart, lay) as plan:
         for it in range(iterations):
             a.copy_(a0)
             c.copy_(c0)
@@ -83,11 +87,11 @@ def run_case(case, iterations: int = 10, device=None):
                 x = d.double()
                 s, ss, linf = float(x.sum()), float((x * x).sum()), float(x.abs().max())
                 samples = [float(x[pb.vector_index(0, z, 0, nsys, nrow)]) for z in (0, n3 // 2, n3 - 1)]
-                signature = ",".join([head, MPI_MODE] + [_fmt(v, 16) for v in
+                signature = ",".join([head, mode] + [_fmt(v, 16) for v in
                                                         (s, math.sqrt(ss), linf, *samples)])
             fields = [t.total, t.total, t.local_compute, t.forward_exchange, t.reduced_compute,
                       t.backward_exchange, t.update_compute, t.computation(), t.communication()]
-            timing.append(",".join([head, str(it), str(iterations), MPI_MODE] + [_fmt(v, 12) for v in fields]))
+            timing.append(",".join([head, str(it), str(iterations), mode] + [_fmt(v, 12) for v in fields]))
     del a, b, c, d, a0, c0, d0
     torch.cuda.empty_cache()
     return timing, signature
@@ -109,12 +113,14 @@ def main(argv=None) -> int:
     ap.add_argument("--iterations", type=int, default=10)
     ap.add_argument("--case", action="append", default=[],
                     help="nranks,n1,n2,n3,m (repeatable; default: the reference Study's case matrix)")
+    ap.add_argument("--npart", choices=["nranks", "auto"], default="nranks",
+                    help="partitions per system: the case's rank count, or the library's choice")
     args = ap.parse_args(argv)
     cases = [tuple(int(v) for v in c.split(",")) for c in args.case] or STUDY_CASES
     for case in cases:
         if len(case) != 5:
             raise SystemExit(f"--case needs nranks,n1,n2,n3,m, got {case}")
-        timing, signature = run_case(case, args.iterations)
+        timing, signature = run_case(case, args.iterations, auto_npart=args.npart == "auto")
         _append(args.profile_out, PROFILE_HEADER, timing)
         _append(args.signature_out, SIGNATURE_HEADER, [signature])
         print(f"[study] case np={case[0]} {case[1]}x{case[2]}x{case[3]} m={case[4]} done", flush=True)

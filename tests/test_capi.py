@@ -63,12 +63,28 @@ def test_layout_elems():
     ((3, 0, 10, 1, pb.Layout.TILED), ValueError),
     ((3, 10, 1, 1, pb.Layout.TILED), ValueError),
     ((3, 10, 10, 4, pb.Layout.TILED), ValueError),   # needs nrow >= 3*npart
-    ((3, 10, 10, 0, pb.Layout.TILED), ValueError),
+    ((3, 10, 10, -1, pb.Layout.TILED), ValueError),
     ((3, 10, 10, 1, pb.Layout.REF_CPU), ValueError),
 ])
 def test_plan_create_rejects_bad_arguments(args, exc):
     with pytest.raises(exc):
         pb.BtdmaGpuPlan().create(*args)
+
+
+def test_auto_npart_rule():
+    """npart = 0: the single-kernel solve for large batches, about one wave per SIMD
+    (65536 lanes) for small ones, at least 8 rows per partition."""
+    auto = lambda m, nsys, nrow: pb._capi.lib().btd_auto_npart(m, nsys, nrow)
+    assert auto(5, 131072, 512) == 1          # the headline batch
+    assert auto(5, 32768, 512) == 1
+    assert auto(5, 1024, 128) == 16           # the Study's single-GPU case: nrow / 8
+    assert auto(5, 512, 65536) == 128         # long systems: 65536 lanes
+    assert auto(8, 4096, 2048) == 16
+    assert auto(3, 1024, 64) == 8
+    assert auto(3, 1024, 15) == 1             # fewer than 2 partitions of 8 rows
+    for m, nsys, nrow in [(2, 7, 40), (10, 3000, 4096), (5, 1, 2)]:
+        p = auto(m, nsys, nrow)
+        assert p == 1 or (p >= 2 and nrow >= 8 * p)
 
 
 def test_uncreated_plan_errors():

@@ -138,6 +138,22 @@ def test_study_signatures_on_device(cuda, row):
         assert abs(x[pb.vector_index(0, z, 0, nsys, n)] - float(row[key])) < 1e-14
 
 
+@pytest.mark.parametrize("m,nsys,n,expect", [(3, 1024, 64, 8), (5, 700, 200, 25), (2, 40000, 6, 1)])
+def test_auto_npart_plan(cuda, oracle, m, nsys, n, expect):
+    """A plan created with npart = 0 takes btd_auto_npart's count and still matches the oracle."""
+    import torch
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 50 + m, m, nsys, n)
+    lay = int(pb.Layout.TILED)
+    ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d))
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 0, pb.Layout.TILED) as plan:
+        assert plan.npart == expect == pb._capi.lib().btd_auto_npart(m, nsys, n)
+        pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    x = layout_to_vec(td.cpu().numpy(), lay, n, nsys, m)
+    assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
+
+
 def test_singular_block_is_reported(cuda):
     import torch
     m, nsys, n = 3, 64, 8

@@ -39,17 +39,23 @@ def test_number_format_is_iostream_setprecision():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("auto", [False, True])
 @pytest.mark.parametrize("case", [(1, 32, 32, 128, 5), (2, 64, 64, 1024, 2), (8, 32, 32, 2048, 8)])
-def test_study_rows_on_device(cuda, case, tmp_path):
-    timing, signature = study.run_case(case, iterations=3)
+def test_study_rows_on_device(cuda, case, auto, tmp_path):
+    timing, signature = study.run_case(case, iterations=3, auto_npart=auto)
     prof = tmp_path / "p.csv"
     sig = tmp_path / "s.csv"
     study._append(str(prof), study.PROFILE_HEADER, timing)
     study._append(str(sig), study.SIGNATURE_HEADER, [signature])
     rows = list(csv.DictReader(io.StringIO(prof.read_text())))
     assert [int(r["iter"]) for r in rows] == [0, 1, 2]
+    P = case[0]
+    if auto:
+        import pascal_btdma_amd as pb
+        P = pb._capi.lib().btd_auto_npart(case[4], case[1] * case[2], case[3])
     for r in rows:
-        assert r["implementation"] == "hip-mi355x" and r["mpi_mode"] == "in-hbm"
+        assert r["implementation"] == "hip-mi355x"
+        assert r["mpi_mode"] == (f"in-hbm-np{P}" if auto else "in-hbm")
         total, comp = float(r["total_s_max"]), float(r["compute_s_max"])
         parts = sum(float(r[k]) for k in ("local_compute_s_max", "reduced_compute_s_max",
                                           "update_compute_s_max"))
@@ -67,4 +73,4 @@ def test_study_rows_on_device(cuda, case, tmp_path):
     for k in ("solution_linf", "sample_z0", "sample_zmid", "sample_zlast"):
         assert abs(float(srow[k]) - float(g[k])) < 1e-14
     nrows = [int(srow["nrow_min"]), int(srow["nrow_max"])]
-    assert nrows == [case[3] // case[0]] * 2
+    assert nrows == [case[3] // P] * 2
