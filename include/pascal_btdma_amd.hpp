@@ -95,6 +95,7 @@ struct BtdmaGpuPlan {
         nsys = nsys_in;
         nrow = nrow_in;
         npart = npart_in;
+        if (npart == 0) check_status(btd_plan_npart(handle, &npart));  // the library's choice
         layout = layout_in;
         created = true;
     }
@@ -111,6 +112,10 @@ struct BtdmaGpuPlan {
         check_status(btd_plan_check(handle, &s));
         return s != 0;
     }
+
+    // Synchronises; info[s] = 1 for every system that met a zero pivot since the last check
+    // (info must hold nsys ints).
+    void singular_systems(int* info) { check_status(btd_plan_info(handle, info)); }
 
    private:
     void swap(BtdmaGpuPlan& o) noexcept {
@@ -150,6 +155,37 @@ inline void solve_cyclic(BtdmaGpuPlan& plan, double* a_dev, double* b_dev, doubl
                          double* d_dev, int m, int nsys, int nrow, hipStream_t stream = nullptr) {
     check_status(btd_solve_cyclic(plan.handle, a_dev, b_dev, c_dev, d_dev, m, nsys, nrow,
                                   (void*)stream));
+}
+
+// Entry points with no counterpart in the reference (see the C header for their contracts).
+
+// FP32-stored blocks, FP64 arithmetic; work: btd_layout_elems(layout, nsys, nrow, m*m) doubles.
+inline void solve_noncyclic_f32s(BtdmaGpuPlan& plan, const float* a_dev, const float* b_dev,
+                                 const float* c_dev, double* d_dev, double* work_dev,
+                                 hipStream_t stream = nullptr) {
+    check_status(btd_solve_noncyclic_f32s(plan.handle, a_dev, b_dev, c_dev, d_dev, work_dev,
+                                          (void*)stream));
+}
+
+// euler3d sweeps: axis 0/1/2 = x/y/z; U(0:n1+1,0:n2+1,0:n3+1,5), fields (n1,n2,n3,5).
+inline void euler_assemble(BtdmaGpuPlan& plan, int axis, const double* U_dev, const double* rhs_dev,
+                           int n1, int n2, int n3, double h, double dt, double eps, double* a_dev,
+                           double* b_dev, double* c_dev, double* d_dev, hipStream_t stream = nullptr) {
+    check_status(btd_euler_assemble(plan.handle, axis, U_dev, rhs_dev, n1, n2, n3, h, dt, eps, a_dev,
+                                    b_dev, c_dev, d_dev, (void*)stream));
+}
+
+inline void sweep_scatter(BtdmaGpuPlan& plan, int axis, const double* d_dev, int n1, int n2, int n3,
+                          double* field_dev, hipStream_t stream = nullptr) {
+    check_status(btd_sweep_scatter(plan.handle, axis, d_dev, n1, n2, n3, field_dev, (void*)stream));
+}
+
+inline void euler_sweep_cyclic(BtdmaGpuPlan& plan, int axis, const double* U_dev,
+                               const double* rhs_dev, int n1, int n2, int n3, double h, double dt,
+                               double eps, double* work_dev, double* out_dev,
+                               hipStream_t stream = nullptr) {
+    check_status(btd_euler_sweep_cyclic(plan.handle, axis, U_dev, rhs_dev, n1, n2, n3, h, dt, eps,
+                                        work_dev, out_dev, (void*)stream));
 }
 
 }  // namespace pascal_btdma

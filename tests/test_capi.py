@@ -138,3 +138,26 @@ def test_generator_diag_is_study_workload():
     np.testing.assert_array_equal(B, np.broadcast_to(2 * np.eye(3), B.shape))
     np.testing.assert_array_equal(A, np.broadcast_to(0.25 * np.eye(3), A.shape))
     np.testing.assert_array_equal(d, 1.0)
+
+
+def test_new_entry_points_reject_an_uncreated_plan():
+    """Every entry point without a reference counterpart checks its plan before any device
+    use: a NULL plan gives BTD_ERR_NOT_CREATED (no GPU needed)."""
+    lib = _capi.lib()
+    NOT_CREATED = 2
+    null = None
+    calls = {
+        "btd_solve_noncyclic_f32s": (null, 1, 1, 1, 1, 1, null),
+        "btd_solve_transposed_f32r": (null, 1, 1, 1, 1, 1, 1, 2, null),
+        "btd_outer_grads_f32": (null, 1, 1, 1, 1, 1, null),
+        "btd_euler_assemble": (null, 0, 1, 1, 4, 4, 4, 0.1, 0.1, 0.1, 1, 1, 1, 1, null),
+        "btd_euler_sweep_cyclic": (null, 0, 1, 1, 4, 4, 4, 0.1, 0.1, 0.1, 1, 1, null),
+        "btd_sweep_gather": (null, 0, 1, 4, 4, 4, 1, null),
+        "btd_sweep_scatter": (null, 0, 1, 4, 4, 4, 1, null),
+        "btd_plan_info": (null, 1),
+        "btd_plan_npart": (null, None),
+        "btd_plan_check": (null, None),
+    }
+    for name, args in calls.items():
+        assert getattr(lib, name)(*args) == NOT_CREATED, name
+        assert lib.btd_last_error() != b"", name
