@@ -62,11 +62,15 @@ def rel_err_sys(x: np.ndarray, xo: np.ndarray) -> np.ndarray:
 
 
 def event_time(fn, restore, steps: int, stream) -> np.ndarray:
-    """Per-step HIP-event times (ms) of fn on `stream`, restore() outside the events."""
+    """Per-step HIP-event times (ms) of fn on `stream`, restore() outside the events. The
+    device is drained before every restore: restore() works on torch's current stream, which
+    need not be `stream`, and must neither overlap the previous step nor change its inputs
+    while it runs."""
     import torch
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
     for e0, e1 in evs:
+        torch.cuda.synchronize()
         restore()
         e0.record(stream)
         fn()
