@@ -254,8 +254,9 @@ def extra_config0(dev, checks: ParityChecks) -> dict:
 def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
     """Config 2: 20000 systems with m uniform in {2,3,5,8,10}, n log-uniform in [32,1024],
     plus 512 long systems (m = 5, n = 65536) split into P partitions, all replayed as one
-    captured graph (the batch's persistent kernel and the long systems' three kernels on a
-    forked branch). The autotuner picks the batch placement/grid and P in {8,16,32}."""
+    captured graph (the batch's persistent kernel and the long systems' three kernels, one
+    after the other or on forked branches). The autotuner picks the batch placement/grid and
+    P; the graph's topology and, when forked, the batch's grid are picked by timing them."""
     import torch
     import pascal_btdma_amd as pb
     from pascal_btdma_amd.tuner import tune_batch, tune_partitions
@@ -288,9 +289,10 @@ def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
 
     s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
 
-    def capture(forked: bool):
-        """One graph holding the batch's persistent kernel and the long systems' three
-        kernels, either on two forked branches or one after the other."""
+    def capture(forked: bool, grid: int):
+        """One graph holding the batch's persistent kernel (at `grid` workgroups) and the
+        long systems' three kernels, either on two forked branches or one after the other."""
+        bt.set_params(tb["placement"], grid)
         g = pb.Graph()
         fork, join = torch.cuda.Event(), torch.cuda.Event()
         torch.cuda.synchronize()
@@ -311,12 +313,23 @@ def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
         t = event_time(lambda: g.launch(s1), lambda: (restore(), torch.cuda.synchronize()), steps, s1)
         return g, float(np.median(t))
 
-    g_fork, ms_fork = capture(True)
-    g_seq, ms_seq = capture(False)
-    g, topology = (g_fork, "forked") if ms_fork <= ms_seq else (g_seq, "sequential")
-    (g_seq if g is g_fork else g_fork).destroy()
-    t = np.array([min(ms_fork, ms_seq)])
-    # the batch alone and the long systems alone, for the record
+    # sequential at the tuned grid; forked also with the batch on one workgroup per CU, which
+    # leaves the other SIMDs to the long systems' kernels
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    forked_grids = sorted({tb["grid"], cus})
+    candidates = [("sequential", tb["grid"])] + [("forked", gr) for gr in forked_grids]
+    graphs = {}
+    for topo, gr in candidates:
+        graphs[(topo, gr)] = capture(topo == "forked", gr)
+    (topology, grid_used), (g, ms_best) = min(graphs.items(), key=lambda kv: kv[1][1])
+    for key, (gg, _) in graphs.items():
+        if gg is not g:
+            gg.destroy()
+    ms_seq = graphs[("sequential", tb["grid"])][1]
+    ms_fork = min(v[1] for k, v in graphs.items() if k[0] == "forked")
+    t = np.array([ms_best])
+    # the batch alone (at the tuned grid) and the long systems alone, for the record
+    bt.set_params(tb["placement"], tb["grid"])
     tb_only = event_time(lambda: bt.solve(a, b, c, d), restore, 3, torch.cuda.current_stream())
     tl_only = event_time(lambda: pb.solve_noncyclic(plan, la, lb, lc, ld, lm, lnsys, ln), restore, 3,
                          torch.cuda.current_stream())
@@ -340,7 +353,9 @@ def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
             "block_rows_per_s": (rows_b + rows_l) / (ms * 1e-3),
             "algorithmic_GBs": (bytes_b + bytes_l) / (ms * 1e-3) / 1e9,
             "hbm_frac": (bytes_b + bytes_l) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "graph_topology": topology, "graph_forked_ms": ms_fork, "graph_sequential_ms": ms_seq,
+            "graph_topology": topology, "graph_batch_grid": grid_used, "graph_forked_ms": ms_fork,
+            "graph_sequential_ms": ms_seq,
+            "graph_candidates_ms": {f"{k[0]},grid={k[1]}": v[1] for k, v in graphs.items()},
             "batch_only_ms": float(np.median(tb_only)), "long_only_ms": float(np.median(tl_only)),
             "long_only_GBs": bytes_l / (np.median(tl_only) * 1e-3) / 1e9,
             "tuned_batch": tb, "tuned_partitions": tp, "batch_tiles": bt.ntiles,

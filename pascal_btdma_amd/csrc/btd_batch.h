@@ -5,8 +5,9 @@
 // to 64 systems of the same m. Each tile is stored like the tiled layout with its own row
 // count (the longest system of the tile): element e of row `row` of the system in lane l of
 // tile t is at tile_mat_off[t] + (row*E + e)*64 + l (E = m*m), vectors likewise with E = m.
-// Work items: a whole tile (one lane per system) or, for the cooperative placement, a
-// chunk of floor(64/m) systems of a tile with m lanes per system. Items are ordered by
+// Work items: a whole tile (one lane per system) or, for the cooperative placements, a
+// chunk of floor(64/m) systems of a tile with m lanes per system, or of floor(64/(2m))
+// systems with two groups of m lanes per system (twisted). Items are ordered by
 // decreasing estimated cost (longest-processing-time first) and consumed through an
 // atomic counter by a grid sized to the device; every wave exits when the counter passes
 // the number of items.
@@ -216,13 +217,14 @@ __device__ bool plain_lane_lds(const double* __restrict__ A, const double* __res
 }
 
 // Orders the LDS accesses of the lanes of one wavefront. The batch kernel runs one-wave
-// (64-lane) workgroups, so LDS-only acquire/release fences around a wave barrier suffice and
-// outstanding global loads (the prefetched next row) are not waited for, unlike with a full
-// workgroup barrier.
+// (64-lane) workgroups, and the LDS performs the operations of a wavefront in the order they
+// are issued, so wavefront-scope fences (which keep the compiler from reordering the accesses
+// and emit no wait) suffice: an exchange costs one LDS round trip, not a drain of every
+// outstanding load.
 __device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // 1/x from the hardware reciprocal refined by two Newton steps (error ~1 ulp); shorter on
@@ -235,166 +237,379 @@ __device__ __forceinline__ double fast_rcp(double x) {
     return fma(r, e, r);
 }
 
-// Cooperative solve: m lanes per system, lane i owns block row i of every block. LDS
-// (sh, [slot][lane], 5m+5 slots) holds the carried C'_{q-1} (column j in lane base+j) and
-// d'_{q-1}, and two alternating exchange buffers, so every pivot step needs one barrier:
-// each lane posts its candidate |r(k)| with its whole augmented row, every lane picks the
-// pivot (largest |r(k)| among rows not yet chosen, lowest lane on a tie) and eliminates
-// with it; the buffer written at step k is rewritten only at step k+2, after the barrier
-// of step k+1 that every lane passes once done reading. After the elimination each lane
-// posts its row once more and back-substitutes one right-hand-side column on its own
-// (lane i: column i of C'_q; every lane: d'_q). The next row's blocks are loaded while the
-// current row is eliminated. All lanes of the wave run the same number of row steps (nmax),
-// so every barrier is reached by the whole workgroup; lanes past their system's n, and
-// lanes of no system (n = 0), compute on zeros and store nothing.
+// Rows of the forward (kFwdAhead) and backward (kBwdAhead) sweeps of the cooperative solve
+// kept in flight: one wavefront per work item has no other waves to hide HBM latency
+// behind, so the next rows' blocks are loaded while the current row is eliminated.
+// Rows of the forward (kFwdAhead) and backward (kBwdAhead) sweeps of the cooperative solve
+// kept in flight: one wavefront per work item has no other waves to hide HBM latency
+// behind, so the next rows' blocks are loaded while the current row is eliminated.
+template <int M>
+struct CoopDepth {
+    static constexpr int kFwdAhead = M <= 2 ? 8 : M <= 3 ? 6 : M <= 5 ? 4 : 1;
+    static constexpr int kBwdAhead = M <= 3 ? 16 : M <= 5 ? 12 : 8;
+};
+
+// LDS of a cooperative work item, [slot][lane] (64 doubles a slot): slots [0, M) the carried
+// block of each group (slot k, lane base+j: element (k, j)), slot M its carried right-hand
+// side (lane base+i: element i), then the 2M+2 slots of the elimination exchange.
+constexpr int kCoopSlots = 3 * kMaxM + 3;
+
+// Elimination of one block row by a group of M lanes: lane i holds row i of R and of the
+// right-hand sides rhs = (d, S) (S the block whose R^{-1} S is carried to the next row).
+// Pivot step k: every lane posts its row's columns k..M-1, its rhs and its candidate |r(k)|
+// (-1 once chosen); every lane reads the M candidates, picks the pivot (largest, lowest lane
+// on a tie), reads the pivot row and eliminates with it, a lane that is already chosen or is
+// the pivot with multiplier 0. Then each lane posts its row once more and back-substitutes
+// one rhs column on its own: xc = column i of R^{-1} S (xc[k] = (R^{-1} S)(k, i)) and, in
+// every lane, the whole xd = R^{-1} d. Returns 1 if a pivot was zero. The code is free of
+// lane-dependent branches so that LDS reads issue in batches.
+template <int M>
+__device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[M + 1], double* __restrict__ xb,
+                                              int i, int base, double (&xc)[M], double (&xd)[M]) {
+    const int lane = threadIdx.x;
+    int rank = -1;
+    int plane[M];
+    int zero_pivot = 0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+#pragma unroll
+        for (int j = k; j < M; ++j) xb[j * kTile + lane] = r[j];
+#pragma unroll
+        for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
+        xb[(2 * M + 1) * kTile + lane] = rank < 0 ? fabs(r[k]) : -1.0;
+        wave_lds_sync();
+        double cand[M];
+#pragma unroll
+        for (int t = 0; t < M; ++t) cand[t] = xb[(2 * M + 1) * kTile + base + t];
+        int p = 0;
+        double best = cand[0];
+#pragma unroll
+        for (int t = 1; t < M; ++t) {
+            const bool c = cand[t] > best;
+            best = c ? cand[t] : best;
+            p = c ? t : p;
+        }
+        plane[k] = p;
+        const int pl = base + p;
+        double prow[M], prhs[M + 1];
+#pragma unroll
+        for (int j = k; j < M; ++j) prow[j] = xb[j * kTile + pl];
+#pragma unroll
+        for (int j = 0; j <= M; ++j) prhs[j] = xb[(M + j) * kTile + pl];
+        zero_pivot |= prow[k] == 0.0 ? 1 : 0;
+        const bool elim = rank < 0 && i != p;
+        const double l = elim ? r[k] * fast_rcp(prow[k]) : 0.0;
+        rank = (rank < 0 && i == p) ? k : rank;
+#pragma unroll
+        for (int j = k + 1; j < M; ++j) r[j] = fma(-l, prow[j], r[j]);
+#pragma unroll
+        for (int j = 0; j <= M; ++j) rhs[j] = fma(-l, prhs[j], rhs[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) xb[j * kTile + lane] = r[j];
+#pragma unroll
+    for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
+    wave_lds_sync();
+#pragma unroll
+    for (int k = M - 1; k >= 0; --k) {
+        const int pl = base + plane[k];
+        double sc = xb[(M + 1 + i) * kTile + pl];
+        double sd = xb[M * kTile + pl];
+#pragma unroll
+        for (int j = k + 1; j < M; ++j) {
+            const double u = xb[j * kTile + pl];
+            sc = fma(-u, xc[j], sc);
+            sd = fma(-u, xd[j], sd);
+        }
+        const double rk = fast_rcp(xb[k * kTile + pl]);
+        xc[k] = sc * rk;
+        xd[k] = sd * rk;
+    }
+    return zero_pivot;
+}
+
+// Element i of a vector held whole in every lane.
+template <int M>
+__device__ __forceinline__ double lane_elem(const double (&v)[M], int i) {
+    double x = v[0];
+#pragma unroll
+    for (int t = 1; t < M; ++t) x = (i == t) ? v[t] : x;
+    return x;
+}
+
+// Cooperative solve: M lanes per system, lane i owns block row i of every block; row q is
+// R_q = B_q - A_q C'_{q-1} eliminated with rhs (d_q - A_q d'_{q-1}, C_q) (coop_eliminate),
+// C'_q stored over C_q and carried in LDS, d'_q over d_q and carried whole in registers;
+// then x_q = d'_q - C'_q x_{q+1} upwards, x_{q+1} exchanged through LDS. Exchanges are
+// ordered by wave_lds_sync only. Rows are loaded kFwdAhead (forward) and kBwdAhead
+// (backward) rows ahead; every row below nmax is allocated for every lane of the tile, so
+// loads need no guard (rows past the ends are clamped). All lanes of the wave run the same
+// row steps (nmax rounded up to the prefetch depth); lanes past their system's n compute on
+// what the rows hold and store nothing.
 template <int M>
 __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict__ B, double* C,
                           double* D, int64_t mo, int64_t vo, int i, int base, int n, int nmax,
                           double* __restrict__ sh) {
     constexpr int E = M * M;
-    constexpr int W = 2 * M + 2;          // exchange slots: row (M), rhs (M+1), candidate (1)
-    double* carry = sh;                   // slots [0, M]: C'(k, lane-base) and d'(lane-base)
-    double* xbuf0 = sh + (M + 1) * kTile; // two exchange buffers of W slots
-    double* xbuf1 = xbuf0 + W * kTile;
+    constexpr int F = CoopDepth<M>::kFwdAhead;
+    constexpr int G = CoopDepth<M>::kBwdAhead;
+    double* carry = sh;
+    double* xb = sh + (M + 1) * kTile;
     const int lane = threadIdx.x;
     const int64_t es = kTile;
     bool ok = true;
-    double pa[M], pb[M], pc[M], pd;
-    auto fetch = [&](int q) {
-        const bool act = q < n, has_next = q + 1 < n;
+    double pa[F][M], pb[F][M], pc[F][M], pd[F];
+    auto fetch = [&](double (&fa)[M], double (&fb)[M], double (&fc)[M], double& fd, int q) {
+        q = q < nmax ? q : nmax - 1;
         const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            pa[k] = (act && q > 0) ? __builtin_nontemporal_load(A + rm + (i + M * k) * es) : 0.0;
-            pb[k] = act ? __builtin_nontemporal_load(B + rm + (i + M * k) * es) : 0.0;
-            pc[k] = has_next ? C[rm + (i + M * k) * es] : 0.0;
+            fa[k] = __builtin_nontemporal_load(A + rm + (i + M * k) * es);
+            fb[k] = __builtin_nontemporal_load(B + rm + (i + M * k) * es);
+            fc[k] = C[rm + (i + M * k) * es];
         }
-        pd = act ? D[rv + i * es] : 0.0;
+        fd = D[rv + i * es];
     };
-    fetch(0);
-    double xlast = 0.0;
-    for (int q = 0; q < nmax; ++q) {
-        const bool act = q < n;
-        const bool has_next = q + 1 < n;
-        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
-        double r[M], rhs[M + 1];
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-            r[j] = pb[j];
-            rhs[1 + j] = pc[j];
-        }
-        rhs[0] = pd;
-        if (q > 0) {
+    for (int s = 0; s < F; ++s) fetch(pa[s], pb[s], pc[s], pd[s], s);
+    double dp[M];  // d'_{q-1}, whole, in every lane of the system
+#pragma unroll
+    for (int k = 0; k < M; ++k) dp[k] = 0.0;
+    double xlast = 0.0;
+    for (int q0 = 0; q0 < nmax; q0 += F) {
+#pragma unroll
+        for (int s = 0; s < F; ++s) {
+            const int q = q0 + s;
+            const bool act = q < n;
+            const bool has_next = q + 1 < n;
+            const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+            double r[M], rhs[M + 1];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                r[j] = pb[s][j];
+                rhs[1 + j] = pc[s][j];
+            }
+            rhs[0] = pd[s];
+            if (q > 0) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    const double a = pa[s][k];
+                    rhs[0] = fma(-a, dp[k], rhs[0]);
+#pragma unroll
+                    for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
+                }
+            }
+            fetch(pa[s], pb[s], pc[s], pd[s], q + F);
+            double xc[M], xd[M];
+            const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
+            const double di = lane_elem<M>(xd, i);
+            ok = ok && !(act && zero_pivot);
+            if (act) {
+                D[rv + i * es] = di;
+                xlast = di;
+            }
+            if (has_next) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) C[rm + (k + M * i) * es] = xc[k];
+            }
 #pragma unroll
             for (int k = 0; k < M; ++k) {
-                rhs[0] = fma(-pa[k], carry[M * kTile + base + k], rhs[0]);
-#pragma unroll
-                for (int j = 0; j < M; ++j) r[j] = fma(-pa[k], carry[k * kTile + base + j], r[j]);
+                carry[k * kTile + lane] = xc[k];  // C'(k, i) in lane base+i
+                dp[k] = xd[k];
             }
+            wave_lds_sync();
         }
-        if (q + 1 < nmax) fetch(q + 1);
-        int rank = -1;
-        int plane[M];
+    }
+    // back substitution over the rows: x_q(i) = d'_q(i) - sum_j C'_q(i,j) x_{q+1}(j), with
+    // C'_q(i, :) and d'_q(i) loaded kBwdAhead rows ahead (rows below 0 clamp to row 0).
+    double xnext = xlast;
+    double cn[G][M], dn[G];
+    auto bfetch = [&](double (&fc)[M], double& fd, int q) {
+        q = q < 0 ? 0 : q;
+        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+#pragma unroll
+        for (int j = 0; j < M; ++j) fc[j] = C[rm + (i + M * j) * es];
+        fd = D[rv + i * es];
+    };
+#pragma unroll
+    for (int s = 0; s < G; ++s) bfetch(cn[s], dn[s], nmax - 2 - s);
+    for (int q0 = nmax - 2; q0 >= 0; q0 -= G) {
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            const int q = q0 - s;
+            const bool act = q >= 0 && q < n - 1;
+            double cq[M], x = dn[s];
+#pragma unroll
+            for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
+            bfetch(cn[s], dn[s], q - G);
+            sh[lane] = xnext;
+            wave_lds_sync();
+#pragma unroll
+            for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
+            if (act) {
+                D[vo + (int64_t)q * M * kTile + i * es] = x;
+                xnext = x;
+            }
+            wave_lds_sync();
+        }
+    }
+    return ok;
+}
+
+// Twisted cooperative solve: two groups of M lanes per system (lane i of each owns block row
+// i). With h = (n-1)/2 the top group (`top`) eliminates rows 0..h-1 downwards as coop_rows
+// does (C'_q = R_q^{-1} C_q over C_q), while the bottom group eliminates rows n-1..h+1
+// upwards: S_q = B_q - C_q A''_{q+1}, A''_q = S_q^{-1} A_q stored over C_q (dead once row q
+// is eliminated), d''_q = S_q^{-1}(d_q - C_q d''_{q+1}) over d_q. Both groups then solve the
+// meeting row, (B_h - A_h C'_{h-1} - C_h A''_{h+1}) x_h = d_h - A_h d'_{h-1} - C_h d''_{h+1},
+// and substitute outwards: x_q = d'_q - C'_q x_{q+1} (q < h), x_q = d''_q - A''_q x_{q-1}
+// (q > h). Each group handles half of the rows, so a long system takes half the time of
+// coop_rows for the same arithmetic. The groups of a system are base and pbase; `steps`
+// (uniform over the wave) is the largest row count of a group of the item (the bottom
+// group of its longest system, whose rows nmax bound every row loaded). A group's carried
+// block and rhs stay as its last row left them while the other groups finish.
+template <int M>
+__device__ bool coop_twisted(const double* A, const double* __restrict__ B, double* C, double* D,
+                             int64_t mo, int64_t vo, int i, int base, int pbase, bool top, int n,
+                             int steps, int nmax, double* __restrict__ sh) {
+    constexpr int E = M * M;
+    constexpr int F = CoopDepth<M>::kFwdAhead;
+    constexpr int G = CoopDepth<M>::kBwdAhead;
+    double* carry = sh;
+    double* xb = sh + (M + 1) * kTile;
+    const int lane = threadIdx.x;
+    const int64_t es = kTile;
+    const int h = n > 0 ? (n - 1) / 2 : 0;
+    const int cnt = n > 0 ? (top ? h : n - 1 - h) : 0;
+    const double* sub = top ? A : C;  // the block coupling row q to the row eliminated before it
+    const double* sup = top ? C : A;  // the block coupling row q to the row eliminated after it
+    auto clamp_row = [&](int q) { return q < 0 ? 0 : (q >= nmax ? nmax - 1 : q); };
+    bool ok = true;
+    double ps[F][M], pg[F][M], pu[F][M], pd[F];
+    auto fetch = [&](double (&fs)[M], double (&fg)[M], double (&fu)[M], double& fd, int t) {
+        const int q = clamp_row(top ? t : n - 1 - t);
+        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            double* xb = (k & 1) ? xbuf1 : xbuf0;
-#pragma unroll
-            for (int j = 0; j < M; ++j) xb[j * kTile + lane] = r[j];
-#pragma unroll
-            for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
-            xb[(2 * M + 1) * kTile + lane] = rank < 0 ? fabs(r[k]) : -1.0;
-            wave_lds_sync();
-            int p = 0;
-            double best = -2.0;
-#pragma unroll
-            for (int t = 0; t < M; ++t) {
-                const double v = xb[(2 * M + 1) * kTile + base + t];
-                const bool c = v > best;
-                best = c ? v : best;
-                p = c ? t : p;
-            }
-            plane[k] = p;
-            const int pl = base + p;
-            const double pk = xb[k * kTile + pl];
-            ok = ok && (!act || pk != 0.0);
-            if (i == p) {
-                rank = k;
-            } else if (rank < 0) {
-                const double l = r[k] * fast_rcp(pk);
-#pragma unroll
-                for (int j = k + 1; j < M; ++j) r[j] = fma(-l, xb[j * kTile + pl], r[j]);
-#pragma unroll
-                for (int j = 0; j <= M; ++j) rhs[j] = fma(-l, xb[(M + j) * kTile + pl], rhs[j]);
-            }
+            fs[k] = sub[rm + (i + M * k) * es];
+            fg[k] = __builtin_nontemporal_load(B + rm + (i + M * k) * es);
+            fu[k] = sup[rm + (i + M * k) * es];
         }
-        // post the eliminated rows; lane i solves column i+1 (C'(:, i)), every lane column 0
-        double* xb = (M & 1) ? xbuf1 : xbuf0;
-#pragma unroll
-        for (int j = 0; j < M; ++j) xb[j * kTile + lane] = r[j];
-#pragma unroll
-        for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
-        wave_lds_sync();
-        double xc[M], xd[M];
-#pragma unroll
-        for (int k = M - 1; k >= 0; --k) {
-            const int pl = base + plane[k];
-            double sc = 0.0;
-#pragma unroll
-            for (int t = 0; t < M; ++t) sc = (i == t) ? xb[(M + 1 + t) * kTile + pl] : sc;
-            double sd = xb[M * kTile + pl];
-#pragma unroll
-            for (int j = k + 1; j < M; ++j) {
-                const double u = xb[j * kTile + pl];
-                sc = fma(-u, xc[j], sc);
-                sd = fma(-u, xd[j], sd);
-            }
-            const double rk = fast_rcp(xb[k * kTile + pl]);
-            xc[k] = sc * rk;
-            xd[k] = sd * rk;
-        }
-        double di = xd[0];
-#pragma unroll
-        for (int t = 1; t < M; ++t) di = (i == t) ? xd[t] : di;
-        if (act) {
-            D[rv + i * es] = di;
-            xlast = di;
-        }
-        if (has_next) {
-#pragma unroll
-            for (int k = 0; k < M; ++k) C[rm + (k + M * i) * es] = xc[k];
-        }
-#pragma unroll
-        for (int k = 0; k < M; ++k) carry[k * kTile + lane] = xc[k];  // C'(k, i) in lane base+i
-        carry[M * kTile + lane] = di;                                   // d'(i) in lane base+i
-        wave_lds_sync();
-    }
-    // back substitution over the rows: x_q(i) = d'_q(i) - sum_j C'_q(i,j) x_{q+1}(j); the
-    // next row's C'(i,:) and d'(i) are loaded while the current row is finished.
-    double xnext = xlast;
-    double cn[M], dn = 0.0;
-    auto bfetch = [&](int q) {
-        const bool act = q >= 0 && q < n - 1;
-        const int64_t rm = mo + (int64_t)(act ? q : 0) * E * kTile, rv = vo + (int64_t)(act ? q : 0) * M * kTile;
-#pragma unroll
-        for (int j = 0; j < M; ++j) cn[j] = act ? C[rm + (i + M * j) * es] : 0.0;
-        dn = act ? D[rv + i * es] : 0.0;
+        fd = D[rv + i * es];
     };
-    bfetch(nmax - 2);
-    for (int q = nmax - 2; q >= 0; --q) {
-        const bool act = q < n - 1;
-        double cq[M], x = dn;
 #pragma unroll
-        for (int j = 0; j < M; ++j) cq[j] = cn[j];
-        bfetch(q - 1);
-        sh[lane] = xnext;
-        wave_lds_sync();
+    for (int s = 0; s < F; ++s) fetch(ps[s], pg[s], pu[s], pd[s], s);
+    double dp[M];
 #pragma unroll
-        for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
-        if (act) {
-            D[vo + (int64_t)q * M * kTile + i * es] = x;
-            xnext = x;
+    for (int k = 0; k < M; ++k) {
+        dp[k] = 0.0;
+        carry[k * kTile + lane] = 0.0;
+    }
+    for (int t0 = 0; t0 < steps; t0 += F) {
+#pragma unroll
+        for (int s = 0; s < F; ++s) {
+            const int t = t0 + s;
+            const bool act = t < cnt;
+            const int q = clamp_row(top ? t : n - 1 - t);
+            const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+            double r[M], rhs[M + 1];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                r[j] = pg[s][j];
+                rhs[1 + j] = pu[s][j];
+            }
+            rhs[0] = pd[s];
+            if (t > 0) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    const double a = ps[s][k];
+                    rhs[0] = fma(-a, dp[k], rhs[0]);
+#pragma unroll
+                    for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
+                }
+            }
+            fetch(ps[s], pg[s], pu[s], pd[s], t + F);
+            double xc[M], xd[M];
+            const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
+            ok = ok && !(act && zero_pivot);
+            if (act) {
+                D[rv + i * es] = lane_elem<M>(xd, i);
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    C[rm + (k + M * i) * es] = xc[k];
+                    carry[k * kTile + lane] = xc[k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < M; ++k) dp[k] = act ? xd[k] : dp[k];
+            wave_lds_sync();
         }
-        wave_lds_sync();
+    }
+    // meeting row h: both groups of a system solve it from both carries
+    carry[M * kTile + lane] = lane_elem<M>(dp, i);
+    wave_lds_sync();
+    const int tb = top ? base : pbase, bb = top ? pbase : base;
+    double xh[M];
+    {
+        const int64_t rm = mo + (int64_t)h * E * kTile, rv = vo + (int64_t)h * M * kTile;
+        double r[M], rhs[M + 1];
+        double ah[M], ch[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            ah[k] = h > 0 ? A[rm + (i + M * k) * es] : 0.0;  // row 0 has no sub-diagonal block
+            ch[k] = C[rm + (i + M * k) * es];
+            r[k] = B[rm + (i + M * k) * es];
+            rhs[1 + k] = 0.0;
+        }
+        rhs[0] = D[rv + i * es];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            rhs[0] = fma(-ah[k], carry[M * kTile + tb + k], rhs[0]);
+            rhs[0] = fma(-ch[k], carry[M * kTile + bb + k], rhs[0]);
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                r[j] = fma(-ah[k], carry[k * kTile + tb + j], r[j]);
+                r[j] = fma(-ch[k], carry[k * kTile + bb + j], r[j]);
+            }
+        }
+        double xc[M];
+        const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xh);
+        ok = ok && !(n > 0 && zero_pivot);
+        if (n > 0 && top) D[rv + i * es] = lane_elem<M>(xh, i);
+    }
+    // outward substitution, kBwdAhead rows ahead
+    double xprev = lane_elem<M>(xh, i);
+    double cn[G][M], dn[G];
+    auto bfetch = [&](double (&fc)[M], double& fd, int t) {
+        const int q = clamp_row(top ? h - 1 - t : h + 1 + t);
+        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+#pragma unroll
+        for (int j = 0; j < M; ++j) fc[j] = C[rm + (i + M * j) * es];
+        fd = D[rv + i * es];
+    };
+    wave_lds_sync();
+#pragma unroll
+    for (int s = 0; s < G; ++s) bfetch(cn[s], dn[s], s);
+    for (int t0 = 0; t0 < steps; t0 += G) {
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            const int t = t0 + s;
+            const bool act = t < cnt;
+            const int q = clamp_row(top ? h - 1 - t : h + 1 + t);
+            double cq[M], x = dn[s];
+#pragma unroll
+            for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
+            bfetch(cn[s], dn[s], t + G);
+            sh[lane] = xprev;
+            wave_lds_sync();
+#pragma unroll
+            for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
+            if (act) {
+                D[vo + (int64_t)q * M * kTile + i * es] = x;
+                xprev = x;
+            }
+            wave_lds_sync();
+        }
     }
     return ok;
 }
@@ -411,43 +626,87 @@ __device__ __forceinline__ bool coop_item(const BatchDev& b, const double* A, co
     const int n = mine ? b.lane_rows[tile * kTile + slot] : 0;
     const int nmax = b.lane_rows[tile * kTile + lane0];  // systems are sorted longest first
     const int64_t mo = b.mat_off[tile] + (mine ? slot : 0), vo = b.vec_off[tile] + (mine ? slot : 0);
-    return coop_rows<M>(A, B, C, D, mo, vo, mine ? i : 0, mine ? g * M : lane, n, nmax, sh);
+    // lanes past the last whole group read group 0's posts and write only their own lane's
+    // slots, which no group reads
+    return coop_rows<M>(A, B, C, D, mo, vo, i, g < G ? g * M : 0, n, nmax, sh);
+}
+
+template <int M>
+__device__ __forceinline__ bool twisted_item(const BatchDev& b, const double* A, const double* B,
+                                             double* C, double* D, int tile, int lane0, double* sh) {
+    constexpr int G2 = kTile / (2 * M);  // systems per wave
+    const int lane = threadIdx.x;
+    const int g = lane / M, i = lane % M;
+    const int pair = g / 2;
+    const bool grouped = pair < G2;
+    const bool top = grouped ? (g % 2 == 0) : true;
+    const int slot = lane0 + pair;
+    const int count = b.tile_count[tile];
+    const bool mine = grouped && slot < count;
+    const int n = mine ? b.lane_rows[tile * kTile + slot] : 0;
+    const int nmax = b.lane_rows[tile * kTile + lane0];
+    const int steps = (nmax - 1) - (nmax - 1) / 2;
+    const int64_t mo = b.mat_off[tile] + (mine ? slot : 0), vo = b.vec_off[tile] + (mine ? slot : 0);
+    // lanes past the last pair act as a top group over groups 0 and 1, writing only their
+    // own lane's slots
+    const int base = grouped ? g * M : 0;
+    const int pbase = grouped ? (top ? base + M : base - M) : M;
+    return coop_twisted<M>(A, B, C, D, mo, vo, i, base, pbase, top, n, steps, nmax, sh);
 }
 
 // Persistent solve of a packed batch. KIND selects the code a placement needs, so that the
 // register allocation of the kernel is that of the paths it can run: 0 one lane per system
 // for every m (VGPRs); 1 one lane per system, m >= 6 with blocks in LDS (105 KB of LDS);
-// 2 one lane per system for m <= 5 and m lanes per system (cooperative items).
+// 2 one lane per system for m <= 5, m lanes per system (cooperative items, mode 2) and two
+// groups of m lanes per system (twisted items, mode 3).
 template <int KIND>
 __global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restrict__ A,
                                               const double* __restrict__ B, double* C, double* D,
                                               int* __restrict__ counter, int* __restrict__ flag) {
     bool ok = true;
     const int lane = threadIdx.x;
+    const int nitems = b.nitems[0];
     for (;;) {
         int t = 0;
         if (lane == 0) t = atomicAdd(counter, 1);
         t = __shfl(t, 0);
-        if (t >= b.nitems) break;
+        if (t >= nitems) break;
         const int tile = b.item_tile[t];
         const int mode = b.item_mode[t];
         const int m = b.tile_m[tile];
         if constexpr (KIND == 2) {
-            if (mode == 2) {
-                __shared__ double sh[(5 * kMaxM + 5) * kTile];
+            if (mode >= 2) {
+                __shared__ double sh[kCoopSlots * kTile];
                 const int l0 = b.item_lane0[t];
-                switch (m) {
-                    case 2: ok = coop_item<2>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                    case 3: ok = coop_item<3>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                    case 4: ok = coop_item<4>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                    case 5: ok = coop_item<5>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                    case 6: ok = coop_item<6>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                    case 7: ok = coop_item<7>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                    case 8: ok = coop_item<8>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                    case 9: ok = coop_item<9>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                    case 10: ok = coop_item<10>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                    default: break;
+                if (mode == 2) {
+                    switch (m) {
+                        case 2: ok = coop_item<2>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 3: ok = coop_item<3>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 4: ok = coop_item<4>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 5: ok = coop_item<5>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 6: ok = coop_item<6>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 7: ok = coop_item<7>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 8: ok = coop_item<8>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 9: ok = coop_item<9>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 10: ok = coop_item<10>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        default: break;
+                    }
+                } else {
+                    switch (m) {
+                        case 2: ok = twisted_item<2>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 3: ok = twisted_item<3>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 4: ok = twisted_item<4>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 5: ok = twisted_item<5>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 6: ok = twisted_item<6>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 7: ok = twisted_item<7>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 8: ok = twisted_item<8>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 9: ok = twisted_item<9>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 10: ok = twisted_item<10>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        default: break;
+                    }
                 }
+                // the next item's first posts must not overtake this item's last reads
+                wave_lds_sync();
                 continue;
             }
         }

@@ -47,22 +47,43 @@ static void batch_free(btd_batch_s* b) {
 
 // Work items for a placement policy: 0 every tile one lane per system in VGPRs; 1 m >= 6 in
 // LDS (one lane per system); 2 m >= 6 cooperative (m lanes per system, floor(64/m) systems
-// per item); 3 m >= 4 cooperative; 4 every m >= 2 cooperative. Longest estimated work first.
-static btd_status build_items(btd_batch_s* b, int placement) {
+// per item); 3 m >= 4 cooperative; 4 every m >= 2 cooperative; 5 as 4, with every item whose
+// cost exceeds a wave's mean share of the batch (total cost / grid) made twisted (two groups
+// of m lanes per system, floor(64/(2m)) systems per item, half the rows per group), so that
+// the longest systems stop setting the end of the launch. Longest estimated work first.
+static btd_status build_items(btd_batch_s* b, int placement, int grid) {
     struct Item {
         int tile, lane0, mode;
         double cost;
     };
     std::vector<Item> items;
-    const int coop_from = placement == 2 ? 6 : placement == 3 ? 4 : placement == 4 ? 2 : 1000;
+    const int coop_from = placement == 2 ? 6 : placement == 3 ? 4 : placement >= 4 ? 2 : 1000;
+    auto coop_cost = [](double rows, double mm) { return rows * mm * (mm + 4) * 4.0; };
+    double share = 0.0;  // a wave's mean share of the cooperative cost
+    if (placement == 5) {
+        for (int t = 0; t < b->ntiles; ++t) {
+            const int m = b->tile_m[t];
+            if (m < coop_from) continue;
+            const int G = btd::kTile / m;
+            for (int l0 = 0; l0 < b->tile_count[t]; l0 += G)
+                share += coop_cost(b->lane_rows[t * btd::kTile + l0], m);
+        }
+        share /= grid;
+    }
     for (int t = 0; t < b->ntiles; ++t) {
         const int m = b->tile_m[t];
         const double mm = m;
         if (m >= coop_from) {
-            const int G = btd::kTile / m;
-            for (int l0 = 0; l0 < b->tile_count[t]; l0 += G) {
-                const double rows = b->lane_rows[t * btd::kTile + l0];
-                items.push_back({t, l0, 2, rows * mm * (mm + 4) * 4.0});
+            const int G = btd::kTile / m, G2 = btd::kTile / (2 * m);
+            for (int l0 = 0; l0 < b->tile_count[t];) {
+                const int rows = b->lane_rows[t * btd::kTile + l0];
+                if (placement == 5 && G2 >= 1 && coop_cost(rows, mm) > share) {
+                    items.push_back({t, l0, 3, coop_cost((rows - 1) - (rows - 1) / 2 + 1, mm)});
+                    l0 += G2;
+                } else {
+                    items.push_back({t, l0, 2, coop_cost(rows, mm)});
+                    l0 += G;
+                }
             }
         } else {
             const int mode = (placement == 1 && m >= 6) ? 1 : 0;
@@ -71,15 +92,17 @@ static btd_status build_items(btd_batch_s* b, int placement) {
     }
     std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
     if ((int)items.size() > b->max_items) return bfail(BTD_ERR_INVALID_ARG, "batch: too many work items");
-    std::vector<int> host(3 * (size_t)b->max_items, 0);
+    // the count goes with the table (the int before it): a graph captured earlier replays
+    // whatever item list is current, never a count that does not match it
+    std::vector<int> host(1 + 3 * (size_t)b->max_items, 0);
+    host[0] = (int)items.size();
     for (size_t k = 0; k < items.size(); ++k) {
-        host[k] = items[k].tile;
-        host[b->max_items + k] = items[k].lane0;
-        host[2 * b->max_items + k] = items[k].mode;
+        host[1 + k] = items[k].tile;
+        host[1 + b->max_items + k] = items[k].lane0;
+        host[1 + 2 * b->max_items + k] = items[k].mode;
     }
-    hipError_t e = hipMemcpy(b->d_items, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(b->d_items - 1, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bfail(BTD_ERR_HIP, std::string("batch items: ") + hipGetErrorString(e));
-    b->bd.nitems = (int)items.size();
     b->placement = placement;
     return BTD_OK;
 }
@@ -138,7 +161,7 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
     // device tables in one allocation
     const size_t T = b->ntiles;
     b->max_items = (int)T * ((btd::kTile + 1) / 2);  // cooperative m = 2 cuts a tile into 2 items
-    const size_t bytes_i = sizeof(int) * (T * 3 + T * btd::kTile * 2 + 2 + 3 * (size_t)b->max_items);
+    const size_t bytes_i = sizeof(int) * (T * 3 + T * btd::kTile * 2 + 3 + 3 * (size_t)b->max_items);
     const size_t bytes = ((bytes_i + 15) / 16) * 16 + sizeof(int64_t) * 2 * T;
     std::vector<char> host(bytes, 0);
     int* hi = (int*)host.data();
@@ -151,7 +174,7 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
     };
     const size_t o_m = put(b->tile_m), o_rows = put(b->tile_rows), o_cnt = put(b->tile_count),
                  o_lr = put(b->lane_rows), o_ls = put(b->lane_sys);
-    const size_t o_counter = o, o_flag = o + 1, o_items = o + 2;
+    const size_t o_counter = o, o_flag = o + 1, o_items = o + 3;  // o + 2: the item count
     int64_t* h64 = (int64_t*)(host.data() + ((bytes_i + 15) / 16) * 16);
     std::memcpy(h64, b->mat_off.data(), sizeof(int64_t) * T);
     std::memcpy(h64 + T, b->vec_off.data(), sizeof(int64_t) * T);
@@ -165,7 +188,8 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
     int64_t* d64 = (int64_t*)((char*)b->dev + ((bytes_i + 15) / 16) * 16);
     b->d_items = di + o_items;
     b->bd = btd::BatchDev{b->d_items, b->d_items + b->max_items, b->d_items + 2 * b->max_items,
-                          di + o_m, di + o_rows, di + o_cnt, di + o_lr, d64, d64 + T, b->ntiles, 0};
+                          di + o_m, di + o_rows, di + o_cnt, di + o_lr, d64, d64 + T, b->d_items - 1,
+                          b->ntiles};
     b->d_lane_sys = di + o_ls;
     b->d_counter = di + o_counter;
     b->d_flag = di + o_flag;
@@ -175,7 +199,7 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
     }
     b->grid = 4 * cus;  // one 64-lane wave per SIMD
-    btd_status st = build_items(b, 0);
+    btd_status st = build_items(b, 0, b->grid);
     if (st != BTD_OK) {
         batch_free(b);
         return st;
@@ -208,9 +232,9 @@ btd_status btd_batch_locate(btd_batch_t b, int sys, size_t* mat_off, size_t* vec
 
 btd_status btd_batch_set_params(btd_batch_t b, int placement, int grid) {
     if (!b) return bfail(BTD_ERR_NOT_CREATED, "btd_batch_set_params: NULL batch");
-    if (placement < 0 || placement > 4 || grid < 1)
-        return bfail(BTD_ERR_INVALID_ARG, "btd_batch_set_params: placement in 0..4, grid >= 1");
-    btd_status st = build_items(b, placement);
+    if (placement < 0 || placement > 5 || grid < 1)
+        return bfail(BTD_ERR_INVALID_ARG, "btd_batch_set_params: placement in 0..5, grid >= 1");
+    btd_status st = build_items(b, placement, grid);
     if (st != BTD_OK) return st;
     b->grid = grid;
     return BTD_OK;

@@ -60,15 +60,16 @@ struct BatchDev {
     const int* item_tile;      // work items in work order: tile id,
     const int* item_lane0;     // first system slot of the item in its tile,
     const int* item_mode;      // 0 lane per system (VGPR), 1 lane per system (LDS),
-                               // 2 m lanes per system (one block row each)
+                               // 2 m lanes per system (one block row each), 3 two groups
+                               // of m lanes per system (twisted)
     const int* tile_m;         // block size of each tile
     const int* tile_rows;      // rows of the longest system of each tile
     const int* tile_count;     // systems in the tile (<= 64)
     const int* lane_rows;      // rows of each (tile, lane): [tile*64 + lane]
     const int64_t* mat_off;    // first matrix element of each tile
     const int64_t* vec_off;    // first vector element of each tile
+    const int* nitems;         // number of work items (device, written with the item table)
     int ntiles;
-    int nitems;
 };
 
 hipError_t launch_batch(const BatchDev& b, int kind, int grid, const double* A, const double* B,

@@ -1,8 +1,9 @@
 """Autotuner for the launch choices of the solver, by measurement on the device.
 
 * ``tune_batch``: for a mixed-size batch, the placement (one lane per system with blocks
-  in VGPRs or in LDS, or m lanes per system from a block size up) and the persistent grid
-  size (64-lane workgroups per CU).
+  in VGPRs or in LDS, m lanes per system from a block size up, or the same with the longest
+  systems solved twisted from both ends) and the persistent grid size (64-lane workgroups
+  per CU).
 * ``tune_partitions``: for long systems, the number of row partitions P of the two-stage
   solve (P partitions per system become P lanes; the reduced system has 2P rows).
 
@@ -38,7 +39,7 @@ def _time(fn, restore, reps: int) -> float:
 
 
 def tune_batch(batch: BtdmaBatch, a, b, c, d, reps: int = 3,
-               placements=(0, 1, 2, 3, 4), grids_per_cu=(1, 2, 4)) -> dict:
+               placements=(0, 1, 2, 3, 4, 5), grids_per_cu=(1, 2, 4)) -> dict:
     """Pick (placement, grid) for ``batch``; a, b, c, d are its packed inputs (unchanged)."""
     cw, dw = c.clone(), d.clone()
 

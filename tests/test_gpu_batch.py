@@ -46,12 +46,15 @@ def test_batch_generator_matches_host(cuda):
         np.testing.assert_array_equal(bt.extract(hd, s), dd[:, 0])
 
 
-@pytest.mark.parametrize("placement", [0, 1, 2, 3, 4])
-def test_batch_matches_oracle(cuda, oracle, placement):
+@pytest.mark.parametrize("placement,grid", [(0, 64), (1, 64), (2, 64), (3, 64), (4, 64), (5, 64),
+                                            (5, 100000)])
+def test_batch_matches_oracle(cuda, oracle, placement, grid):
+    """Every placement against the oracle; placement 5 twists the items above total/grid, so
+    grid 100000 makes every cooperative item twisted (n from 2: meeting rows 0 and 1)."""
     rng = np.random.default_rng(placement + 1)
     m, n = rand_sizes(rng, 300)
     bt, a, b, c, d = make_batch(cuda, m, n, pb.GenKind.DDPERM, 5)
-    bt.set_params(placement, 64)
+    bt.set_params(placement, grid)
     bt.solve(a, b, c, d)
     assert not bt.check()
     hd = d.cpu().numpy()
@@ -60,7 +63,62 @@ def test_batch_matches_oracle(cuda, oracle, placement):
         _, xo = oracle_system(oracle, bt, pb.GenKind.DDPERM, 5, s)
         x = bt.extract(hd, s)
         worst = max(worst, np.abs(x - xo).max() / np.abs(xo).max())
-    print(f"placement {placement}: {bt.ntiles} tiles, max rel err {worst:.2e}")
+    print(f"placement {placement} grid {grid}: {bt.ntiles} tiles, max rel err {worst:.2e}")
+    assert worst <= TOL
+
+
+@pytest.mark.parametrize("placement,grid", [(0, 64), (1, 64), (4, 64), (5, 100000)])
+def test_batch_zero_pivot_flag(cuda, placement, grid):
+    """A system with an all-zero block row is reported by btd_batch_check in every placement
+    (twisted: the row falls in the bottom half), a clean batch is not, and the report clears."""
+    import torch
+    rng = np.random.default_rng(3)
+    m, n = rand_sizes(rng, 120, mset=(2, 5, 8, 10))
+    bt, a, b, c0, d0 = make_batch(cuda, m, n)
+    bt.set_params(placement, grid)
+    c, d = c0.clone(), d0.clone()
+    bt.solve(a, b, c, d)
+    assert not bt.check()
+    s = int(np.argmax(n))
+    mm, row = int(m[s]), 2 * int(n[s]) // 3
+    mo, _ = bt.locate(s)
+    idx = torch.tensor([mo + (row * mm * mm + e) * 64 for e in range(mm * mm)], device=cuda)
+    for t in (a, b, c0):
+        t[idx] = 0.0
+    c, d = c0.clone(), d0.clone()
+    bt.solve(a, b, c, d)
+    assert bt.check()
+    assert not bt.check()
+
+
+def test_graph_replays_current_item_list(cuda, oracle):
+    """A graph captured before set_params changes the work items replays the current item
+    table with its own count (both live on the device), so every system is still solved."""
+    import torch
+    rng = np.random.default_rng(21)
+    m, n = rand_sizes(rng, 200, mset=(2, 5, 10), nmax=120)
+    bt, a, b, c0, d0 = make_batch(cuda, m, n)
+    bt.set_params(5, 64)
+    c, d = c0.clone(), d0.clone()
+    s1 = torch.cuda.Stream()
+    g = pb.Graph()
+    torch.cuda.synchronize()
+    g.begin(s1)
+    bt.solve(a, b, c, d, stream=s1)
+    g.end(s1)
+    bt.set_params(5, 100000)  # every cooperative item twisted: a longer item list
+    c.copy_(c0)
+    d.copy_(d0)
+    torch.cuda.synchronize()
+    g.launch(s1)
+    s1.synchronize()
+    g.destroy()
+    assert not bt.check()
+    hd = d.cpu().numpy()
+    worst = 0.0
+    for s in range(bt.nsys):
+        _, xo = oracle_system(oracle, bt, pb.GenKind.DDPERM, 5, s)
+        worst = max(worst, np.abs(bt.extract(hd, s) - xo).max() / np.abs(xo).max())
     assert worst <= TOL
 
 
