@@ -358,7 +358,7 @@ def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
             "graph_candidates_ms": {f"{k[0]},grid={k[1]}": v[1] for k, v in graphs.items()},
             "batch_only_ms": float(np.median(tb_only)), "long_only_ms": float(np.median(tl_only)),
             "long_only_GBs": bytes_l / (np.median(tl_only) * 1e-3) / 1e9,
-            "tuned_batch": tb, "tuned_partitions": tp, "batch_tiles": bt.ntiles,
+            "tuned_batch": tb, "tuned_partitions": tp, "batch_packs": bt.npacks,
             "max_rel_err_vs_oracle_sampled": None,
             "parity_sample": "every 997th batch system and the first 2 long systems"}
     sink = max_sink(out, "max_rel_err_vs_oracle_sampled")

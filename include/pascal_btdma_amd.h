@@ -256,18 +256,20 @@ btd_status btd_generate_host(int kind, uint64_t seed, int m, int nsys, int nrow,
 
 /* ---- Mixed-size batches (no counterpart in the reference, whose plan has one m and one
  * nrow): systems of different block size m (1..10) and row count n (>= 2) solved by one
- * persistent kernel that takes tiles of 64 same-m systems from a work queue in
- * longest-first order. Packed layout: element e of row `row` of system s is at
- * mat_off(s) + (row*m*m + e)*64 for blocks and vec_off(s) + (row*m + e)*64 for vectors
- * (btd_batch_locate). In place like btd_solve_noncyclic. */
+ * persistent kernel that takes work items (packs of same-m systems) from a queue in
+ * longest-first order. Packed layout: systems of one m, longest first, are interleaved in
+ * packs of W systems (W = 64 for m = 1, floor(32/m) otherwise): element e of row `row` of
+ * system s is at mat_off(s) + (row*m*m + e)*W for blocks and vec_off(s) + (row*m + e)*W for
+ * vectors (btd_batch_locate gives both offsets and W). In place like btd_solve_noncyclic. */
 typedef struct btd_batch_s *btd_batch_t;
 
 /* m[s], n[s] for s < nsys (host arrays, copied). */
 btd_status btd_batch_create(btd_batch_t *batch, int nsys, const int *m, const int *n);
 btd_status btd_batch_destroy(btd_batch_t batch);
 /* Doubles needed by the packed block arrays (A, B, C) and vector arrays (D). */
-btd_status btd_batch_sizes(btd_batch_t batch, size_t *mat_elems, size_t *vec_elems, int *ntiles);
-btd_status btd_batch_locate(btd_batch_t batch, int sys, size_t *mat_off, size_t *vec_off);
+btd_status btd_batch_sizes(btd_batch_t batch, size_t *mat_elems, size_t *vec_elems, int *npacks);
+/* Where system sys starts in the packed arrays and the stride W between its elements. */
+btd_status btd_batch_locate(btd_batch_t batch, int sys, size_t *mat_off, size_t *vec_off, int *stride);
 /* Launch parameters (what the autotuner picks): placement 0 = one lane per system, blocks
  * in VGPRs; 1 = as 0 but m >= 6 with blocks in LDS; 2, 3, 4 = m lanes per system (one block
  * row per lane, pivot search and row exchange through LDS) for m >= 6, m >= 4, m >= 2;

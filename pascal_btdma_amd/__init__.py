@@ -408,9 +408,10 @@ def sweep_scatter(plan: BtdmaGpuPlan, axis: int, d, n1: int, n2: int, n3: int, f
 
 class BtdmaBatch:
     """A batch of systems with their own block size m[s] (1..10) and row count n[s] (>= 2),
-    packed into 64-system tiles of equal m and solved by one persistent kernel.
-    Element e of block row `row` of system s sits at locate(s)[0] + (row*m*m + e)*64 in the
-    block arrays and at locate(s)[1] + (row*m + e)*64 in the vector array."""
+    packed into interleaved packs of equal m (64 systems for m = 1, floor(32/m) otherwise)
+    and solved by one persistent kernel. With (mo, vo, w) = locate(s), element e of block row
+    `row` of system s sits at mo + (row*m*m + e)*w in the block arrays and at
+    vo + (row*m + e)*w in the vector array."""
 
     def __init__(self, m, n):
         import numpy as np
@@ -424,17 +425,19 @@ class BtdmaBatch:
                                             self.m.ctypes.data_as(ip), self.n.ctypes.data_as(ip)))
         me, ve, nt = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
         _check(_capi.lib().btd_batch_sizes(self._h, ctypes.byref(me), ctypes.byref(ve), ctypes.byref(nt)))
-        self.mat_elems, self.vec_elems, self.ntiles = me.value, ve.value, nt.value
+        self.mat_elems, self.vec_elems, self.npacks = me.value, ve.value, nt.value
         self.placement, self.grid = 0, None
 
     @property
     def nsys(self) -> int:
         return len(self.m)
 
-    def locate(self, sys: int) -> tuple[int, int]:
-        mo, vo = ctypes.c_size_t(), ctypes.c_size_t()
-        _check(_capi.lib().btd_batch_locate(self._h, int(sys), ctypes.byref(mo), ctypes.byref(vo)))
-        return mo.value, vo.value
+    def locate(self, sys: int) -> tuple[int, int, int]:
+        """(first block element, first vector element, element stride) of system sys."""
+        mo, vo, w = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
+        _check(_capi.lib().btd_batch_locate(self._h, int(sys), ctypes.byref(mo), ctypes.byref(vo),
+                                            ctypes.byref(w)))
+        return mo.value, vo.value, w.value
 
     def set_params(self, placement: int, grid: int) -> None:
         _check(_capi.lib().btd_batch_set_params(self._h, int(placement), int(grid)))
@@ -461,10 +464,10 @@ class BtdmaBatch:
         """Host copy of one system's vector (n, m) or blocks (n, m, m) from a packed array."""
         import numpy as np
         m, n = int(self.m[sys]), int(self.n[sys])
-        mo, vo = self.locate(sys)
+        mo, vo, w = self.locate(sys)
         E = m if vector else m * m
         base = vo if vector else mo
-        idx = base + (np.arange(n)[:, None] * E + np.arange(E)[None, :]) * 64
+        idx = base + (np.arange(n)[:, None] * E + np.arange(E)[None, :]) * w
         vals = flat[idx]
         return vals if vector else np.swapaxes(vals.reshape(n, m, m), -1, -2)
 

@@ -63,7 +63,7 @@ SIGNATURES = {
     "btd_batch_sizes": (_i, [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
                              ctypes.POINTER(_i)]),
     "btd_batch_locate": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_size_t),
-                              ctypes.POINTER(ctypes.c_size_t)]),
+                              ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(_i)]),
     "btd_batch_set_params": (_i, [_vp, _i, _i]),
     "btd_batch_generate": (_i, [_vp, _i, _u64, _dp, _dp, _dp, _dp, _vp]),
     "btd_batch_solve": (_i, [_vp, _dp, _dp, _dp, _dp, _vp]),

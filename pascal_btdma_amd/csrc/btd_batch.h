@@ -1,16 +1,18 @@
 // Mixed-size batches: systems of different block size m and row count n solved by one
 // persistent kernel.
 //
-// Packing: systems are grouped by m, sorted by n (longest first) and cut into tiles of up
-// to 64 systems of the same m. Each tile is stored like the tiled layout with its own row
-// count (the longest system of the tile): element e of row `row` of the system in lane l of
-// tile t is at tile_mat_off[t] + (row*E + e)*64 + l (E = m*m), vectors likewise with E = m.
-// Work items: a whole tile (one lane per system) or, for the cooperative placements, a
-// chunk of floor(64/m) systems of a tile with m lanes per system, or of floor(64/(2m))
-// systems with two groups of m lanes per system (twisted). Items are ordered by
-// decreasing estimated cost (longest-processing-time first) and consumed through an
-// atomic counter by a grid sized to the device; every wave exits when the counter passes
-// the number of items.
+// Packing: systems are grouped by m, sorted by n (longest first) and cut into packs of
+// W = batch_width(m) systems (64 for m = 1, else floor(32/m): the systems of one twisted work
+// item). A pack is stored interleaved with its own row count (its longest system's):
+// element e of row `row` of the system in slot s of pack p is at mat_off[p] + (row*E + e)*W + s
+// (E = m*m), vectors likewise with E = m. So the W systems of a pack that a work item solves
+// together occupy whole, contiguous cache lines: a cooperative load instruction (one block
+// element per lane, m lanes per system) reads M*W consecutive doubles.
+// Work items (consecutive packs of one m): one lane per system over floor(64/W) packs;
+// cooperative, m lanes per system over 2 packs; or twisted, two groups of m lanes per system
+// over 1 pack. Items are ordered by decreasing estimated cost (longest-processing-time
+// first) and consumed through an atomic counter by a grid sized to the device; every wave
+// exits when the counter passes the number of items.
 #pragma once
 
 #include "btd_kernels.h"
@@ -23,11 +25,11 @@ template <int M>
 __device__ __forceinline__ bool plain_lane(const double* __restrict__ A, const double* __restrict__ B,
                                            double* C, double* D, int64_t mo, int64_t vo, int n) {
     constexpr int E = M * M;
-    const int64_t es = kTile;
+    constexpr int64_t es = batch_width(M);
     double Cp[M][M], dp[M];
     bool ok = true;
     for (int q = 0; q < n; ++q) {
-        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
         double R[M][M], v[M];
 #pragma unroll
         for (int j = 0; j < M; ++j)
@@ -73,7 +75,7 @@ __device__ __forceinline__ bool plain_lane(const double* __restrict__ A, const d
         }
     }
     for (int q = n - 2; q >= 0; --q) {
-        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
         double x[M];
 #pragma unroll
         for (int i = 0; i < M; ++i) x[i] = D[rv + i * es];
@@ -100,7 +102,7 @@ __device__ bool plain_lane_lds(const double* __restrict__ A, const double* __res
                                double* __restrict__ sR, double* __restrict__ sC,
                                int* __restrict__ sP) {
     constexpr int E = M * M;
-    const int64_t es = kTile;
+    constexpr int64_t es = batch_width(M);
     const int lane = threadIdx.x;
 #define BTD_R(i, j) sR[((i) + M * (j)) * kTile + lane]
 #define BTD_CP(i, j) sC[((i) + M * (j)) * kTile + lane]
@@ -108,7 +110,7 @@ __device__ bool plain_lane_lds(const double* __restrict__ A, const double* __res
     double dp[M];
     bool ok = true;
     for (int q = 0; q < n; ++q) {
-        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
         double v[M];
         for (int e = 0; e < E; ++e) sR[e * kTile + lane] = __builtin_nontemporal_load(B + rm + e * es);
 #pragma unroll
@@ -199,7 +201,7 @@ __device__ bool plain_lane_lds(const double* __restrict__ A, const double* __res
 #undef BTD_CP
 #undef BTD_P
     for (int q = n - 2; q >= 0; --q) {
-        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
         double x[M];
 #pragma unroll
         for (int i = 0; i < M; ++i) x[i] = D[rv + i * es];
@@ -342,13 +344,14 @@ __device__ __forceinline__ double lane_elem(const double (&v)[M], int i) {
 // C'_q stored over C_q and carried in LDS, d'_q over d_q and carried whole in registers;
 // then x_q = d'_q - C'_q x_{q+1} upwards, x_{q+1} exchanged through LDS. Exchanges are
 // ordered by wave_lds_sync only. Rows are loaded kFwdAhead (forward) and kBwdAhead
-// (backward) rows ahead; every row below nmax is allocated for every lane of the tile, so
-// loads need no guard (rows past the ends are clamped). All lanes of the wave run the same
-// row steps (nmax rounded up to the prefetch depth); lanes past their system's n compute on
-// what the rows hold and store nothing.
+// (backward) rows ahead; every row below `rows` (the lane's pack's) is allocated for every
+// slot of the pack, so loads are clamped to that range and need no other guard. All lanes of
+// the wave run the same row steps (nmax, the item's longest system, rounded up to the
+// prefetch depth); lanes past their system's n compute on what the rows hold and store
+// nothing.
 template <int M>
 __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict__ B, double* C,
-                          double* D, int64_t mo, int64_t vo, int i, int base, int n, int nmax,
+                          double* D, int64_t mo, int64_t vo, int i, int base, int n, int nmax, int rows,
                           double* __restrict__ sh) {
     constexpr int E = M * M;
     constexpr int F = CoopDepth<M>::kFwdAhead;
@@ -356,12 +359,12 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
     double* carry = sh;
     double* xb = sh + (M + 1) * kTile;
     const int lane = threadIdx.x;
-    const int64_t es = kTile;
+    constexpr int64_t es = batch_width(M);
     bool ok = true;
     double pa[F][M], pb[F][M], pc[F][M], pd[F];
     auto fetch = [&](double (&fa)[M], double (&fb)[M], double (&fc)[M], double& fd, int q) {
-        q = q < nmax ? q : nmax - 1;
-        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        q = q < rows ? q : rows - 1;
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             fa[k] = __builtin_nontemporal_load(A + rm + (i + M * k) * es);
@@ -382,7 +385,7 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
             const int q = q0 + s;
             const bool act = q < n;
             const bool has_next = q + 1 < n;
-            const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+            const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
             double r[M], rhs[M + 1];
 #pragma unroll
             for (int j = 0; j < M; ++j) {
@@ -421,12 +424,12 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
         }
     }
     // back substitution over the rows: x_q(i) = d'_q(i) - sum_j C'_q(i,j) x_{q+1}(j), with
-    // C'_q(i, :) and d'_q(i) loaded kBwdAhead rows ahead (rows below 0 clamp to row 0).
+    // C'_q(i, :) and d'_q(i) loaded kBwdAhead rows ahead (rows are clamped to the lane's pack).
     double xnext = xlast;
     double cn[G][M], dn[G];
     auto bfetch = [&](double (&fc)[M], double& fd, int q) {
-        q = q < 0 ? 0 : q;
-        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        q = q < 0 ? 0 : (q < rows ? q : rows - 1);  // this lane's pack may be shorter than nmax
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
 #pragma unroll
         for (int j = 0; j < M; ++j) fc[j] = C[rm + (i + M * j) * es];
         fd = D[rv + i * es];
@@ -447,7 +450,7 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
 #pragma unroll
             for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
             if (act) {
-                D[vo + (int64_t)q * M * kTile + i * es] = x;
+                D[vo + (int64_t)q * M * es + i * es] = x;
                 xnext = x;
             }
             wave_lds_sync();
@@ -466,29 +469,30 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
 // (q > h). Each group handles half of the rows, so a long system takes half the time of
 // coop_rows for the same arithmetic. The groups of a system are base and pbase; `steps`
 // (uniform over the wave) is the largest row count of a group of the item (the bottom
-// group of its longest system, whose rows nmax bound every row loaded). A group's carried
-// block and rhs stay as its last row left them while the other groups finish.
+// group of its longest system); every row loaded is clamped below `rows`, the pack's row
+// count. A group's carried block and rhs stay as its last row left them while the other
+// groups finish.
 template <int M>
 __device__ bool coop_twisted(const double* A, const double* __restrict__ B, double* C, double* D,
                              int64_t mo, int64_t vo, int i, int base, int pbase, bool top, int n,
-                             int steps, int nmax, double* __restrict__ sh) {
+                             int steps, int rows, double* __restrict__ sh) {
     constexpr int E = M * M;
     constexpr int F = CoopDepth<M>::kFwdAhead;
     constexpr int G = CoopDepth<M>::kBwdAhead;
     double* carry = sh;
     double* xb = sh + (M + 1) * kTile;
     const int lane = threadIdx.x;
-    const int64_t es = kTile;
+    constexpr int64_t es = batch_width(M);
     const int h = n > 0 ? (n - 1) / 2 : 0;
     const int cnt = n > 0 ? (top ? h : n - 1 - h) : 0;
     const double* sub = top ? A : C;  // the block coupling row q to the row eliminated before it
     const double* sup = top ? C : A;  // the block coupling row q to the row eliminated after it
-    auto clamp_row = [&](int q) { return q < 0 ? 0 : (q >= nmax ? nmax - 1 : q); };
+    auto clamp_row = [&](int q) { return q < 0 ? 0 : (q >= rows ? rows - 1 : q); };
     bool ok = true;
     double ps[F][M], pg[F][M], pu[F][M], pd[F];
     auto fetch = [&](double (&fs)[M], double (&fg)[M], double (&fu)[M], double& fd, int t) {
         const int q = clamp_row(top ? t : n - 1 - t);
-        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             fs[k] = sub[rm + (i + M * k) * es];
@@ -511,7 +515,7 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
             const int t = t0 + s;
             const bool act = t < cnt;
             const int q = clamp_row(top ? t : n - 1 - t);
-            const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+            const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
             double r[M], rhs[M + 1];
 #pragma unroll
             for (int j = 0; j < M; ++j) {
@@ -551,7 +555,7 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
     const int tb = top ? base : pbase, bb = top ? pbase : base;
     double xh[M];
     {
-        const int64_t rm = mo + (int64_t)h * E * kTile, rv = vo + (int64_t)h * M * kTile;
+        const int64_t rm = mo + (int64_t)h * E * es, rv = vo + (int64_t)h * M * es;
         double r[M], rhs[M + 1];
         double ah[M], ch[M];
 #pragma unroll
@@ -582,7 +586,7 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
     double cn[G][M], dn[G];
     auto bfetch = [&](double (&fc)[M], double& fd, int t) {
         const int q = clamp_row(top ? h - 1 - t : h + 1 + t);
-        const int64_t rm = mo + (int64_t)q * E * kTile, rv = vo + (int64_t)q * M * kTile;
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
 #pragma unroll
         for (int j = 0; j < M; ++j) fc[j] = C[rm + (i + M * j) * es];
         fd = D[rv + i * es];
@@ -605,7 +609,7 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
 #pragma unroll
             for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
             if (act) {
-                D[vo + (int64_t)q * M * kTile + i * es] = x;
+                D[vo + (int64_t)q * M * es + i * es] = x;
                 xprev = x;
             }
             wave_lds_sync();
@@ -614,39 +618,39 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
     return ok;
 }
 
+// A cooperative item: the systems of packs p0 (and p0 + 1 when npack = 2), m lanes each.
 template <int M>
 __device__ __forceinline__ bool coop_item(const BatchDev& b, const double* A, const double* B,
-                                          double* C, double* D, int tile, int lane0, double* sh) {
-    constexpr int G = kTile / M;  // systems per wave
+                                          double* C, double* D, int p0, int npack, double* sh) {
+    constexpr int W = batch_width(M);
     const int lane = threadIdx.x;
     const int g = lane / M, i = lane % M;
-    const int slot = lane0 + g;
-    const int count = b.tile_count[tile];
-    const bool mine = g < G && slot < count;
-    const int n = mine ? b.lane_rows[tile * kTile + slot] : 0;
-    const int nmax = b.lane_rows[tile * kTile + lane0];  // systems are sorted longest first
-    const int64_t mo = b.mat_off[tile] + (mine ? slot : 0), vo = b.vec_off[tile] + (mine ? slot : 0);
+    const bool grouped = g < 2 * W && g / W < npack;
+    const int pk = grouped ? p0 + g / W : p0, slot = g % W;
+    const bool mine = grouped && slot < b.pack_count[pk];
+    const int n = mine ? b.slot_rows[b.pack_slot0[pk] + slot] : 0;
+    const int nmax = b.pack_rows[p0];  // systems are sorted longest first
+    const int64_t mo = b.mat_off[pk] + (mine ? slot : 0), vo = b.vec_off[pk] + (mine ? slot : 0);
     // lanes past the last whole group read group 0's posts and write only their own lane's
     // slots, which no group reads
-    return coop_rows<M>(A, B, C, D, mo, vo, i, g < G ? g * M : 0, n, nmax, sh);
+    return coop_rows<M>(A, B, C, D, mo, vo, i, grouped ? g * M : 0, n, nmax, b.pack_rows[pk], sh);
 }
 
+// A twisted item: the systems of pack p, two groups of m lanes each.
 template <int M>
 __device__ __forceinline__ bool twisted_item(const BatchDev& b, const double* A, const double* B,
-                                             double* C, double* D, int tile, int lane0, double* sh) {
-    constexpr int G2 = kTile / (2 * M);  // systems per wave
+                                             double* C, double* D, int p, double* sh) {
+    constexpr int W = batch_width(M);
     const int lane = threadIdx.x;
     const int g = lane / M, i = lane % M;
-    const int pair = g / 2;
-    const bool grouped = pair < G2;
+    const int slot = g / 2;
+    const bool grouped = slot < W;
     const bool top = grouped ? (g % 2 == 0) : true;
-    const int slot = lane0 + pair;
-    const int count = b.tile_count[tile];
-    const bool mine = grouped && slot < count;
-    const int n = mine ? b.lane_rows[tile * kTile + slot] : 0;
-    const int nmax = b.lane_rows[tile * kTile + lane0];
+    const bool mine = grouped && slot < b.pack_count[p];
+    const int n = mine ? b.slot_rows[b.pack_slot0[p] + slot] : 0;
+    const int nmax = b.pack_rows[p];
     const int steps = (nmax - 1) - (nmax - 1) / 2;
-    const int64_t mo = b.mat_off[tile] + (mine ? slot : 0), vo = b.vec_off[tile] + (mine ? slot : 0);
+    const int64_t mo = b.mat_off[p] + (mine ? slot : 0), vo = b.vec_off[p] + (mine ? slot : 0);
     // lanes past the last pair act as a top group over groups 0 and 1, writing only their
     // own lane's slots
     const int base = grouped ? g * M : 0;
@@ -671,37 +675,36 @@ __global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restri
         if (lane == 0) t = atomicAdd(counter, 1);
         t = __shfl(t, 0);
         if (t >= nitems) break;
-        const int tile = b.item_tile[t];
+        const int p0 = b.item_pack[t], npack = b.item_npack[t];
         const int mode = b.item_mode[t];
-        const int m = b.tile_m[tile];
+        const int m = b.pack_m[p0];
         if constexpr (KIND == 2) {
             if (mode >= 2) {
                 __shared__ double sh[kCoopSlots * kTile];
-                const int l0 = b.item_lane0[t];
                 if (mode == 2) {
                     switch (m) {
-                        case 2: ok = coop_item<2>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 3: ok = coop_item<3>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 4: ok = coop_item<4>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 5: ok = coop_item<5>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 6: ok = coop_item<6>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 7: ok = coop_item<7>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 8: ok = coop_item<8>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 9: ok = coop_item<9>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 10: ok = coop_item<10>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 2: ok = coop_item<2>(b, A, B, C, D, p0, npack, sh) && ok; break;
+                        case 3: ok = coop_item<3>(b, A, B, C, D, p0, npack, sh) && ok; break;
+                        case 4: ok = coop_item<4>(b, A, B, C, D, p0, npack, sh) && ok; break;
+                        case 5: ok = coop_item<5>(b, A, B, C, D, p0, npack, sh) && ok; break;
+                        case 6: ok = coop_item<6>(b, A, B, C, D, p0, npack, sh) && ok; break;
+                        case 7: ok = coop_item<7>(b, A, B, C, D, p0, npack, sh) && ok; break;
+                        case 8: ok = coop_item<8>(b, A, B, C, D, p0, npack, sh) && ok; break;
+                        case 9: ok = coop_item<9>(b, A, B, C, D, p0, npack, sh) && ok; break;
+                        case 10: ok = coop_item<10>(b, A, B, C, D, p0, npack, sh) && ok; break;
                         default: break;
                     }
                 } else {
                     switch (m) {
-                        case 2: ok = twisted_item<2>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 3: ok = twisted_item<3>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 4: ok = twisted_item<4>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 5: ok = twisted_item<5>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 6: ok = twisted_item<6>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 7: ok = twisted_item<7>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 8: ok = twisted_item<8>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 9: ok = twisted_item<9>(b, A, B, C, D, tile, l0, sh) && ok; break;
-                        case 10: ok = twisted_item<10>(b, A, B, C, D, tile, l0, sh) && ok; break;
+                        case 2: ok = twisted_item<2>(b, A, B, C, D, p0, sh) && ok; break;
+                        case 3: ok = twisted_item<3>(b, A, B, C, D, p0, sh) && ok; break;
+                        case 4: ok = twisted_item<4>(b, A, B, C, D, p0, sh) && ok; break;
+                        case 5: ok = twisted_item<5>(b, A, B, C, D, p0, sh) && ok; break;
+                        case 6: ok = twisted_item<6>(b, A, B, C, D, p0, sh) && ok; break;
+                        case 7: ok = twisted_item<7>(b, A, B, C, D, p0, sh) && ok; break;
+                        case 8: ok = twisted_item<8>(b, A, B, C, D, p0, sh) && ok; break;
+                        case 9: ok = twisted_item<9>(b, A, B, C, D, p0, sh) && ok; break;
+                        case 10: ok = twisted_item<10>(b, A, B, C, D, p0, sh) && ok; break;
                         default: break;
                     }
                 }
@@ -710,9 +713,12 @@ __global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restri
                 continue;
             }
         }
-        if (lane < b.tile_count[tile]) {
-            const int n = b.lane_rows[tile * kTile + lane];
-            const int64_t mo = b.mat_off[tile] + lane, vo = b.vec_off[tile] + lane;
+        // one lane per system: lane l holds slot l % W of pack p0 + l / W
+        const int W = batch_width(m);
+        const int pk = p0 + lane / W, slot = lane % W;
+        if (lane / W < npack && slot < b.pack_count[pk]) {
+            const int n = b.slot_rows[b.pack_slot0[pk] + slot];
+            const int64_t mo = b.mat_off[pk] + slot, vo = b.vec_off[pk] + slot;
             if constexpr (KIND == 1) {
                 if (mode == 1) {
                     __shared__ double sR[kMaxM * kMaxM * kTile];
@@ -756,45 +762,47 @@ __global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restri
 // Work-counter reset as a kernel (a memset node would not replay reliably in a graph).
 __global__ void k_batch_reset(int* counter) { counter[0] = 0; }
 
-// Generator for a packed batch: one thread per (tile, row, lane).
+// Generator for a packed batch: one thread per (pack, row, slot).
 template <int M>
 __device__ __forceinline__ void gen_batch_elem(int kind, uint64_t seed, int64_t sys, int row, int n,
                                                int64_t mo, int64_t vo, double* A, double* B,
                                                double* C, double* D) {
+    constexpr int64_t es = batch_width(M);
     double a[M][M], bb[M][M], c[M][M], d[M];
     gen_row<M>(kind, seed, sys, row, n, a, bb, c, d);
-    const int64_t rm = mo + (int64_t)row * M * M * kTile, rv = vo + (int64_t)row * M * kTile;
+    const int64_t rm = mo + (int64_t)row * M * M * es, rv = vo + (int64_t)row * M * es;
     for (int j = 0; j < M; ++j)
         for (int i = 0; i < M; ++i) {
-            A[rm + (i + M * j) * kTile] = a[i][j];
-            B[rm + (i + M * j) * kTile] = bb[i][j];
-            C[rm + (i + M * j) * kTile] = c[i][j];
+            A[rm + (i + M * j) * es] = a[i][j];
+            B[rm + (i + M * j) * es] = bb[i][j];
+            C[rm + (i + M * j) * es] = c[i][j];
         }
-    for (int i = 0; i < M; ++i) D[rv + i * kTile] = d[i];
+    for (int i = 0; i < M; ++i) D[rv + i * es] = d[i];
 }
 
-__global__ __launch_bounds__(256) void k_batch_generate(BatchDev b, const int* __restrict__ lane_sys,
+__global__ __launch_bounds__(256) void k_batch_generate(BatchDev b, const int* __restrict__ slot_sys,
                                                         int kind, uint64_t seed, double* A, double* B,
                                                         double* C, double* D) {
-    const int tile = blockIdx.y;
-    const int row = blockIdx.x * 4 + threadIdx.x / kTile, lane = threadIdx.x % kTile;
-    if (lane >= b.tile_count[tile]) return;
-    const int n = b.lane_rows[tile * kTile + lane];
-    if (row >= n) return;
-    const int64_t sys = lane_sys[tile * kTile + lane];
-    const int64_t mo = b.mat_off[tile] + lane, vo = b.vec_off[tile] + lane;
-    switch (b.tile_m[tile]) {
-        case 1: gen_batch_elem<1>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        case 2: gen_batch_elem<2>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        case 3: gen_batch_elem<3>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        case 4: gen_batch_elem<4>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        case 5: gen_batch_elem<5>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        case 6: gen_batch_elem<6>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        case 7: gen_batch_elem<7>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        case 8: gen_batch_elem<8>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        case 9: gen_batch_elem<9>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        case 10: gen_batch_elem<10>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-        default: break;
+    const int row = blockIdx.x * 4 + threadIdx.x / kTile, slot = threadIdx.x % kTile;
+    for (int p = blockIdx.y; p < b.npacks; p += gridDim.y) {
+        if (slot >= b.pack_count[p]) continue;
+        const int n = b.slot_rows[b.pack_slot0[p] + slot];
+        if (row >= n) continue;
+        const int64_t sys = slot_sys[b.pack_slot0[p] + slot];
+        const int64_t mo = b.mat_off[p] + slot, vo = b.vec_off[p] + slot;
+        switch (b.pack_m[p]) {
+            case 1: gen_batch_elem<1>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 2: gen_batch_elem<2>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 3: gen_batch_elem<3>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 4: gen_batch_elem<4>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 5: gen_batch_elem<5>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 6: gen_batch_elem<6>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 7: gen_batch_elem<7>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 8: gen_batch_elem<8>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 9: gen_batch_elem<9>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 10: gen_batch_elem<10>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            default: break;
+        }
     }
 }
 

@@ -1,4 +1,6 @@
 // Launchers for the mixed-size batch kernels (btd_batch.h).
+#include <algorithm>
+
 #include "btd_batch.h"
 
 namespace btd {
@@ -15,11 +17,11 @@ hipError_t launch_batch(const BatchDev& b, int kind, int grid, const double* A, 
     return hipGetLastError();
 }
 
-hipError_t launch_batch_generate(const BatchDev& b, const int* lane_sys, int max_rows, int kind,
+hipError_t launch_batch_generate(const BatchDev& b, const int* slot_sys, int max_rows, int kind,
                                  uint64_t seed, double* A, double* B, double* C, double* D,
                                  hipStream_t s) {
-    const dim3 grid((unsigned)((max_rows + 3) / 4), (unsigned)b.ntiles), block(256);
-    hipLaunchKernelGGL(k_batch_generate, grid, block, 0, s, b, lane_sys, kind, seed, A, B, C, D);
+    const dim3 grid((unsigned)((max_rows + 3) / 4), (unsigned)std::min(b.npacks, 65535)), block(256);
+    hipLaunchKernelGGL(k_batch_generate, grid, block, 0, s, b, slot_sys, kind, seed, A, B, C, D);
     return hipGetLastError();
 }
 

@@ -21,18 +21,19 @@ btd_status bfail(btd_status s, const std::string& m) {
 }  // namespace
 
 struct btd_batch_s {
-    int nsys = 0, ntiles = 0, max_rows = 0;
+    int nsys = 0, npacks = 0, max_rows = 0;
     std::vector<int> m, n;
-    std::vector<int> tile_m, tile_rows, tile_count, lane_rows, lane_sys;
+    // packs: systems of one m, sorted longest first, batch_width(m) per pack
+    std::vector<int> pack_m, pack_rows, pack_count, pack_slot0, slot_rows, slot_sys;
     std::vector<int64_t> mat_off, vec_off;
-    std::vector<int> sys_tile, sys_lane;
+    std::vector<int> sys_pack, sys_slot;
     int64_t mat_elems = 0, vec_elems = 0;
     void* dev = nullptr;  // one allocation holding every device table
     btd::BatchDev bd{};
-    const int* d_lane_sys = nullptr;
+    const int* d_slot_sys = nullptr;
     int* d_counter = nullptr;
     int* d_flag = nullptr;
-    int* d_items = nullptr;  // [tile | lane0 | mode] x max_items
+    int* d_items = nullptr;  // [count] [pack0 | npack | mode] x max_items
     int max_items = 0;
     int placement = 0;
     int grid = 0;
@@ -45,49 +46,57 @@ static void batch_free(btd_batch_s* b) {
     delete b;
 }
 
-// Work items for a placement policy: 0 every tile one lane per system in VGPRs; 1 m >= 6 in
-// LDS (one lane per system); 2 m >= 6 cooperative (m lanes per system, floor(64/m) systems
-// per item); 3 m >= 4 cooperative; 4 every m >= 2 cooperative; 5 as 4, with every item whose
-// cost exceeds a wave's mean share of the batch (total cost / grid) made twisted (two groups
-// of m lanes per system, floor(64/(2m)) systems per item, half the rows per group), so that
-// the longest systems stop setting the end of the launch. Longest estimated work first.
+// Work items for a placement policy: 0 every pack one lane per system in VGPRs; 1 m >= 6 in
+// LDS (one lane per system); 2 m >= 6 cooperative (m lanes per system, two packs per item);
+// 3 m >= 4 cooperative; 4 every m >= 2 cooperative; 5 as 4, with every item whose cost
+// exceeds a wave's mean share of the batch (total cost / grid) split into two twisted items
+// (one pack each, two groups of m lanes per system, half the rows per group), so that the
+// longest systems stop setting the end of the launch. One-lane items take floor(64/W)
+// packs. Longest estimated work first.
 static btd_status build_items(btd_batch_s* b, int placement, int grid) {
     struct Item {
-        int tile, lane0, mode;
+        int pack, npack, mode;
         double cost;
     };
     std::vector<Item> items;
     const int coop_from = placement == 2 ? 6 : placement == 3 ? 4 : placement >= 4 ? 2 : 1000;
     auto coop_cost = [](double rows, double mm) { return rows * mm * (mm + 4) * 4.0; };
+    // runs of consecutive packs with the same m
+    std::vector<std::pair<int, int>> runs;
+    for (int p = 0; p < b->npacks;) {
+        int q = p;
+        while (q < b->npacks && b->pack_m[q] == b->pack_m[p]) ++q;
+        runs.push_back({p, q});
+        p = q;
+    }
     double share = 0.0;  // a wave's mean share of the cooperative cost
     if (placement == 5) {
-        for (int t = 0; t < b->ntiles; ++t) {
-            const int m = b->tile_m[t];
+        for (auto [p, q] : runs) {
+            const int m = b->pack_m[p];
             if (m < coop_from) continue;
-            const int G = btd::kTile / m;
-            for (int l0 = 0; l0 < b->tile_count[t]; l0 += G)
-                share += coop_cost(b->lane_rows[t * btd::kTile + l0], m);
+            for (int k = p; k < q; k += 2) share += coop_cost(b->pack_rows[k], m);
         }
         share /= grid;
     }
-    for (int t = 0; t < b->ntiles; ++t) {
-        const int m = b->tile_m[t];
+    for (auto [p, q] : runs) {
+        const int m = b->pack_m[p];
         const double mm = m;
         if (m >= coop_from) {
-            const int G = btd::kTile / m, G2 = btd::kTile / (2 * m);
-            for (int l0 = 0; l0 < b->tile_count[t];) {
-                const int rows = b->lane_rows[t * btd::kTile + l0];
-                if (placement == 5 && G2 >= 1 && coop_cost(rows, mm) > share) {
-                    items.push_back({t, l0, 3, coop_cost((rows - 1) - (rows - 1) / 2 + 1, mm)});
-                    l0 += G2;
+            for (int k = p; k < q; k += 2) {
+                const int np = std::min(2, q - k);
+                const int rows = b->pack_rows[k];
+                if (placement == 5 && coop_cost(rows, mm) > share) {
+                    for (int j = 0; j < np; ++j)
+                        items.push_back({k + j, 1, 3, coop_cost((b->pack_rows[k + j] + 1) / 2, mm)});
                 } else {
-                    items.push_back({t, l0, 2, coop_cost(rows, mm)});
-                    l0 += G;
+                    items.push_back({k, np, 2, coop_cost(rows, mm)});
                 }
             }
         } else {
+            const int per = btd::kTile / btd::batch_width(m);
             const int mode = (placement == 1 && m >= 6) ? 1 : 0;
-            items.push_back({t, 0, mode, (double)b->tile_rows[t] * mm * mm * (mm + 4)});
+            for (int k = p; k < q; k += per)
+                items.push_back({k, std::min(per, q - k), mode, (double)b->pack_rows[k] * mm * mm * (mm + 4)});
         }
     }
     std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
@@ -97,8 +106,8 @@ static btd_status build_items(btd_batch_s* b, int placement, int grid) {
     std::vector<int> host(1 + 3 * (size_t)b->max_items, 0);
     host[0] = (int)items.size();
     for (size_t k = 0; k < items.size(); ++k) {
-        host[1 + k] = items[k].tile;
-        host[1 + b->max_items + k] = items[k].lane0;
+        host[1 + k] = items[k].pack;
+        host[1 + b->max_items + k] = items[k].npack;
         host[1 + 2 * b->max_items + k] = items[k].mode;
     }
     hipError_t e = hipMemcpy(b->d_items - 1, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice);
@@ -127,42 +136,41 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
         if (m[a] != m[c]) return m[a] < m[c];
         return n[a] > n[c];
     });
-    b->sys_tile.assign(nsys, 0);
-    b->sys_lane.assign(nsys, 0);
+    b->sys_pack.assign(nsys, 0);
+    b->sys_slot.assign(nsys, 0);
     int64_t mo = 0, vo = 0;
     for (size_t i = 0; i < idx.size();) {
         const int mm = m[idx[i]];
+        const int W = btd::batch_width(mm);
         size_t j = i;
-        while (j < idx.size() && j - i < (size_t)btd::kTile && m[idx[j]] == mm) ++j;
-        const int t = (int)b->tile_m.size();
+        while (j < idx.size() && j - i < (size_t)W && m[idx[j]] == mm) ++j;
+        const int p = (int)b->pack_m.size();
         const int rows = n[idx[i]];
-        b->tile_m.push_back(mm);
-        b->tile_rows.push_back(rows);
-        b->tile_count.push_back((int)(j - i));
+        b->pack_m.push_back(mm);
+        b->pack_rows.push_back(rows);
+        b->pack_count.push_back((int)(j - i));
+        b->pack_slot0.push_back((int)b->slot_rows.size());
         b->mat_off.push_back(mo);
         b->vec_off.push_back(vo);
-        for (int l = 0; l < btd::kTile; ++l) {
-            const bool used = i + l < j;
-            b->lane_rows.push_back(used ? n[idx[i + l]] : 0);
-            b->lane_sys.push_back(used ? idx[i + l] : -1);
-            if (used) {
-                b->sys_tile[idx[i + l]] = t;
-                b->sys_lane[idx[i + l]] = l;
-            }
+        for (size_t l = i; l < j; ++l) {
+            b->slot_rows.push_back(n[idx[l]]);
+            b->slot_sys.push_back(idx[l]);
+            b->sys_pack[idx[l]] = p;
+            b->sys_slot[idx[l]] = (int)(l - i);
         }
-        mo += (int64_t)rows * mm * mm * btd::kTile;
-        vo += (int64_t)rows * mm * btd::kTile;
+        mo += (int64_t)rows * mm * mm * W;
+        vo += (int64_t)rows * mm * W;
         b->max_rows = std::max(b->max_rows, rows);
         i = j;
     }
-    b->ntiles = (int)b->tile_m.size();
+    b->npacks = (int)b->pack_m.size();
     b->mat_elems = mo;
     b->vec_elems = vo;
     // device tables in one allocation
-    const size_t T = b->ntiles;
-    b->max_items = (int)T * ((btd::kTile + 1) / 2);  // cooperative m = 2 cuts a tile into 2 items
-    const size_t bytes_i = sizeof(int) * (T * 3 + T * btd::kTile * 2 + 3 + 3 * (size_t)b->max_items);
-    const size_t bytes = ((bytes_i + 15) / 16) * 16 + sizeof(int64_t) * 2 * T;
+    const size_t P = b->npacks, S = b->slot_rows.size();
+    b->max_items = (int)P;  // at most one item per pack (twisted); fewer otherwise
+    const size_t bytes_i = sizeof(int) * (P * 4 + S * 2 + 3 + 3 * (size_t)b->max_items);
+    const size_t bytes = ((bytes_i + 15) / 16) * 16 + sizeof(int64_t) * 2 * P;
     std::vector<char> host(bytes, 0);
     int* hi = (int*)host.data();
     size_t o = 0;
@@ -172,12 +180,12 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
         o += v.size();
         return at;
     };
-    const size_t o_m = put(b->tile_m), o_rows = put(b->tile_rows), o_cnt = put(b->tile_count),
-                 o_lr = put(b->lane_rows), o_ls = put(b->lane_sys);
+    const size_t o_m = put(b->pack_m), o_rows = put(b->pack_rows), o_cnt = put(b->pack_count),
+                 o_s0 = put(b->pack_slot0), o_sr = put(b->slot_rows), o_ss = put(b->slot_sys);
     const size_t o_counter = o, o_flag = o + 1, o_items = o + 3;  // o + 2: the item count
     int64_t* h64 = (int64_t*)(host.data() + ((bytes_i + 15) / 16) * 16);
-    std::memcpy(h64, b->mat_off.data(), sizeof(int64_t) * T);
-    std::memcpy(h64 + T, b->vec_off.data(), sizeof(int64_t) * T);
+    std::memcpy(h64, b->mat_off.data(), sizeof(int64_t) * P);
+    std::memcpy(h64 + P, b->vec_off.data(), sizeof(int64_t) * P);
     hipError_t e = hipMalloc(&b->dev, bytes);
     if (e == hipSuccess) e = hipMemcpy(b->dev, host.data(), bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -188,9 +196,9 @@ btd_status btd_batch_create(btd_batch_t* batch, int nsys, const int* m, const in
     int64_t* d64 = (int64_t*)((char*)b->dev + ((bytes_i + 15) / 16) * 16);
     b->d_items = di + o_items;
     b->bd = btd::BatchDev{b->d_items, b->d_items + b->max_items, b->d_items + 2 * b->max_items,
-                          di + o_m, di + o_rows, di + o_cnt, di + o_lr, d64, d64 + T, b->d_items - 1,
-                          b->ntiles};
-    b->d_lane_sys = di + o_ls;
+                          di + o_m, di + o_rows, di + o_cnt, di + o_s0, di + o_sr, d64, d64 + P,
+                          b->d_items - 1, b->npacks};
+    b->d_slot_sys = di + o_ss;
     b->d_counter = di + o_counter;
     b->d_flag = di + o_flag;
     int dev = 0, cus = 256;
@@ -213,20 +221,21 @@ btd_status btd_batch_destroy(btd_batch_t batch) {
     return BTD_OK;
 }
 
-btd_status btd_batch_sizes(btd_batch_t b, size_t* mat_elems, size_t* vec_elems, int* ntiles) {
+btd_status btd_batch_sizes(btd_batch_t b, size_t* mat_elems, size_t* vec_elems, int* npacks) {
     if (!b) return bfail(BTD_ERR_NOT_CREATED, "btd_batch_sizes: NULL batch");
     if (mat_elems) *mat_elems = (size_t)b->mat_elems;
     if (vec_elems) *vec_elems = (size_t)b->vec_elems;
-    if (ntiles) *ntiles = b->ntiles;
+    if (npacks) *npacks = b->npacks;
     return BTD_OK;
 }
 
-btd_status btd_batch_locate(btd_batch_t b, int sys, size_t* mat_off, size_t* vec_off) {
+btd_status btd_batch_locate(btd_batch_t b, int sys, size_t* mat_off, size_t* vec_off, int* stride) {
     if (!b) return bfail(BTD_ERR_NOT_CREATED, "btd_batch_locate: NULL batch");
     if (sys < 0 || sys >= b->nsys) return bfail(BTD_ERR_INVALID_ARG, "btd_batch_locate: sys out of range");
-    const int t = b->sys_tile[sys], l = b->sys_lane[sys];
-    if (mat_off) *mat_off = (size_t)(b->mat_off[t] + l);
-    if (vec_off) *vec_off = (size_t)(b->vec_off[t] + l);
+    const int p = b->sys_pack[sys], l = b->sys_slot[sys];
+    if (mat_off) *mat_off = (size_t)(b->mat_off[p] + l);
+    if (vec_off) *vec_off = (size_t)(b->vec_off[p] + l);
+    if (stride) *stride = btd::batch_width(b->pack_m[p]);
     return BTD_OK;
 }
 
@@ -245,7 +254,7 @@ btd_status btd_batch_generate(btd_batch_t b, int kind, uint64_t seed, double* a,
     if (!b) return bfail(BTD_ERR_NOT_CREATED, "btd_batch_generate: NULL batch");
     if (!a || !bb || !c || !d || kind < 0 || kind > 3)
         return bfail(BTD_ERR_INVALID_ARG, "btd_batch_generate: bad argument");
-    hipError_t e = btd::launch_batch_generate(b->bd, b->d_lane_sys, b->max_rows, kind, seed, a, bb, c,
+    hipError_t e = btd::launch_batch_generate(b->bd, b->d_slot_sys, b->max_rows, kind, seed, a, bb, c,
                                               d, (hipStream_t)stream);
     if (e != hipSuccess) return bfail(BTD_ERR_HIP, std::string("btd_batch_generate: ") + hipGetErrorString(e));
     return BTD_OK;

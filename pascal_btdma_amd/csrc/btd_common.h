@@ -16,6 +16,10 @@ namespace btd {
 constexpr int kTile = 64;     // systems per tile = lanes per wavefront
 constexpr int kMaxM = 10;     // largest block size handled
 
+// Systems interleaved in one pack of a mixed-size batch: 64 for m = 1 (one lane each), else
+// the systems of a twisted work item (two groups of m lanes per system in a wavefront).
+BTD_HD constexpr int batch_width(int m) { return m == 1 ? kTile : kTile / (2 * m); }
+
 // ------------------------------------------------------------------------------------
 // Layouts. Element e of a block is p + m*q (p = block row, q = block column), the
 // column-major order of the reference. A vector element is e = p.

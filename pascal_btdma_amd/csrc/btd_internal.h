@@ -57,25 +57,26 @@ hipError_t launch_generate(int m, int kind, uint64_t seed, int layout, int nsys,
 
 // Device tables of a packed mixed-size batch (see btd_batch.h).
 struct BatchDev {
-    const int* item_tile;      // work items in work order: tile id,
-    const int* item_lane0;     // first system slot of the item in its tile,
+    const int* item_pack;      // work items in work order: first pack,
+    const int* item_npack;     // number of packs (consecutive, same m),
     const int* item_mode;      // 0 lane per system (VGPR), 1 lane per system (LDS),
                                // 2 m lanes per system (one block row each), 3 two groups
                                // of m lanes per system (twisted)
-    const int* tile_m;         // block size of each tile
-    const int* tile_rows;      // rows of the longest system of each tile
-    const int* tile_count;     // systems in the tile (<= 64)
-    const int* lane_rows;      // rows of each (tile, lane): [tile*64 + lane]
-    const int64_t* mat_off;    // first matrix element of each tile
-    const int64_t* vec_off;    // first vector element of each tile
+    const int* pack_m;         // block size of each pack
+    const int* pack_rows;      // rows of the longest system of each pack (its slot 0)
+    const int* pack_count;     // systems in the pack (<= batch_width(m))
+    const int* pack_slot0;     // first entry of the pack in slot_rows
+    const int* slot_rows;      // rows of each system slot of each pack
+    const int64_t* mat_off;    // first matrix element of each pack
+    const int64_t* vec_off;    // first vector element of each pack
     const int* nitems;         // number of work items (device, written with the item table)
-    int ntiles;
+    int npacks;
 };
 
 hipError_t launch_batch(const BatchDev& b, int kind, int grid, const double* A, const double* B,
                         double* C, double* D, int* counter, int* flag, hipStream_t s);
 
-hipError_t launch_batch_generate(const BatchDev& b, const int* lane_sys, int max_rows, int kind,
+hipError_t launch_batch_generate(const BatchDev& b, const int* slot_sys, int max_rows, int kind,
                                  uint64_t seed, double* A, double* B, double* C, double* D,
                                  hipStream_t s);
 

@@ -63,7 +63,7 @@ def test_batch_matches_oracle(cuda, oracle, placement, grid):
         _, xo = oracle_system(oracle, bt, pb.GenKind.DDPERM, 5, s)
         x = bt.extract(hd, s)
         worst = max(worst, np.abs(x - xo).max() / np.abs(xo).max())
-    print(f"placement {placement} grid {grid}: {bt.ntiles} tiles, max rel err {worst:.2e}")
+    print(f"placement {placement} grid {grid}: {bt.npacks} packs, max rel err {worst:.2e}")
     assert worst <= TOL
 
 
@@ -81,8 +81,8 @@ def test_batch_zero_pivot_flag(cuda, placement, grid):
     assert not bt.check()
     s = int(np.argmax(n))
     mm, row = int(m[s]), 2 * int(n[s]) // 3
-    mo, _ = bt.locate(s)
-    idx = torch.tensor([mo + (row * mm * mm + e) * 64 for e in range(mm * mm)], device=cuda)
+    mo, _, w = bt.locate(s)
+    idx = torch.tensor([mo + (row * mm * mm + e) * w for e in range(mm * mm)], device=cuda)
     for t in (a, b, c0):
         t[idx] = 0.0
     c, d = c0.clone(), d0.clone()
