@@ -275,7 +275,8 @@ btd_status btd_batch_locate(btd_batch_t batch, int sys, size_t *mat_off, size_t 
  * row per lane, pivot search and row exchange through LDS) for m >= 6, m >= 4, m >= 2;
  * 5 = as 4, with the items whose cost exceeds total / grid solved twisted (two groups of m
  * lanes per system eliminating from both ends, half the rows each; results agree with the
- * other placements to rounding, not bit for bit); grid = persistent workgroups of 64 lanes. */
+ * other placements to rounding, not bit for bit); 6 = as 4 with every cooperative item
+ * twisted; grid = persistent workgroups of 64 lanes. */
 btd_status btd_batch_set_params(btd_batch_t batch, int placement, int grid);
 /* The seeded generators of btd_generate, per system (system index s, its own n). */
 btd_status btd_batch_generate(btd_batch_t batch, int kind, uint64_t seed, double *a, double *b,

@@ -51,8 +51,8 @@ static void batch_free(btd_batch_s* b) {
 // 3 m >= 4 cooperative; 4 every m >= 2 cooperative; 5 as 4, with every item whose cost
 // exceeds a wave's mean share of the batch (total cost / grid) split into two twisted items
 // (one pack each, two groups of m lanes per system, half the rows per group), so that the
-// longest systems stop setting the end of the launch. One-lane items take floor(64/W)
-// packs. Longest estimated work first.
+// longest systems stop setting the end of the launch; 6 as 4 with every cooperative item
+// twisted. One-lane items take floor(64/W) packs. Longest estimated work first.
 static btd_status build_items(btd_batch_s* b, int placement, int grid) {
     struct Item {
         int pack, npack, mode;
@@ -85,7 +85,7 @@ static btd_status build_items(btd_batch_s* b, int placement, int grid) {
             for (int k = p; k < q; k += 2) {
                 const int np = std::min(2, q - k);
                 const int rows = b->pack_rows[k];
-                if (placement == 5 && coop_cost(rows, mm) > share) {
+                if (placement == 6 || (placement == 5 && coop_cost(rows, mm) > share)) {
                     for (int j = 0; j < np; ++j)
                         items.push_back({k + j, 1, 3, coop_cost((b->pack_rows[k + j] + 1) / 2, mm)});
                 } else {
@@ -241,8 +241,8 @@ btd_status btd_batch_locate(btd_batch_t b, int sys, size_t* mat_off, size_t* vec
 
 btd_status btd_batch_set_params(btd_batch_t b, int placement, int grid) {
     if (!b) return bfail(BTD_ERR_NOT_CREATED, "btd_batch_set_params: NULL batch");
-    if (placement < 0 || placement > 5 || grid < 1)
-        return bfail(BTD_ERR_INVALID_ARG, "btd_batch_set_params: placement in 0..5, grid >= 1");
+    if (placement < 0 || placement > 6 || grid < 1)
+        return bfail(BTD_ERR_INVALID_ARG, "btd_batch_set_params: placement in 0..6, grid >= 1");
     btd_status st = build_items(b, placement, grid);
     if (st != BTD_OK) return st;
     b->grid = grid;

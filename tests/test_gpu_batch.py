@@ -47,7 +47,7 @@ def test_batch_generator_matches_host(cuda):
 
 
 @pytest.mark.parametrize("placement,grid", [(0, 64), (1, 64), (2, 64), (3, 64), (4, 64), (5, 64),
-                                            (5, 100000)])
+                                            (5, 100000), (6, 64)])
 def test_batch_matches_oracle(cuda, oracle, placement, grid):
     """Every placement against the oracle; placement 5 twists the items above total/grid, so
     grid 100000 makes every cooperative item twisted (n from 2: meeting rows 0 and 1)."""
@@ -67,7 +67,7 @@ def test_batch_matches_oracle(cuda, oracle, placement, grid):
     assert worst <= TOL
 
 
-@pytest.mark.parametrize("placement,grid", [(0, 64), (1, 64), (4, 64), (5, 100000)])
+@pytest.mark.parametrize("placement,grid", [(0, 64), (1, 64), (4, 64), (5, 100000), (6, 64)])
 def test_batch_zero_pivot_flag(cuda, placement, grid):
     """A system with an all-zero block row is reported by btd_batch_check in every placement
     (twisted: the row falls in the bottom half), a clean batch is not, and the report clears."""
