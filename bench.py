@@ -251,6 +251,34 @@ def extra_config0(dev, checks: ParityChecks) -> dict:
     return out
 
 
+def config2_batch(dev):
+    """The 20000 mixed systems of config 2 (m uniform in {2,3,5,8,10}, n log-uniform in
+    [32,1024], seeded), packed and generated: (batch, m, n, a, b, c0, d0)."""
+    import torch
+    import pascal_btdma_amd as pb
+    rng = np.random.default_rng(2026)
+    nb = 20000
+    m = rng.choice(np.array([2, 3, 5, 8, 10]), nb)
+    n = np.rint(np.exp(rng.uniform(np.log(32), np.log(1024), nb))).astype(np.int32)
+    bt = pb.BtdmaBatch(m, n)
+    a, b, c0 = (torch.empty(bt.mat_elems, dtype=torch.float64, device=dev) for _ in range(3))
+    d0 = torch.empty(bt.vec_elems, dtype=torch.float64, device=dev)
+    bt.generate(pb.GenKind.DDPERM, 7, a, b, c0, d0)
+    return bt, m, n, a, b, c0, d0
+
+
+def batch_probe(dev, placement: int, grid: int, reps: int = 3) -> dict:
+    """Only the config-2 batch, `reps` solves at fixed launch parameters (for profilers)."""
+    import torch
+    bt, m, n, a, b, c0, d0 = config2_batch(dev)
+    bt.set_params(placement, grid)
+    c, d = c0.clone(), d0.clone()
+    t = event_time(lambda: bt.solve(a, b, c, d), lambda: (c.copy_(c0), d.copy_(d0)), reps,
+                   torch.cuda.current_stream())
+    return {"batch_probe": {"placement": placement, "grid": grid, "ms": [float(x) for x in t],
+                            "singular": bt.check()}}
+
+
 def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
     """Config 2: 20000 systems with m uniform in {2,3,5,8,10}, n log-uniform in [32,1024],
     plus 512 long systems (m = 5, n = 65536) split into P partitions, all replayed as one
@@ -260,14 +288,8 @@ def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
     import torch
     import pascal_btdma_amd as pb
     from pascal_btdma_amd.tuner import tune_batch, tune_partitions
-    rng = np.random.default_rng(2026)
-    nb = 20000
-    m = rng.choice(np.array([2, 3, 5, 8, 10]), nb)
-    n = np.rint(np.exp(rng.uniform(np.log(32), np.log(1024), nb))).astype(np.int32)
-    bt = pb.BtdmaBatch(m, n)
-    a, b, c0 = (torch.empty(bt.mat_elems, dtype=torch.float64, device=dev) for _ in range(3))
-    d0 = torch.empty(bt.vec_elems, dtype=torch.float64, device=dev)
-    bt.generate(pb.GenKind.DDPERM, 7, a, b, c0, d0)
+    bt, m, n, a, b, c0, d0 = config2_batch(dev)
+    nb = len(m)
     tb = tune_batch(bt, a, b, c0, d0, reps=2)
     lm, lnsys, ln = 5, 512, 65536
     lay = pb.Layout.TILED
@@ -643,6 +665,9 @@ def main() -> int:
     ap.add_argument("--only", default="",
                     help="comma list of extras to run (study,config0,batch,f32r,adjoint,euler)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--batch-probe", default="",
+                    help="PLACEMENT,GRID: only solve the config-2 batch 3 times at these launch "
+                         "parameters and print the times (for profilers)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="only copy one buffer of this many GiB to another (counter calibration)")
     args = ap.parse_args()
@@ -655,6 +680,11 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": nel * 8, "copy_probe_bytes_written": nel * 8}))
+        return 0
+    if args.batch_probe:
+        import torch
+        pl, gr = (int(v) for v in args.batch_probe.split(","))
+        print(json.dumps(batch_probe(torch.device("cuda:0"), pl, gr)))
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
