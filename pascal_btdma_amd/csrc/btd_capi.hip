@@ -3,6 +3,8 @@
 // (CUDA_CXX_Port/src/pascal_btdma_cuda.cu:853-1075).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -35,6 +37,21 @@ btd_status hip_fail(hipError_t e, const char* what) {
 }  // namespace
 
 void btd::set_last_error(const char* msg) { g_err = msg; }
+
+int btd::device_cus() {
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 256;
+    if (dev < 64) {
+        const int c = cached[dev].load(std::memory_order_relaxed);
+        if (c > 0) return c;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    if (dev < 64) cached[dev].store(cus, std::memory_order_relaxed);
+    return cus;
+}
 
 struct btd_plan_s {
     int m = 0, nsys = 0, nrow = 0, npart = 1, layout = BTD_LAYOUT_TILED;

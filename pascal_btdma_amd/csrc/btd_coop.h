@@ -1,0 +1,422 @@
+// Cooperative block-Thomas rows: a system's block rows spread over m lanes of a wavefront
+// (lane i owns row i of every block), pivot search and row exchange through LDS. Used by the
+// mixed-size batch (btd_batch.h) and by the twisted solve of the two-stage reduced system
+// (btd_kernels.h). Storage: element e of row q of a system is at base + (q*E + e)*ES
+// (E = m*m for blocks, m for vectors), base being the system's own offset.
+#pragma once
+
+#include "btd_common.h"
+
+namespace btd {
+
+// Orders the LDS accesses of the lanes of one wavefront. The cooperative kernels run one-wave
+// (64-lane) workgroups, and the LDS performs the operations of a wavefront in the order they
+// are issued, so wavefront-scope fences (which keep the compiler from reordering the accesses
+// and emit no wait) suffice: an exchange costs one LDS round trip, not a drain of every
+// outstanding load.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 1/x from the hardware reciprocal refined by two Newton steps (error ~1 ulp); shorter on
+// the pivot's critical path than an IEEE division.
+__device__ __forceinline__ double fast_rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Rows of the forward (kFwdAhead) and backward (kBwdAhead) sweeps of the cooperative solve
+// kept in flight: one wavefront per work item has no other waves to hide HBM latency
+// behind, so the next rows' blocks are loaded while the current row is eliminated.
+// Rows of the forward (kFwdAhead) and backward (kBwdAhead) sweeps of the cooperative solve
+// kept in flight: one wavefront per work item has no other waves to hide HBM latency
+// behind, so the next rows' blocks are loaded while the current row is eliminated.
+template <int M>
+struct CoopDepth {
+    static constexpr int kFwdAhead = M <= 2 ? 8 : M <= 3 ? 6 : M <= 5 ? 4 : 1;
+    static constexpr int kBwdAhead = M <= 3 ? 16 : M <= 5 ? 12 : 8;
+};
+
+// LDS of a cooperative work item, [slot][lane] (64 doubles a slot): slots [0, M) the carried
+// block of each group (slot k, lane base+j: element (k, j)), slot M its carried right-hand
+// side (lane base+i: element i), then the 2M+2 slots of the elimination exchange.
+constexpr int kCoopSlots = 3 * kMaxM + 3;
+
+// Elimination of one block row by a group of M lanes: lane i holds row i of R and of the
+// right-hand sides rhs = (d, S) (S the block whose R^{-1} S is carried to the next row).
+// Pivot step k: every lane posts its row's columns k..M-1, its rhs and its candidate |r(k)|
+// (-1 once chosen); every lane reads the M candidates, picks the pivot (largest, lowest lane
+// on a tie), reads the pivot row and eliminates with it, a lane that is already chosen or is
+// the pivot with multiplier 0. Then each lane posts its row once more and back-substitutes
+// one rhs column on its own: xc = column i of R^{-1} S (xc[k] = (R^{-1} S)(k, i)) and, in
+// every lane, the whole xd = R^{-1} d. Returns 1 if a pivot was zero. The code is free of
+// lane-dependent branches so that LDS reads issue in batches.
+template <int M>
+__device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[M + 1], double* __restrict__ xb,
+                                              int i, int base, double (&xc)[M], double (&xd)[M]) {
+    const int lane = threadIdx.x;
+    int rank = -1;
+    int plane[M];
+    int zero_pivot = 0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+#pragma unroll
+        for (int j = k; j < M; ++j) xb[j * kTile + lane] = r[j];
+#pragma unroll
+        for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
+        xb[(2 * M + 1) * kTile + lane] = rank < 0 ? fabs(r[k]) : -1.0;
+        wave_lds_sync();
+        double cand[M];
+#pragma unroll
+        for (int t = 0; t < M; ++t) cand[t] = xb[(2 * M + 1) * kTile + base + t];
+        int p = 0;
+        double best = cand[0];
+#pragma unroll
+        for (int t = 1; t < M; ++t) {
+            const bool c = cand[t] > best;
+            best = c ? cand[t] : best;
+            p = c ? t : p;
+        }
+        plane[k] = p;
+        const int pl = base + p;
+        double prow[M], prhs[M + 1];
+#pragma unroll
+        for (int j = k; j < M; ++j) prow[j] = xb[j * kTile + pl];
+#pragma unroll
+        for (int j = 0; j <= M; ++j) prhs[j] = xb[(M + j) * kTile + pl];
+        zero_pivot |= prow[k] == 0.0 ? 1 : 0;
+        const bool elim = rank < 0 && i != p;
+        const double l = elim ? r[k] * fast_rcp(prow[k]) : 0.0;
+        rank = (rank < 0 && i == p) ? k : rank;
+#pragma unroll
+        for (int j = k + 1; j < M; ++j) r[j] = fma(-l, prow[j], r[j]);
+#pragma unroll
+        for (int j = 0; j <= M; ++j) rhs[j] = fma(-l, prhs[j], rhs[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) xb[j * kTile + lane] = r[j];
+#pragma unroll
+    for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
+    wave_lds_sync();
+#pragma unroll
+    for (int k = M - 1; k >= 0; --k) {
+        const int pl = base + plane[k];
+        double sc = xb[(M + 1 + i) * kTile + pl];
+        double sd = xb[M * kTile + pl];
+#pragma unroll
+        for (int j = k + 1; j < M; ++j) {
+            const double u = xb[j * kTile + pl];
+            sc = fma(-u, xc[j], sc);
+            sd = fma(-u, xd[j], sd);
+        }
+        const double rk = fast_rcp(xb[k * kTile + pl]);
+        xc[k] = sc * rk;
+        xd[k] = sd * rk;
+    }
+    return zero_pivot;
+}
+
+// Element i of a vector held whole in every lane.
+template <int M>
+__device__ __forceinline__ double lane_elem(const double (&v)[M], int i) {
+    double x = v[0];
+#pragma unroll
+    for (int t = 1; t < M; ++t) x = (i == t) ? v[t] : x;
+    return x;
+}
+
+// Cooperative solve: M lanes per system, lane i owns block row i of every block; row q is
+// R_q = B_q - A_q C'_{q-1} eliminated with rhs (d_q - A_q d'_{q-1}, C_q) (coop_eliminate),
+// C'_q stored over C_q and carried in LDS, d'_q over d_q and carried whole in registers;
+// then x_q = d'_q - C'_q x_{q+1} upwards, x_{q+1} exchanged through LDS. Exchanges are
+// ordered by wave_lds_sync only. Rows are loaded kFwdAhead (forward) and kBwdAhead
+// (backward) rows ahead; every row below `rows` (the lane's pack's) is allocated for every
+// slot of the pack, so loads are clamped to that range and need no other guard. All lanes of
+// the wave run the same row steps (nmax, the item's longest system); lanes past their
+// system's n compute on what the rows hold and store nothing.
+template <int M, int ES = batch_width(M)>
+__device__ bool coop_rows(const double* __restrict__ A, const double* __restrict__ B, double* C,
+                          double* D, int64_t mo, int64_t vo, int i, int base, int n, int nmax, int rows,
+                          double* __restrict__ sh) {
+    constexpr int E = M * M;
+    constexpr int F = CoopDepth<M>::kFwdAhead;
+    constexpr int G = CoopDepth<M>::kBwdAhead;
+    double* carry = sh;
+    double* xb = sh + (M + 1) * kTile;
+    const int lane = threadIdx.x;
+    constexpr int64_t es = ES;
+    bool ok = true;
+    double pa[F][M], pb[F][M], pc[F][M], pd[F];
+    auto fetch = [&](double (&fa)[M], double (&fb)[M], double (&fc)[M], double& fd, int q) {
+        q = q < rows ? q : rows - 1;
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            fa[k] = __builtin_nontemporal_load(A + rm + (i + M * k) * es);
+            fb[k] = __builtin_nontemporal_load(B + rm + (i + M * k) * es);
+            fc[k] = C[rm + (i + M * k) * es];
+        }
+        fd = D[rv + i * es];
+    };
+#pragma unroll
+    for (int s = 0; s < F; ++s) fetch(pa[s], pb[s], pc[s], pd[s], s);
+    double dp[M];  // d'_{q-1}, whole, in every lane of the system
+#pragma unroll
+    for (int k = 0; k < M; ++k) dp[k] = 0.0;
+    double xlast = 0.0;
+    for (int q0 = 0; q0 < nmax; q0 += F) {
+#pragma unroll
+        for (int s = 0; s < F; ++s) {
+            const int q = q0 + s;
+            if (q >= nmax) break;  // uniform: the last chunk may be partial
+            const bool act = q < n;
+            const bool has_next = q + 1 < n;
+            const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
+            double r[M], rhs[M + 1];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                r[j] = pb[s][j];
+                rhs[1 + j] = pc[s][j];
+            }
+            rhs[0] = pd[s];
+            if (q > 0) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    const double a = pa[s][k];
+                    rhs[0] = fma(-a, dp[k], rhs[0]);
+#pragma unroll
+                    for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
+                }
+            }
+            fetch(pa[s], pb[s], pc[s], pd[s], q + F);
+            double xc[M], xd[M];
+            const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
+            const double di = lane_elem<M>(xd, i);
+            ok = ok && !(act && zero_pivot);
+            if (act) {
+                D[rv + i * es] = di;
+                xlast = di;
+            }
+            if (has_next) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) C[rm + (k + M * i) * es] = xc[k];
+            }
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                carry[k * kTile + lane] = xc[k];  // C'(k, i) in lane base+i
+                dp[k] = xd[k];
+            }
+            wave_lds_sync();
+        }
+    }
+    // back substitution over the rows: x_q(i) = d'_q(i) - sum_j C'_q(i,j) x_{q+1}(j), with
+    // C'_q(i, :) and d'_q(i) loaded kBwdAhead rows ahead (rows are clamped to the lane's pack).
+    double xnext = xlast;
+    double cn[G][M], dn[G];
+    auto bfetch = [&](double (&fc)[M], double& fd, int q) {
+        q = q < 0 ? 0 : (q < rows ? q : rows - 1);  // this lane's pack may be shorter than nmax
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
+#pragma unroll
+        for (int j = 0; j < M; ++j) fc[j] = C[rm + (i + M * j) * es];
+        fd = D[rv + i * es];
+    };
+#pragma unroll
+    for (int s = 0; s < G; ++s) bfetch(cn[s], dn[s], nmax - 2 - s);
+    for (int q0 = nmax - 2; q0 >= 0; q0 -= G) {
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            const int q = q0 - s;
+            if (q < 0) break;  // uniform
+            const bool act = q >= 0 && q < n - 1;
+            double cq[M], x = dn[s];
+#pragma unroll
+            for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
+            bfetch(cn[s], dn[s], q - G);
+            sh[lane] = xnext;
+            wave_lds_sync();
+#pragma unroll
+            for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
+            if (act) {
+                D[vo + (int64_t)q * M * es + i * es] = x;
+                xnext = x;
+            }
+            wave_lds_sync();
+        }
+    }
+    return ok;
+}
+
+// Twisted cooperative solve: two groups of M lanes per system (lane i of each owns block row
+// i). With h = (n-1)/2 the top group (`top`) eliminates rows 0..h-1 downwards as coop_rows
+// does (C'_q = R_q^{-1} C_q over C_q), while the bottom group eliminates rows n-1..h+1
+// upwards: S_q = B_q - C_q A''_{q+1}, A''_q = S_q^{-1} A_q stored over C_q (dead once row q
+// is eliminated), d''_q = S_q^{-1}(d_q - C_q d''_{q+1}) over d_q. Both groups then solve the
+// meeting row, (B_h - A_h C'_{h-1} - C_h A''_{h+1}) x_h = d_h - A_h d'_{h-1} - C_h d''_{h+1},
+// and substitute outwards: x_q = d'_q - C'_q x_{q+1} (q < h), x_q = d''_q - A''_q x_{q-1}
+// (q > h). Each group handles half of the rows, so a long system takes half the time of
+// coop_rows for the same arithmetic. The groups of a system are base and pbase; `steps`
+// (uniform over the wave) is the largest row count of a group of the item (the bottom
+// group of its longest system); every row loaded is clamped below `rows`, the pack's row
+// count. A group's carried block and rhs stay as its last row left them while the other
+// groups finish. ES is the element stride of the storage; with UNIT_B every B_q is the
+// identity and B is not read (the reduced system of the two-stage solve).
+template <int M, int ES = batch_width(M), bool UNIT_B = false>
+__device__ bool coop_twisted(const double* A, const double* __restrict__ B, double* C, double* D,
+                             int64_t mo, int64_t vo, int i, int base, int pbase, bool top, int n,
+                             int steps, int rows, double* __restrict__ sh) {
+    constexpr int E = M * M;
+    constexpr int F = CoopDepth<M>::kFwdAhead;
+    constexpr int G = CoopDepth<M>::kBwdAhead;
+    double* carry = sh;
+    double* xb = sh + (M + 1) * kTile;
+    const int lane = threadIdx.x;
+    constexpr int64_t es = ES;
+    const int h = n > 0 ? (n - 1) / 2 : 0;
+    const int cnt = n > 0 ? (top ? h : n - 1 - h) : 0;
+    const double* sub = top ? A : C;  // the block coupling row q to the row eliminated before it
+    const double* sup = top ? C : A;  // the block coupling row q to the row eliminated after it
+    auto clamp_row = [&](int q) { return q < 0 ? 0 : (q >= rows ? rows - 1 : q); };
+    bool ok = true;
+    double ps[F][M], pg[F][M], pu[F][M], pd[F];
+    auto fetch = [&](double (&fs)[M], double (&fg)[M], double (&fu)[M], double& fd, int t) {
+        const int q = clamp_row(top ? t : n - 1 - t);
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            fs[k] = sub[rm + (i + M * k) * es];
+            if constexpr (UNIT_B)
+                fg[k] = i == k ? 1.0 : 0.0;
+            else
+                fg[k] = __builtin_nontemporal_load(B + rm + (i + M * k) * es);
+            fu[k] = sup[rm + (i + M * k) * es];
+        }
+        fd = D[rv + i * es];
+    };
+#pragma unroll
+    for (int s = 0; s < F; ++s) fetch(ps[s], pg[s], pu[s], pd[s], s);
+    double dp[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        dp[k] = 0.0;
+        carry[k * kTile + lane] = 0.0;
+    }
+    for (int t0 = 0; t0 < steps; t0 += F) {
+#pragma unroll
+        for (int s = 0; s < F; ++s) {
+            const int t = t0 + s;
+            if (t >= steps) break;  // uniform: the last chunk may be partial
+            const bool act = t < cnt;
+            const int q = clamp_row(top ? t : n - 1 - t);
+            const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
+            double r[M], rhs[M + 1];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                r[j] = pg[s][j];
+                rhs[1 + j] = pu[s][j];
+            }
+            rhs[0] = pd[s];
+            if (t > 0) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    const double a = ps[s][k];
+                    rhs[0] = fma(-a, dp[k], rhs[0]);
+#pragma unroll
+                    for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
+                }
+            }
+            fetch(ps[s], pg[s], pu[s], pd[s], t + F);
+            double xc[M], xd[M];
+            const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
+            ok = ok && !(act && zero_pivot);
+            if (act) {
+                D[rv + i * es] = lane_elem<M>(xd, i);
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    C[rm + (k + M * i) * es] = xc[k];
+                    carry[k * kTile + lane] = xc[k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < M; ++k) dp[k] = act ? xd[k] : dp[k];
+            wave_lds_sync();
+        }
+    }
+    // meeting row h: both groups of a system solve it from both carries
+    carry[M * kTile + lane] = lane_elem<M>(dp, i);
+    wave_lds_sync();
+    const int tb = top ? base : pbase, bb = top ? pbase : base;
+    double xh[M];
+    {
+        const int64_t rm = mo + (int64_t)h * E * es, rv = vo + (int64_t)h * M * es;
+        double r[M], rhs[M + 1];
+        double ah[M], ch[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            ah[k] = h > 0 ? A[rm + (i + M * k) * es] : 0.0;  // row 0 has no sub-diagonal block
+            ch[k] = C[rm + (i + M * k) * es];
+            if constexpr (UNIT_B)
+                r[k] = i == k ? 1.0 : 0.0;
+            else
+                r[k] = B[rm + (i + M * k) * es];
+            rhs[1 + k] = 0.0;
+        }
+        rhs[0] = D[rv + i * es];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            rhs[0] = fma(-ah[k], carry[M * kTile + tb + k], rhs[0]);
+            rhs[0] = fma(-ch[k], carry[M * kTile + bb + k], rhs[0]);
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                r[j] = fma(-ah[k], carry[k * kTile + tb + j], r[j]);
+                r[j] = fma(-ch[k], carry[k * kTile + bb + j], r[j]);
+            }
+        }
+        double xc[M];
+        const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xh);
+        ok = ok && !(n > 0 && zero_pivot);
+        if (n > 0 && top) D[rv + i * es] = lane_elem<M>(xh, i);
+    }
+    // outward substitution, kBwdAhead rows ahead
+    double xprev = lane_elem<M>(xh, i);
+    double cn[G][M], dn[G];
+    auto bfetch = [&](double (&fc)[M], double& fd, int t) {
+        const int q = clamp_row(top ? h - 1 - t : h + 1 + t);
+        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
+#pragma unroll
+        for (int j = 0; j < M; ++j) fc[j] = C[rm + (i + M * j) * es];
+        fd = D[rv + i * es];
+    };
+    wave_lds_sync();
+#pragma unroll
+    for (int s = 0; s < G; ++s) bfetch(cn[s], dn[s], s);
+    for (int t0 = 0; t0 < steps; t0 += G) {
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            const int t = t0 + s;
+            if (t >= steps) break;  // uniform
+            const bool act = t < cnt;
+            const int q = clamp_row(top ? h - 1 - t : h + 1 + t);
+            double cq[M], x = dn[s];
+#pragma unroll
+            for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
+            bfetch(cn[s], dn[s], t + G);
+            sh[lane] = xprev;
+            wave_lds_sync();
+#pragma unroll
+            for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
+            if (act) {
+                D[vo + (int64_t)q * M * es + i * es] = x;
+                xprev = x;
+            }
+            wave_lds_sync();
+        }
+    }
+    return ok;
+}
+
+}  // namespace btd

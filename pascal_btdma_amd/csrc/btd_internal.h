@@ -83,6 +83,9 @@ hipError_t launch_batch_generate(const BatchDev& b, const int* slot_sys, int max
 // Sets the thread-local message returned by btd_last_error().
 void set_last_error(const char* msg);
 
+// Compute units of the current device (queried once per device).
+int device_cus();
+
 hipError_t launch_convert(int src_layout, int dst_layout, int E, int nsys, int nrow,
                           const double* src, double* dst, hipStream_t s);
 

@@ -11,10 +11,13 @@
 //                GPU: mod_btdma_gpu_v2.f90:241-350, pascal_btdma_cuda.cu:274-334).
 //   k_modi       stage 1 of the two-stage solve over one partition per lane,
 //                btdma_many_modi (mod_btdma_cpu.f90:305-443; mod_btdma_gpu_v2.f90:618-866).
-//   k_reduced    plain solve of the 2P-row reduced system (B = I), on-device.
+//   k_reduced    plain solve of the 2P-row reduced system (B = I), on-device;
+//   k_reduced_twisted  the same system solved twisted by two groups of m lanes per system
+//                (btd_coop.h), for few systems with many partitions.
 //   k_update     stage 3, btdma_many_update (mod_btdma_cpu.f90:511-535).
 //   k_generate   seeded input generator (bench and tests).
 #pragma once
+#include "btd_coop.h"
 #include "btd_internal.h"
 
 namespace btd {
@@ -966,6 +969,42 @@ __global__ __launch_bounds__(64) void k_reduced(const double* __restrict__ redA,
     if (!ok) report_singular(flag, sys);
 }
 
+// Largest number of wavefronts per CU for which the reduced system is solved twisted.
+#ifndef BTD_TWISTED_REDUCED_WAVES_PER_CU
+#define BTD_TWISTED_REDUCED_WAVES_PER_CU 4
+#endif
+constexpr int kTwistedReducedWavesPerCU = BTD_TWISTED_REDUCED_WAVES_PER_CU;
+
+// The reduced system (nrr = 2P rows, B = I, tiled reduced buffers) solved twisted: per system
+// two groups of M lanes, one eliminating from the first row down, the other from the last row
+// up, meeting in the middle (coop_twisted), batch_width(M) systems per wavefront. k_reduced
+// gives each system one lane and takes nrr row latencies; this takes about nrr/2 shorter
+// ones, which is what matters when there are few systems with many partitions. The result
+// differs from k_reduced's only by rounding; redC is left overwritten.
+template <int M>
+__global__ __launch_bounds__(64) void k_reduced_twisted(const double* __restrict__ redA, double* redC,
+                                                        double* redD, int nsys, int nrr,
+                                                        int* __restrict__ flag) {
+    __shared__ double sh[kCoopSlots * kTile];
+    constexpr int W = batch_width(M);
+    const int lane = threadIdx.x;
+    const int g = lane / M, i = lane % M, slot = g / 2;
+    const bool grouped = slot < W;
+    const bool top = grouped ? (g % 2 == 0) : true;
+    const int64_t first = (int64_t)blockIdx.x * W;  // < nsys: the grid is ceil(nsys / W)
+    const int64_t sys = first + (grouped ? slot : 0);
+    const bool mine = grouped && sys < nsys;
+    const int64_t s_eff = mine ? sys : first;
+    const int64_t tile = s_eff / kTile, l = s_eff % kTile;
+    const int64_t mo = tile * nrr * M * M * kTile + l, vo = tile * nrr * M * kTile + l;
+    const int steps = (nrr - 1) - (nrr - 1) / 2;
+    const int base = grouped ? g * M : 0;
+    const int pbase = grouped ? (top ? base + M : base - M) : M;
+    const bool ok = coop_twisted<M, kTile, true>(redA, nullptr, redC, redD, mo, vo, i, base, pbase, top,
+                                                 mine ? nrr : 0, steps, nrr, sh);
+    if (!ok && mine) report_singular(flag, sys);
+}
+
 // Banded outer-product gradients of the FP32-stored solve, computed in FP64 and stored in
 // FP32: dA_q = -lambda_q x_{q-1}^T, dB_q = -lambda_q x_q^T, dC_q = -lambda_q x_{q+1}^T.
 template <int M, int LAYOUT>
@@ -1377,9 +1416,20 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
         hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
                            npart, redA, redC, redD, flag);
     if (ev) (void)hipEventRecord(ev[1], s);
+    // the twisted reduced solve when its wavefronts all fit on the device at once (one per
+    // SIMD); otherwise one lane per system does more systems per instruction
+    bool twisted = false;
+    if constexpr (M >= 2) {
+        const int64_t waves = ((int64_t)nsys + batch_width(M) - 1) / batch_width(M);
+        if (!cyclic && waves <= kTwistedReducedWavesPerCU * (int64_t)device_cus()) {
+            hipLaunchKernelGGL((k_reduced_twisted<M>), dim3((unsigned)waves), block, 0, s, redA, redC, redD,
+                               nsys, 2 * npart, flag);
+            twisted = true;
+        }
+    }
     if (cyclic)
         hipLaunchKernelGGL((k_reduced_cyclic<M>), gridr, block, 0, s, redA, redC, redD, nsys, 2 * npart, flag);
-    else
+    else if (!twisted)
         hipLaunchKernelGGL((k_reduced<M>), gridr, block, 0, s, redA, redC, redD, nsys, 2 * npart, flag);
     if (ev) (void)hipEventRecord(ev[2], s);
     if (layout == LAYOUT_TILED)
