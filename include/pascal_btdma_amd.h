@@ -56,6 +56,12 @@ typedef enum btd_layout {
     BTD_LAYOUT_TILED = 2
 } btd_layout;
 
+/* How a plan's single-partition solve runs (btd_plan_set_form). */
+typedef enum btd_form {
+    BTD_FORM_LANE = 0,    /* one lane per system, blocks in VGPRs */
+    BTD_FORM_TWISTED = 1  /* two groups of m lanes per system, one eliminating from each end */
+} btd_form;
+
 /* Phase times in seconds; the fields of BtdmaSolveTimings (pascal_btdma_cuda.hpp:24-39)
  * and BTDMA_TIMING_gpu_v2 (mod_btdma_gpu_v2.f90:38-45). On one GPU the two exchange
  * phases do not exist and read 0. Measured with HIP events on the solve stream. */
@@ -73,7 +79,8 @@ typedef struct btd_timings {
  * (mod_btdma_gpu_v2.f90:70-109). `npart` takes the place of the communicator size:
  * npart = 1 is the single-partition solve; npart > 1 splits every system's rows with
  * btd_partition_1d and needs nrow >= 3*npart; npart = 0 lets the library choose
- * (btd_auto_npart; read the choice back with btd_plan_npart). layout is BTD_LAYOUT_REF_GPU
+ * (btd_auto_form, then btd_auto_npart; read the choices back with btd_plan_form and
+ * btd_plan_npart). layout is BTD_LAYOUT_REF_GPU
  * or BTD_LAYOUT_TILED. 1 <= m <= BTD_MAX_M, nsys >= 1, nrow >= 2. Allocates device scratch
  * on the current device. */
 btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npart, int layout);
@@ -84,8 +91,24 @@ btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npar
  * Pure function (no device use). */
 int btd_auto_npart(int m, int nsys, int nrow);
 
+/* New: the form a plan created with npart = 0 gets: BTD_FORM_TWISTED (with npart = 1) for tiled
+ * plans of m >= 8 with at most 4.5M block rows in all and either nrow <= 256 or nsys >= 4096
+ * (m = 8) / 1024 (m >= 9), where it measured fastest; otherwise BTD_FORM_LANE with
+ * btd_auto_npart's partition count. Pure function (no device use). */
+int btd_auto_form(int m, int nsys, int nrow, int layout);
+
 /* The plan's partition count (the one chosen when it was created with npart = 0). */
 btd_status btd_plan_npart(btd_plan_t plan, int *npart);
+
+/* New (no counterpart in the reference): choose how an npart = 1 plan's non-cyclic solve runs.
+ * BTD_FORM_LANE (the default) gives each system one lane; BTD_FORM_TWISTED (tiled layout,
+ * m >= 2) gives each system two groups of m lanes, one eliminating downwards from row 0 and
+ * one upwards from row nrow-1, meeting in the middle: 2m lanes and half the row latency per
+ * system, for plans with too few systems to fill the device with one lane each. Same zero-
+ * pivot reporting; results agree with BTD_FORM_LANE to rounding; C is left overwritten in
+ * both forms. Cyclic and the other solve variants are unaffected. */
+btd_status btd_plan_set_form(btd_plan_t plan, int form);
+btd_status btd_plan_form(btd_plan_t plan, int *form);
 
 /* Replaces BtdmaGpuPlan::destroy (pascal_btdma_cuda.hpp:134, pascal_btdma_cuda.cu:1007-1033)
  * and btdma_cleanplan_gpu_v2 (mod_btdma_gpu_v2.f90:111-129). NULL is accepted. */

@@ -18,7 +18,7 @@ from dataclasses import dataclass
 from . import _capi
 
 __all__ = [
-    "Layout", "GenKind", "BtdmaGpuPlan", "BtdmaSolveTimings", "BtdmaError", "HipError",
+    "Layout", "Form", "GenKind", "BtdmaGpuPlan", "BtdmaSolveTimings", "BtdmaError", "HipError",
     "SingularError", "solve_noncyclic", "solve_noncyclic_profiled", "solve_cyclic",
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
     "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
@@ -34,6 +34,12 @@ class Layout(enum.IntEnum):
     REF_GPU = 0   # sys + nsys*(row + nrow*e)   (pascal_btdma_cuda.hpp:46-65)
     REF_CPU = 1   # e + E*(sys + nsys*row)      (mod_btdma_cpu.f90:18)
     TILED = 2     # ((tile*nrow + row)*E + e)*64 + lane
+
+
+class Form(enum.IntEnum):
+    """How a plan's single-partition solve runs (btd_form)."""
+    LANE = 0
+    TWISTED = 1
 
 
 class GenKind(enum.IntEnum):
@@ -146,7 +152,7 @@ class BtdmaGpuPlan:
         _check(_capi.lib().btd_plan_create(ctypes.byref(h), int(m), int(nsys), int(nrow),
                                            int(npart), int(layout)))
         self._h = h
-        if int(npart) == 0:  # chosen by the library
+        if int(npart) == 0:  # chosen by the library (with the form: see btd_auto_form)
             p = ctypes.c_int(0)
             _check(_capi.lib().btd_plan_npart(h, ctypes.byref(p)))
             npart = p.value
@@ -160,6 +166,18 @@ class BtdmaGpuPlan:
             _capi.lib().btd_plan_destroy(self._h)
         self._h = ctypes.c_void_p(None)
         self.created = False
+
+    @property
+    def form(self) -> int:
+        """How the single-partition solve runs: Form.LANE or Form.TWISTED (btd_plan_form)."""
+        f = ctypes.c_int(0)
+        _check(_capi.lib().btd_plan_form(self._h, ctypes.byref(f)))
+        return Form(f.value)
+
+    def set_form(self, form: "Form") -> "BtdmaGpuPlan":
+        """btd_plan_set_form: Form.TWISTED needs npart == 1, the tiled layout and m >= 2."""
+        _check(_capi.lib().btd_plan_set_form(self._h, int(form)))
+        return self
 
     def check(self) -> bool:
         """Synchronise; True if any block LU met a zero pivot since the last check."""

@@ -4,6 +4,20 @@
 
 namespace btd {
 
+#define BTD_DECL_PLAIN_TWISTED(M) \
+    hipError_t launch_plain_twisted_##M(const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+BTD_DECL_PLAIN_TWISTED(1)
+BTD_DECL_PLAIN_TWISTED(2)
+BTD_DECL_PLAIN_TWISTED(3)
+BTD_DECL_PLAIN_TWISTED(4)
+BTD_DECL_PLAIN_TWISTED(5)
+BTD_DECL_PLAIN_TWISTED(6)
+BTD_DECL_PLAIN_TWISTED(7)
+BTD_DECL_PLAIN_TWISTED(8)
+BTD_DECL_PLAIN_TWISTED(9)
+BTD_DECL_PLAIN_TWISTED(10)
+#undef BTD_DECL_PLAIN_TWISTED
+
 hipError_t launch_plain_1(int, const double*, const double*, double*, double*, int, int, int*,
                           hipStream_t);
 hipError_t launch_two_stage_1(int, double*, const double*, double*, double*, int, int, const int*,
@@ -246,6 +260,13 @@ __global__ __launch_bounds__(256) void k_convert(int src_layout, int dst_layout,
 hipError_t launch_plain(int m, int layout, const double* A, const double* B, double* C, double* D,
                         int nsys, int nrow, int* flag, hipStream_t s) {
 #define CALL(M) launch_plain_##M(layout, A, B, C, D, nsys, nrow, flag, s)
+    BTD_DISPATCH_M(m, CALL)
+#undef CALL
+}
+
+hipError_t launch_plain_twisted(int m, const double* A, const double* B, double* C, double* D, int nsys,
+                                int nrow, int* flag, hipStream_t s) {
+#define CALL(M) launch_plain_twisted_##M(A, B, C, D, nsys, nrow, flag, s)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }

@@ -14,6 +14,8 @@
 //   k_reduced    plain solve of the 2P-row reduced system (B = I), on-device;
 //   k_reduced_twisted  the same system solved twisted by two groups of m lanes per system
 //                (btd_coop.h), for few systems with many partitions.
+//   k_plain_twisted  the single-partition solve of a tiled plan, twisted, m lanes per half
+//                system (btd_coop.h), for plans with too few systems to fill the device.
 //   k_update     stage 3, btdma_many_update (mod_btdma_cpu.f90:511-535).
 //   k_generate   seeded input generator (bench and tests).
 #pragma once
@@ -1003,6 +1005,58 @@ __global__ __launch_bounds__(64) void k_reduced_twisted(const double* __restrict
     const bool ok = coop_twisted<M, kTile, true>(redA, nullptr, redC, redD, mo, vo, i, base, pbase, top,
                                                  mine ? nrr : 0, steps, nrr, sh);
     if (!ok && mine) report_singular(flag, sys);
+}
+
+// XCDs of the device: workgroups are dispatched to them round robin (workgroup b runs on XCD
+// b mod kXcds), each XCD with its own L2.
+constexpr int kXcds = 8;
+
+// Cooperative plain solve (npart = 1) of a tiled plan, twisted: per system two groups of M
+// lanes (coop_twisted), batch_width(M) consecutive systems of one tile per wavefront, so a
+// plan of few systems fills the device with m lanes per system instead of one. The items of
+// a tile (they share the tile's cache lines) go to workgroups of one XCD, adjacent in its
+// dispatch order, so those lines are fetched into one L2 once. C is left overwritten.
+template <int M>
+__global__ __launch_bounds__(64) void k_plain_twisted(const double* A, const double* __restrict__ B,
+                                                      double* C, double* D, int nsys, int nrow,
+                                                      int ntiles, int* __restrict__ flag) {
+    __shared__ double sh[kCoopSlots * kTile];
+    constexpr int W = batch_width(M);
+    constexpr int kItems = (kTile + W - 1) / W;  // items per tile
+    const int x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    const int tile = x + kXcds * (j / kItems), item = j % kItems;
+    if (tile >= ntiles) return;
+    const int lane = threadIdx.x;
+    const int g = lane / M, i = lane % M, slot = g / 2;
+    const bool grouped = slot < W;
+    const bool top = grouped ? (g % 2 == 0) : true;
+    const int s0 = item * W;
+    const int ts = s0 + (grouped ? slot : 0);  // the lane's slot in the tile
+    const int64_t sys = (int64_t)tile * kTile + ts;
+    const bool mine = grouped && ts < kTile && sys < nsys;
+    const int64_t l = mine ? ts : s0;  // a slot of the (padded) tile either way
+    const int64_t mo = (int64_t)tile * nrow * M * M * kTile + l, vo = (int64_t)tile * nrow * M * kTile + l;
+    const int steps = (nrow - 1) - (nrow - 1) / 2;
+    const int base = grouped ? g * M : 0;
+    const int pbase = grouped ? (top ? base + M : base - M) : M;
+    const bool ok = coop_twisted<M, kTile, false>(A, B, C, D, mo, vo, i, base, pbase, top, mine ? nrow : 0,
+                                                  steps, nrow, sh);
+    if (!ok && mine) report_singular(flag, sys);
+}
+
+template <int M>
+hipError_t launch_plain_twisted_m(const double* A, const double* B, double* C, double* D, int nsys,
+                                  int nrow, int* flag, hipStream_t s) {
+    if constexpr (M < 2) {
+        return hipErrorInvalidValue;
+    } else {
+        constexpr int kItems = (kTile + batch_width(M) - 1) / batch_width(M);
+        const int ntiles = (nsys + kTile - 1) / kTile;
+        const int64_t blocks = (int64_t)kXcds * ((ntiles + kXcds - 1) / kXcds) * kItems;
+        hipLaunchKernelGGL((k_plain_twisted<M>), dim3((unsigned)blocks), dim3(64), 0, s, A, B, C, D, nsys,
+                           nrow, ntiles, flag);
+        return hipGetLastError();
+    }
 }
 
 // Banded outer-product gradients of the FP32-stored solve, computed in FP64 and stored in

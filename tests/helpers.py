@@ -63,9 +63,9 @@ def rel_err(x: np.ndarray, xref: np.ndarray) -> np.ndarray:
     return num / den
 
 
-def gpu_solve(A, B, C, d, layout: int, npart: int = 1, profiled: bool = False):
+def gpu_solve(A, B, C, d, layout: int, npart: int = 1, profiled: bool = False, form: int = 0):
     """Upload natural-form numpy inputs, solve in place through the C ABI, return x (n,nsys,m)
-    (and the timings when profiled)."""
+    (and the timings when profiled); `form` is the plan's btd_form."""
     import torch
     import pascal_btdma_amd as pb
     n, nsys, m = d.shape
@@ -75,6 +75,8 @@ def gpu_solve(A, B, C, d, layout: int, npart: int = 1, profiled: bool = False):
     tc = torch.from_numpy(nat_to_layout(C, layout)).to(dev)
     td = torch.from_numpy(nat_to_layout(d, layout)).to(dev)
     with pb.BtdmaGpuPlan().create(m, nsys, n, npart, pb.Layout(layout)) as plan:
+        if form:
+            plan.set_form(pb.Form(form))
         if profiled:
             t = pb.solve_noncyclic_profiled(plan, ta, tb, tc, td, m, nsys, n)
         else:

@@ -87,6 +87,23 @@ def test_auto_npart_rule():
         assert p == 1 or (p >= 2 and nrow >= 8 * p)
 
 
+def test_auto_form_rule():
+    """npart = 0 on a tiled plan: the twisted form for m >= 8 where it measured fastest (short
+    systems, or enough of them), one lane per system (with btd_auto_npart) otherwise."""
+    tw, lane = int(pb.Form.TWISTED), int(pb.Form.LANE)
+    tiled, ref = int(pb.Layout.TILED), int(pb.Layout.REF_GPU)
+    form = lambda m, nsys, nrow, lay=tiled: pb._capi.lib().btd_auto_form(m, nsys, nrow, lay)
+    assert form(8, 1024, 128) == tw
+    assert form(10, 16384, 128) == tw
+    assert form(8, 4096, 1024) == tw
+    assert form(10, 1024, 1024) == tw
+    assert form(8, 1024, 1024) == lane       # the two-stage solve with 64 partitions wins
+    assert form(8, 16384, 1024) == lane      # above 4.5M block rows
+    assert form(5, 1024, 128) == lane        # m < 8: partitions win
+    assert form(8, 1024, 128, ref) == lane   # the twisted form needs the tiled layout
+    assert form(1, 64, 8) == lane
+
+
 def test_uncreated_plan_errors():
     plan = pb.BtdmaGpuPlan()
     with pytest.raises(RuntimeError):
