@@ -121,13 +121,16 @@ __device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[M + 
     return zero_pivot;
 }
 
-// Element i of a vector held whole in every lane.
+// Element i of a vector held whole in every lane, picked with bit masks: written as a chain
+// of selects, the compiler turns it into an indexed load and keeps the whole vector in
+// scratch memory (stored on every row step).
 template <int M>
 __device__ __forceinline__ double lane_elem(const double (&v)[M], int i) {
-    double x = v[0];
+    unsigned long long bits = 0;
 #pragma unroll
-    for (int t = 1; t < M; ++t) x = (i == t) ? v[t] : x;
-    return x;
+    for (int t = 0; t < M; ++t)
+        bits |= (unsigned long long)__double_as_longlong(v[t]) & (0ull - (unsigned long long)(i == t));
+    return __longlong_as_double((long long)bits);
 }
 
 // Cooperative solve: M lanes per system, lane i owns block row i of every block; row q is
@@ -173,45 +176,46 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
 #pragma unroll
         for (int s = 0; s < F; ++s) {
             const int q = q0 + s;
-            if (q >= nmax) break;  // uniform: the last chunk may be partial
-            const bool act = q < n;
-            const bool has_next = q + 1 < n;
-            const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
-            double r[M], rhs[M + 1];
+            if (q < nmax) {  // uniform: the last chunk may be partial
+                const bool act = q < n;
+                const bool has_next = q + 1 < n;
+                const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
+                double r[M], rhs[M + 1];
 #pragma unroll
-            for (int j = 0; j < M; ++j) {
-                r[j] = pb[s][j];
-                rhs[1 + j] = pc[s][j];
-            }
-            rhs[0] = pd[s];
-            if (q > 0) {
+                for (int j = 0; j < M; ++j) {
+                    r[j] = pb[s][j];
+                    rhs[1 + j] = pc[s][j];
+                }
+                rhs[0] = pd[s];
+                if (q > 0) {
+#pragma unroll
+                    for (int k = 0; k < M; ++k) {
+                        const double a = pa[s][k];
+                        rhs[0] = fma(-a, dp[k], rhs[0]);
+#pragma unroll
+                        for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
+                    }
+                }
+                fetch(pa[s], pb[s], pc[s], pd[s], q + F);
+                double xc[M], xd[M];
+                const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
+                const double di = lane_elem<M>(xd, i);
+                ok = ok && !(act && zero_pivot);
+                if (act) {
+                    D[rv + i * es] = di;
+                    xlast = di;
+                }
+                if (has_next) {
+#pragma unroll
+                    for (int k = 0; k < M; ++k) C[rm + (k + M * i) * es] = xc[k];
+                }
 #pragma unroll
                 for (int k = 0; k < M; ++k) {
-                    const double a = pa[s][k];
-                    rhs[0] = fma(-a, dp[k], rhs[0]);
-#pragma unroll
-                    for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
+                    carry[k * kTile + lane] = xc[k];  // C'(k, i) in lane base+i
+                    dp[k] = xd[k];
                 }
+                wave_lds_sync();
             }
-            fetch(pa[s], pb[s], pc[s], pd[s], q + F);
-            double xc[M], xd[M];
-            const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
-            const double di = lane_elem<M>(xd, i);
-            ok = ok && !(act && zero_pivot);
-            if (act) {
-                D[rv + i * es] = di;
-                xlast = di;
-            }
-            if (has_next) {
-#pragma unroll
-                for (int k = 0; k < M; ++k) C[rm + (k + M * i) * es] = xc[k];
-            }
-#pragma unroll
-            for (int k = 0; k < M; ++k) {
-                carry[k * kTile + lane] = xc[k];  // C'(k, i) in lane base+i
-                dp[k] = xd[k];
-            }
-            wave_lds_sync();
         }
     }
     // back substitution over the rows: x_q(i) = d'_q(i) - sum_j C'_q(i,j) x_{q+1}(j), with
@@ -231,21 +235,22 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
 #pragma unroll
         for (int s = 0; s < G; ++s) {
             const int q = q0 - s;
-            if (q < 0) break;  // uniform
-            const bool act = q >= 0 && q < n - 1;
-            double cq[M], x = dn[s];
+            if (q >= 0) {  // uniform: the last chunk may be partial
+                const bool act = q >= 0 && q < n - 1;
+                double cq[M], x = dn[s];
 #pragma unroll
-            for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
-            bfetch(cn[s], dn[s], q - G);
-            sh[lane] = xnext;
-            wave_lds_sync();
+                for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
+                bfetch(cn[s], dn[s], q - G);
+                sh[lane] = xnext;
+                wave_lds_sync();
 #pragma unroll
-            for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
-            if (act) {
-                D[vo + (int64_t)q * M * es + i * es] = x;
-                xnext = x;
+                for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
+                if (act) {
+                    D[vo + (int64_t)q * M * es + i * es] = x;
+                    xnext = x;
+                }
+                wave_lds_sync();
             }
-            wave_lds_sync();
         }
     }
     return ok;
@@ -309,41 +314,42 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
 #pragma unroll
         for (int s = 0; s < F; ++s) {
             const int t = t0 + s;
-            if (t >= steps) break;  // uniform: the last chunk may be partial
-            const bool act = t < cnt;
-            const int q = clamp_row(top ? t : n - 1 - t);
-            const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
-            double r[M], rhs[M + 1];
+            if (t < steps) {  // uniform: the last chunk may be partial
+                const bool act = t < cnt;
+                const int q = clamp_row(top ? t : n - 1 - t);
+                const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
+                double r[M], rhs[M + 1];
 #pragma unroll
-            for (int j = 0; j < M; ++j) {
-                r[j] = pg[s][j];
-                rhs[1 + j] = pu[s][j];
-            }
-            rhs[0] = pd[s];
-            if (t > 0) {
-#pragma unroll
-                for (int k = 0; k < M; ++k) {
-                    const double a = ps[s][k];
-                    rhs[0] = fma(-a, dp[k], rhs[0]);
-#pragma unroll
-                    for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
+                for (int j = 0; j < M; ++j) {
+                    r[j] = pg[s][j];
+                    rhs[1 + j] = pu[s][j];
                 }
-            }
-            fetch(ps[s], pg[s], pu[s], pd[s], t + F);
-            double xc[M], xd[M];
-            const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
-            ok = ok && !(act && zero_pivot);
-            if (act) {
-                D[rv + i * es] = lane_elem<M>(xd, i);
+                rhs[0] = pd[s];
+                if (t > 0) {
 #pragma unroll
-                for (int k = 0; k < M; ++k) {
-                    C[rm + (k + M * i) * es] = xc[k];
-                    carry[k * kTile + lane] = xc[k];
+                    for (int k = 0; k < M; ++k) {
+                        const double a = ps[s][k];
+                        rhs[0] = fma(-a, dp[k], rhs[0]);
+#pragma unroll
+                        for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
+                    }
                 }
-            }
+                fetch(ps[s], pg[s], pu[s], pd[s], t + F);
+                double xc[M], xd[M];
+                const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
+                ok = ok && !(act && zero_pivot);
+                if (act) {
+                    D[rv + i * es] = lane_elem<M>(xd, i);
 #pragma unroll
-            for (int k = 0; k < M; ++k) dp[k] = act ? xd[k] : dp[k];
-            wave_lds_sync();
+                    for (int k = 0; k < M; ++k) {
+                        C[rm + (k + M * i) * es] = xc[k];
+                        carry[k * kTile + lane] = xc[k];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < M; ++k) dp[k] = act ? xd[k] : dp[k];
+                wave_lds_sync();
+            }
         }
     }
     // meeting row h: both groups of a system solve it from both carries
@@ -398,22 +404,23 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
 #pragma unroll
         for (int s = 0; s < G; ++s) {
             const int t = t0 + s;
-            if (t >= steps) break;  // uniform
-            const bool act = t < cnt;
-            const int q = clamp_row(top ? h - 1 - t : h + 1 + t);
-            double cq[M], x = dn[s];
+            if (t < steps) {  // uniform: the last chunk may be partial
+                const bool act = t < cnt;
+                const int q = clamp_row(top ? h - 1 - t : h + 1 + t);
+                double cq[M], x = dn[s];
 #pragma unroll
-            for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
-            bfetch(cn[s], dn[s], t + G);
-            sh[lane] = xprev;
-            wave_lds_sync();
+                for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
+                bfetch(cn[s], dn[s], t + G);
+                sh[lane] = xprev;
+                wave_lds_sync();
 #pragma unroll
-            for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
-            if (act) {
-                D[vo + (int64_t)q * M * es + i * es] = x;
-                xprev = x;
+                for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
+                if (act) {
+                    D[vo + (int64_t)q * M * es + i * es] = x;
+                    xprev = x;
+                }
+                wave_lds_sync();
             }
-            wave_lds_sync();
         }
     }
     return ok;

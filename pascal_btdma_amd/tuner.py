@@ -39,7 +39,7 @@ def _time(fn, restore, reps: int) -> float:
 
 
 def tune_batch(batch: BtdmaBatch, a, b, c, d, reps: int = 3,
-               placements=(0, 1, 2, 3, 4, 5, 6), grids_per_cu=(1, 2, 4)) -> dict:
+               placements=(0, 1, 2, 3, 4, 5, 6), grids_per_cu=(1, 2, 3, 4)) -> dict:
     """Pick (placement, grid) for ``batch``; a, b, c, d are its packed inputs (unchanged)."""
     cw, dw = c.clone(), d.clone()
 
