@@ -335,10 +335,10 @@ def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
         t = event_time(lambda: g.launch(s1), lambda: (restore(), torch.cuda.synchronize()), steps, s1)
         return g, float(np.median(t))
 
-    # sequential at the tuned grid; forked also with the batch on one workgroup per CU, which
-    # leaves the other SIMDs to the long systems' kernels
+    # sequential at the tuned grid; forked also with the batch on one or two workgroups per CU,
+    # which leaves the other SIMDs to the long systems' kernels
     cus = torch.cuda.get_device_properties(dev).multi_processor_count
-    forked_grids = sorted({tb["grid"], cus})
+    forked_grids = sorted({tb["grid"], cus, 2 * cus})
     candidates = [("sequential", tb["grid"])] + [("forked", gr) for gr in forked_grids]
     graphs = {}
     for topo, gr in candidates:
