@@ -48,17 +48,21 @@ struct CoopDepth {
 constexpr int kCoopSlots = 3 * kMaxM + 3;
 
 // Elimination of one block row by a group of M lanes: lane i holds row i of R and of the
-// right-hand sides rhs = (d, S) (S the block whose R^{-1} S is carried to the next row).
+// right-hand sides rhs = (d, S), S of NS blocks of M columns whose R^{-1} S is carried on.
 // Pivot step k: every lane posts its row's columns k..M-1, its rhs and its candidate |r(k)|
 // (-1 once chosen); every lane reads the M candidates, picks the pivot (largest, lowest lane
 // on a tie), reads the pivot row and eliminates with it, a lane that is already chosen or is
 // the pivot with multiplier 0. Then each lane posts its row once more and back-substitutes
-// one rhs column on its own: xc = column i of R^{-1} S (xc[k] = (R^{-1} S)(k, i)) and, in
-// every lane, the whole xd = R^{-1} d. Returns 1 if a pivot was zero. The code is free of
-// lane-dependent branches so that LDS reads issue in batches.
-template <int M>
-__device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[M + 1], double* __restrict__ xb,
-                                              int i, int base, double (&xc)[M], double (&xd)[M]) {
+// column i of every block of S on its own: xc[c*M + k] = (R^{-1} S)(k, c*M + i); and, in every
+// lane, the whole xd = R^{-1} d. Returns 1 if a pivot was zero. The code is free of
+// lane-dependent branches so that LDS reads issue in batches. LDS: xb holds (NS+1)*M + 2
+// slots: the row [0, M), d at M, S at [M+1, M+1+NS*M), the candidate after them.
+template <int M, int NS = 1>
+__device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[NS * M + 1],
+                                              double* __restrict__ xb, int i, int base,
+                                              double (&xc)[NS * M], double (&xd)[M]) {
+    constexpr int NR = NS * M + 1;     // right-hand-side entries of a row
+    constexpr int CAND = M + NR;       // the candidate's slot
     const int lane = threadIdx.x;
     int rank = -1;
     int plane[M];
@@ -68,12 +72,12 @@ __device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[M + 
 #pragma unroll
         for (int j = k; j < M; ++j) xb[j * kTile + lane] = r[j];
 #pragma unroll
-        for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
-        xb[(2 * M + 1) * kTile + lane] = rank < 0 ? fabs(r[k]) : -1.0;
+        for (int j = 0; j < NR; ++j) xb[(M + j) * kTile + lane] = rhs[j];
+        xb[CAND * kTile + lane] = rank < 0 ? fabs(r[k]) : -1.0;
         wave_lds_sync();
         double cand[M];
 #pragma unroll
-        for (int t = 0; t < M; ++t) cand[t] = xb[(2 * M + 1) * kTile + base + t];
+        for (int t = 0; t < M; ++t) cand[t] = xb[CAND * kTile + base + t];
         int p = 0;
         double best = cand[0];
 #pragma unroll
@@ -84,11 +88,11 @@ __device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[M + 
         }
         plane[k] = p;
         const int pl = base + p;
-        double prow[M], prhs[M + 1];
+        double prow[M], prhs[NR];
 #pragma unroll
         for (int j = k; j < M; ++j) prow[j] = xb[j * kTile + pl];
 #pragma unroll
-        for (int j = 0; j <= M; ++j) prhs[j] = xb[(M + j) * kTile + pl];
+        for (int j = 0; j < NR; ++j) prhs[j] = xb[(M + j) * kTile + pl];
         zero_pivot |= prow[k] == 0.0 ? 1 : 0;
         const bool elim = rank < 0 && i != p;
         const double l = elim ? r[k] * fast_rcp(prow[k]) : 0.0;
@@ -96,26 +100,30 @@ __device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[M + 
 #pragma unroll
         for (int j = k + 1; j < M; ++j) r[j] = fma(-l, prow[j], r[j]);
 #pragma unroll
-        for (int j = 0; j <= M; ++j) rhs[j] = fma(-l, prhs[j], rhs[j]);
+        for (int j = 0; j < NR; ++j) rhs[j] = fma(-l, prhs[j], rhs[j]);
     }
 #pragma unroll
     for (int j = 0; j < M; ++j) xb[j * kTile + lane] = r[j];
 #pragma unroll
-    for (int j = 0; j <= M; ++j) xb[(M + j) * kTile + lane] = rhs[j];
+    for (int j = 0; j < NR; ++j) xb[(M + j) * kTile + lane] = rhs[j];
     wave_lds_sync();
 #pragma unroll
     for (int k = M - 1; k >= 0; --k) {
         const int pl = base + plane[k];
-        double sc = xb[(M + 1 + i) * kTile + pl];
+        double sc[NS];
+#pragma unroll
+        for (int c = 0; c < NS; ++c) sc[c] = xb[(M + 1 + c * M + i) * kTile + pl];
         double sd = xb[M * kTile + pl];
 #pragma unroll
         for (int j = k + 1; j < M; ++j) {
             const double u = xb[j * kTile + pl];
-            sc = fma(-u, xc[j], sc);
+#pragma unroll
+            for (int c = 0; c < NS; ++c) sc[c] = fma(-u, xc[c * M + j], sc[c]);
             sd = fma(-u, xd[j], sd);
         }
         const double rk = fast_rcp(xb[k * kTile + pl]);
-        xc[k] = sc * rk;
+#pragma unroll
+        for (int c = 0; c < NS; ++c) xc[c * M + k] = sc[c] * rk;
         xd[k] = sd * rk;
     }
     return zero_pivot;

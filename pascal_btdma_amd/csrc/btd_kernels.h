@@ -918,6 +918,203 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
     }
 }
 
+// XCDs of the device: workgroups are dispatched to them round robin (workgroup b runs on XCD
+// b mod kXcds), each XCD with its own L2.
+constexpr int kXcds = 8;
+
+// Stage 1, cooperative: the arithmetic of k_modi (forward modified sweep, row 1's relation
+// composed on the way, the close, the two reduced rows) with a partition's block rows spread
+// over m lanes (lane i owns row i of every block). Each row is eliminated by coop_eliminate
+// with the right-hand sides [d | X | C], X = A_q for q < 2 and -A_q Â_{q-1} after; Ĉ and Â
+// are carried to the next row in LDS, d̂ whole in every lane; alpha, beta, gamma are kept
+// row-wise (lane i: alpha_i, beta(i,:), gamma(i,:)). floor(64/m) consecutive systems of one
+// partition per wavefront; the items of one partition go to the workgroups of one XCD in
+// order, so neighbouring systems' shared lines meet in one L2. Used for m >= 7, where the
+// one-lane form spills its blocks to scratch memory.
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_modi_coop(double* A, const double* __restrict__ B, double* C,
+                                                  double* D, int nsys, int nrow,
+                                                  const int* __restrict__ part_first,
+                                                  const int* __restrict__ part_len, int npart,
+                                                  double* __restrict__ redA, double* __restrict__ redC,
+                                                  double* __restrict__ redD, int* __restrict__ flag) {
+    constexpr int G = kTile / M;  // systems per wavefront
+    constexpr int E = M * M;
+    __shared__ double sh[(5 * M + 3) * kTile];
+    const int nchunks = (nsys + G - 1) / G;
+    const int nitems = npart * nchunks;
+    const int per = (nitems + kXcds - 1) / kXcds;
+    const int u = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (u >= nitems) return;
+    const int part = u / nchunks, chunk = u % nchunks;
+    const int lane = threadIdx.x;
+    const int g = lane / M, i = lane % M;
+    const bool grouped = g < G;
+    const int64_t s0 = (int64_t)chunk * G;
+    const int64_t sys = s0 + (grouped ? g : 0);
+    const bool mine = grouped && sys < nsys;
+    const int64_t se = mine ? sys : s0;  // a real system either way
+    // lanes past the last whole group read group 0's posts and write only their own slots
+    const int base = grouped ? g * M : 0;
+    const Addr<LAYOUT> am(se, nsys, nrow, E), av(se, nsys, nrow, M);
+    const Addr<LAYOUT_TILED> rm(se, 0, 2 * npart, E), rv(se, 0, 2 * npart, M);
+    const int64_t r0 = part_first[part];
+    const int N = part_len[part];
+    double* carry = sh;                     // slots [0, M) Ĉ, [M, 2M) Â (slot k, lane base+j: (k, j))
+    double* xb = sh + (2 * M + 1) * kTile;  // the elimination exchange, 3M + 2 slots
+    bool ok = true;
+    double pa[M], pb[M], pc[M], pd;
+    auto fetch = [&](int q) {
+        const int64_t gr = r0 + (q < N ? q : N - 1);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            pa[k] = A[am.at(gr, i + M * k)];
+            pb[k] = __builtin_nontemporal_load(B + am.at(gr, i + M * k));
+            pc[k] = C[am.at(gr, i + M * k)];
+        }
+        pd = D[av.at(gr, i)];
+    };
+    fetch(0);
+    double dp[M], beta[M], gamma[M], alpha = 0.0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) dp[k] = beta[k] = gamma[k] = 0.0;
+    for (int q = 0; q < N; ++q) {
+        const int64_t gr = r0 + q;
+        double r[M], rhs[2 * M + 1];
+        rhs[0] = pd;
+        if (q >= 2) {
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                r[j] = pb[j];
+                rhs[1 + j] = 0.0;
+                rhs[1 + M + j] = pc[j];
+            }
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                const double a = pa[k];
+                rhs[0] = fma(-a, dp[k], rhs[0]);
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
+                    rhs[1 + j] = fma(-a, carry[(M + k) * kTile + base + j], rhs[1 + j]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                r[j] = pb[j];
+                rhs[1 + j] = pa[j];
+                rhs[1 + M + j] = pc[j];
+            }
+        }
+        fetch(q + 1);
+        double xc[2 * M], xd[M];
+        ok = !coop_eliminate<M, 2>(r, rhs, xb, i, base, xc, xd) && ok;
+        const bool keep = q + 1 < N;  // the last row only feeds the reduced system
+        if (mine) {
+            if (keep) {
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    A[am.at(gr, k + M * i)] = xc[k];
+                    C[am.at(gr, k + M * i)] = xc[M + k];
+                }
+                D[av.at(gr, i)] = lane_elem<M>(xd, i);
+            } else {
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    redA[rm.at(2 * part + 1, k + M * i)] = xc[k];
+                    redC[rm.at(2 * part + 1, k + M * i)] = xc[M + k];
+                }
+                redD[rv.at(2 * part + 1, i)] = lane_elem<M>(xd, i);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            carry[k * kTile + lane] = xc[M + k];  // Ĉ(k, i)
+            carry[(M + k) * kTile + lane] = xc[k];  // Â(k, i)
+            dp[k] = xd[k];
+        }
+        wave_lds_sync();
+        if (q == 1) {
+            alpha = lane_elem<M>(xd, i);
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                beta[j] = carry[(M + i) * kTile + base + j];
+                gamma[j] = carry[i * kTile + base + j];
+            }
+        } else if (q >= 2 && keep) {
+            // alpha -= gamma d̂_q, beta -= gamma Â_q, gamma = -gamma Ĉ_q (row i)
+#pragma unroll
+            for (int k = 0; k < M; ++k) alpha = fma(-gamma[k], xd[k], alpha);
+            double nb[M], ng[M];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double b = beta[j], c = 0.0;
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    b = fma(-gamma[k], carry[(M + k) * kTile + base + j], b);
+                    c = fma(-gamma[k], carry[k * kTile + base + j], c);
+                }
+                nb[j] = b;
+                ng[j] = c;
+            }
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                beta[j] = nb[j];
+                gamma[j] = ng[j];
+            }
+        }
+    }
+    // close, row 0: R = I - Ĉ_0 beta; [d̂|Â|Ĉ]_0 = R^{-1}[d̂_0 - Ĉ_0 alpha | Â_0 | -Ĉ_0 gamma], with
+    // Â_0, Ĉ_0, d̂_0 read back where the group stored them and alpha, beta, gamma through LDS
+    {
+        double a0[M], c0[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            a0[j] = A[am.at(r0, i + M * j)];
+            c0[j] = C[am.at(r0, i + M * j)];
+        }
+        const double d0 = D[av.at(r0, i)];
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            carry[j * kTile + lane] = gamma[j];        // gamma(i, j): slot j, lane base+i
+            carry[(M + j) * kTile + lane] = beta[j];   // beta(i, j)
+        }
+        carry[2 * M * kTile + lane] = alpha;
+        wave_lds_sync();
+        double r[M], rhs[2 * M + 1];
+        rhs[0] = d0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            r[j] = i == j ? 1.0 : 0.0;
+            rhs[1 + j] = a0[j];
+            rhs[1 + M + j] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const double ck = c0[k];
+            rhs[0] = fma(-ck, carry[2 * M * kTile + base + k], rhs[0]);
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                r[j] = fma(-ck, carry[(M + j) * kTile + base + k], r[j]);
+                rhs[1 + M + j] = fma(-ck, carry[j * kTile + base + k], rhs[1 + M + j]);
+            }
+        }
+        double xc[2 * M], xd[M];
+        ok = !coop_eliminate<M, 2>(r, rhs, xb, i, base, xc, xd) && ok;
+        if (mine) {
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                redA[rm.at(2 * part, k + M * i)] = xc[k];
+                redC[rm.at(2 * part, k + M * i)] = xc[M + k];
+            }
+            redD[rv.at(2 * part, i)] = lane_elem<M>(xd, i);
+        }
+    }
+    if (mine && !ok) report_singular(flag, sys);
+}
+
 // ---------------------------------------------------------------------------- reduced --
 // Plain solve of the reduced system, 2P rows with B = I, in the tiled reduced buffers.
 // The solution replaces redD.
@@ -971,6 +1168,12 @@ __global__ __launch_bounds__(64) void k_reduced(const double* __restrict__ redA,
     if (!ok) report_singular(flag, sys);
 }
 
+// Smallest m whose stage 1 runs cooperatively (k_modi_coop) instead of one lane per partition.
+#ifndef BTD_MODI_COOP_FROM
+#define BTD_MODI_COOP_FROM 7
+#endif
+constexpr int kModiCoopFrom = BTD_MODI_COOP_FROM;
+
 // Largest number of wavefronts per CU for which the reduced system is solved twisted.
 #ifndef BTD_TWISTED_REDUCED_WAVES_PER_CU
 #define BTD_TWISTED_REDUCED_WAVES_PER_CU 4
@@ -1006,10 +1209,6 @@ __global__ __launch_bounds__(64) void k_reduced_twisted(const double* __restrict
                                                  mine ? nrr : 0, steps, nrr, sh);
     if (!ok && mine) report_singular(flag, sys);
 }
-
-// XCDs of the device: workgroups are dispatched to them round robin (workgroup b runs on XCD
-// b mod kXcds), each XCD with its own L2.
-constexpr int kXcds = 8;
 
 // Cooperative plain solve (npart = 1) of a tiled plan, twisted: per system two groups of M
 // lanes (coop_twisted), batch_width(M) consecutive systems of one tile per wavefront, so a
@@ -1463,12 +1662,23 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
         (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_REF_GPU>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     }
-    if (layout == LAYOUT_TILED)
+    if constexpr (M >= kModiCoopFrom) {
+        // m lanes per partition (no spills); the items of a partition on one XCD
+        const int64_t items = (int64_t)npart * ((nsys + kTile / M - 1) / (kTile / M));
+        const dim3 gridc((unsigned)(kXcds * ((items + kXcds - 1) / kXcds)));
+        if (layout == LAYOUT_TILED)
+            hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_TILED>), gridc, block, 0, s, A, B, C, D, nsys, nrow, pf,
+                               pl, npart, redA, redC, redD, flag);
+        else
+            hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_REF_GPU>), gridc, block, 0, s, A, B, C, D, nsys, nrow, pf,
+                               pl, npart, redA, redC, redD, flag);
+    } else if (layout == LAYOUT_TILED) {
         hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
                            npart, redA, redC, redD, flag);
-    else
+    } else {
         hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
                            npart, redA, redC, redD, flag);
+    }
     if (ev) (void)hipEventRecord(ev[1], s);
     // the twisted reduced solve when its wavefronts all fit on the device at once (one per
     // SIMD); otherwise one lane per system does more systems per instruction

@@ -47,7 +47,8 @@ def test_cyclic_single_partition(cuda, oracle, m, layout):
     assert rel_err(x, xo).max() <= TOL
 
 
-@pytest.mark.parametrize("m,n,npart", [(2, 24, 2), (3, 96, 4), (5, 96, 8), (3, 100, 7), (5, 9, 3)])
+@pytest.mark.parametrize("m,n,npart", [(2, 24, 2), (3, 96, 4), (5, 96, 8), (3, 100, 7), (5, 9, 3),
+                                        (8, 40, 3), (10, 30, 4)])
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
 def test_cyclic_two_stage(cuda, oracle, m, n, npart, layout):
     rng = np.random.default_rng(90 + n + m)

@@ -194,6 +194,24 @@ def test_per_system_info(cuda, npart, cyclic):
         assert not plan.check()
 
 
+@pytest.mark.parametrize("m,npart", [(8, 3), (10, 2), (7, 4)])
+def test_per_system_info_cooperative_stage1(cuda, m, npart):
+    """m >= 7 runs stage 1 cooperatively (m lanes per partition): zero block rows in a middle
+    row, in a partition's first row and in its last row are named per system."""
+    import torch
+    nsys, n = 70, 4 * npart + 3
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 12, m, nsys, n)
+    firsts = [pb.partition_1d(0, n - 1, npart, r)[0] for r in range(npart)]
+    bad = [(2, firsts[-1] + 1), (30, firsts[1]), (69, n - 1)]
+    for s_, q in bad:
+        A[q, s_] = B[q, s_] = C[q, s_] = 0.0
+    for lay in (pb.Layout.TILED, pb.Layout.REF_GPU):
+        ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, int(lay))).to(cuda) for X in (A, B, C, d))
+        with pb.BtdmaGpuPlan().create(m, nsys, n, npart, lay) as plan:
+            pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
+            assert plan.info().nonzero()[0].tolist() == [2, 30, 69]
+
+
 def test_api_errors_on_device(cuda):
     import torch
     with pb.BtdmaGpuPlan().create(3, 64, 8) as plan:
