@@ -58,7 +58,8 @@ typedef enum btd_layout {
 
 /* How a plan's single-partition solve runs (btd_plan_set_form). */
 typedef enum btd_form {
-    BTD_FORM_LANE = 0,    /* one lane per system, blocks in VGPRs */
+    BTD_FORM_PLAIN = 0,   /* one lane per system (m <= 7; blocks in VGPRs) or m lanes per system
+                           * (m >= 8; one block row each, pivoting through LDS) */
     BTD_FORM_TWISTED = 1  /* two groups of m lanes per system, one eliminating from each end */
 } btd_form;
 
@@ -79,34 +80,30 @@ typedef struct btd_timings {
  * (mod_btdma_gpu_v2.f90:70-109). `npart` takes the place of the communicator size:
  * npart = 1 is the single-partition solve; npart > 1 splits every system's rows with
  * btd_partition_1d and needs nrow >= 3*npart; npart = 0 lets the library choose
- * (btd_auto_form, then btd_auto_npart; read the choices back with btd_plan_form and
- * btd_plan_npart). layout is BTD_LAYOUT_REF_GPU
+ * (btd_auto_npart; read the choice back with btd_plan_npart). layout is BTD_LAYOUT_REF_GPU
  * or BTD_LAYOUT_TILED. 1 <= m <= BTD_MAX_M, nsys >= 1, nrow >= 2. Allocates device scratch
  * on the current device. */
 btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npart, int layout);
 
 /* New: the partition count a plan created with npart = 0 gets on one GPU, where partitions are
- * an occupancy choice rather than a rank count: 1 (the single-kernel solve) for nsys >= 32768,
- * otherwise about 65536 / nsys (one wave per SIMD), at most nrow / 8, and 1 when that is < 2.
+ * an occupancy choice rather than a rank count: 1 (the single-kernel solve) for nsys >= 32768
+ * (nsys >= 4096 for m >= 8, whose single-partition solve runs m lanes per system), otherwise
+ * about 65536 / nsys (one wave per SIMD), at most nrow / 8, and 1 when that is < 2.
  * Pure function (no device use). */
 int btd_auto_npart(int m, int nsys, int nrow);
-
-/* New: the form a plan created with npart = 0 gets: BTD_FORM_TWISTED (with npart = 1) for tiled
- * plans of m >= 8 with at most 4.5M block rows in all and either nrow <= 256 or nsys >= 4096
- * (m = 8) / 1024 (m >= 9), where it measured fastest; otherwise BTD_FORM_LANE with
- * btd_auto_npart's partition count. Pure function (no device use). */
-int btd_auto_form(int m, int nsys, int nrow, int layout);
 
 /* The plan's partition count (the one chosen when it was created with npart = 0). */
 btd_status btd_plan_npart(btd_plan_t plan, int *npart);
 
 /* New (no counterpart in the reference): choose how an npart = 1 plan's non-cyclic solve runs.
- * BTD_FORM_LANE (the default) gives each system one lane; BTD_FORM_TWISTED (tiled layout,
- * m >= 2) gives each system two groups of m lanes, one eliminating downwards from row 0 and
- * one upwards from row nrow-1, meeting in the middle: 2m lanes and half the row latency per
- * system, for plans with too few systems to fill the device with one lane each. Same zero-
- * pivot reporting; results agree with BTD_FORM_LANE to rounding; C is left overwritten in
- * both forms. Cyclic and the other solve variants are unaffected. */
+ * BTD_FORM_PLAIN (the default) is the library's single-partition kernel: one lane per system
+ * for m <= 7, m lanes per system (one block row each) for m >= 8. BTD_FORM_TWISTED (tiled
+ * layout, m >= 2) gives each system two groups of m lanes, one eliminating downwards from row
+ * 0 and one upwards from row nrow-1, meeting in the middle: half the row latency per system,
+ * for plans with few, long systems (it reads the tiled lines several times over, so it does
+ * not pay for large batches; it is never chosen automatically). Same zero-pivot reporting;
+ * results agree with BTD_FORM_PLAIN to rounding; C is left overwritten. Cyclic and the other
+ * solve variants are unaffected. */
 btd_status btd_plan_set_form(btd_plan_t plan, int form);
 btd_status btd_plan_form(btd_plan_t plan, int *form);
 

@@ -38,7 +38,7 @@ class Layout(enum.IntEnum):
 
 class Form(enum.IntEnum):
     """How a plan's single-partition solve runs (btd_form)."""
-    LANE = 0
+    PLAIN = 0
     TWISTED = 1
 
 
@@ -152,7 +152,7 @@ class BtdmaGpuPlan:
         _check(_capi.lib().btd_plan_create(ctypes.byref(h), int(m), int(nsys), int(nrow),
                                            int(npart), int(layout)))
         self._h = h
-        if int(npart) == 0:  # chosen by the library (with the form: see btd_auto_form)
+        if int(npart) == 0:  # chosen by the library
             p = ctypes.c_int(0)
             _check(_capi.lib().btd_plan_npart(h, ctypes.byref(p)))
             npart = p.value
@@ -169,7 +169,7 @@ class BtdmaGpuPlan:
 
     @property
     def form(self) -> int:
-        """How the single-partition solve runs: Form.LANE or Form.TWISTED (btd_plan_form)."""
+        """How the single-partition solve runs: Form.PLAIN or Form.TWISTED (btd_plan_form)."""
         f = ctypes.c_int(0)
         _check(_capi.lib().btd_plan_form(self._h, ctypes.byref(f)))
         return Form(f.value)

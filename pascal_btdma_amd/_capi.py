@@ -47,7 +47,6 @@ SIGNATURES = {
     "btd_plan_check": (_i, [_vp, ctypes.POINTER(_i)]),
     "btd_auto_npart": (_i, [_i, _i, _i]),
     "btd_plan_npart": (_i, [_vp, ctypes.POINTER(_i)]),
-    "btd_auto_form": (_i, [_i, _i, _i, _i]),
     "btd_plan_set_form": (_i, [_vp, _i]),
     "btd_plan_form": (_i, [_vp, ctypes.POINTER(_i)]),
     "btd_plan_info": (_i, [_vp, _vp]),

@@ -233,7 +233,9 @@ __device__ __forceinline__ bool coop_item(const BatchDev& b, const double* A, co
     const int64_t mo = b.mat_off[pk] + (mine ? slot : 0), vo = b.vec_off[pk] + (mine ? slot : 0);
     // lanes past the last whole group read group 0's posts and write only their own lane's
     // slots, which no group reads
-    return coop_rows<M>(A, B, C, D, mo, vo, i, grouped ? g * M : 0, n, nmax, b.pack_rows[pk], sh);
+    const StridedAddr<M * M * W, W> am{mo};
+    const StridedAddr<M * W, W> av{vo};
+    return coop_rows<M>(A, B, C, D, am, av, i, grouped ? g * M : 0, n, nmax, b.pack_rows[pk], sh);
 }
 
 // A twisted item: the systems of pack p, two groups of m lanes each.
@@ -255,7 +257,9 @@ __device__ __forceinline__ bool twisted_item(const BatchDev& b, const double* A,
     // own lane's slots
     const int base = grouped ? g * M : 0;
     const int pbase = grouped ? (top ? base + M : base - M) : M;
-    return coop_twisted<M>(A, B, C, D, mo, vo, i, base, pbase, top, n, steps, nmax, sh);
+    const StridedAddr<M * M * W, W> am{mo};
+    const StridedAddr<M * W, W> av{vo};
+    return coop_twisted<M>(A, B, C, D, am, av, i, base, pbase, top, n, steps, nmax, sh);
 }
 
 // Persistent solve of a packed batch. KIND selects the code a placement needs, so that the

@@ -55,7 +55,7 @@ int btd::device_cus() {
 
 struct btd_plan_s {
     int m = 0, nsys = 0, nrow = 0, npart = 1, layout = BTD_LAYOUT_TILED;
-    int form = BTD_FORM_LANE;  // how an npart = 1 solve runs (btd_plan_set_form)
+    int form = BTD_FORM_PLAIN;  // how an npart = 1 solve runs (btd_plan_set_form)
     int device = 0;
     bool created = false;
     std::vector<int> first, len;  // partition rows (host)
@@ -106,32 +106,19 @@ size_t btd_layout_elems(int layout, int nsys, int nrow, int elems_per_block) {
 }
 
 int btd_auto_npart(int m, int nsys, int nrow) {
-    // One GPU: partitions are an occupancy choice. A batch of at least half a wave per SIMD
-    // (32768 systems) is solved by the single-kernel solve; a smaller one gets enough
-    // partitions for about one wave per SIMD (65536 lanes), keeping >= 8 rows per partition
-    // so that the reduced system of 2P rows stays small against the partition work.
-    (void)m;
+    // One GPU: partitions are an occupancy choice. The single-kernel solve takes a batch that
+    // fills the device on its own: at least half a wave per SIMD (32768 systems) with one lane
+    // per system, or 4096 systems for m >= 8, whose single-partition solve gives every system
+    // m lanes. A smaller batch gets enough partitions for about one wave per SIMD (65536
+    // lanes), keeping >= 8 rows per partition so that the reduced system of 2P rows stays
+    // small against the partition work.
     constexpr int64_t kLanes = 65536;
     if (nsys < 1 || nrow < 2) return 1;
-    if ((int64_t)nsys >= kLanes / 2) return 1;
+    if ((int64_t)nsys >= (m >= 8 ? 4096 : kLanes / 2)) return 1;
     const int64_t want = (kLanes + nsys - 1) / nsys;
     const int64_t most = nrow / 8;
     const int64_t p = want < most ? want : most;
     return p < 2 ? 1 : (int)p;
-}
-
-int btd_auto_form(int m, int nsys, int nrow, int layout) {
-    // The twisted single-partition solve gives every system 2m lanes and half the row latency
-    // but reads the tiled layout in 16-byte-per-system pieces whose lines are shared by other
-    // work items (measured 4.2x the compulsory bytes at m = 8, 16384 x 1024). It is chosen
-    // where it measured fastest of the one-lane, library two-stage and twisted solves: m >= 8
-    // (the one-lane LU of large blocks is slow), batches of at most 4.5M block rows, with short
-    // systems (nrow <= 256) or enough systems to fill the device (nsys >= 4096 for m = 8,
-    // >= 1024 for m >= 9); otherwise the partitions of btd_auto_npart win.
-    if (layout != BTD_LAYOUT_TILED || m < 8 || m > BTD_MAX_M || nsys < 1 || nrow < 2) return BTD_FORM_LANE;
-    if ((int64_t)nsys * nrow > 4500000) return BTD_FORM_LANE;
-    if (nrow <= 256 || nsys >= (m >= 9 ? 1024 : 4096)) return BTD_FORM_TWISTED;
-    return BTD_FORM_LANE;
 }
 
 btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npart, int layout) {
@@ -144,11 +131,7 @@ btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npar
         return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: requires nsys > 0 and nrow >= 2");
     if (npart < 0)
         return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: requires npart >= 1 (or 0: chosen by the library)");
-    int form = BTD_FORM_LANE;
-    if (npart == 0) {
-        form = btd_auto_form(m, nsys, nrow, layout);
-        npart = form == BTD_FORM_TWISTED ? 1 : btd_auto_npart(m, nsys, nrow);
-    }
+    if (npart == 0) npart = btd_auto_npart(m, nsys, nrow);
     if (npart > 1 && nrow < 3 * npart)
         return fail(BTD_ERR_INVALID_ARG,
                     "btd_plan_create: the two-stage solve needs at least 3 rows per partition "
@@ -163,7 +146,6 @@ btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npar
     p->nrow = nrow;
     p->npart = npart;
     p->layout = layout;
-    p->form = form;
     hipError_t e = hipGetDevice(&p->device);
     if (e == hipSuccess) e = hipMalloc(&p->d_flag, sizeof(int) * (1 + (size_t)nsys));
     if (e == hipSuccess) e = hipMemset(p->d_flag, 0, sizeof(int) * (1 + (size_t)nsys));
@@ -208,7 +190,7 @@ btd_status btd_plan_npart(btd_plan_t plan, int* npart) {
 btd_status btd_plan_set_form(btd_plan_t plan, int form) {
     g_err.clear();
     if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_plan_set_form: uncreated plan");
-    if (form == BTD_FORM_LANE) {
+    if (form == BTD_FORM_PLAIN) {
         plan->form = form;
         return BTD_OK;
     }

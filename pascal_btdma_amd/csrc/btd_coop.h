@@ -1,13 +1,21 @@
 // Cooperative block-Thomas rows: a system's block rows spread over m lanes of a wavefront
 // (lane i owns row i of every block), pivot search and row exchange through LDS. Used by the
 // mixed-size batch (btd_batch.h) and by the twisted solve of the two-stage reduced system
-// (btd_kernels.h). Storage: element e of row q of a system is at base + (q*E + e)*ES
-// (E = m*m for blocks, m for vectors), base being the system's own offset.
+// (btd_kernels.h). Storage: element e (column-major within the block) of row q of a lane's
+// system is at am.at(q, e) for blocks and av.at(q, e) for vectors; StridedAddr gives the
+// pack and tiled forms with compile-time strides, the plans' Addr the reference GPU layout.
 #pragma once
 
 #include "btd_common.h"
 
 namespace btd {
+
+// Element e of row q at base + q*RS + e*ES, strides fixed at compile time.
+template <int64_t RS, int64_t ES>
+struct StridedAddr {
+    int64_t base;
+    __device__ __forceinline__ int64_t at(int64_t q, int e) const { return base + q * RS + (int64_t)e * ES; }
+};
 
 // Orders the LDS accesses of the lanes of one wavefront. The cooperative kernels run one-wave
 // (64-lane) workgroups, and the LDS performs the operations of a wavefront in the order they
@@ -150,29 +158,26 @@ __device__ __forceinline__ double lane_elem(const double (&v)[M], int i) {
 // slot of the pack, so loads are clamped to that range and need no other guard. All lanes of
 // the wave run the same row steps (nmax, the item's longest system); lanes past their
 // system's n compute on what the rows hold and store nothing.
-template <int M, int ES = batch_width(M)>
+template <int M, class AM, class AV>
 __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict__ B, double* C,
-                          double* D, int64_t mo, int64_t vo, int i, int base, int n, int nmax, int rows,
+                          double* D, const AM& am, const AV& av, int i, int base, int n, int nmax, int rows,
                           double* __restrict__ sh) {
-    constexpr int E = M * M;
     constexpr int F = CoopDepth<M>::kFwdAhead;
     constexpr int G = CoopDepth<M>::kBwdAhead;
     double* carry = sh;
     double* xb = sh + (M + 1) * kTile;
     const int lane = threadIdx.x;
-    constexpr int64_t es = ES;
     bool ok = true;
     double pa[F][M], pb[F][M], pc[F][M], pd[F];
     auto fetch = [&](double (&fa)[M], double (&fb)[M], double (&fc)[M], double& fd, int q) {
         q = q < rows ? q : rows - 1;
-        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            fa[k] = __builtin_nontemporal_load(A + rm + (i + M * k) * es);
-            fb[k] = __builtin_nontemporal_load(B + rm + (i + M * k) * es);
-            fc[k] = C[rm + (i + M * k) * es];
+            fa[k] = __builtin_nontemporal_load(A + am.at(q, i + M * k));
+            fb[k] = __builtin_nontemporal_load(B + am.at(q, i + M * k));
+            fc[k] = C[am.at(q, i + M * k)];
         }
-        fd = D[rv + i * es];
+        fd = D[av.at(q, i)];
     };
 #pragma unroll
     for (int s = 0; s < F; ++s) fetch(pa[s], pb[s], pc[s], pd[s], s);
@@ -187,7 +192,6 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
             if (q < nmax) {  // uniform: the last chunk may be partial
                 const bool act = q < n;
                 const bool has_next = q + 1 < n;
-                const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
                 double r[M], rhs[M + 1];
 #pragma unroll
                 for (int j = 0; j < M; ++j) {
@@ -210,12 +214,12 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
                 const double di = lane_elem<M>(xd, i);
                 ok = ok && !(act && zero_pivot);
                 if (act) {
-                    D[rv + i * es] = di;
+                    D[av.at(q, i)] = di;
                     xlast = di;
                 }
                 if (has_next) {
 #pragma unroll
-                    for (int k = 0; k < M; ++k) C[rm + (k + M * i) * es] = xc[k];
+                    for (int k = 0; k < M; ++k) C[am.at(q, k + M * i)] = xc[k];
                 }
 #pragma unroll
                 for (int k = 0; k < M; ++k) {
@@ -232,10 +236,9 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
     double cn[G][M], dn[G];
     auto bfetch = [&](double (&fc)[M], double& fd, int q) {
         q = q < 0 ? 0 : (q < rows ? q : rows - 1);  // this lane's pack may be shorter than nmax
-        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
 #pragma unroll
-        for (int j = 0; j < M; ++j) fc[j] = C[rm + (i + M * j) * es];
-        fd = D[rv + i * es];
+        for (int j = 0; j < M; ++j) fc[j] = C[am.at(q, i + M * j)];
+        fd = D[av.at(q, i)];
     };
 #pragma unroll
     for (int s = 0; s < G; ++s) bfetch(cn[s], dn[s], nmax - 2 - s);
@@ -254,7 +257,7 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
 #pragma unroll
                 for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
                 if (act) {
-                    D[vo + (int64_t)q * M * es + i * es] = x;
+                    D[av.at(q, i)] = x;
                     xnext = x;
                 }
                 wave_lds_sync();
@@ -278,17 +281,15 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
 // count. A group's carried block and rhs stay as its last row left them while the other
 // groups finish. ES is the element stride of the storage; with UNIT_B every B_q is the
 // identity and B is not read (the reduced system of the two-stage solve).
-template <int M, int ES = batch_width(M), bool UNIT_B = false>
+template <int M, bool UNIT_B = false, class AM, class AV>
 __device__ bool coop_twisted(const double* A, const double* __restrict__ B, double* C, double* D,
-                             int64_t mo, int64_t vo, int i, int base, int pbase, bool top, int n,
+                             const AM& am, const AV& av, int i, int base, int pbase, bool top, int n,
                              int steps, int rows, double* __restrict__ sh) {
-    constexpr int E = M * M;
     constexpr int F = CoopDepth<M>::kFwdAhead;
     constexpr int G = CoopDepth<M>::kBwdAhead;
     double* carry = sh;
     double* xb = sh + (M + 1) * kTile;
     const int lane = threadIdx.x;
-    constexpr int64_t es = ES;
     const int h = n > 0 ? (n - 1) / 2 : 0;
     const int cnt = n > 0 ? (top ? h : n - 1 - h) : 0;
     const double* sub = top ? A : C;  // the block coupling row q to the row eliminated before it
@@ -298,17 +299,16 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
     double ps[F][M], pg[F][M], pu[F][M], pd[F];
     auto fetch = [&](double (&fs)[M], double (&fg)[M], double (&fu)[M], double& fd, int t) {
         const int q = clamp_row(top ? t : n - 1 - t);
-        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            fs[k] = sub[rm + (i + M * k) * es];
+            fs[k] = sub[am.at(q, i + M * k)];
             if constexpr (UNIT_B)
                 fg[k] = i == k ? 1.0 : 0.0;
             else
-                fg[k] = __builtin_nontemporal_load(B + rm + (i + M * k) * es);
-            fu[k] = sup[rm + (i + M * k) * es];
+                fg[k] = __builtin_nontemporal_load(B + am.at(q, i + M * k));
+            fu[k] = sup[am.at(q, i + M * k)];
         }
-        fd = D[rv + i * es];
+        fd = D[av.at(q, i)];
     };
 #pragma unroll
     for (int s = 0; s < F; ++s) fetch(ps[s], pg[s], pu[s], pd[s], s);
@@ -325,7 +325,6 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
             if (t < steps) {  // uniform: the last chunk may be partial
                 const bool act = t < cnt;
                 const int q = clamp_row(top ? t : n - 1 - t);
-                const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
                 double r[M], rhs[M + 1];
 #pragma unroll
                 for (int j = 0; j < M; ++j) {
@@ -347,10 +346,10 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
                 const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
                 ok = ok && !(act && zero_pivot);
                 if (act) {
-                    D[rv + i * es] = lane_elem<M>(xd, i);
+                    D[av.at(q, i)] = lane_elem<M>(xd, i);
 #pragma unroll
                     for (int k = 0; k < M; ++k) {
-                        C[rm + (k + M * i) * es] = xc[k];
+                        C[am.at(q, k + M * i)] = xc[k];
                         carry[k * kTile + lane] = xc[k];
                     }
                 }
@@ -366,20 +365,20 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
     const int tb = top ? base : pbase, bb = top ? pbase : base;
     double xh[M];
     {
-        const int64_t rm = mo + (int64_t)h * E * es, rv = vo + (int64_t)h * M * es;
+        const int q = h;  // the meeting row
         double r[M], rhs[M + 1];
         double ah[M], ch[M];
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            ah[k] = h > 0 ? A[rm + (i + M * k) * es] : 0.0;  // row 0 has no sub-diagonal block
-            ch[k] = C[rm + (i + M * k) * es];
+            ah[k] = h > 0 ? A[am.at(q, i + M * k)] : 0.0;  // row 0 has no sub-diagonal block
+            ch[k] = C[am.at(q, i + M * k)];
             if constexpr (UNIT_B)
                 r[k] = i == k ? 1.0 : 0.0;
             else
-                r[k] = B[rm + (i + M * k) * es];
+                r[k] = B[am.at(q, i + M * k)];
             rhs[1 + k] = 0.0;
         }
-        rhs[0] = D[rv + i * es];
+        rhs[0] = D[av.at(q, i)];
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             rhs[0] = fma(-ah[k], carry[M * kTile + tb + k], rhs[0]);
@@ -393,17 +392,16 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
         double xc[M];
         const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xh);
         ok = ok && !(n > 0 && zero_pivot);
-        if (n > 0 && top) D[rv + i * es] = lane_elem<M>(xh, i);
+        if (n > 0 && top) D[av.at(q, i)] = lane_elem<M>(xh, i);
     }
     // outward substitution, kBwdAhead rows ahead
     double xprev = lane_elem<M>(xh, i);
     double cn[G][M], dn[G];
     auto bfetch = [&](double (&fc)[M], double& fd, int t) {
         const int q = clamp_row(top ? h - 1 - t : h + 1 + t);
-        const int64_t rm = mo + (int64_t)q * E * es, rv = vo + (int64_t)q * M * es;
 #pragma unroll
-        for (int j = 0; j < M; ++j) fc[j] = C[rm + (i + M * j) * es];
-        fd = D[rv + i * es];
+        for (int j = 0; j < M; ++j) fc[j] = C[am.at(q, i + M * j)];
+        fd = D[av.at(q, i)];
     };
     wave_lds_sync();
 #pragma unroll
@@ -424,7 +422,7 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
 #pragma unroll
                 for (int j = 0; j < M; ++j) x = fma(-cq[j], sh[base + j], x);
                 if (act) {
-                    D[vo + (int64_t)q * M * es + i * es] = x;
+                    D[av.at(q, i)] = x;
                     xprev = x;
                 }
                 wave_lds_sync();

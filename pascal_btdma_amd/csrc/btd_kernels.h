@@ -1201,12 +1201,47 @@ __global__ __launch_bounds__(64) void k_reduced_twisted(const double* __restrict
     const bool mine = grouped && sys < nsys;
     const int64_t s_eff = mine ? sys : first;
     const int64_t tile = s_eff / kTile, l = s_eff % kTile;
-    const int64_t mo = tile * nrr * M * M * kTile + l, vo = tile * nrr * M * kTile + l;
+    const StridedAddr<M * M * kTile, kTile> am{tile * nrr * M * M * kTile + l};
+    const StridedAddr<M * kTile, kTile> av{tile * nrr * M * kTile + l};
     const int steps = (nrr - 1) - (nrr - 1) / 2;
     const int base = grouped ? g * M : 0;
     const int pbase = grouped ? (top ? base + M : base - M) : M;
-    const bool ok = coop_twisted<M, kTile, true>(redA, nullptr, redC, redD, mo, vo, i, base, pbase, top,
-                                                 mine ? nrr : 0, steps, nrr, sh);
+    const bool ok = coop_twisted<M, true>(redA, nullptr, redC, redD, am, av, i, base, pbase, top,
+                                          mine ? nrr : 0, steps, nrr, sh);
+    if (!ok && mine) report_singular(flag, sys);
+}
+
+// Smallest m whose single-partition solve runs cooperatively (k_plain_coop).
+#ifndef BTD_PLAIN_COOP_FROM
+#define BTD_PLAIN_COOP_FROM 8
+#endif
+constexpr int kPlainCoopFrom = BTD_PLAIN_COOP_FROM;
+
+// Single-partition solve, cooperative: coop_rows with m lanes per system (lane i owns row i of
+// every block), floor(64/m) consecutive systems per wavefront, either layout; the items are
+// cut into one contiguous range per XCD, taken in order, so neighbouring systems' shared
+// lines meet in one L2. For m >= 8, where one lane per system holds too many registers to
+// keep more than one wavefront per SIMD. Same results as k_plain to rounding; C is left
+// holding C'.
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_plain_coop(const double* A, const double* __restrict__ B, double* C,
+                                                   double* D, int nsys, int nrow, int* __restrict__ flag) {
+    __shared__ double sh[kCoopSlots * kTile];
+    constexpr int G = kTile / M;
+    const int nitems = (nsys + G - 1) / G;
+    const int per = (nitems + kXcds - 1) / kXcds;
+    const int u = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (u >= nitems) return;
+    const int lane = threadIdx.x;
+    const int g = lane / M, i = lane % M;
+    const bool grouped = g < G;
+    const int64_t s0 = (int64_t)u * G;
+    const int64_t sys = s0 + (grouped ? g : 0);
+    const bool mine = grouped && sys < nsys;
+    const int64_t se = mine ? sys : s0;  // a real system either way
+    const Addr<LAYOUT> am(se, nsys, nrow, M * M), av(se, nsys, nrow, M);
+    // lanes past the last whole group read group 0's posts and write only their own slots
+    const bool ok = coop_rows<M>(A, B, C, D, am, av, i, grouped ? g * M : 0, mine ? nrow : 0, nrow, nrow, sh);
     if (!ok && mine) report_singular(flag, sys);
 }
 
@@ -1234,12 +1269,13 @@ __global__ __launch_bounds__(64) void k_plain_twisted(const double* A, const dou
     const int64_t sys = (int64_t)tile * kTile + ts;
     const bool mine = grouped && ts < kTile && sys < nsys;
     const int64_t l = mine ? ts : s0;  // a slot of the (padded) tile either way
-    const int64_t mo = (int64_t)tile * nrow * M * M * kTile + l, vo = (int64_t)tile * nrow * M * kTile + l;
+    const StridedAddr<M * M * kTile, kTile> am{(int64_t)tile * nrow * M * M * kTile + l};
+    const StridedAddr<M * kTile, kTile> av{(int64_t)tile * nrow * M * kTile + l};
     const int steps = (nrow - 1) - (nrow - 1) / 2;
     const int base = grouped ? g * M : 0;
     const int pbase = grouped ? (top ? base + M : base - M) : M;
-    const bool ok = coop_twisted<M, kTile, false>(A, B, C, D, mo, vo, i, base, pbase, top, mine ? nrow : 0,
-                                                  steps, nrow, sh);
+    const bool ok = coop_twisted<M, false>(A, B, C, D, am, av, i, base, pbase, top, mine ? nrow : 0, steps,
+                                           nrow, sh);
     if (!ok && mine) report_singular(flag, sys);
 }
 
@@ -1553,11 +1589,21 @@ __global__ __launch_bounds__(256) void k_generate(int kind, uint64_t seed, int l
 template <int M>
 hipError_t launch_plain_m(int layout, const double* A, const double* B, double* C, double* D,
                                  int nsys, int nrow, int* flag, hipStream_t s) {
-    const dim3 grid((nsys + 63) / 64), block(64);
-    if (layout == LAYOUT_TILED)
-        hipLaunchKernelGGL((k_plain<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
-    else
-        hipLaunchKernelGGL((k_plain<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    const dim3 block(64);
+    if constexpr (M >= kPlainCoopFrom) {
+        const int64_t items = ((int64_t)nsys + kTile / M - 1) / (kTile / M);
+        const dim3 grid((unsigned)(kXcds * ((items + kXcds - 1) / kXcds)));
+        if (layout == LAYOUT_TILED)
+            hipLaunchKernelGGL((k_plain_coop<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+        else
+            hipLaunchKernelGGL((k_plain_coop<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    } else {
+        const dim3 grid((nsys + 63) / 64);
+        if (layout == LAYOUT_TILED)
+            hipLaunchKernelGGL((k_plain<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+        else
+            hipLaunchKernelGGL((k_plain<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    }
     return hipGetLastError();
 }
 

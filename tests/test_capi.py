@@ -72,36 +72,21 @@ def test_plan_create_rejects_bad_arguments(args, exc):
 
 
 def test_auto_npart_rule():
-    """npart = 0: the single-kernel solve for large batches, about one wave per SIMD
-    (65536 lanes) for small ones, at least 8 rows per partition."""
+    """npart = 0: the single-kernel solve for large batches (4096 systems suffice for m >= 8),
+    about one wave per SIMD (65536 lanes) for small ones, at least 8 rows per partition."""
     auto = lambda m, nsys, nrow: pb._capi.lib().btd_auto_npart(m, nsys, nrow)
     assert auto(5, 131072, 512) == 1          # the headline batch
     assert auto(5, 32768, 512) == 1
     assert auto(5, 1024, 128) == 16           # the Study's single-GPU case: nrow / 8
     assert auto(5, 512, 65536) == 128         # long systems: 65536 lanes
-    assert auto(8, 4096, 2048) == 16
+    assert auto(8, 4096, 2048) == 1           # m >= 8: m lanes per system fill the device
+    assert auto(8, 2048, 2048) == 32
+    assert auto(7, 4096, 2048) == 16
     assert auto(3, 1024, 64) == 8
     assert auto(3, 1024, 15) == 1             # fewer than 2 partitions of 8 rows
     for m, nsys, nrow in [(2, 7, 40), (10, 3000, 4096), (5, 1, 2)]:
         p = auto(m, nsys, nrow)
         assert p == 1 or (p >= 2 and nrow >= 8 * p)
-
-
-def test_auto_form_rule():
-    """npart = 0 on a tiled plan: the twisted form for m >= 8 where it measured fastest (short
-    systems, or enough of them), one lane per system (with btd_auto_npart) otherwise."""
-    tw, lane = int(pb.Form.TWISTED), int(pb.Form.LANE)
-    tiled, ref = int(pb.Layout.TILED), int(pb.Layout.REF_GPU)
-    form = lambda m, nsys, nrow, lay=tiled: pb._capi.lib().btd_auto_form(m, nsys, nrow, lay)
-    assert form(8, 1024, 128) == tw
-    assert form(10, 16384, 128) == tw
-    assert form(8, 4096, 1024) == tw
-    assert form(10, 1024, 1024) == tw
-    assert form(8, 1024, 1024) == lane       # the two-stage solve with 64 partitions wins
-    assert form(8, 16384, 1024) == lane      # above 4.5M block rows
-    assert form(5, 1024, 128) == lane        # m < 8: partitions win
-    assert form(8, 1024, 128, ref) == lane   # the twisted form needs the tiled layout
-    assert form(1, 64, 8) == lane
 
 
 def test_uncreated_plan_errors():

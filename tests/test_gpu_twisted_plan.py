@@ -60,11 +60,11 @@ def test_twisted_profiled_and_info(cuda):
 
 
 def test_twisted_form_rules(cuda):
-    """The twisted form needs npart == 1, the tiled layout and m >= 2; LANE is always allowed."""
+    """The twisted form needs npart == 1, the tiled layout and m >= 2; PLAIN is always allowed."""
     with pb.BtdmaGpuPlan().create(3, 64, 12, 1, pb.Layout.REF_GPU) as plan:
         with pytest.raises(ValueError):
             plan.set_form(pb.Form.TWISTED)
-        plan.set_form(pb.Form.LANE)
+        plan.set_form(pb.Form.PLAIN)
     with pb.BtdmaGpuPlan().create(3, 64, 12, 2, pb.Layout.TILED) as plan:
         with pytest.raises(ValueError):
             plan.set_form(pb.Form.TWISTED)
@@ -72,22 +72,6 @@ def test_twisted_form_rules(cuda):
         with pytest.raises(ValueError):
             plan.set_form(pb.Form.TWISTED)
     with pb.BtdmaGpuPlan().create(3, 64, 12, 1, pb.Layout.TILED) as plan:
-        assert plan.form == pb.Form.LANE
+        assert plan.form == pb.Form.PLAIN
         with pytest.raises(ValueError):
             plan.set_form(7)
-
-
-def test_auto_form_plan(cuda, oracle):
-    """npart = 0 on a tiled m = 8 plan of short systems picks the twisted form (btd_auto_form)
-    with one partition, and the solve matches the oracle."""
-    import torch
-    from helpers import layout_to_vec
-    m, nsys, n = 8, 300, 64
-    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 71, m, nsys, n)
-    ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, TILED)).to(cuda) for X in (A, B, C, d))
-    with pb.BtdmaGpuPlan().create(m, nsys, n, 0, pb.Layout.TILED) as plan:
-        assert plan.npart == 1 and plan.form == pb.Form.TWISTED
-        pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
-        assert not plan.check()
-    x = layout_to_vec(td.cpu().numpy(), TILED, n, nsys, m)
-    assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
