@@ -588,166 +588,8 @@ __global__ __launch_bounds__(64) void k_f32r(const float* __restrict__ A, const 
 }
 
 // ------------------------------------------------------------------------------ modi --
-// Register-lean stage 1 for m >= 7, where holding Â_{q-1}, Ĉ_{q-1} next to R and X would spill
-// to scratch: the same arithmetic in the same order as the register form below, but the previous
-// row's Ĉ, Â, d̂ are read back (this lane stored them one row earlier), the last row goes to the
-// reduced row 2*part+1 as it is solved, the composition of row 1 reads Ĉ_q back once R is dead,
-// and the close reads alpha, beta, gamma from LDS. At most R, X and a few vectors are live.
-template <int M, int LAYOUT>
-__device__ __forceinline__ void modi_lean(double* A, const double* __restrict__ B, double* C,
-                                          double* D, int nsys, int nrow,
-                                          const int* __restrict__ part_first,
-                                          const int* __restrict__ part_len, int npart,
-                                          double* __restrict__ redA, double* __restrict__ redC,
-                                          double* __restrict__ redD, int* __restrict__ flag,
-                                          int part, int64_t sys) {
-    const int64_t r0 = part_first[part];
-    const int N = part_len[part];
-    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
-    const Addr<LAYOUT_TILED> rm(sys, 0, 2 * npart, M * M), rv(sys, 0, 2 * npart, M);
-    const int64_t es = am.estride(), vs = av.estride();
-    extern __shared__ double sp[];  // alpha (M), beta (M*M), gamma (M*M), [element][lane]
-    const int lane = threadIdx.x;
-    auto SA = [&](int i) -> double& { return sp[i * 64 + lane]; };
-    auto SB = [&](int i, int j) -> double& { return sp[(M + i * M + j) * 64 + lane]; };
-    auto SG = [&](int i, int j) -> double& { return sp[(M + M * M + i * M + j) * 64 + lane]; };
-    bool ok = true;
-    for (int q = 0; q < N; ++q) {
-        const int64_t g = r0 + q;
-        const bool keep = q + 1 < N;  // the last row only feeds the reduced system
-        double R[M][M], v[M], X[M][M];
-        load_mat<M>(B, am, g, R);
-        load_vec<M>(D, av, g, v);
-        if (q >= 2) {
-            const double* pc = C + am.at(g - 1, 0);  // Ĉ_{q-1}
-            const double* pa = A + am.at(g - 1, 0);  // Â_{q-1}
-            const double* pd = D + av.at(g - 1, 0);  // d̂_{q-1}
-#pragma unroll
-            for (int i = 0; i < M; ++i)
-#pragma unroll
-                for (int j = 0; j < M; ++j) X[i][j] = 0.0;
-#pragma unroll 1
-            for (int k = 0; k < M; ++k) {  // rolled: one column of A and one row of Ĉ, Â live
-                double ak[M], ck[M], hk[M];
-                load_col<M>(A, am, g, k, ak);
-#pragma unroll
-                for (int j = 0; j < M; ++j) {
-                    ck[j] = pc[(k + M * j) * es];
-                    hk[j] = pa[(k + M * j) * es];
-                }
-                const double dk = pd[k * vs];
-#pragma unroll
-                for (int i = 0; i < M; ++i) {
-                    v[i] = fma(-ak[i], dk, v[i]);
-#pragma unroll
-                    for (int j = 0; j < M; ++j) {
-                        R[i][j] = fma(-ak[i], ck[j], R[i][j]);
-                        X[i][j] = fma(-ak[i], hk[j], X[i][j]);
-                    }
-                }
-            }
-        } else {
-            load_mat_c<M>(A, am, g, X);
-        }
-        int piv[M];
-        ok = lu_factor<M>(R, piv) && ok;
-        lu_solve<M>(R, piv, v);
-        solve_rows<M>(R, piv, X);
-        if (keep) {
-            store_mat<M>(A, am, g, X);
-            store_vec<M>(D, av, g, v);
-        } else {
-            store_mat<M>(redA, rm, 2 * part + 1, X);
-            store_vec<M>(redD, rv, 2 * part + 1, v);
-        }
-#pragma unroll 1
-        for (int j = 0; j < M; ++j) {  // rolled: one column of C live
-            double c[M];
-            load_col<M>(C, am, g, j, c);
-            lu_solve<M>(R, piv, c);
-            if (keep)
-                store_col<M>(C, am, g, j, c);
-            else
-                store_col<M>(redC, rm, 2 * part + 1, j, c);
-        }
-        if (q == 1) {
-            double Ch[M][M];
-            load_mat_c<M>(C, am, g, Ch);
-#pragma unroll
-            for (int i = 0; i < M; ++i) {
-                SA(i) = v[i];
-#pragma unroll
-                for (int j = 0; j < M; ++j) {
-                    SB(i, j) = X[i][j];
-                    SG(i, j) = Ch[i][j];
-                }
-            }
-        } else if (q >= 2 && keep) {
-            double Ch[M][M];
-            load_mat_c<M>(C, am, g, Ch);
-#pragma unroll 1
-            for (int i = 0; i < M; ++i) {
-                double gi[M];
-#pragma unroll
-                for (int k = 0; k < M; ++k) gi[k] = SG(i, k);
-                double a = SA(i);
-#pragma unroll
-                for (int k = 0; k < M; ++k) a = fma(-gi[k], v[k], a);
-                SA(i) = a;
-#pragma unroll
-                for (int j = 0; j < M; ++j) {
-                    double b = SB(i, j), c = 0.0;
-#pragma unroll
-                    for (int k = 0; k < M; ++k) {
-                        b = fma(-gi[k], X[k][j], b);
-                        c = fma(-gi[k], Ch[k][j], c);
-                    }
-                    SB(i, j) = b;
-                    SG(i, j) = c;
-                }
-            }
-        }
-    }
-    // close, q = 0 (as in the register form): R = I - Ĉ_0 beta,
-    // [d̂|Â|Ĉ]_0 = R^{-1}[d̂_0 - Ĉ_0 alpha | Â_0 | -Ĉ_0 gamma]
-    {
-        double C0[M][M], d0[M], R[M][M], X[M][M];
-        load_mat_c<M>(C, am, r0, C0);
-        load_vec<M>(D, av, r0, d0);
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            double t = C0[i][0] * SA(0);
-#pragma unroll
-            for (int k = 1; k < M; ++k) t = fma(C0[i][k], SA(k), t);
-            d0[i] -= t;
-        }
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int j = 0; j < M; ++j) {
-                double t = C0[i][0] * SB(0, j), x = C0[i][0] * SG(0, j);
-#pragma unroll
-                for (int k = 1; k < M; ++k) {
-                    t = fma(C0[i][k], SB(k, j), t);
-                    x = fma(C0[i][k], SG(k, j), x);
-                }
-                R[i][j] = (i == j ? 1.0 : 0.0) - t;
-                X[i][j] = -x;
-            }
-        int piv[M];
-        ok = lu_factor<M>(R, piv) && ok;
-        lu_solve<M>(R, piv, d0);
-        solve_rows<M>(R, piv, X);
-        store_mat<M>(redC, rm, 2 * part, X);
-        store_vec<M>(redD, rv, 2 * part, d0);
-        load_mat_c<M>(A, am, r0, X);  // Â_0
-        solve_rows<M>(R, piv, X);
-        store_mat<M>(redA, rm, 2 * part, X);
-    }
-    if (!ok) report_singular(flag, sys);
-}
-
-// Stage 1 over partition `part` (rows [r0, r0+N)) of each system, mod_btdma_cpu.f90:305-443.
+// Stage 1 over partition `part` (rows [r0, r0+N)) of each system, one lane per partition
+// (m <= 6; k_modi_coop below for larger m), mod_btdma_cpu.f90:305-443.
 // Forward sweep (in place: Â over A, Ĉ over C, d̂ over D for rows 0..N-2):
 //   [d̂|Â|Ĉ]_q = R^{-1}[d_q - A_q d̂_{q-1} | -A_q Â_{q-1} | C_q],  R = B_q - A_q Ĉ_{q-1}
 // (rows 0 and 1: R = B_q and the right-hand side [d_q | A_q | C_q]), so that
@@ -773,10 +615,7 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
     const int tile = blockIdx.x % ntiles, part = blockIdx.x / ntiles;
     const int64_t sys = (int64_t)tile * 64 + threadIdx.x;
     if (sys >= nsys) return;
-    if constexpr (M >= 7) {
-        modi_lean<M, LAYOUT>(A, B, C, D, nsys, nrow, part_first, part_len, npart, redA, redC, redD,
-                             flag, part, sys);
-    } else {
+    {
         const int64_t r0 = part_first[part];
         const int N = part_len[part];
         const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
