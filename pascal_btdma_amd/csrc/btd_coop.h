@@ -17,11 +17,11 @@ struct StridedAddr {
     __device__ __forceinline__ int64_t at(int64_t q, int e) const { return base + q * RS + (int64_t)e * ES; }
 };
 
-// Orders the LDS accesses of the lanes of one wavefront. The cooperative kernels run one-wave
-// (64-lane) workgroups, and the LDS performs the operations of a wavefront in the order they
-// are issued, so wavefront-scope fences (which keep the compiler from reordering the accesses
-// and emit no wait) suffice: an exchange costs one LDS round trip, not a drain of every
-// outstanding load.
+// Orders the LDS accesses of the lanes of one wavefront. In the cooperative kernels every
+// wavefront exchanges data only among its own lanes, through its own LDS region, and the LDS
+// performs the operations of a wavefront in the order they are issued, so wavefront-scope
+// fences (which keep the compiler from reordering the accesses and emit no wait) suffice: an
+// exchange costs one LDS round trip, not a drain of every outstanding load.
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -41,13 +41,13 @@ __device__ __forceinline__ double fast_rcp(double x) {
 // Rows of the forward (kFwdAhead) and backward (kBwdAhead) sweeps of the cooperative solve
 // kept in flight: one wavefront per work item has no other waves to hide HBM latency
 // behind, so the next rows' blocks are loaded while the current row is eliminated.
-// Rows of the forward (kFwdAhead) and backward (kBwdAhead) sweeps of the cooperative solve
-// kept in flight: one wavefront per work item has no other waves to hide HBM latency
-// behind, so the next rows' blocks are loaded while the current row is eliminated.
+#ifndef BTD_COOP_BWD_WIDE
+#define BTD_COOP_BWD_WIDE 8
+#endif
 template <int M>
 struct CoopDepth {
     static constexpr int kFwdAhead = M <= 2 ? 8 : M <= 3 ? 6 : M <= 5 ? 4 : 1;
-    static constexpr int kBwdAhead = M <= 3 ? 16 : M <= 5 ? 12 : 8;
+    static constexpr int kBwdAhead = M <= 3 ? 16 : M <= 5 ? 12 : BTD_COOP_BWD_WIDE;
 };
 
 // LDS of a cooperative work item, [slot][lane] (64 doubles a slot): slots [0, M) the carried
@@ -71,7 +71,7 @@ __device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[NS *
                                               double (&xc)[NS * M], double (&xd)[M]) {
     constexpr int NR = NS * M + 1;     // right-hand-side entries of a row
     constexpr int CAND = M + NR;       // the candidate's slot
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kTile - 1);  // the lane in its wavefront
     int rank = -1;
     int plane[M];
     int zero_pivot = 0;
@@ -158,7 +158,9 @@ __device__ __forceinline__ double lane_elem(const double (&v)[M], int i) {
 // slot of the pack, so loads are clamped to that range and need no other guard. All lanes of
 // the wave run the same row steps (nmax, the item's longest system); lanes past their
 // system's n compute on what the rows hold and store nothing.
-template <int M, class AM, class AV>
+// With LOCK each row step's loads are issued after a workgroup barrier (s_barrier, no memory
+// wait), so the wavefronts of a workgroup, which must all run nmax steps, load each row together.
+template <int M, bool LOCK = false, class AM, class AV>
 __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict__ B, double* C,
                           double* D, const AM& am, const AV& av, int i, int base, int n, int nmax, int rows,
                           double* __restrict__ sh) {
@@ -166,7 +168,7 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
     constexpr int G = CoopDepth<M>::kBwdAhead;
     double* carry = sh;
     double* xb = sh + (M + 1) * kTile;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kTile - 1);  // the lane in its wavefront
     bool ok = true;
     double pa[F][M], pb[F][M], pc[F][M], pd[F];
     auto fetch = [&](double (&fa)[M], double (&fb)[M], double (&fc)[M], double& fd, int q) {
@@ -208,6 +210,7 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
                         for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
                     }
                 }
+                if constexpr (LOCK) __builtin_amdgcn_s_barrier();
                 fetch(pa[s], pb[s], pc[s], pd[s], q + F);
                 double xc[M], xd[M];
                 const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
@@ -251,6 +254,7 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
                 double cq[M], x = dn[s];
 #pragma unroll
                 for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
+                if constexpr (LOCK) __builtin_amdgcn_s_barrier();
                 bfetch(cn[s], dn[s], q - G);
                 sh[lane] = xnext;
                 wave_lds_sync();
@@ -289,7 +293,7 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
     constexpr int G = CoopDepth<M>::kBwdAhead;
     double* carry = sh;
     double* xb = sh + (M + 1) * kTile;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kTile - 1);  // the lane in its wavefront
     const int h = n > 0 ? (n - 1) / 2 : 0;
     const int cnt = n > 0 ? (top ? h : n - 1 - h) : 0;
     const double* sub = top ? A : C;  // the block coupling row q to the row eliminated before it

@@ -1057,30 +1057,34 @@ __global__ __launch_bounds__(64) void k_reduced_twisted(const double* __restrict
 constexpr int kPlainCoopFrom = BTD_PLAIN_COOP_FROM;
 
 // Single-partition solve, cooperative: coop_rows with m lanes per system (lane i owns row i of
-// every block), floor(64/m) consecutive systems per wavefront, either layout; the items are
-// cut into one contiguous range per XCD, taken in order, so neighbouring systems' shared
-// lines meet in one L2. For m >= 8, where one lane per system holds too many registers to
-// keep more than one wavefront per SIMD. Same results as k_plain to rounding; C is left
-// holding C'.
-template <int M, int LAYOUT>
-__global__ __launch_bounds__(64) void k_plain_coop(const double* A, const double* __restrict__ B, double* C,
-                                                   double* D, int nsys, int nrow, int* __restrict__ flag) {
-    __shared__ double sh[kCoopSlots * kTile];
+// every block), floor(64/m) consecutive systems per wavefront, WV wavefronts per workgroup on
+// consecutive systems, either layout; the workgroups are cut into one contiguous
+// range per XCD, taken in order. For m >= 8, where one lane per system holds too many
+// registers to keep more than one wavefront per SIMD. Same results as k_plain to rounding; C
+// is left holding C'. With LOCK the wavefronts of a workgroup meet at a barrier before each
+// row's loads, so the parts of a cache line that they read are requested together and the
+// line is fetched once; unlocked, they drift apart by more than the line stays in L2 (one
+// wavefront reads 8 systems, half a line, at m = 8: twice the compulsory bytes).
+template <int M, int LAYOUT, int WV, bool LOCK>
+__global__ __launch_bounds__(64 * WV) void k_plain_coop(const double* A, const double* __restrict__ B,
+                                                        double* C, double* D, int nsys, int nrow,
+                                                        int* __restrict__ flag) {
+    __shared__ double sh[WV][kCoopSlots * kTile];
     constexpr int G = kTile / M;
-    const int nitems = (nsys + G - 1) / G;
+    const int nitems = (nsys + G * WV - 1) / (G * WV);
     const int per = (nitems + kXcds - 1) / kXcds;
     const int u = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     if (u >= nitems) return;
-    const int lane = threadIdx.x;
+    const int w = threadIdx.x / kTile, lane = threadIdx.x % kTile;
     const int g = lane / M, i = lane % M;
     const bool grouped = g < G;
-    const int64_t s0 = (int64_t)u * G;
+    const int64_t s0 = ((int64_t)u * WV + w) * G;
     const int64_t sys = s0 + (grouped ? g : 0);
     const bool mine = grouped && sys < nsys;
-    const int64_t se = mine ? sys : s0;  // a real system either way
+    const int64_t se = mine ? sys : (s0 < nsys ? s0 : (int64_t)u * WV * G);  // a real system either way
     const Addr<LAYOUT> am(se, nsys, nrow, M * M), av(se, nsys, nrow, M);
     // lanes past the last whole group read group 0's posts and write only their own slots
-    const bool ok = coop_rows<M>(A, B, C, D, am, av, i, grouped ? g * M : 0, mine ? nrow : 0, nrow, nrow, sh);
+    const bool ok = coop_rows<M, LOCK>(A, B, C, D, am, av, i, grouped ? g * M : 0, mine ? nrow : 0, nrow, nrow, sh[w]);
     if (!ok && mine) report_singular(flag, sys);
 }
 
@@ -1425,17 +1429,54 @@ __global__ __launch_bounds__(256) void k_generate(int kind, uint64_t seed, int l
 }
 
 // ---------------------------------------------------------------------- launch templates --
+template <int M, int LAYOUT, int WV, bool LOCK>
+void launch_plain_coop_as(const double* A, const double* B, double* C, double* D, int nsys, int nrow,
+                          int* flag, hipStream_t s) {
+    constexpr int per_item = (kTile / M) * WV;
+    const int64_t items = ((int64_t)nsys + per_item - 1) / per_item;
+    const dim3 grid((unsigned)(kXcds * ((items + kXcds - 1) / kXcds))), block(64 * WV);
+    hipLaunchKernelGGL((k_plain_coop<M, LAYOUT, WV, LOCK>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+}
+
+// Workgroup shape of the cooperative single-partition solve, as measured (m = 8-10, 1024-32768
+// systems of 128-2048 rows): at m >= 9 the kernel holds all 512 registers of a lane (one
+// wavefront per SIMD) and four unlocked wavefronts per workgroup ran 5-29% faster than one. At
+// m = 8 (two wavefronts per SIMD) locked workgroups of 2 (tiled layout) or 4 (reference layout)
+// wavefronts ran 13-33% faster, reading the compulsory bytes instead of twice them, once the
+// wavefronts of work reach kCoopLockFrom per CU (bandwidth-bound: 8 tiled, 4 reference) and
+// slower on fewer systems (1.9x at 1024 systems of 128 rows), where every wavefront waiting at
+// the barrier is latency; those run one wavefront per workgroup.
+#ifndef BTD_COOP_LOCK_FROM
+#define BTD_COOP_LOCK_FROM 4
+#endif
+template <int LAYOUT>
+constexpr int kCoopLockFrom = (LAYOUT == LAYOUT_TILED ? 2 : 1) * BTD_COOP_LOCK_FROM;
+
+template <int M, int LAYOUT>
+void launch_plain_coop(const double* A, const double* B, double* C, double* D, int nsys, int nrow,
+                       int* flag, hipStream_t s) {
+    if constexpr (M >= 9) {
+        launch_plain_coop_as<M, LAYOUT, 4, false>(A, B, C, D, nsys, nrow, flag, s);
+    } else {
+        const int64_t waves = ((int64_t)nsys + kTile / M - 1) / (kTile / M);
+        if (waves < (int64_t)kCoopLockFrom<LAYOUT> * device_cus())
+            launch_plain_coop_as<M, LAYOUT, 1, false>(A, B, C, D, nsys, nrow, flag, s);
+        else if constexpr (LAYOUT == LAYOUT_TILED)
+            launch_plain_coop_as<M, LAYOUT, 2, true>(A, B, C, D, nsys, nrow, flag, s);
+        else
+            launch_plain_coop_as<M, LAYOUT, 4, true>(A, B, C, D, nsys, nrow, flag, s);
+    }
+}
+
 template <int M>
 hipError_t launch_plain_m(int layout, const double* A, const double* B, double* C, double* D,
                                  int nsys, int nrow, int* flag, hipStream_t s) {
     const dim3 block(64);
     if constexpr (M >= kPlainCoopFrom) {
-        const int64_t items = ((int64_t)nsys + kTile / M - 1) / (kTile / M);
-        const dim3 grid((unsigned)(kXcds * ((items + kXcds - 1) / kXcds)));
         if (layout == LAYOUT_TILED)
-            hipLaunchKernelGGL((k_plain_coop<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+            launch_plain_coop<M, LAYOUT_TILED>(A, B, C, D, nsys, nrow, flag, s);
         else
-            hipLaunchKernelGGL((k_plain_coop<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+            launch_plain_coop<M, LAYOUT_REF_GPU>(A, B, C, D, nsys, nrow, flag, s);
     } else {
         const dim3 grid((nsys + 63) / 64);
         if (layout == LAYOUT_TILED)

@@ -194,6 +194,27 @@ def test_per_system_info(cuda, npart, cyclic):
         assert not plan.check()
 
 
+@pytest.mark.parametrize("m", [8, 9, 10])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_cooperative_plain_edges(cuda, oracle, m, layout):
+    """m >= 8 single-partition solves run m lanes per system: n = 2 and 3, a partial last
+    wavefront of systems, and zero block rows named per system."""
+    import torch
+    for nsys, n in ((13, 2), (70, 3), (129, 9)):
+        A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 40 + n, m, nsys, n)
+        x, sing = gpu_solve(A, B, C, d, int(layout))
+        assert not sing
+        assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
+    nsys, n = 70, 11
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 9, m, nsys, n)
+    for s_, q in ((0, 0), (33, 5), (69, 10)):
+        A[q, s_] = B[q, s_] = C[q, s_] = 0.0
+    ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, int(layout))).to(cuda) for X in (A, B, C, d))
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
+        pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
+        assert plan.info().nonzero()[0].tolist() == [0, 33, 69]
+
+
 @pytest.mark.parametrize("m,npart", [(8, 3), (10, 2), (7, 4)])
 def test_per_system_info_cooperative_stage1(cuda, m, npart):
     """m >= 7 runs stage 1 cooperatively (m lanes per partition): zero block rows in a middle
