@@ -767,31 +767,37 @@ constexpr int kXcds = 8;
 // with the right-hand sides [d | X | C], X = A_q for q < 2 and -A_q Â_{q-1} after; Ĉ and Â
 // are carried to the next row in LDS, d̂ whole in every lane; alpha, beta, gamma are kept
 // row-wise (lane i: alpha_i, beta(i,:), gamma(i,:)). floor(64/m) consecutive systems of one
-// partition per wavefront; the items of one partition go to the workgroups of one XCD in
-// order, so neighbouring systems' shared lines meet in one L2. Used for m >= 7, where the
-// one-lane form spills its blocks to scratch memory.
-template <int M, int LAYOUT>
-__global__ __launch_bounds__(64) void k_modi_coop(double* A, const double* __restrict__ B, double* C,
-                                                  double* D, int nsys, int nrow,
-                                                  const int* __restrict__ part_first,
-                                                  const int* __restrict__ part_len, int npart,
-                                                  double* __restrict__ redA, double* __restrict__ redC,
-                                                  double* __restrict__ redD, int* __restrict__ flag) {
+// partition per wavefront, WV wavefronts per workgroup on consecutive systems; the items of
+// one partition go to the workgroups of one XCD in order, so neighbouring systems' shared
+// lines meet in one L2. Used for m >= 7, where the one-lane form spills its blocks to scratch
+// memory.
+template <int M, int LAYOUT, int WV>
+__global__ __launch_bounds__(64 * WV) void k_modi_coop(double* A, const double* __restrict__ B, double* C,
+                                                       double* D, int nsys, int nrow,
+                                                       const int* __restrict__ part_first,
+                                                       const int* __restrict__ part_len, int npart,
+                                                       double* __restrict__ redA, double* __restrict__ redC,
+                                                       double* __restrict__ redD, int* __restrict__ flag) {
     constexpr int G = kTile / M;  // systems per wavefront
     constexpr int E = M * M;
-    __shared__ double sh[(5 * M + 3) * kTile];
+    __shared__ double shw[WV][(5 * M + 3) * kTile];
     const int nchunks = (nsys + G - 1) / G;
-    const int nitems = npart * nchunks;
+    const int ngroups = (nchunks + WV - 1) / WV;
+    const int nitems = npart * ngroups;
     const int per = (nitems + kXcds - 1) / kXcds;
     const int u = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     if (u >= nitems) return;
-    const int part = u / nchunks, chunk = u % nchunks;
-    const int lane = threadIdx.x;
+    const int w = threadIdx.x / kTile, lane = threadIdx.x % kTile;
+    const int part = u / ngroups, chunk = (u % ngroups) * WV + w;
+    // a wavefront past the last chunk runs the partition's steps on the last chunk's systems
+    // and stores nothing
+    const bool live = chunk < nchunks;
+    double* sh = shw[w];
     const int g = lane / M, i = lane % M;
     const bool grouped = g < G;
-    const int64_t s0 = (int64_t)chunk * G;
+    const int64_t s0 = (int64_t)(live ? chunk : nchunks - 1) * G;
     const int64_t sys = s0 + (grouped ? g : 0);
-    const bool mine = grouped && sys < nsys;
+    const bool mine = live && grouped && sys < nsys;
     const int64_t se = mine ? sys : s0;  // a real system either way
     // lanes past the last whole group read group 0's posts and write only their own slots
     const int base = grouped ? g * M : 0;
@@ -1006,6 +1012,13 @@ __global__ __launch_bounds__(64) void k_reduced(const double* __restrict__ redA,
     }
     if (!ok) report_singular(flag, sys);
 }
+
+// Wavefronts per workgroup of k_modi_coop (each on its own systems): 2 at m >= 9, where the
+// kernel holds one wavefront per SIMD, measured 2-11% faster on m = 10 (256-4096 systems,
+// 1024-2048 rows, P = 16-128); at m = 7, 8 one and two were within 3%, and workgroups whose
+// wavefronts meet at a barrier before each row's loads were 3-20% slower in every case.
+template <int M>
+constexpr int modi_coop_waves() { return M >= 9 ? 2 : 1; }
 
 // Smallest m whose stage 1 runs cooperatively (k_modi_coop) instead of one lane per partition.
 #ifndef BTD_MODI_COOP_FROM
@@ -1590,14 +1603,16 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     }
     if constexpr (M >= kModiCoopFrom) {
         // m lanes per partition (no spills); the items of a partition on one XCD
-        const int64_t items = (int64_t)npart * ((nsys + kTile / M - 1) / (kTile / M));
-        const dim3 gridc((unsigned)(kXcds * ((items + kXcds - 1) / kXcds)));
+        constexpr int WV = modi_coop_waves<M>();
+        const int64_t chunks = (nsys + kTile / M - 1) / (kTile / M);
+        const int64_t items = (int64_t)npart * ((chunks + WV - 1) / WV);
+        const dim3 gridc((unsigned)(kXcds * ((items + kXcds - 1) / kXcds))), blockc(64 * WV);
         if (layout == LAYOUT_TILED)
-            hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_TILED>), gridc, block, 0, s, A, B, C, D, nsys, nrow, pf,
-                               pl, npart, redA, redC, redD, flag);
+            hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_TILED, WV>), gridc, blockc, 0, s, A, B, C, D, nsys,
+                               nrow, pf, pl, npart, redA, redC, redD, flag);
         else
-            hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_REF_GPU>), gridc, block, 0, s, A, B, C, D, nsys, nrow, pf,
-                               pl, npart, redA, redC, redD, flag);
+            hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_REF_GPU, WV>), gridc, blockc, 0, s, A, B, C, D, nsys,
+                               nrow, pf, pl, npart, redA, redC, redD, flag);
     } else if (layout == LAYOUT_TILED) {
         hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
                            npart, redA, redC, redD, flag);

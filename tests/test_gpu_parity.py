@@ -215,6 +215,21 @@ def test_cooperative_plain_edges(cuda, oracle, m, layout):
         assert plan.info().nonzero()[0].tolist() == [0, 33, 69]
 
 
+@pytest.mark.parametrize("m,nsys", [(9, 61), (10, 61), (10, 5)])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_two_stage_cooperative_odd_chunks(cuda, oracle, m, nsys, layout):
+    """At m >= 9 the cooperative stage 1 runs two wavefronts of floor(64/m) systems per
+    workgroup; an odd number of such chunks (9 at m = 9, 11 and 1 at m = 10) leaves the last
+    workgroup's second wavefront without systems, which must store nothing."""
+    n, npart = 40, 5
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 77 + m + nsys, m, nsys, n)
+    x, sing = gpu_solve(A, B, C, d, int(layout), npart=npart)
+    assert not sing
+    xo, info = oracle.solve_two_stage(cpu_layout(A), cpu_layout(B), cpu_layout(C), d, npart)
+    assert info == 0
+    assert rel_err(x, xo).max() <= TOL
+
+
 @pytest.mark.parametrize("m,npart", [(8, 3), (10, 2), (7, 4)])
 def test_per_system_info_cooperative_stage1(cuda, m, npart):
     """m >= 7 runs stage 1 cooperatively (m lanes per partition): zero block rows in a middle
