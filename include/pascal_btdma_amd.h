@@ -58,8 +58,9 @@ typedef enum btd_layout {
 
 /* How a plan's single-partition solve runs (btd_plan_set_form). */
 typedef enum btd_form {
-    BTD_FORM_PLAIN = 0,   /* one lane per system (m <= 7; blocks in VGPRs) or m lanes per system
-                           * (m >= 8; one block row each, pivoting through LDS) */
+    BTD_FORM_PLAIN = 0,   /* the library's choice: one lane per system (m <= 7; blocks in VGPRs),
+                           * m lanes per system (m >= 8; one block row each, pivoting through
+                           * LDS) or, for small m >= 8 batches, twisted */
     BTD_FORM_TWISTED = 1  /* two groups of m lanes per system, one eliminating from each end */
 } btd_form;
 
@@ -96,14 +97,16 @@ int btd_auto_npart(int m, int nsys, int nrow);
 btd_status btd_plan_npart(btd_plan_t plan, int *npart);
 
 /* New (no counterpart in the reference): choose how an npart = 1 plan's non-cyclic solve runs.
- * BTD_FORM_PLAIN (the default) is the library's single-partition kernel: one lane per system
- * for m <= 7, m lanes per system (one block row each) for m >= 8. BTD_FORM_TWISTED (tiled
- * layout, m >= 2) gives each system two groups of m lanes, one eliminating downwards from row
- * 0 and one upwards from row nrow-1, meeting in the middle: half the row latency per system,
- * for plans with few, long systems (it reads the tiled lines several times over, so it does
- * not pay for large batches; it is never chosen automatically). Same zero-pivot reporting;
- * results agree with BTD_FORM_PLAIN to rounding; C is left overwritten. Cyclic and the other
- * solve variants are unaffected. */
+ * BTD_FORM_PLAIN (the default) is the library's choice: one lane per system for m <= 7; for
+ * m >= 8 m lanes per system (one block row each), or the twisted form below when the batch
+ * is too small to fill the device that way: at m = 8 below 32 systems per CU (reference
+ * layout) or 64 (tiled), i.e. 8192 / 16384 systems on 256 CUs; at m = 9, 10 below 14 / 12
+ * systems per CU. BTD_FORM_TWISTED (m >= 2, either layout) always gives each system two
+ * groups of m lanes, one eliminating downwards from row 0 and one upwards from row nrow-1,
+ * meeting in the middle: half the row latency per system (on large batches its wavefronts
+ * share cache lines that leave L2 between their reads, so it moves more bytes). Same
+ * zero-pivot reporting; results agree with the one-lane solve to rounding; C is left
+ * overwritten. Cyclic and the other solve variants are unaffected. */
 btd_status btd_plan_set_form(btd_plan_t plan, int form);
 btd_status btd_plan_form(btd_plan_t plan, int *form);
 

@@ -175,7 +175,7 @@ class BtdmaGpuPlan:
         return Form(f.value)
 
     def set_form(self, form: "Form") -> "BtdmaGpuPlan":
-        """btd_plan_set_form: Form.TWISTED needs npart == 1, the tiled layout and m >= 2."""
+        """btd_plan_set_form: Form.TWISTED needs npart == 1 and m >= 2 (either layout)."""
         _check(_capi.lib().btd_plan_set_form(self._h, int(form)))
         return self
 

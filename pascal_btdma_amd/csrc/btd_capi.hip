@@ -195,9 +195,8 @@ btd_status btd_plan_set_form(btd_plan_t plan, int form) {
         return BTD_OK;
     }
     if (form != BTD_FORM_TWISTED) return fail(BTD_ERR_INVALID_ARG, "btd_plan_set_form: unknown form");
-    if (plan->npart != 1 || plan->layout != BTD_LAYOUT_TILED || plan->m < 2)
-        return fail(BTD_ERR_INVALID_ARG,
-                    "btd_plan_set_form: the twisted form needs npart == 1, the tiled layout and m >= 2");
+    if (plan->npart != 1 || plan->m < 2)
+        return fail(BTD_ERR_INVALID_ARG, "btd_plan_set_form: the twisted form needs npart == 1 and m >= 2");
     plan->form = form;
     return BTD_OK;
 }
@@ -235,7 +234,8 @@ static btd_status solve_impl(btd_plan_t plan, double* a, const double* b, double
             BTD_HIP(btd::launch_cyclic(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                        plan->d_flag, s));
         else if (plan->form == BTD_FORM_TWISTED)
-            BTD_HIP(btd::launch_plain_twisted(plan->m, a, b, c, d, plan->nsys, plan->nrow, plan->d_flag, s));
+            BTD_HIP(btd::launch_plain_twisted(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
+                                              plan->d_flag, s));
         else
             BTD_HIP(btd::launch_plain(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                       plan->d_flag, s));

@@ -41,13 +41,10 @@ __device__ __forceinline__ double fast_rcp(double x) {
 // Rows of the forward (kFwdAhead) and backward (kBwdAhead) sweeps of the cooperative solve
 // kept in flight: one wavefront per work item has no other waves to hide HBM latency
 // behind, so the next rows' blocks are loaded while the current row is eliminated.
-#ifndef BTD_COOP_BWD_WIDE
-#define BTD_COOP_BWD_WIDE 8
-#endif
 template <int M>
 struct CoopDepth {
     static constexpr int kFwdAhead = M <= 2 ? 8 : M <= 3 ? 6 : M <= 5 ? 4 : 1;
-    static constexpr int kBwdAhead = M <= 3 ? 16 : M <= 5 ? 12 : BTD_COOP_BWD_WIDE;
+    static constexpr int kBwdAhead = M <= 3 ? 16 : M <= 5 ? 12 : 8;
 };
 
 // LDS of a cooperative work item, [slot][lane] (64 doubles a slot): slots [0, M) the carried
@@ -285,7 +282,8 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
 // count. A group's carried block and rhs stay as its last row left them while the other
 // groups finish. ES is the element stride of the storage; with UNIT_B every B_q is the
 // identity and B is not read (the reduced system of the two-stage solve).
-template <int M, bool UNIT_B = false, class AM, class AV>
+// LOCK: as in coop_rows (every wavefront of the workgroup runs the same `steps`).
+template <int M, bool UNIT_B = false, bool LOCK = false, class AM, class AV>
 __device__ bool coop_twisted(const double* A, const double* __restrict__ B, double* C, double* D,
                              const AM& am, const AV& av, int i, int base, int pbase, bool top, int n,
                              int steps, int rows, double* __restrict__ sh) {
@@ -345,6 +343,7 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
                         for (int j = 0; j < M; ++j) r[j] = fma(-a, carry[k * kTile + base + j], r[j]);
                     }
                 }
+                if constexpr (LOCK) __builtin_amdgcn_s_barrier();
                 fetch(ps[s], pg[s], pu[s], pd[s], t + F);
                 double xc[M], xd[M];
                 const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xd);
@@ -420,6 +419,7 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
                 double cq[M], x = dn[s];
 #pragma unroll
                 for (int j = 0; j < M; ++j) cq[j] = cn[s][j];
+                if constexpr (LOCK) __builtin_amdgcn_s_barrier();
                 bfetch(cn[s], dn[s], t + G);
                 sh[lane] = xprev;
                 wave_lds_sync();

@@ -5,7 +5,7 @@
 namespace btd {
 
 #define BTD_DECL_PLAIN_TWISTED(M) \
-    hipError_t launch_plain_twisted_##M(const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+    hipError_t launch_plain_twisted_##M(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
 BTD_DECL_PLAIN_TWISTED(1)
 BTD_DECL_PLAIN_TWISTED(2)
 BTD_DECL_PLAIN_TWISTED(3)
@@ -264,9 +264,9 @@ hipError_t launch_plain(int m, int layout, const double* A, const double* B, dou
 #undef CALL
 }
 
-hipError_t launch_plain_twisted(int m, const double* A, const double* B, double* C, double* D, int nsys,
-                                int nrow, int* flag, hipStream_t s) {
-#define CALL(M) launch_plain_twisted_##M(A, B, C, D, nsys, nrow, flag, s)
+hipError_t launch_plain_twisted(int m, int layout, const double* A, const double* B, double* C, double* D,
+                                int nsys, int nrow, int* flag, hipStream_t s) {
+#define CALL(M) launch_plain_twisted_##M(layout, A, B, C, D, nsys, nrow, flag, s)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }

@@ -17,9 +17,9 @@ hipError_t BTD_NAME(launch_plain_)(int layout, const double* A, const double* B,
     return launch_plain_m<BTD_M>(layout, A, B, C, D, nsys, nrow, flag, s);
 }
 
-hipError_t BTD_NAME(launch_plain_twisted_)(const double* A, const double* B, double* C, double* D,
-                                           int nsys, int nrow, int* flag, hipStream_t s) {
-    return launch_plain_twisted_m<BTD_M>(A, B, C, D, nsys, nrow, flag, s);
+hipError_t BTD_NAME(launch_plain_twisted_)(int layout, const double* A, const double* B, double* C,
+                                           double* D, int nsys, int nrow, int* flag, hipStream_t s) {
+    return launch_plain_twisted_m<BTD_M>(layout, A, B, C, D, nsys, nrow, flag, s);
 }
 
 hipError_t BTD_NAME(launch_two_stage_)(int layout, double* A, const double* B, double* C,

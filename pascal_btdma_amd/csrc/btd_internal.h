@@ -14,9 +14,9 @@ namespace btd {
 hipError_t launch_plain(int m, int layout, const double* A, const double* B, double* C, double* D,
                         int nsys, int nrow, int* flag, hipStream_t s);
 
-// The twisted cooperative single-partition solve of a tiled plan (m >= 2).
-hipError_t launch_plain_twisted(int m, const double* A, const double* B, double* C, double* D, int nsys,
-                                int nrow, int* flag, hipStream_t s);
+// The twisted cooperative single-partition solve (m >= 2), either layout.
+hipError_t launch_plain_twisted(int m, int layout, const double* A, const double* B, double* C, double* D,
+                                int nsys, int nrow, int* flag, hipStream_t s);
 
 hipError_t launch_plain_save(int m, int layout, const double* A, const double* B, double* C,
                              double* D, double* F, uint64_t* PV, int nsys, int nrow, int* flag,

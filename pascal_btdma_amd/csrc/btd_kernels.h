@@ -789,15 +789,13 @@ __global__ __launch_bounds__(64 * WV) void k_modi_coop(double* A, const double* 
     if (u >= nitems) return;
     const int w = threadIdx.x / kTile, lane = threadIdx.x % kTile;
     const int part = u / ngroups, chunk = (u % ngroups) * WV + w;
-    // a wavefront past the last chunk runs the partition's steps on the last chunk's systems
-    // and stores nothing
-    const bool live = chunk < nchunks;
+    if (chunk >= nchunks) return;  // the wavefronts share no barrier
     double* sh = shw[w];
     const int g = lane / M, i = lane % M;
     const bool grouped = g < G;
-    const int64_t s0 = (int64_t)(live ? chunk : nchunks - 1) * G;
+    const int64_t s0 = (int64_t)chunk * G;
     const int64_t sys = s0 + (grouped ? g : 0);
-    const bool mine = live && grouped && sys < nsys;
+    const bool mine = grouped && sys < nsys;
     const int64_t se = mine ? sys : s0;  // a real system either way
     // lanes past the last whole group read group 0's posts and write only their own slots
     const int base = grouped ? g * M : 0;
@@ -1072,8 +1070,8 @@ constexpr int kPlainCoopFrom = BTD_PLAIN_COOP_FROM;
 // Single-partition solve, cooperative: coop_rows with m lanes per system (lane i owns row i of
 // every block), floor(64/m) consecutive systems per wavefront, WV wavefronts per workgroup on
 // consecutive systems, either layout; the workgroups are cut into one contiguous
-// range per XCD, taken in order. For m >= 8, where one lane per system holds too many
-// registers to keep more than one wavefront per SIMD. Same results as k_plain to rounding; C
+// range per XCD, taken in order. For m >= 8 batches large enough to fill the device this way
+// (one lane per system would hold too many registers). Same results as k_plain to rounding; C
 // is left holding C'. With LOCK the wavefronts of a workgroup meet at a barrier before each
 // row's loads, so the parts of a cache line that they read are requested together and the
 // line is fetched once; unlocked, they drift apart by more than the line stays in L2 (one
@@ -1101,51 +1099,81 @@ __global__ __launch_bounds__(64 * WV) void k_plain_coop(const double* A, const d
     if (!ok && mine) report_singular(flag, sys);
 }
 
-// Cooperative plain solve (npart = 1) of a tiled plan, twisted: per system two groups of M
-// lanes (coop_twisted), batch_width(M) consecutive systems of one tile per wavefront, so a
-// plan of few systems fills the device with m lanes per system instead of one. The items of
-// a tile (they share the tile's cache lines) go to workgroups of one XCD, adjacent in its
-// dispatch order, so those lines are fetched into one L2 once. C is left overwritten.
-template <int M>
-__global__ __launch_bounds__(64) void k_plain_twisted(const double* A, const double* __restrict__ B,
-                                                      double* C, double* D, int nsys, int nrow,
-                                                      int ntiles, int* __restrict__ flag) {
-    __shared__ double sh[kCoopSlots * kTile];
+// Cooperative plain solve (npart = 1), twisted: per system two groups of M lanes
+// (coop_twisted), batch_width(M) consecutive systems per wavefront and WV wavefronts per
+// workgroup on consecutive systems, either layout, so a plan of few systems fills the device
+// with 2m lanes per system instead of m. The workgroups are cut into one contiguous range per
+// XCD, so neighbouring systems' shared lines meet in one L2; with LOCK the wavefronts of a
+// workgroup meet at a barrier before each row's loads (coop_rows). C is left overwritten.
+template <int M, int LAYOUT, int WV, bool LOCK>
+__global__ __launch_bounds__(64 * WV) void k_plain_twisted(const double* A, const double* __restrict__ B,
+                                                           double* C, double* D, int nsys, int nrow,
+                                                           int* __restrict__ flag) {
+    __shared__ double sh[WV][kCoopSlots * kTile];
     constexpr int W = batch_width(M);
-    constexpr int kItems = (kTile + W - 1) / W;  // items per tile
-    const int x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
-    const int tile = x + kXcds * (j / kItems), item = j % kItems;
-    if (tile >= ntiles) return;
-    const int lane = threadIdx.x;
+    const int nitems = (nsys + W * WV - 1) / (W * WV);
+    const int per = (nitems + kXcds - 1) / kXcds;
+    const int u = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (u >= nitems) return;
+    const int w = threadIdx.x / kTile, lane = threadIdx.x % kTile;
     const int g = lane / M, i = lane % M, slot = g / 2;
     const bool grouped = slot < W;
     const bool top = grouped ? (g % 2 == 0) : true;
-    const int s0 = item * W;
-    const int ts = s0 + (grouped ? slot : 0);  // the lane's slot in the tile
-    const int64_t sys = (int64_t)tile * kTile + ts;
-    const bool mine = grouped && ts < kTile && sys < nsys;
-    const int64_t l = mine ? ts : s0;  // a slot of the (padded) tile either way
-    const StridedAddr<M * M * kTile, kTile> am{(int64_t)tile * nrow * M * M * kTile + l};
-    const StridedAddr<M * kTile, kTile> av{(int64_t)tile * nrow * M * kTile + l};
+    const int64_t s0 = ((int64_t)u * WV + w) * W;
+    const int64_t sys = s0 + (grouped ? slot : 0);
+    const bool mine = grouped && sys < nsys;
+    const int64_t se = mine ? sys : (s0 < nsys ? s0 : (int64_t)u * WV * W);  // a real system either way
+    const Addr<LAYOUT> am(se, nsys, nrow, M * M), av(se, nsys, nrow, M);
     const int steps = (nrow - 1) - (nrow - 1) / 2;
+    // lanes past the last pair act as a top group over groups 0 and 1, writing only their
+    // own lane's slots
     const int base = grouped ? g * M : 0;
     const int pbase = grouped ? (top ? base + M : base - M) : M;
-    const bool ok = coop_twisted<M, false>(A, B, C, D, am, av, i, base, pbase, top, mine ? nrow : 0, steps,
-                                           nrow, sh);
+    const bool ok = coop_twisted<M, false, LOCK>(A, B, C, D, am, av, i, base, pbase, top, mine ? nrow : 0,
+                                                 steps, nrow, sh[w]);
     if (!ok && mine) report_singular(flag, sys);
 }
 
+template <int M, int LAYOUT, int WV, bool LOCK>
+void launch_plain_twisted_as(const double* A, const double* B, double* C, double* D, int nsys, int nrow,
+                             int* flag, hipStream_t s) {
+    constexpr int per_item = batch_width(M) * WV;
+    const int64_t items = ((int64_t)nsys + per_item - 1) / per_item;
+    const dim3 grid((unsigned)(kXcds * ((items + kXcds - 1) / kXcds))), block(64 * WV);
+    hipLaunchKernelGGL((k_plain_twisted<M, LAYOUT, WV, LOCK>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+}
+
+// Workgroup shape of the twisted solve, as measured (m = 8, 10; 1024-16384 systems of 512-4096
+// rows, both layouts): m = 8 with 4 or more wavefronts of work per CU (4096 systems and up):
+// four wavefronts locked as in k_plain_coop (4096 x 4096 rows 39.8 -> 25.2 ms, reference
+// layout); fewer: one wavefront (1024 x 2048: 5.7 ms, locked 4 10.6 ms). m >= 9: two locked
+// wavefronts (1024 x 2048: 16.3 -> 13.5 ms). m <= 7: one wavefront.
+template <int M, int LAYOUT>
+void launch_plain_twisted_auto(const double* A, const double* B, double* C, double* D, int nsys, int nrow,
+                               int* flag, hipStream_t s) {
+    if constexpr (M >= 9) {
+        launch_plain_twisted_as<M, LAYOUT, 2, true>(A, B, C, D, nsys, nrow, flag, s);
+    } else if constexpr (M == 8) {
+        const int64_t waves = ((int64_t)nsys + batch_width(M) - 1) / batch_width(M);
+        if (waves >= 4 * (int64_t)device_cus())
+            launch_plain_twisted_as<M, LAYOUT, 4, true>(A, B, C, D, nsys, nrow, flag, s);
+        else
+            launch_plain_twisted_as<M, LAYOUT, 1, false>(A, B, C, D, nsys, nrow, flag, s);
+    } else {
+        launch_plain_twisted_as<M, LAYOUT, 1, false>(A, B, C, D, nsys, nrow, flag, s);
+    }
+}
+
 template <int M>
-hipError_t launch_plain_twisted_m(const double* A, const double* B, double* C, double* D, int nsys,
-                                  int nrow, int* flag, hipStream_t s) {
+hipError_t launch_plain_twisted_m(int layout, const double* A, const double* B, double* C, double* D,
+                                  int nsys, int nrow, int* flag, hipStream_t s) {
     if constexpr (M < 2) {
         return hipErrorInvalidValue;
     } else {
-        constexpr int kItems = (kTile + batch_width(M) - 1) / batch_width(M);
-        const int ntiles = (nsys + kTile - 1) / kTile;
-        const int64_t blocks = (int64_t)kXcds * ((ntiles + kXcds - 1) / kXcds) * kItems;
-        hipLaunchKernelGGL((k_plain_twisted<M>), dim3((unsigned)blocks), dim3(64), 0, s, A, B, C, D, nsys,
-                           nrow, ntiles, flag);
+        if (layout == LAYOUT_TILED)
+            launch_plain_twisted_auto<M, LAYOUT_TILED>(A, B, C, D, nsys, nrow, flag, s);
+        else
+            launch_plain_twisted_auto<M, LAYOUT_REF_GPU>(A, B, C, D, nsys, nrow, flag, s);
         return hipGetLastError();
     }
 }
@@ -1451,14 +1479,18 @@ void launch_plain_coop_as(const double* A, const double* B, double* C, double* D
     hipLaunchKernelGGL((k_plain_coop<M, LAYOUT, WV, LOCK>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
 }
 
-// Workgroup shape of the cooperative single-partition solve, as measured (m = 8-10, 1024-32768
-// systems of 128-2048 rows): at m >= 9 the kernel holds all 512 registers of a lane (one
-// wavefront per SIMD) and four unlocked wavefronts per workgroup ran 5-29% faster than one. At
-// m = 8 (two wavefronts per SIMD) locked workgroups of 2 (tiled layout) or 4 (reference layout)
-// wavefronts ran 13-33% faster, reading the compulsory bytes instead of twice them, once the
-// wavefronts of work reach kCoopLockFrom per CU (bandwidth-bound: 8 tiled, 4 reference) and
-// slower on fewer systems (1.9x at 1024 systems of 128 rows), where every wavefront waiting at
-// the barrier is latency; those run one wavefront per workgroup.
+// The single-partition solve for m >= 8, as measured (m = 8-10, 1024-32768 systems of
+// 128-4096 rows, both layouts), by the wavefronts of cooperative work per CU (floor(64/m)
+// systems a wavefront):
+// * m = 8 (two cooperative wavefronts per SIMD), from kCoopLockFrom per CU (bandwidth-bound: 8
+//   tiled, 4 reference): cooperative, locked workgroups of 2 (tiled) or 4 (reference)
+//   wavefronts, 13-33% faster than one unlocked wavefront, reading the compulsory bytes instead
+//   of twice them (16384 x 2048: 43.4 -> 34.2 ms reference, 36.4 -> 24.6 ms tiled); below
+//   that the twisted form (latency-bound: two groups per system halve the sequential rows):
+//   4096 x 4096 30.8 -> 25.2 ms (reference), 20.6 -> 18.2 ms (tiled), 1024 x 2048 10.3 -> 5.7 ms.
+// * m >= 9 (one wavefront per SIMD): cooperative in unlocked workgroups of 4 wavefronts (5-29%
+//   faster than one) from 2 wavefronts of work per CU; below, twisted (1024 x 2048 26.4 ->
+//   13.5 ms reference, 18.5 -> 10.6 ms tiled; 4096 x 1024 stays cooperative, 15.0 vs 17.6 ms).
 #ifndef BTD_COOP_LOCK_FROM
 #define BTD_COOP_LOCK_FROM 4
 #endif
@@ -1468,12 +1500,15 @@ constexpr int kCoopLockFrom = (LAYOUT == LAYOUT_TILED ? 2 : 1) * BTD_COOP_LOCK_F
 template <int M, int LAYOUT>
 void launch_plain_coop(const double* A, const double* B, double* C, double* D, int nsys, int nrow,
                        int* flag, hipStream_t s) {
+    const int64_t waves = ((int64_t)nsys + kTile / M - 1) / (kTile / M), cus = device_cus();
     if constexpr (M >= 9) {
-        launch_plain_coop_as<M, LAYOUT, 4, false>(A, B, C, D, nsys, nrow, flag, s);
+        if (waves < 2 * cus)
+            launch_plain_twisted_auto<M, LAYOUT>(A, B, C, D, nsys, nrow, flag, s);
+        else
+            launch_plain_coop_as<M, LAYOUT, 4, false>(A, B, C, D, nsys, nrow, flag, s);
     } else {
-        const int64_t waves = ((int64_t)nsys + kTile / M - 1) / (kTile / M);
-        if (waves < (int64_t)kCoopLockFrom<LAYOUT> * device_cus())
-            launch_plain_coop_as<M, LAYOUT, 1, false>(A, B, C, D, nsys, nrow, flag, s);
+        if (waves < (int64_t)kCoopLockFrom<LAYOUT> * cus)
+            launch_plain_twisted_auto<M, LAYOUT>(A, B, C, D, nsys, nrow, flag, s);
         else if constexpr (LAYOUT == LAYOUT_TILED)
             launch_plain_coop_as<M, LAYOUT, 2, true>(A, B, C, D, nsys, nrow, flag, s);
         else

@@ -83,11 +83,28 @@ def test_f32r_rejects_bad_arguments(cuda):
 @pytest.mark.parametrize("m", list(range(1, 11)))
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
 def test_fp32_storage_is_the_fp64_solve_of_the_stored_values(cuda, m, layout):
-    """btd_solve_noncyclic_f32s (FP32 blocks, FP64 arithmetic) equals, bit for bit, the FP64
-    solve btd_solve_noncyclic of the same blocks converted to FP64; the blocks are left as
-    they were."""
+    """btd_solve_noncyclic_f32s (FP32 blocks, FP64 arithmetic, one lane per system) equals,
+    bit for bit, the FP64 solve btd_solve_noncyclic of the same blocks converted to FP64 where
+    that solve does the same arithmetic: one lane per system (m <= 7) or m lanes per system
+    (m >= 8, batches that fill the device that way); the blocks are left as they were. Smaller
+    m >= 8 batches are solved twisted (another elimination order) and agree to rounding."""
     import torch
-    nsys, n = 70, 13
+    n = 13
+    if m <= 7:
+        nsys = 70
+    else:
+        cus, g = torch.cuda.get_device_properties(0).multi_processor_count, 64 // m
+        nsys = ((8 if layout == pb.Layout.TILED else 4) if m == 8 else 2) * cus * g + 5
+        x32, x64 = _f32s_and_fp64(cuda, m, 70, n, layout)
+        assert rel_err(x32, x64).max() <= 1e-12
+    x32, x64 = _f32s_and_fp64(cuda, m, nsys, n, layout)
+    assert np.array_equal(x32, x64)
+
+
+def _f32s_and_fp64(cuda, m, nsys, n, layout):
+    """x from the FP32-storage solve and from the FP64 solve of the same (FP32-rounded) blocks,
+    as natural-form arrays; asserts the FP32 blocks are unchanged."""
+    import torch
     A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 300 + m, m, nsys, n)
     A, B, C = fp32_round(A, B, C)
     lay = int(layout)
@@ -103,9 +120,10 @@ def test_fp32_storage_is_the_fp64_solve_of_the_stored_values(cuda, m, layout):
         pb.solve_noncyclic(plan, a64, b64, c64, x64, m, nsys, n)
         torch.cuda.synchronize()
         assert not plan.check()
-    assert torch.equal(x32, x64)
     for t, k in zip((ta, tb, tc), keep):
         assert torch.equal(t, k)
+    return (layout_to_vec(x32.cpu().numpy(), lay, n, nsys, m),
+            layout_to_vec(x64.cpu().numpy(), lay, n, nsys, m))
 
 
 def test_fp32_storage_on_conditioned_blocks(cuda, oracle):

@@ -197,8 +197,9 @@ def test_per_system_info(cuda, npart, cyclic):
 @pytest.mark.parametrize("m", [8, 9, 10])
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
 def test_cooperative_plain_edges(cuda, oracle, m, layout):
-    """m >= 8 single-partition solves run m lanes per system: n = 2 and 3, a partial last
-    wavefront of systems, and zero block rows named per system."""
+    """m >= 8 single-partition solves of few systems (run twisted, two groups of m lanes per
+    system): n = 2 and 3, a partial last wavefront of systems, and zero block rows named per
+    system."""
     import torch
     for nsys, n in ((13, 2), (70, 3), (129, 9)):
         A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 40 + n, m, nsys, n)
@@ -215,12 +216,47 @@ def test_cooperative_plain_edges(cuda, oracle, m, layout):
         assert plan.info().nonzero()[0].tolist() == [0, 33, 69]
 
 
+def _plain_regimes(m, layout, cus):
+    """Batch sizes that reach each kernel shape of the m >= 8 single-partition solve on a
+    device of `cus` CUs (wavefronts of work per CU decide it), each a few systems past its
+    threshold so that the last workgroup is partial."""
+    g = 64 // m
+    if m == 8:
+        lock_from = (8 if layout == pb.Layout.TILED else 4) * cus * g
+        return {"twisted, 4 locked wavefronts": 4 * cus * 4 + 3, "cooperative, locked": lock_from + 5}
+    return {"cooperative, 4 wavefronts": 2 * cus * g + 7}
+
+
+@pytest.mark.parametrize("m", [8, 9, 10])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_plain_large_batch_shapes(cuda, oracle, m, layout):
+    """The multi-wavefront workgroup shapes of the m >= 8 single-partition solve (cooperative
+    or twisted, locked or not, chosen by the batch size) against the oracle, and zero block
+    rows named per system, the last system included."""
+    import torch
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    n = 5
+    for what, nsys in _plain_regimes(m, layout, cus).items():
+        print(f"[case] m={m} {layout.name} nsys={nsys}: {what}", flush=True)
+        A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 61 + m, m, nsys, n)
+        x, sing = gpu_solve(A, B, C, d, int(layout))
+        assert not sing
+        assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
+        bad = [1, nsys // 2, nsys - 1]
+        for s_, q in zip(bad, (0, 2, 4)):
+            A[q, s_] = B[q, s_] = C[q, s_] = 0.0
+        ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, int(layout))).to(cuda) for X in (A, B, C, d))
+        with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
+            pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
+            assert plan.info().nonzero()[0].tolist() == bad
+
+
 @pytest.mark.parametrize("m,nsys", [(9, 61), (10, 61), (10, 5)])
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
 def test_two_stage_cooperative_odd_chunks(cuda, oracle, m, nsys, layout):
     """At m >= 9 the cooperative stage 1 runs two wavefronts of floor(64/m) systems per
     workgroup; an odd number of such chunks (9 at m = 9, 11 and 1 at m = 10) leaves the last
-    workgroup's second wavefront without systems, which must store nothing."""
+    workgroup's second wavefront without systems."""
     n, npart = 40, 5
     A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 77 + m + nsys, m, nsys, n)
     x, sing = gpu_solve(A, B, C, d, int(layout), npart=npart)

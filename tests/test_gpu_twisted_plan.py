@@ -1,5 +1,6 @@
 """The twisted form of a plan's single-partition solve (btd_plan_set_form, BTD_FORM_TWISTED):
-two groups of m lanes per system, one eliminating from each end, against the oracle."""
+two groups of m lanes per system, one eliminating from each end, against the oracle, in both
+layouts."""
 import numpy as np
 import pytest
 
@@ -18,12 +19,14 @@ def oracle_plain(oracle, A, B, C, d):
 
 
 @pytest.mark.parametrize("m,nsys,n", [(2, 70, 2), (3, 64, 3), (5, 130, 17), (8, 33, 64), (10, 7, 40),
-                                      (4, 200, 101), (6, 65, 5), (7, 100, 9), (9, 20, 31), (5, 1, 4)])
-def test_twisted_matches_oracle(cuda, oracle, m, nsys, n):
+                                      (4, 200, 101), (6, 65, 5), (7, 100, 9), (9, 20, 31), (5, 1, 4),
+                                      (8, 200, 24), (10, 150, 19)])
+@pytest.mark.parametrize("layout", [pb.Layout.TILED, pb.Layout.REF_GPU])
+def test_twisted_matches_oracle(cuda, oracle, m, nsys, n, layout):
     """Every system (nsys not a multiple of 64, the last tile partial; n = 2 and 3 put the
     meeting row at 0 and 1) against the oracle's plain solve."""
     A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 300 + m + n, m, nsys, n)
-    x, sing = gpu_solve(A, B, C, d, TILED, form=int(pb.Form.TWISTED))
+    x, sing = gpu_solve(A, B, C, d, int(layout), form=int(pb.Form.TWISTED))
     assert not sing
     assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
 
@@ -60,10 +63,10 @@ def test_twisted_profiled_and_info(cuda):
 
 
 def test_twisted_form_rules(cuda):
-    """The twisted form needs npart == 1, the tiled layout and m >= 2; PLAIN is always allowed."""
+    """The twisted form needs npart == 1 and m >= 2 (either layout); PLAIN is always allowed."""
     with pb.BtdmaGpuPlan().create(3, 64, 12, 1, pb.Layout.REF_GPU) as plan:
-        with pytest.raises(ValueError):
-            plan.set_form(pb.Form.TWISTED)
+        plan.set_form(pb.Form.TWISTED)
+        assert plan.form == pb.Form.TWISTED
         plan.set_form(pb.Form.PLAIN)
     with pb.BtdmaGpuPlan().create(3, 64, 12, 2, pb.Layout.TILED) as plan:
         with pytest.raises(ValueError):
