@@ -108,13 +108,13 @@ size_t btd_layout_elems(int layout, int nsys, int nrow, int elems_per_block) {
 int btd_auto_npart(int m, int nsys, int nrow) {
     // One GPU: partitions are an occupancy choice. The single-kernel solve takes a batch that
     // fills the device on its own: at least half a wave per SIMD (32768 systems) with one lane
-    // per system, or 4096 systems for m >= 8, whose single-partition solve gives every system
-    // m lanes. A smaller batch gets enough partitions for about one wave per SIMD (65536
-    // lanes), keeping >= 8 rows per partition so that the reduced system of 2P rows stays
-    // small against the partition work.
+    // per system, or 4096 systems for m >= 7, whose single-partition solve gives every system
+    // m or 2m lanes (cooperative or twisted). A smaller batch gets enough partitions for about
+    // one wave per SIMD (65536 lanes), keeping >= 8 rows per partition so that the reduced
+    // system of 2P rows stays small against the partition work.
     constexpr int64_t kLanes = 65536;
     if (nsys < 1 || nrow < 2) return 1;
-    if ((int64_t)nsys >= (m >= 8 ? 4096 : kLanes / 2)) return 1;
+    if ((int64_t)nsys >= (m >= 7 ? 4096 : kLanes / 2)) return 1;
     const int64_t want = (kLanes + nsys - 1) / nsys;
     const int64_t most = nrow / 8;
     const int64_t p = want < most ? want : most;

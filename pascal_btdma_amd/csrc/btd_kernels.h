@@ -1143,24 +1143,27 @@ void launch_plain_twisted_as(const double* A, const double* B, double* C, double
     hipLaunchKernelGGL((k_plain_twisted<M, LAYOUT, WV, LOCK>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
 }
 
-// Workgroup shape of the twisted solve, as measured (m = 8, 10; 1024-16384 systems of 512-4096
-// rows, both layouts): m = 8 with 4 or more wavefronts of work per CU (4096 systems and up):
-// four wavefronts locked as in k_plain_coop (4096 x 4096 rows 39.8 -> 25.2 ms, reference
-// layout); fewer: one wavefront (1024 x 2048: 5.7 ms, locked 4 10.6 ms). m >= 9: two locked
-// wavefronts (1024 x 2048: 16.3 -> 13.5 ms). m <= 7: one wavefront.
+// Workgroup shape of the twisted solve, as measured (m = 2-10; 1024-16384 systems of 512-4096
+// rows, both layouts): m >= 9 two locked wavefronts (1024 x 2048: 16.3 -> 13.5 ms); m <= 8
+// four locked wavefronts (as in k_plain_coop) from kTwistedLockFrom twisted wavefronts of work
+// per CU (m = 8 4096 x 4096 39.8 -> 25.2 ms, m = 7 16384 x 2048 52.5 -> 31.7 ms, m = 3
+// 16384 x 2048 8.6 -> 4.8 ms; reference layout), one wavefront below. Between 2 and 4 per CU
+// the locked shape won at m = 3, 5 (8192 x 1024 m = 3: 2.20 -> 1.32 ms) and lost at m = 2, 7, 8
+// (2048 x 1024 m = 8: 4.44 -> 6.12 ms).
+template <int M>
+constexpr int kTwistedLockFrom = (M >= 3 && M <= 6) ? 2 : 4;
+
 template <int M, int LAYOUT>
 void launch_plain_twisted_auto(const double* A, const double* B, double* C, double* D, int nsys, int nrow,
                                int* flag, hipStream_t s) {
     if constexpr (M >= 9) {
         launch_plain_twisted_as<M, LAYOUT, 2, true>(A, B, C, D, nsys, nrow, flag, s);
-    } else if constexpr (M == 8) {
+    } else {
         const int64_t waves = ((int64_t)nsys + batch_width(M) - 1) / batch_width(M);
-        if (waves >= 4 * (int64_t)device_cus())
+        if (waves >= kTwistedLockFrom<M> * (int64_t)device_cus())
             launch_plain_twisted_as<M, LAYOUT, 4, true>(A, B, C, D, nsys, nrow, flag, s);
         else
             launch_plain_twisted_as<M, LAYOUT, 1, false>(A, B, C, D, nsys, nrow, flag, s);
-    } else {
-        launch_plain_twisted_as<M, LAYOUT, 1, false>(A, B, C, D, nsys, nrow, flag, s);
     }
 }
 
@@ -1516,21 +1519,44 @@ void launch_plain_coop(const double* A, const double* B, double* C, double* D, i
     }
 }
 
+// Smallest m whose single-partition solve runs twisted below kPlainLaneFrom systems instead of
+// one lane per system: at m = 7, 16384 systems, one lane per system fills a wavefront per CU
+// and the twisted solve (locked workgroups) measured 31.7 vs 32.5 ms (2048 rows, reference
+// layout) and 25.9 vs 27.1 ms (tiled), 4096 systems 10.9 vs 30.8 ms; at m <= 5 the two-stage
+// solve, which the library picks for those batches, was as fast or faster.
+#ifndef BTD_PLAIN_TWISTED_FROM
+#define BTD_PLAIN_TWISTED_FROM 7
+#endif
+constexpr int kPlainTwistedFrom = BTD_PLAIN_TWISTED_FROM;
+constexpr int kPlainLaneFrom = 32768;
+
+template <int M>
+void launch_plain_lane(int layout, const double* A, const double* B, double* C, double* D, int nsys, int nrow,
+                       int* flag, hipStream_t s) {
+    const dim3 grid((nsys + 63) / 64), block(64);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_plain<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    else
+        hipLaunchKernelGGL((k_plain<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+}
+
 template <int M>
 hipError_t launch_plain_m(int layout, const double* A, const double* B, double* C, double* D,
                                  int nsys, int nrow, int* flag, hipStream_t s) {
-    const dim3 block(64);
     if constexpr (M >= kPlainCoopFrom) {
         if (layout == LAYOUT_TILED)
             launch_plain_coop<M, LAYOUT_TILED>(A, B, C, D, nsys, nrow, flag, s);
         else
             launch_plain_coop<M, LAYOUT_REF_GPU>(A, B, C, D, nsys, nrow, flag, s);
-    } else {
-        const dim3 grid((nsys + 63) / 64);
-        if (layout == LAYOUT_TILED)
-            hipLaunchKernelGGL((k_plain<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    } else if constexpr (M >= 2 && M >= kPlainTwistedFrom) {
+        if (nsys >= kPlainLaneFrom)
+            launch_plain_lane<M>(layout, A, B, C, D, nsys, nrow, flag, s);
+        else if (layout == LAYOUT_TILED)
+            launch_plain_twisted_auto<M, LAYOUT_TILED>(A, B, C, D, nsys, nrow, flag, s);
         else
-            hipLaunchKernelGGL((k_plain<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+            launch_plain_twisted_auto<M, LAYOUT_REF_GPU>(A, B, C, D, nsys, nrow, flag, s);
+    } else {
+        launch_plain_lane<M>(layout, A, B, C, D, nsys, nrow, flag, s);
     }
     return hipGetLastError();
 }

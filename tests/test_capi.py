@@ -72,16 +72,16 @@ def test_plan_create_rejects_bad_arguments(args, exc):
 
 
 def test_auto_npart_rule():
-    """npart = 0: the single-kernel solve for large batches (4096 systems suffice for m >= 8),
+    """npart = 0: the single-kernel solve for large batches (4096 systems suffice for m >= 7),
     about one wave per SIMD (65536 lanes) for small ones, at least 8 rows per partition."""
     auto = lambda m, nsys, nrow: pb._capi.lib().btd_auto_npart(m, nsys, nrow)
     assert auto(5, 131072, 512) == 1          # the headline batch
     assert auto(5, 32768, 512) == 1
     assert auto(5, 1024, 128) == 16           # the Study's single-GPU case: nrow / 8
     assert auto(5, 512, 65536) == 128         # long systems: 65536 lanes
-    assert auto(8, 4096, 2048) == 1           # m >= 8: m lanes per system fill the device
+    assert auto(8, 4096, 2048) == 1           # m >= 7: m or 2m lanes per system fill the device
     assert auto(8, 2048, 2048) == 32
-    assert auto(7, 4096, 2048) == 16
+    assert auto(7, 4096, 2048) == 1 and auto(7, 2048, 2048) == 32 and auto(6, 4096, 2048) == 16
     assert auto(3, 1024, 64) == 8
     assert auto(3, 1024, 15) == 1             # fewer than 2 partitions of 8 rows
     for m, nsys, nrow in [(2, 7, 40), (10, 3000, 4096), (5, 1, 2)]:

@@ -85,16 +85,20 @@ def test_f32r_rejects_bad_arguments(cuda):
 def test_fp32_storage_is_the_fp64_solve_of_the_stored_values(cuda, m, layout):
     """btd_solve_noncyclic_f32s (FP32 blocks, FP64 arithmetic, one lane per system) equals,
     bit for bit, the FP64 solve btd_solve_noncyclic of the same blocks converted to FP64 where
-    that solve does the same arithmetic: one lane per system (m <= 7) or m lanes per system
-    (m >= 8, batches that fill the device that way); the blocks are left as they were. Smaller
-    m >= 8 batches are solved twisted (another elimination order) and agree to rounding."""
+    that solve does the same arithmetic: one lane per system (m <= 6; m = 7 from 32768 systems)
+    or m lanes per system (m >= 8, batches that fill the device that way); the blocks are left
+    as they were. Smaller m >= 7 batches are solved twisted (another elimination order) and
+    agree to rounding."""
     import torch
     n = 13
-    if m <= 7:
+    if m <= 6:
         nsys = 70
     else:
         cus, g = torch.cuda.get_device_properties(0).multi_processor_count, 64 // m
-        nsys = ((8 if layout == pb.Layout.TILED else 4) if m == 8 else 2) * cus * g + 5
+        if m == 7:
+            nsys = 32768 + 5
+        else:
+            nsys = ((8 if layout == pb.Layout.TILED else 4) if m == 8 else 2) * cus * g + 5
         x32, x64 = _f32s_and_fp64(cuda, m, 70, n, layout)
         assert rel_err(x32, x64).max() <= 1e-12
     x32, x64 = _f32s_and_fp64(cuda, m, nsys, n, layout)

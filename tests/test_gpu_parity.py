@@ -217,22 +217,25 @@ def test_cooperative_plain_edges(cuda, oracle, m, layout):
 
 
 def _plain_regimes(m, layout, cus):
-    """Batch sizes that reach each kernel shape of the m >= 8 single-partition solve on a
+    """Batch sizes that reach each kernel shape of the m >= 7 single-partition solve on a
     device of `cus` CUs (wavefronts of work per CU decide it), each a few systems past its
     threshold so that the last workgroup is partial."""
     g = 64 // m
+    if m == 7:
+        return {"twisted, 1 wavefront": 70, "twisted, 4 locked wavefronts": 4 * cus * 4 + 3,
+                "one lane per system": 32768 + 5}
     if m == 8:
         lock_from = (8 if layout == pb.Layout.TILED else 4) * cus * g
         return {"twisted, 4 locked wavefronts": 4 * cus * 4 + 3, "cooperative, locked": lock_from + 5}
     return {"cooperative, 4 wavefronts": 2 * cus * g + 7}
 
 
-@pytest.mark.parametrize("m", [8, 9, 10])
+@pytest.mark.parametrize("m", [7, 8, 9, 10])
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
 def test_plain_large_batch_shapes(cuda, oracle, m, layout):
-    """The multi-wavefront workgroup shapes of the m >= 8 single-partition solve (cooperative
-    or twisted, locked or not, chosen by the batch size) against the oracle, and zero block
-    rows named per system, the last system included."""
+    """The kernel shapes of the m >= 7 single-partition solve (one lane, cooperative or
+    twisted, locked or not, chosen by the batch size) against the oracle, and zero block rows
+    named per system, the last system included."""
     import torch
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     n = 5

@@ -31,6 +31,21 @@ def test_twisted_matches_oracle(cuda, oracle, m, nsys, n, layout):
     assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
 
 
+@pytest.mark.parametrize("m,per_cu", [(2, 4), (3, 2), (5, 2), (6, 2)])
+@pytest.mark.parametrize("layout", [pb.Layout.TILED, pb.Layout.REF_GPU])
+def test_twisted_locked_workgroups(cuda, oracle, m, per_cu, layout):
+    """From per_cu twisted wavefronts of work per CU (floor(32/m) systems each) the twisted
+    solve runs four wavefronts per workgroup that meet at a barrier before each row's loads;
+    a few systems past that, the last workgroup is partial."""
+    import torch
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    nsys, n = per_cu * cus * (32 // m) + 5, 6
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 500 + m, m, nsys, n)
+    x, sing = gpu_solve(A, B, C, d, int(layout), form=int(pb.Form.TWISTED))
+    assert not sing
+    assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
+
+
 @pytest.mark.parametrize("kind", [pb.GenKind.EULER, pb.GenKind.COND, pb.GenKind.DIAG])
 def test_twisted_other_generators(cuda, oracle, kind):
     A, B, C, d = pb.generate_host(kind, 17, 5, 96, 48)
