@@ -1656,11 +1656,13 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     const dim3 block(64), gridp((unsigned)(ntiles * npart)), gridr((unsigned)ntiles);
     if (ev) (void)hipEventRecord(ev[0], s);
     const size_t lds = sizeof(double) * (2 * M * M + M) * 64;
-    if (lds > 65536) {  // above the default dynamic-LDS limit (m >= 8)
-        (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_TILED>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_REF_GPU>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if constexpr (M < kModiCoopFrom) {
+        if (lds > 65536) {  // above the default dynamic-LDS limit (m >= 8)
+            (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_TILED>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_REF_GPU>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        }
     }
     if constexpr (M >= kModiCoopFrom) {
         // m lanes per partition (no spills); the items of a partition on one XCD
