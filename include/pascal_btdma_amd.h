@@ -134,9 +134,13 @@ btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double *a, const double
 /* Periodic solve: replaces btdma_many_cycl_mpi_gpu_v2 (mod_btdma_gpu_v2.f90:200-239) and
  * solve_cyclic (pascal_btdma_cuda.hpp:158-166), which throws logic_error in the reference
  * C++ port. Row 0 couples to x_{n-1} through A of row 0 and row n-1 to x_0 through C of
- * row n-1. npart = 1: one periodic block-Thomas sweep per system (needs nrow >= 3; A, C, D
- * overwritten); npart > 1: stage 1 as in btd_solve_noncyclic, a periodic reduced system
- * of 2*npart rows, stage 3 (btdma_many_cyclic_mpi, mod_btdma_cpu.f90:273-302). */
+ * row n-1. npart > 1: stage 1 as in btd_solve_noncyclic, a periodic reduced system of
+ * 2*npart rows, stage 3 (btdma_many_cyclic_mpi, mod_btdma_cpu.f90:273-302). npart = 1:
+ * the same over partitions the library chooses (btd_auto_npart's count where that is >= 2,
+ * else 4, or 2 below 32 rows; faster than one periodic sweep per system at every size
+ * measured), with their scratch allocated by the plan's first periodic solve (make that call
+ * before capturing one in a graph); below 6 rows one periodic block-Thomas sweep per system
+ * (needs nrow >= 3). A, C, D are overwritten. */
 btd_status btd_solve_cyclic(btd_plan_t plan, double *a, const double *b, double *c, double *d,
                             int m, int nsys, int nrow, void *stream);
 

@@ -66,14 +66,18 @@ struct btd_plan_s {
     int* d_flag = nullptr;  // zero-pivot flag (word 0), then one word per system
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     hipStream_t last_stream = nullptr;
+    // The periodic solve of an npart = 1 plan runs as a two-stage solve over cyc_npart
+    // partitions (0 until its first call sets them up; 1: too few rows, one periodic sweep).
+    int cyc_npart = 0;
+    int* cyc_part = nullptr;
+    double *cyc_redA = nullptr, *cyc_redC = nullptr, *cyc_redD = nullptr;
 };
 
 static void plan_free(btd_plan_s* p) {
     if (!p) return;
-    if (p->d_part) (void)hipFree(p->d_part);
-    if (p->redA) (void)hipFree(p->redA);
-    if (p->redC) (void)hipFree(p->redC);
-    if (p->redD) (void)hipFree(p->redD);
+    for (void* q : {(void*)p->d_part, (void*)p->redA, (void*)p->redC, (void*)p->redD, (void*)p->cyc_part,
+                    (void*)p->cyc_redA, (void*)p->cyc_redC, (void*)p->cyc_redD})
+        if (q) (void)hipFree(q);
     if (p->d_flag) (void)hipFree(p->d_flag);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
@@ -121,6 +125,44 @@ int btd_auto_npart(int m, int nsys, int nrow) {
     return p < 2 ? 1 : (int)p;
 }
 
+// Row partitions of every system (btd_partition_1d) on the host and as a device table
+// [first..., len...], and the tiled reduced system of 2*npart rows per system.
+static hipError_t alloc_partitions(int m, int nsys, int nrow, int npart, std::vector<int>* first,
+                                   std::vector<int>* len, int** d_part, double** redA, double** redC,
+                                   double** redD) {
+    std::vector<int> f(npart), n(npart);
+    for (int r = 0; r < npart; ++r) {
+        int a, b;
+        btd_partition_1d(0, nrow - 1, npart, r, &a, &b);
+        f[r] = a;
+        n[r] = b - a + 1;
+    }
+    std::vector<int> tab(f);
+    tab.insert(tab.end(), n.begin(), n.end());
+    hipError_t e = hipMalloc(d_part, sizeof(int) * tab.size());
+    if (e == hipSuccess) e = hipMemcpy(*d_part, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice);
+    const size_t tiles = (size_t)btd::pad_nsys(nsys);
+    const size_t rm = tiles * 2 * npart * m * m, rv = tiles * 2 * npart * m;
+    if (e == hipSuccess) e = hipMalloc(redA, rm * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(redC, rm * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(redD, rv * sizeof(double));
+    if (first) *first = f;
+    if (len) *len = n;
+    return e;
+}
+
+// Partitions of the periodic solve of an npart = 1 plan. The two-stage periodic solve
+// (k_modi / k_modi_coop, a periodic reduced system, k_update) measured faster than one
+// periodic sweep per lane (k_cyclic) at every size tried (m = 2-10, 256-131072 systems of
+// 12-1024 rows: 1.3-1.9x at m <= 5 on large batches, 3-13x at m >= 7), with the partition
+// count of btd_auto_npart where that is >= 2 and otherwise 4 (2 for fewer than 32 rows; P = 2,
+// 3, 4, 8 were within 5% of each other on large batches). Fewer than 6 rows keep k_cyclic.
+static int cyclic_npart(int m, int nsys, int nrow) {
+    int p = btd_auto_npart(m, nsys, nrow);
+    if (p < 2) p = nrow >= 32 ? 4 : 2;
+    return nrow >= 3 * p ? p : 1;
+}
+
 btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npart, int layout) {
     g_err.clear();
     if (!plan) return fail(BTD_ERR_INVALID_ARG, "btd_plan_create: plan pointer is NULL");
@@ -150,26 +192,9 @@ btd_status btd_plan_create(btd_plan_t* plan, int m, int nsys, int nrow, int npar
     if (e == hipSuccess) e = hipMalloc(&p->d_flag, sizeof(int) * (1 + (size_t)nsys));
     if (e == hipSuccess) e = hipMemset(p->d_flag, 0, sizeof(int) * (1 + (size_t)nsys));
     for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&p->ev[i]);
-    if (e == hipSuccess && npart > 1) {
-        p->first.resize(npart);
-        p->len.resize(npart);
-        for (int r = 0; r < npart; ++r) {
-            int f, l;
-            btd_partition_1d(0, nrow - 1, npart, r, &f, &l);
-            p->first[r] = f;
-            p->len[r] = l - f + 1;
-        }
-        std::vector<int> tab(p->first);
-        tab.insert(tab.end(), p->len.begin(), p->len.end());
-        e = hipMalloc(&p->d_part, sizeof(int) * tab.size());
-        if (e == hipSuccess)
-            e = hipMemcpy(p->d_part, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice);
-        const size_t tiles = (size_t)btd::pad_nsys(nsys);
-        const size_t rm = tiles * 2 * npart * m * m, rv = tiles * 2 * npart * m;
-        if (e == hipSuccess) e = hipMalloc(&p->redA, rm * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc(&p->redC, rm * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc(&p->redD, rv * sizeof(double));
-    }
+    if (e == hipSuccess && npart > 1)
+        e = alloc_partitions(m, nsys, nrow, npart, &p->first, &p->len, &p->d_part, &p->redA, &p->redC,
+                             &p->redD);
     if (e != hipSuccess) {
         plan_free(p);
         return hip_fail(e, "btd_plan_create");
@@ -228,12 +253,28 @@ static btd_status check_solve(btd_plan_t plan, const void* a, const void* b, con
 static btd_status solve_impl(btd_plan_t plan, double* a, const double* b, double* c, double* d,
                              hipStream_t s, bool timed, bool cyclic = false) {
     plan->last_stream = s;
-    if (plan->npart == 1) {
-        if (timed) BTD_HIP(hipEventRecord(plan->ev[0], s));
-        if (cyclic)
+    if (plan->npart == 1 && cyclic) {
+        if (plan->cyc_npart == 0) {
+            const int pc = cyclic_npart(plan->m, plan->nsys, plan->nrow);
+            if (pc > 1) {
+                hipError_t e = alloc_partitions(plan->m, plan->nsys, plan->nrow, pc, nullptr, nullptr,
+                                                &plan->cyc_part, &plan->cyc_redA, &plan->cyc_redC,
+                                                &plan->cyc_redD);
+                if (e != hipSuccess) return hip_fail(e, "btd_solve_cyclic: partition scratch");
+            }
+            plan->cyc_npart = pc;
+        }
+        if (plan->cyc_npart > 1)
+            BTD_HIP(btd::launch_two_stage(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
+                                          plan->cyc_part, plan->cyc_part + plan->cyc_npart, plan->cyc_npart,
+                                          plan->cyc_redA, plan->cyc_redC, plan->cyc_redD, plan->d_flag, s,
+                                          nullptr, true));
+        else
             BTD_HIP(btd::launch_cyclic(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                        plan->d_flag, s));
-        else if (plan->form == BTD_FORM_TWISTED)
+    } else if (plan->npart == 1) {
+        if (timed) BTD_HIP(hipEventRecord(plan->ev[0], s));
+        if (plan->form == BTD_FORM_TWISTED)
             BTD_HIP(btd::launch_plain_twisted(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                               plan->d_flag, s));
         else

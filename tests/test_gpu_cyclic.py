@@ -67,3 +67,52 @@ def test_cyclic_three_rows(cuda, oracle):
     x = gpu_cyclic(cuda, A, B, C, d, pb.Layout.TILED, 1)
     xo, _ = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
     assert rel_err(x, xo).max() <= TOL
+
+
+@pytest.mark.parametrize("m", [2, 5, 8])
+@pytest.mark.parametrize("n", [3, 4, 5, 6, 7])
+def test_cyclic_single_partition_row_counts(cuda, oracle, m, n):
+    """npart = 1: fewer than 6 rows run one periodic sweep per system (k_cyclic); from 6 rows
+    the periodic solve runs over partitions the library chooses (here 2)."""
+    rng = np.random.default_rng(7 * n + m)
+    A, B, C, d = periodic_system(rng, n, 66, m)
+    x = gpu_cyclic(cuda, A, B, C, d, pb.Layout.TILED, 1)
+    xo, info = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    assert rel_err(x, xo).max() <= TOL
+
+
+@pytest.mark.parametrize("m,nsys,n", [(5, 3000, 64), (8, 4100, 40), (3, 200, 33)])
+def test_cyclic_single_partition_partition_counts(cuda, oracle, m, nsys, n):
+    """npart = 1 periodic solves whose library-chosen partition count is 8 (btd_auto_npart), 4
+    (a batch the single-partition solve fills the device with) and 4 (33 rows)."""
+    rng = np.random.default_rng(m + n)
+    A, B, C, d = periodic_system(rng, n, nsys, m)
+    x = gpu_cyclic(cuda, A, B, C, d, pb.Layout.REF_GPU, 1)
+    xo, info = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    assert rel_err(x, xo).max() <= TOL
+
+
+def test_cyclic_and_plain_on_one_plan(cuda, oracle):
+    """One npart = 1 plan: a periodic solve (which sets up its partitions), a non-periodic
+    solve, then a periodic solve of other data reusing them; each against the oracle."""
+    import torch
+    m, nsys, n = 4, 90, 20
+    lay = int(pb.Layout.TILED)
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, pb.Layout.TILED) as plan:
+        for k, cyc in enumerate((True, False, True)):
+            rng = np.random.default_rng(500 + k)
+            A, B, C, d = periodic_system(rng, n, nsys, m)
+            t = [torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d)]
+            (pb.solve_cyclic if cyc else pb.solve_noncyclic)(plan, *t, m, nsys, n)
+            torch.cuda.synchronize()
+            assert not plan.check()
+            x = layout_to_vec(t[3].cpu().numpy(), lay, n, nsys, m)
+            if cyc:
+                xo, _ = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+            else:
+                A[0] = 0.0  # row 0's sub-diagonal block is outside a non-periodic system
+                C[-1] = 0.0
+                xo, _ = oracle.solve_plain(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+            assert rel_err(x, xo).max() <= TOL
