@@ -649,7 +649,8 @@ def extra_euler(dev, steps: int, n: int = 256) -> dict:
     out["cells_per_s_matrix_free_adi_step"] = out["cells"] / (tot["matrix_free"] * 1e-3)
     out["traffic_model"] = ("matrix-free sweep: 125 doubles per block row (U 5, rhs 5, C'/E'/d' "
                             "55 written and 55 read, x 5); assembled: assembly 90 + periodic solve "
-                            "255 + scatter 10")
+                            "195 (two-stage over the plan's periodic partitions: A, B, C, d 80 read, "
+                            "Â/Ĉ/d̂ 55 written and 55 read, x 5) + scatter 10")
     return out
 
 
