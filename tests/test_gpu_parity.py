@@ -238,13 +238,13 @@ def test_plain_large_batch_shapes(cuda, oracle, m, layout):
     named per system, the last system included."""
     import torch
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    n = 5
     for what, nsys in _plain_regimes(m, layout, cus).items():
         print(f"[case] m={m} {layout.name} nsys={nsys}: {what}", flush=True)
-        A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 61 + m, m, nsys, n)
-        x, sing = gpu_solve(A, B, C, d, int(layout))
-        assert not sing
-        assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
+        for n in (2, 5):  # n = 2: one forward step after row 0 and one backward step
+            A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 61 + m + n, m, nsys, n)
+            x, sing = gpu_solve(A, B, C, d, int(layout))
+            assert not sing
+            assert rel_err(x, oracle_plain(oracle, A, B, C, d)).max() <= TOL
         bad = [1, nsys // 2, nsys - 1]
         for s_, q in zip(bad, (0, 2, 4)):
             A[q, s_] = B[q, s_] = C[q, s_] = 0.0
