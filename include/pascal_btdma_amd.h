@@ -58,10 +58,10 @@ typedef enum btd_layout {
 
 /* How a plan's single-partition solve runs (btd_plan_set_form). */
 typedef enum btd_form {
-    BTD_FORM_PLAIN = 0,   /* the library's choice: one lane per system (m <= 6, or m = 7 in
+    BTD_FORM_PLAIN = 0,   /* the library's choice: one lane per system (m = 1, or m <= 7 in
                            * large batches; blocks in VGPRs), m lanes per system (m >= 8; one
-                           * block row each, pivoting through LDS) or, for smaller m >= 7
-                           * batches, twisted */
+                           * block row each, pivoting through LDS) or, for smaller batches,
+                           * twisted */
     BTD_FORM_TWISTED = 1  /* two groups of m lanes per system, one eliminating from each end */
 } btd_form;
 
@@ -89,7 +89,8 @@ btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npar
 
 /* New: the partition count a plan created with npart = 0 gets on one GPU, where partitions are
  * an occupancy choice rather than a rank count: 1 (the single-kernel solve) for nsys >= 32768
- * (nsys >= 4096 for m >= 7, whose single-partition solve runs m or 2m lanes per system), otherwise
+ * (nsys >= 4096 for m >= 7 and 8192 for m = 2, 4, whose single-partition solve then runs m or
+ * 2m lanes per system), otherwise
  * about 65536 / nsys (one wave per SIMD), at most nrow / 8, and 1 when that is < 2.
  * Pure function (no device use). */
 int btd_auto_npart(int m, int nsys, int nrow);
@@ -98,11 +99,11 @@ int btd_auto_npart(int m, int nsys, int nrow);
 btd_status btd_plan_npart(btd_plan_t plan, int *npart);
 
 /* New (no counterpart in the reference): choose how an npart = 1 plan's non-cyclic solve runs.
- * BTD_FORM_PLAIN (the default) is the library's choice: one lane per system for m <= 6 and for
- * m = 7 from 32768 systems; for m >= 8 m lanes per system (one block row each); and the
- * twisted form below for batches too small to fill the device those ways: m = 7 below 32768
- * systems, m = 8 below 32 systems per CU (reference layout) or 64 (tiled), i.e. 8192 / 16384
- * systems on 256 CUs, m = 9, 10 below 14 / 12 systems per CU. BTD_FORM_TWISTED (m >= 2, either layout) always gives each system two
+ * BTD_FORM_PLAIN (the default) is the library's choice: one lane per system for m = 1 and for
+ * m <= 7 from 32768 systems; for m >= 8 m lanes per system (one block row each); and the
+ * twisted form below for batches too small to fill the device those ways: 2 <= m <= 7 below
+ * 32768 systems, m = 8 below 32 systems per CU (reference layout) or 64 (tiled), i.e. 8192 /
+ * 16384 systems on 256 CUs, m = 9, 10 below 14 / 12 systems per CU. BTD_FORM_TWISTED (m >= 2, either layout) always gives each system two
  * groups of m lanes, one eliminating downwards from row 0 and one upwards from row nrow-1,
  * meeting in the middle: half the row latency per system (on large batches its wavefronts
  * share cache lines that leave L2 between their reads, so it moves more bytes). Same

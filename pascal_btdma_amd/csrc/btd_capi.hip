@@ -112,13 +112,16 @@ size_t btd_layout_elems(int layout, int nsys, int nrow, int elems_per_block) {
 int btd_auto_npart(int m, int nsys, int nrow) {
     // One GPU: partitions are an occupancy choice. The single-kernel solve takes a batch that
     // fills the device on its own: at least half a wave per SIMD (32768 systems) with one lane
-    // per system, or 4096 systems for m >= 7, whose single-partition solve gives every system
-    // m or 2m lanes (cooperative or twisted). A smaller batch gets enough partitions for about
-    // one wave per SIMD (65536 lanes), keeping >= 8 rows per partition so that the reduced
-    // system of 2P rows stays small against the partition work.
+    // per system; 4096 systems for m >= 7, whose single-partition solve gives every system m
+    // or 2m lanes (cooperative or twisted); 8192 for m = 2 and 4, where the twisted solve's
+    // wavefronts read whole halves or quarters of lines (floor(32/m) systems) and measured up
+    // to 21% faster than two-stage from 8192 systems (m = 3, 5, 6 were not). A smaller batch
+    // gets enough partitions for about one wave per SIMD (65536 lanes), keeping >= 8 rows per
+    // partition so that the reduced system of 2P rows stays small against the partition work.
     constexpr int64_t kLanes = 65536;
     if (nsys < 1 || nrow < 2) return 1;
-    if ((int64_t)nsys >= (m >= 7 ? 4096 : kLanes / 2)) return 1;
+    const int64_t one = m >= 7 ? 4096 : (m == 2 || m == 4) ? 8192 : kLanes / 2;
+    if ((int64_t)nsys >= one) return 1;
     const int64_t want = (kLanes + nsys - 1) / nsys;
     const int64_t most = nrow / 8;
     const int64_t p = want < most ? want : most;

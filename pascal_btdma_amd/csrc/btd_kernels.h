@@ -1520,12 +1520,12 @@ void launch_plain_coop(const double* A, const double* B, double* C, double* D, i
 }
 
 // Smallest m whose single-partition solve runs twisted below kPlainLaneFrom systems instead of
-// one lane per system: at m = 7, 16384 systems, one lane per system fills a wavefront per CU
-// and the twisted solve (locked workgroups) measured 31.7 vs 32.5 ms (2048 rows, reference
-// layout) and 25.9 vs 27.1 ms (tiled), 4096 systems 10.9 vs 30.8 ms; at m <= 5 the two-stage
-// solve, which the library picks for those batches, was as fast or faster.
+// one lane per system. Below 32768 systems one lane per system leaves CUs idle, and the
+// twisted solve (2m lanes per system) measured faster at every m tried: m = 3 16384 x 2048
+// 9.35 -> 4.93 ms, m = 5 22.6 -> 14.6 ms, m = 7 32.5 -> 31.7 ms (reference layout), m = 2, 3
+// 4096 x 512 1.06 -> 0.21, 2.06 -> 0.47 ms.
 #ifndef BTD_PLAIN_TWISTED_FROM
-#define BTD_PLAIN_TWISTED_FROM 7
+#define BTD_PLAIN_TWISTED_FROM 2
 #endif
 constexpr int kPlainTwistedFrom = BTD_PLAIN_TWISTED_FROM;
 constexpr int kPlainLaneFrom = 32768;

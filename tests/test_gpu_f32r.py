@@ -85,17 +85,17 @@ def test_f32r_rejects_bad_arguments(cuda):
 def test_fp32_storage_is_the_fp64_solve_of_the_stored_values(cuda, m, layout):
     """btd_solve_noncyclic_f32s (FP32 blocks, FP64 arithmetic, one lane per system) equals,
     bit for bit, the FP64 solve btd_solve_noncyclic of the same blocks converted to FP64 where
-    that solve does the same arithmetic: one lane per system (m <= 6; m = 7 from 32768 systems)
-    or m lanes per system (m >= 8, batches that fill the device that way); the blocks are left
-    as they were. Smaller m >= 7 batches are solved twisted (another elimination order) and
-    agree to rounding."""
+    that solve does the same arithmetic: one lane per system (m = 1; m <= 7 from 32768
+    systems) or m lanes per system (m >= 8, batches that fill the device that way); the blocks
+    are left as they were. Smaller batches (m >= 2) are solved twisted (another elimination
+    order) and agree to rounding."""
     import torch
     n = 13
-    if m <= 6:
+    if m == 1:
         nsys = 70
     else:
         cus, g = torch.cuda.get_device_properties(0).multi_processor_count, 64 // m
-        if m == 7:
+        if m <= 7:
             nsys = 32768 + 5
         else:
             nsys = ((8 if layout == pb.Layout.TILED else 4) if m == 8 else 2) * cus * g + 5
@@ -131,8 +131,9 @@ def _f32s_and_fp64(cuda, m, nsys, n, layout):
 
 
 def test_fp32_storage_on_conditioned_blocks(cuda, oracle):
-    """cond(B_q) log-uniform in [1e1, 1e6]: the FP32-storage solve needs no refinement; it is the
-    FP64 solve of the stored values, within the FP64 floor of the oracle on those values."""
+    """cond(B_q) log-uniform in [1e1, 1e6]: the FP32-storage solve needs no refinement; it agrees
+    with the FP64 solve of the stored values and with the oracle on those values to the FP64
+    floor at this conditioning."""
     import torch
     m, nsys, n = 5, 128, 64
     A, B, C, d = pb.generate_host(pb.GenKind.COND, 9, m, nsys, n)
@@ -148,7 +149,9 @@ def test_fp32_storage_on_conditioned_blocks(cuda, oracle):
         pb.solve_noncyclic_f32s(plan, ta, tb, tc, td, work)
         torch.cuda.synchronize()
     x = layout_to_vec(td.cpu().numpy(), lay, n, nsys, m)
-    np.testing.assert_array_equal(x, x64)
+    # 128 systems: the FP64 solve runs twisted (another elimination order), so the two agree to
+    # the FP64 floor at this conditioning rather than bit for bit
+    assert rel_err(x, x64).max() <= 1e-9
     err = rel_err(x, xo)
     print(f"FP32 storage, FP64 arithmetic, cond up to 1e6: max rel err vs oracle {err.max():.2e}")
     assert err.max() <= 1e-9  # the two FP64 solvers' floor at cond 1e6 (6.5e-11 measured)

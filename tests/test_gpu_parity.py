@@ -158,8 +158,9 @@ def test_singular_block_is_reported(cuda):
     import torch
     m, nsys, n = 3, 64, 8
     A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 5, m, nsys, n)
-    B[4, 17] = 0.0
-    A[4, 17] = 0.0  # row 4 of system 17 is then all zero in its diagonal block
+    # block row 4 of system 17 all zero: the matrix is singular, so every elimination order
+    # (one lane per system, or twisted from both ends, which 64 systems get) meets a zero pivot
+    A[4, 17] = B[4, 17] = C[4, 17] = 0.0
     lay = 2
     ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d))
     with pb.BtdmaGpuPlan().create(m, nsys, n, 1, pb.Layout.TILED) as plan:
@@ -217,12 +218,13 @@ def test_cooperative_plain_edges(cuda, oracle, m, layout):
 
 
 def _plain_regimes(m, layout, cus):
-    """Batch sizes that reach each kernel shape of the m >= 7 single-partition solve on a
+    """Batch sizes that reach each kernel shape of the m >= 2 single-partition solve on a
     device of `cus` CUs (wavefronts of work per CU decide it), each a few systems past its
     threshold so that the last workgroup is partial."""
     g = 64 // m
-    if m == 7:
-        return {"twisted, 1 wavefront": 70, "twisted, 4 locked wavefronts": 4 * cus * 4 + 3,
+    if m <= 7:
+        lock_from = 2 if 3 <= m <= 6 else 4
+        return {"twisted, 1 wavefront": 70, "twisted, 4 locked wavefronts": lock_from * cus * (32 // m) + 3,
                 "one lane per system": 32768 + 5}
     if m == 8:
         lock_from = (8 if layout == pb.Layout.TILED else 4) * cus * g
@@ -230,11 +232,11 @@ def _plain_regimes(m, layout, cus):
     return {"cooperative, 4 wavefronts": 2 * cus * g + 7}
 
 
-@pytest.mark.parametrize("m", [7, 8, 9, 10])
+@pytest.mark.parametrize("m", [2, 3, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
 def test_plain_large_batch_shapes(cuda, oracle, m, layout):
-    """The kernel shapes of the m >= 7 single-partition solve (one lane, cooperative or
-    twisted, locked or not, chosen by the batch size) against the oracle, and zero block rows
+    """The kernel shapes of the single-partition solve (one lane, cooperative or twisted,
+    locked or not, chosen by m and the batch size) against the oracle, and zero block rows
     named per system, the last system included."""
     import torch
     cus = torch.cuda.get_device_properties(0).multi_processor_count
