@@ -50,7 +50,10 @@ struct CoopDepth {
 // LDS of a cooperative work item, [slot][lane] (64 doubles a slot): slots [0, M) the carried
 // block of each group (slot k, lane base+j: element (k, j)), slot M its carried right-hand
 // side (lane base+i: element i), then the 2M+2 slots of the elimination exchange.
-constexpr int kCoopSlots = 3 * kMaxM + 3;
+// Sized per m where the kernel is (a cooperative workgroup of m = 5 needs 18 slots, not 33).
+template <int M>
+constexpr int kCoopSlotsFor = 3 * M + 3;
+constexpr int kCoopSlots = kCoopSlotsFor<kMaxM>;  // any m (the mixed-size batch kernel)
 
 // Elimination of one block row by a group of M lanes: lane i holds row i of R and of the
 // right-hand sides rhs = (d, S), S of NS blocks of M columns whose R^{-1} S is carried on.

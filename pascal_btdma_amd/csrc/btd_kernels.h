@@ -1040,7 +1040,7 @@ template <int M>
 __global__ __launch_bounds__(64) void k_reduced_twisted(const double* __restrict__ redA, double* redC,
                                                         double* redD, int nsys, int nrr,
                                                         int* __restrict__ flag) {
-    __shared__ double sh[kCoopSlots * kTile];
+    __shared__ double sh[kCoopSlotsFor<M> * kTile];
     constexpr int W = batch_width(M);
     const int lane = threadIdx.x;
     const int g = lane / M, i = lane % M, slot = g / 2;
@@ -1080,7 +1080,7 @@ template <int M, int LAYOUT, int WV, bool LOCK>
 __global__ __launch_bounds__(64 * WV) void k_plain_coop(const double* A, const double* __restrict__ B,
                                                         double* C, double* D, int nsys, int nrow,
                                                         int* __restrict__ flag) {
-    __shared__ double sh[WV][kCoopSlots * kTile];
+    __shared__ double sh[WV][kCoopSlotsFor<M> * kTile];
     constexpr int G = kTile / M;
     const int nitems = (nsys + G * WV - 1) / (G * WV);
     const int per = (nitems + kXcds - 1) / kXcds;
@@ -1109,7 +1109,7 @@ template <int M, int LAYOUT, int WV, bool LOCK>
 __global__ __launch_bounds__(64 * WV) void k_plain_twisted(const double* A, const double* __restrict__ B,
                                                            double* C, double* D, int nsys, int nrow,
                                                            int* __restrict__ flag) {
-    __shared__ double sh[WV][kCoopSlots * kTile];
+    __shared__ double sh[WV][kCoopSlotsFor<M> * kTile];
     constexpr int W = batch_width(M);
     const int nitems = (nsys + W * WV - 1) / (W * WV);
     const int per = (nitems + kXcds - 1) / kXcds;
