@@ -117,6 +117,15 @@ struct BtdmaGpuPlan {
     // (info must hold nsys ints).
     void singular_systems(int* info) { check_status(btd_plan_info(handle, info)); }
 
+    // How an npart = 1 plan's non-cyclic solve runs (btd_plan_set_form): BTD_FORM_PLAIN, the
+    // library's choice (the default), or BTD_FORM_TWISTED, always from both ends.
+    void set_form(btd_form f) { check_status(btd_plan_set_form(handle, (int)f)); }
+    btd_form form() {
+        int f = 0;
+        check_status(btd_plan_form(handle, &f));
+        return (btd_form)f;
+    }
+
    private:
     void swap(BtdmaGpuPlan& o) noexcept {
         std::swap(handle, o.handle);
