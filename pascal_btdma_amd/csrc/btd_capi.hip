@@ -263,7 +263,14 @@ static btd_status solve_impl(btd_plan_t plan, double* a, const double* b, double
                 hipError_t e = alloc_partitions(plan->m, plan->nsys, plan->nrow, pc, nullptr, nullptr,
                                                 &plan->cyc_part, &plan->cyc_redA, &plan->cyc_redC,
                                                 &plan->cyc_redD);
-                if (e != hipSuccess) return hip_fail(e, "btd_solve_cyclic: partition scratch");
+                if (e != hipSuccess) {  // free what was allocated; the next call tries again
+                    for (void** q : {(void**)&plan->cyc_part, (void**)&plan->cyc_redA, (void**)&plan->cyc_redC,
+                                     (void**)&plan->cyc_redD}) {
+                        if (*q) (void)hipFree(*q);
+                        *q = nullptr;
+                    }
+                    return hip_fail(e, "btd_solve_cyclic: partition scratch");
+                }
             }
             plan->cyc_npart = pc;
         }
