@@ -1,7 +1,7 @@
 // Cooperative block-Thomas rows: a system's block rows spread over m lanes of a wavefront
 // (lane i owns row i of every block), pivot search and row exchange through LDS. Used by the
-// mixed-size batch (btd_batch.h) and by the twisted solve of the two-stage reduced system
-// (btd_kernels.h). Storage: element e (column-major within the block) of row q of a lane's
+// mixed-size batch (btd_batch.h), the single-partition cooperative and twisted solves, stage 1
+// of the two-stage solve for m >= 7 and the twisted solve of its reduced system (btd_kernels.h). Storage: element e (column-major within the block) of row q of a lane's
 // system is at am.at(q, e) for blocks and av.at(q, e) for vectors; StridedAddr gives the
 // pack and tiled forms with compile-time strides, the plans' Addr the reference GPU layout.
 #pragma once
