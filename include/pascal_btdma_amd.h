@@ -90,7 +90,7 @@ btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npar
 /* New: the partition count a plan created with npart = 0 gets on one GPU, where partitions are
  * an occupancy choice rather than a rank count: 1 (the single-kernel solve) for nsys >= 32768
  * (nsys >= 4096 for m >= 7 and 8192 for m = 2, 4, whose single-partition solve then runs m or
- * 2m lanes per system), otherwise
+ * 2m lanes per system; and for m >= 5 systems of fewer than 128 rows), otherwise
  * about 65536 / nsys (one wave per SIMD), at most nrow / 8, and 1 when that is < 2.
  * Pure function (no device use). */
 int btd_auto_npart(int m, int nsys, int nrow);

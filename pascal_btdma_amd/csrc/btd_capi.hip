@@ -117,11 +117,14 @@ int btd_auto_npart(int m, int nsys, int nrow) {
     // wavefronts read whole halves or quarters of lines (floor(32/m) systems) and measured up
     // to 21% faster than two-stage from 8192 systems (m = 3, 5, 6 were not). A smaller batch
     // gets enough partitions for about one wave per SIMD (65536 lanes), keeping >= 8 rows per
-    // partition so that the reduced system of 2P rows stays small against the partition work.
+    // partition so that the reduced system of 2P rows stays small against the partition work;
+    // except systems of fewer than 128 rows at m >= 5, where the three launches and the reduced
+    // solve of the two-stage form cost more than the twisted solve's n/2 row steps (64 rows,
+    // 256-2048 systems: m = 5 1.2-1.65x, m = 8 0.96-1.43x faster single-partition).
     constexpr int64_t kLanes = 65536;
     if (nsys < 1 || nrow < 2) return 1;
     const int64_t one = m >= 7 ? 4096 : (m == 2 || m == 4) ? 8192 : kLanes / 2;
-    if ((int64_t)nsys >= one) return 1;
+    if ((int64_t)nsys >= one || (m >= 5 && nrow < 128)) return 1;
     const int64_t want = (kLanes + nsys - 1) / nsys;
     const int64_t most = nrow / 8;
     const int64_t p = want < most ? want : most;

@@ -84,6 +84,7 @@ def test_auto_npart_rule():
     assert auto(7, 4096, 2048) == 1 and auto(7, 2048, 2048) == 32 and auto(6, 4096, 2048) == 16
     assert auto(2, 8192, 512) == 1 and auto(4, 8192, 512) == 1 and auto(3, 8192, 512) == 8
     assert auto(4, 4096, 2048) == 16
+    assert auto(5, 1024, 64) == 1 and auto(8, 256, 127) == 1 and auto(4, 1024, 64) == 8  # short systems
     assert auto(3, 1024, 64) == 8
     assert auto(3, 1024, 15) == 1             # fewer than 2 partitions of 8 rows
     for m, nsys, nrow in [(2, 7, 40), (10, 3000, 4096), (5, 1, 2)]:

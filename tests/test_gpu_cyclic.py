@@ -82,7 +82,7 @@ def test_cyclic_single_partition_row_counts(cuda, oracle, m, n):
     assert rel_err(x, xo).max() <= TOL
 
 
-@pytest.mark.parametrize("m,nsys,n", [(5, 3000, 64), (8, 4100, 40), (3, 200, 33)])
+@pytest.mark.parametrize("m,nsys,n", [(3, 3000, 64), (8, 4100, 40), (3, 200, 33)])
 def test_cyclic_single_partition_partition_counts(cuda, oracle, m, nsys, n):
     """npart = 1 periodic solves whose library-chosen partition count is 8 (btd_auto_npart), 4
     (a batch the single-partition solve fills the device with) and 4 (33 rows)."""

@@ -138,7 +138,7 @@ def test_study_signatures_on_device(cuda, row):
         assert abs(x[pb.vector_index(0, z, 0, nsys, n)] - float(row[key])) < 1e-14
 
 
-@pytest.mark.parametrize("m,nsys,n,expect", [(3, 1024, 64, 8), (5, 700, 200, 25), (2, 40000, 6, 1)])
+@pytest.mark.parametrize("m,nsys,n,expect", [(3, 1024, 64, 8), (5, 700, 200, 25), (2, 40000, 6, 1), (5, 700, 64, 1)])
 def test_auto_npart_plan(cuda, oracle, m, nsys, n, expect):
     """A plan created with npart = 0 takes btd_auto_npart's count and still matches the oracle."""
     import torch
