@@ -227,6 +227,20 @@ btd_status btd_euler_sweep_cyclic(btd_plan_t plan, int axis, const double *U, co
                                   int n1, int n2, int n3, double h, double dt, double eps,
                                   double *work, double *out, void *stream);
 
+/* ---- weno2d implicit sweeps: assembly straight into the solve's layout ------------------
+ * Replaces compute_weno_coeffs_{x,y}_kernel + build_matrix_{x,y}_kernel_weno5
+ * (Fortran_Original/examples/weno2d/weno_mpi_gpu.f90:430-485, :504-719): writes the blocks a,
+ * b, c of the sweep's periodic system (I - gdt L) with m = 3 (three consecutive cells per block
+ * row), L the frozen-WENO5 upwind advection operator of a scalar field; the five stencil
+ * coefficients of each interface are computed in the kernel and never stored. U is the
+ * Fortran-ordered field with its three-cell halo filled, U(1-3:n1+3, 1-3:n2+3); vel[s] the
+ * advection speed of line s (x sweep, axis 0: c_x of row j = s+1, nsys = n2, nrow = n1/3;
+ * y sweep, axis 1: c_y of column i = s+1, nsys = n1, nrow = n2/3); vel >= 0 upwinds from the
+ * left. h is the cell width, gdt = gamma * dt. The plan must have m = 3 and that nsys and
+ * nrow. Asynchronous. No counterpart to the reference's API (its weno2d example's kernels). */
+btd_status btd_weno_assemble(btd_plan_t plan, int axis, const double *U, const double *vel, int n1, int n2,
+                             double h, double gdt, double *a, double *b, double *c, void *stream);
+
 /* Field F(n1, n2, n3, m) -> the sweep's vector d in the plan's layout (transpose_jk_kernel /
  * transpose_ik_kernel, :931-943, :976-988, for callers that assemble their own blocks). */
 btd_status btd_sweep_gather(btd_plan_t plan, int axis, const double *field, int n1, int n2, int n3,

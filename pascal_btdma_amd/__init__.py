@@ -23,7 +23,7 @@ __all__ = [
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
     "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
     "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "solve_transposed_f32r", "solve_noncyclic_f32s",
-    "outer_grads_f32", "euler_assemble", "euler_sweep_cyclic",
+    "outer_grads_f32", "euler_assemble", "euler_sweep_cyclic", "weno_assemble",
     "euler_sweep_work_elems", "sweep_gather", "sweep_scatter", "sweep_shape", "BtdmaBatch", "Graph",
 ]
 
@@ -380,6 +380,23 @@ def euler_assemble(plan: BtdmaGpuPlan, axis: int, U, rhs, n1: int, n2: int, n3: 
     _check(_capi.lib().btd_euler_assemble(plan._h, int(axis), _ptr(U), _ptr(rhs), int(n1), int(n2),
                                           int(n3), float(h), float(dt), float(eps), _ptr(a), _ptr(b),
                                           _ptr(c), _ptr(d), _stream(stream)))
+
+
+def weno_assemble(plan: BtdmaGpuPlan, axis: int, U, vel, n1: int, n2: int, h: float, gdt: float,
+                  a, b, c, stream=None) -> None:
+    """Blocks (m = 3) of the weno2d implicit sweep along `axis` (0 = x, 1 = y) from the scalar
+    field U(1-3:n1+3, 1-3:n2+3) with its halo (Fortran order, flat float64 device tensor) and
+    each line's advection speed `vel` (n2 values for x, n1 for y), in the plan's layout;
+    replaces the reference's WENO5 coefficient and build_matrix kernels."""
+    import torch
+    if plan.created:
+        nm = plan.matrix_elems()
+        _check_tensor("U", U, (n1 + 6) * (n2 + 6), torch.float64)
+        _check_tensor("vel", vel, n2 if axis == 0 else n1, torch.float64)
+        for name, t in (("a", a), ("b", b), ("c", c)):
+            _check_tensor(name, t, nm, torch.float64)
+    _check(_capi.lib().btd_weno_assemble(plan._h, int(axis), _ptr(U), _ptr(vel), int(n1), int(n2), float(h),
+                                         float(gdt), _ptr(a), _ptr(b), _ptr(c), _stream(stream)))
 
 
 def euler_sweep_work_elems(plan: BtdmaGpuPlan) -> int:

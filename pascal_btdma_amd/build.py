@@ -36,6 +36,7 @@ UNITS = [("btd_inst.hip", [f"-DBTD_M={m}"], f"btd_inst_m{m}.o") for m in range(1
     ("btd_batch.hip", [], "btd_batch.o"),
     ("btd_batch_api.hip", [], "btd_batch_api.o"),
     ("btd_euler.hip", [], "btd_euler.o"),
+    ("btd_weno.hip", [], "btd_weno.o"),
 ]
 
 

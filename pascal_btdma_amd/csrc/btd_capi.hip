@@ -368,6 +368,25 @@ btd_status btd_euler_assemble(btd_plan_t plan, int axis, const double* U, const 
     return BTD_OK;
 }
 
+btd_status btd_weno_assemble(btd_plan_t plan, int axis, const double* U, const double* vel, int n1, int n2,
+                             double h, double gdt, double* a, double* b, double* c, void* stream) {
+    g_err.clear();
+    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_weno_assemble: uncreated plan");
+    if (axis != 0 && axis != 1) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: axis must be 0 or 1");
+    if (n1 < 1 || n2 < 1) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: grid sizes must be >= 1");
+    if (plan->m != 3) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: the block rows are cell triples (m = 3)");
+    const int n = axis == 0 ? n1 : n2, nsys = axis == 0 ? n2 : n1;
+    if (n % 3 != 0) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: the sweep's cell count must be a multiple of 3");
+    if (plan->nrow != n / 3 || plan->nsys != nsys)
+        return fail(BTD_ERR_DIM_MISMATCH, "btd_weno_assemble: the plan's nrow/nsys do not match the sweep");
+    if (!U || !vel || !a || !b || !c) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: NULL array");
+    if (!(h > 0.0)) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: h must be > 0");
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_weno_assemble(plan->layout, axis, U, vel, n1, n2, h, gdt, a, b, c, plan->nsys, plan->nrow,
+                                      (hipStream_t)stream));
+    return BTD_OK;
+}
+
 btd_status btd_euler_sweep_cyclic(btd_plan_t plan, int axis, const double* U, const double* rhs,
                                   int n1, int n2, int n3, double h, double dt, double eps,
                                   double* work, double* out, void* stream) {

@@ -34,6 +34,11 @@ hipError_t launch_euler_assemble(int layout, int axis, const double* U, const do
                                  int n2, int n3, double cc, double cd, double* A, double* B,
                                  double* C, double* D, int nsys, int nrow, hipStream_t s);
 
+// weno2d sweep blocks (m = 3) from the haloed scalar field and each line's speed (btd_weno.hip).
+hipError_t launch_weno_assemble(int layout, int axis, const double* U, const double* vel, int n1, int n2,
+                                double h, double gdt, double* A, double* B, double* C, int nsys, int nrow,
+                                hipStream_t s);
+
 hipError_t launch_euler_sweep_cyclic(int layout, int axis, const double* U, const double* rhs,
                                      int n1, int n2, int n3, double cc, double cd, double* work,
                                      double* F, int nsys, int nrow, int* flag, hipStream_t s);

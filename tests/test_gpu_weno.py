@@ -1,0 +1,89 @@
+"""weno2d sweep assembly on the device (btd_weno_assemble) against the oracle
+(oracle/weno_assembly.py): the blocks bit for bit in both layouts and both sweeps, and the
+periodic solve of the assembled system against a dense solve of the scalar lines."""
+import numpy as np
+import pytest
+
+import pascal_btdma_amd as pb
+from helpers import layout_to_vec, nat_to_layout, rel_err
+from oracle import weno_assembly as wa
+
+pytestmark = pytest.mark.gpu
+N1, N2 = 54, 69  # x sweep: 69 systems of 18 block rows; y sweep: 54 systems of 23
+
+
+def setup(cuda, axis, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    U = np.pad(rng.uniform(-1, 1, (N2, N1)), wa.NHALO, mode="wrap")
+    U[wa.NHALO + 10:wa.NHALO + 20, wa.NHALO + 5:wa.NHALO + 9] += 2.0  # a plateau: nonlinear weights
+    U = np.pad(U[wa.NHALO:-wa.NHALO, wa.NHALO:-wa.NHALO], wa.NHALO, mode="wrap")
+    nsys = N2 if axis == 0 else N1
+    vel = rng.uniform(-2, 2, nsys)
+    vel[:3] = (0.0, -0.0, -1e-3)  # zero speeds take the left-biased stencil
+    tU = torch.from_numpy(U.ravel().copy()).to(cuda)
+    tv = torch.from_numpy(vel).to(cuda)
+    return U, vel, tU, tv
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_weno_blocks_match_oracle(cuda, axis, layout):
+    import torch
+    U, vel, tU, tv = setup(cuda, axis, 11 + axis)
+    h, gdt = 1.0 / N1, 0.37 / N1
+    n = N1 if axis == 0 else N2
+    nsys = N2 if axis == 0 else N1
+    with pb.BtdmaGpuPlan().create(3, nsys, n // 3, 1, layout) as plan:
+        a, b, c = (torch.zeros(plan.matrix_elems(), dtype=torch.float64, device=cuda) for _ in range(3))
+        pb.weno_assemble(plan, axis, tU, tv, N1, N2, h, gdt, a, b, c)
+        torch.cuda.synchronize()
+    ref = wa.assemble(U, vel, axis, N1, N2, h, gdt)
+    for got, X in zip((a, b, c), ref):
+        np.testing.assert_array_equal(got.cpu().numpy(), nat_to_layout(X, int(layout)))
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_weno_sweep_solve(cuda, axis):
+    """Assemble on the device, solve the periodic system (btd_solve_cyclic), compare with a dense
+    solve of each line's scalar operator."""
+    import torch
+    U, vel, tU, tv = setup(cuda, axis, 21 + axis)
+    h, gdt = 1.0 / N1, 0.5 / N1
+    n = N1 if axis == 0 else N2
+    nsys = N2 if axis == 0 else N1
+    rng = np.random.default_rng(5)
+    rhs = rng.uniform(-1, 1, (nsys, n))  # line s, cell i
+    lay = int(pb.Layout.TILED)
+    d_nat = rhs.reshape(nsys, n // 3, 3).transpose(1, 0, 2)  # (row, sys, p): cell 3 row + p
+    with pb.BtdmaGpuPlan().create(3, nsys, n // 3, 1, pb.Layout.TILED) as plan:
+        a, b, c = (torch.zeros(plan.matrix_elems(), dtype=torch.float64, device=cuda) for _ in range(3))
+        pb.weno_assemble(plan, axis, tU, tv, N1, N2, h, gdt, a, b, c)
+        d = torch.from_numpy(nat_to_layout(np.ascontiguousarray(d_nat), lay)).to(cuda)
+        pb.solve_cyclic(plan, a, b, c, d, 3, nsys, n // 3)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    x = layout_to_vec(d.cpu().numpy(), lay, n // 3, nsys, 3)  # (row, sys, p)
+    x = x.transpose(1, 0, 2).reshape(nsys, n)
+    T = wa.dense_operator(U, vel, axis, N1, N2, h, gdt)
+    xo = np.linalg.solve(T, rhs[..., None])[..., 0]
+    assert rel_err(x.T[:, :, None], xo.T[:, :, None]).max() <= 1e-12
+
+
+def test_weno_argument_errors(cuda):
+    import torch
+    t = torch.zeros(10, dtype=torch.float64, device=cuda)
+    with pb.BtdmaGpuPlan().create(3, N2, N1 // 3, 1, pb.Layout.TILED) as plan:
+        with pytest.raises(ValueError):  # axis
+            pb._check(pb._capi.lib().btd_weno_assemble(plan._h, 2, t.data_ptr(), t.data_ptr(), N1, N2, 1.0, 1.0,
+                                                       t.data_ptr(), t.data_ptr(), t.data_ptr(), None))
+        with pytest.raises(ValueError):  # the y sweep has other dimensions than this plan
+            pb._check(pb._capi.lib().btd_weno_assemble(plan._h, 1, t.data_ptr(), t.data_ptr(), N1, N2, 1.0, 1.0,
+                                                       t.data_ptr(), t.data_ptr(), t.data_ptr(), None))
+        with pytest.raises(ValueError):  # h <= 0
+            pb._check(pb._capi.lib().btd_weno_assemble(plan._h, 0, t.data_ptr(), t.data_ptr(), N1, N2, 0.0, 1.0,
+                                                       t.data_ptr(), t.data_ptr(), t.data_ptr(), None))
+    with pb.BtdmaGpuPlan().create(5, N2, N1 // 3, 1, pb.Layout.TILED) as plan:
+        with pytest.raises(ValueError):  # m must be 3
+            pb._check(pb._capi.lib().btd_weno_assemble(plan._h, 0, t.data_ptr(), t.data_ptr(), N1, N2, 1.0, 1.0,
+                                                       t.data_ptr(), t.data_ptr(), t.data_ptr(), None))
