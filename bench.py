@@ -654,6 +654,32 @@ def extra_euler(dev, steps: int, n: int = 256) -> dict:
     return out
 
 
+def extra_weno(dev, steps: int, n: int = 3072) -> dict:
+    """The weno2d example's implicit sweep assembly (btd_weno_assemble) on an n x n grid: one x
+    and one y sweep's m = 3 blocks from the haloed scalar field, per-line speeds of both signs,
+    tiled layout; device time per sweep and the rate of the block bytes written."""
+    import torch
+    import pascal_btdma_amd as pb
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    U = torch.rand((n + 6) * (n + 6), dtype=torch.float64, device=dev, generator=g)
+    vel = torch.linspace(-1.0, 1.0, n, dtype=torch.float64, device=dev)
+    out = {"workload": f"weno2d sweep assembly, {n}^2 grid, m=3, frozen WENO5, FP64"}
+    s = torch.cuda.current_stream()
+    for axis, name in ((0, "x"), (1, "y")):
+        with pb.BtdmaGpuPlan().create(3, n, n // 3, 1, pb.Layout.TILED) as plan:
+            a, b, c = (torch.empty(plan.matrix_elems(), dtype=torch.float64, device=dev) for _ in range(3))
+            run = lambda: pb.weno_assemble(plan, axis, U, vel, n, n, 1.0 / n, 0.5 / n, a, b, c)
+            run()
+            t = event_time(run, lambda: None, steps, s)
+            ms = float(np.median(t))
+            written = 3 * 9 * 8 * n * (n // 3)
+            out[f"sweep_{name}_ms"] = ms
+            out[f"sweep_{name}_blocks_written_GBs"] = written / (ms * 1e-3) / 1e9
+            del a, b, c
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -664,7 +690,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--only", default="",
-                    help="comma list of extras to run (study,config0,batch,f32r,adjoint,euler)")
+                    help="comma list of extras to run (study,config0,batch,f32r,adjoint,euler,weno)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--batch-probe", default="",
                     help="PLACEMENT,GRID: only solve the config-2 batch 3 times at these launch "
@@ -771,7 +797,8 @@ def main() -> int:
                 ("batch", lambda: extra_batch(dev, 3, checks)),
                 ("f32r", lambda: extra_f32r(dev, 3, ms, checks)),
                 ("adjoint", lambda: extra_adjoint(dev, 3)),
-                ("euler", lambda: extra_euler(dev, 3))]
+                ("euler", lambda: extra_euler(dev, 3)),
+                ("weno", lambda: extra_weno(dev, 5))]
         for name, job in jobs:
             if only and name not in only:
                 continue
