@@ -241,6 +241,16 @@ btd_status btd_euler_sweep_cyclic(btd_plan_t plan, int axis, const double *U, co
 btd_status btd_weno_assemble(btd_plan_t plan, int axis, const double *U, const double *vel, int n1, int n2,
                              double h, double gdt, double *a, double *b, double *c, void *stream);
 
+/* Scalar field F(n1, n2) (Fortran order, no halo) -> the weno2d sweep's vector d (m = 3, cell
+ * 3r + p + 1 of line s at element p of block row r of system s), in the plan's layout; and
+ * back. Replace pack_scalar_to_xblock_kernel / unpack_xblock_to_scalar_kernel
+ * (weno_mpi_gpu.f90:279-301) for the x sweep and, for the y sweep, the transposes of dU around
+ * it (transpose_xy_d_kernel / transpose_yx_d_kernel, :332-356). Asynchronous. */
+btd_status btd_weno_gather(btd_plan_t plan, int axis, const double *field, int n1, int n2, double *d,
+                           void *stream);
+btd_status btd_weno_scatter(btd_plan_t plan, int axis, const double *d, int n1, int n2, double *field,
+                            void *stream);
+
 /* Field F(n1, n2, n3, m) -> the sweep's vector d in the plan's layout (transpose_jk_kernel /
  * transpose_ik_kernel, :931-943, :976-988, for callers that assemble their own blocks). */
 btd_status btd_sweep_gather(btd_plan_t plan, int axis, const double *field, int n1, int n2, int n3,

@@ -23,7 +23,7 @@ __all__ = [
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
     "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
     "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "solve_transposed_f32r", "solve_noncyclic_f32s",
-    "outer_grads_f32", "euler_assemble", "euler_sweep_cyclic", "weno_assemble",
+    "outer_grads_f32", "euler_assemble", "euler_sweep_cyclic", "weno_assemble", "weno_gather", "weno_scatter",
     "euler_sweep_work_elems", "sweep_gather", "sweep_scatter", "sweep_shape", "BtdmaBatch", "Graph",
 ]
 
@@ -397,6 +397,25 @@ def weno_assemble(plan: BtdmaGpuPlan, axis: int, U, vel, n1: int, n2: int, h: fl
             _check_tensor(name, t, nm, torch.float64)
     _check(_capi.lib().btd_weno_assemble(plan._h, int(axis), _ptr(U), _ptr(vel), int(n1), int(n2), float(h),
                                          float(gdt), _ptr(a), _ptr(b), _ptr(c), _stream(stream)))
+
+
+def weno_gather(plan: BtdmaGpuPlan, axis: int, field, n1: int, n2: int, d, stream=None) -> None:
+    """Scalar field F(n1, n2) (Fortran order) -> the weno2d sweep's vector d (m = 3) in the plan's
+    layout."""
+    import torch
+    if plan.created:
+        _check_tensor("field", field, n1 * n2, torch.float64)
+        _check_tensor("d", d, plan.vector_elems(), torch.float64)
+    _check(_capi.lib().btd_weno_gather(plan._h, int(axis), _ptr(field), int(n1), int(n2), _ptr(d), _stream(stream)))
+
+
+def weno_scatter(plan: BtdmaGpuPlan, axis: int, d, n1: int, n2: int, field, stream=None) -> None:
+    """The weno2d sweep's vector d -> the scalar field F(n1, n2)."""
+    import torch
+    if plan.created:
+        _check_tensor("field", field, n1 * n2, torch.float64)
+        _check_tensor("d", d, plan.vector_elems(), torch.float64)
+    _check(_capi.lib().btd_weno_scatter(plan._h, int(axis), _ptr(d), int(n1), int(n2), _ptr(field), _stream(stream)))
 
 
 def euler_sweep_work_elems(plan: BtdmaGpuPlan) -> int:

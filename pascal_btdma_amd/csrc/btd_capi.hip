@@ -368,23 +368,53 @@ btd_status btd_euler_assemble(btd_plan_t plan, int axis, const double* U, const 
     return BTD_OK;
 }
 
+// The weno2d sweep along `axis` of an n1 x n2 grid against the plan (m = 3, cell triples).
+static btd_status check_weno(btd_plan_t plan, int axis, int n1, int n2, const char* what) {
+    const std::string w(what);
+    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, w + ": uncreated plan");
+    if (axis != 0 && axis != 1) return fail(BTD_ERR_INVALID_ARG, w + ": axis must be 0 or 1");
+    if (n1 < 1 || n2 < 1) return fail(BTD_ERR_INVALID_ARG, w + ": grid sizes must be >= 1");
+    if (plan->m != 3) return fail(BTD_ERR_INVALID_ARG, w + ": the block rows are cell triples (m = 3)");
+    const int n = axis == 0 ? n1 : n2, nsys = axis == 0 ? n2 : n1;
+    if (n % 3 != 0) return fail(BTD_ERR_INVALID_ARG, w + ": the sweep's cell count must be a multiple of 3");
+    if (plan->nrow != n / 3 || plan->nsys != nsys)
+        return fail(BTD_ERR_DIM_MISMATCH, w + ": the plan's nrow/nsys do not match the sweep");
+    return BTD_OK;
+}
+
 btd_status btd_weno_assemble(btd_plan_t plan, int axis, const double* U, const double* vel, int n1, int n2,
                              double h, double gdt, double* a, double* b, double* c, void* stream) {
     g_err.clear();
-    if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_weno_assemble: uncreated plan");
-    if (axis != 0 && axis != 1) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: axis must be 0 or 1");
-    if (n1 < 1 || n2 < 1) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: grid sizes must be >= 1");
-    if (plan->m != 3) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: the block rows are cell triples (m = 3)");
-    const int n = axis == 0 ? n1 : n2, nsys = axis == 0 ? n2 : n1;
-    if (n % 3 != 0) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: the sweep's cell count must be a multiple of 3");
-    if (plan->nrow != n / 3 || plan->nsys != nsys)
-        return fail(BTD_ERR_DIM_MISMATCH, "btd_weno_assemble: the plan's nrow/nsys do not match the sweep");
+    btd_status st = check_weno(plan, axis, n1, n2, "btd_weno_assemble");
+    if (st != BTD_OK) return st;
     if (!U || !vel || !a || !b || !c) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: NULL array");
     if (!(h > 0.0)) return fail(BTD_ERR_INVALID_ARG, "btd_weno_assemble: h must be > 0");
     plan->last_stream = (hipStream_t)stream;
     BTD_HIP(btd::launch_weno_assemble(plan->layout, axis, U, vel, n1, n2, h, gdt, a, b, c, plan->nsys, plan->nrow,
                                       (hipStream_t)stream));
     return BTD_OK;
+}
+
+static btd_status weno_move(btd_plan_t plan, int axis, bool to_d, double* field, double* d, int n1, int n2,
+                            void* stream, const char* what) {
+    g_err.clear();
+    btd_status st = check_weno(plan, axis, n1, n2, what);
+    if (st != BTD_OK) return st;
+    if (!field || !d) return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": NULL array");
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_weno_move(plan->layout, axis, to_d, field, d, n1, plan->nsys, plan->nrow,
+                                  (hipStream_t)stream));
+    return BTD_OK;
+}
+
+btd_status btd_weno_gather(btd_plan_t plan, int axis, const double* field, int n1, int n2, double* d,
+                           void* stream) {
+    return weno_move(plan, axis, true, const_cast<double*>(field), d, n1, n2, stream, "btd_weno_gather");
+}
+
+btd_status btd_weno_scatter(btd_plan_t plan, int axis, const double* d, int n1, int n2, double* field,
+                            void* stream) {
+    return weno_move(plan, axis, false, field, const_cast<double*>(d), n1, n2, stream, "btd_weno_scatter");
 }
 
 btd_status btd_euler_sweep_cyclic(btd_plan_t plan, int axis, const double* U, const double* rhs,

@@ -38,6 +38,9 @@ hipError_t launch_euler_assemble(int layout, int axis, const double* U, const do
 hipError_t launch_weno_assemble(int layout, int axis, const double* U, const double* vel, int n1, int n2,
                                 double h, double gdt, double* A, double* B, double* C, int nsys, int nrow,
                                 hipStream_t s);
+// weno2d scalar field F(n1, n2) <-> sweep vector (m = 3) in the plan's layout (btd_weno.hip).
+hipError_t launch_weno_move(int layout, int axis, bool to_d, double* F, double* D, int n1, int nsys, int nrow,
+                            hipStream_t s);
 
 hipError_t launch_euler_sweep_cyclic(int layout, int axis, const double* U, const double* rhs,
                                      int n1, int n2, int n3, double cc, double cd, double* work,

@@ -146,7 +146,51 @@ hipError_t weno_l(int axis, const double* U, const double* vel, int n1, int n2, 
     return weno_la<LAYOUT, 1>(U, vel, n1, n2, h, gdt, A, B, C, nsys, nrow, s);
 }
 
+// Field F(n1, n2) (Fortran order, no halo) <-> the sweep's vector d in the solve's layout:
+// element p of block row r of line sys is cell 3r + p + 1 of that line (pack_scalar_to_xblock
+// / unpack_xblock_to_scalar, weno_mpi_gpu.f90:279-301, and for the y sweep the transposes
+// around it, :332-356). One lane per (line, block row); TO_D gathers, else scatters.
+template <int LAYOUT, int AXIS, bool TO_D>
+__global__ __launch_bounds__(64) void k_weno_move(double* __restrict__ F, double* __restrict__ D, int n1,
+                                                  int64_t nsys, int64_t nrow) {
+    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (sys >= nsys) return;
+    const int64_t r0 = (int64_t)blockIdx.y * kWenoRows;
+    const int64_t r1 = r0 + kWenoRows < nrow ? r0 + kWenoRows : nrow;
+    const Addr<LAYOUT> av(sys, nsys, nrow, 3);
+    for (int64_t r = r0; r < r1; ++r) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const int64_t c = 3 * r + p;  // 0-based cell along the line
+            const int64_t f = AXIS == 0 ? c + (int64_t)n1 * sys : sys + (int64_t)n1 * c;
+            if constexpr (TO_D)
+                D[av.at(r, p)] = F[f];
+            else
+                F[f] = D[av.at(r, p)];
+        }
+    }
+}
+
+template <int LAYOUT, int AXIS>
+hipError_t move_la(bool to_d, double* F, double* D, int n1, int64_t nsys, int64_t nrow, hipStream_t s) {
+    const dim3 grid((unsigned)((nsys + 63) / 64), (unsigned)((nrow + kWenoRows - 1) / kWenoRows));
+    if (to_d)
+        hipLaunchKernelGGL((k_weno_move<LAYOUT, AXIS, true>), grid, dim3(64), 0, s, F, D, n1, nsys, nrow);
+    else
+        hipLaunchKernelGGL((k_weno_move<LAYOUT, AXIS, false>), grid, dim3(64), 0, s, F, D, n1, nsys, nrow);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_weno_move(int layout, int axis, bool to_d, double* F, double* D, int n1, int nsys, int nrow,
+                            hipStream_t s) {
+    if (layout == LAYOUT_TILED)
+        return axis == 0 ? move_la<LAYOUT_TILED, 0>(to_d, F, D, n1, nsys, nrow, s)
+                         : move_la<LAYOUT_TILED, 1>(to_d, F, D, n1, nsys, nrow, s);
+    return axis == 0 ? move_la<LAYOUT_REF_GPU, 0>(to_d, F, D, n1, nsys, nrow, s)
+                     : move_la<LAYOUT_REF_GPU, 1>(to_d, F, D, n1, nsys, nrow, s);
+}
 
 hipError_t launch_weno_assemble(int layout, int axis, const double* U, const double* vel, int n1, int n2,
                                 double h, double gdt, double* A, double* B, double* C, int nsys, int nrow,

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import pascal_btdma_amd as pb
-from helpers import layout_to_vec, nat_to_layout, rel_err
+from helpers import nat_to_layout, rel_err
 from oracle import weno_assembly as wa
 
 pytestmark = pytest.mark.gpu
@@ -45,8 +45,9 @@ def test_weno_blocks_match_oracle(cuda, axis, layout):
 
 @pytest.mark.parametrize("axis", [0, 1])
 def test_weno_sweep_solve(cuda, axis):
-    """Assemble on the device, solve the periodic system (btd_solve_cyclic), compare with a dense
-    solve of each line's scalar operator."""
+    """Gather the right-hand side field, assemble, solve the periodic system (btd_solve_cyclic)
+    and scatter the solution back, all on the device; compare with a dense solve of each line's
+    scalar operator."""
     import torch
     U, vel, tU, tv = setup(cuda, axis, 21 + axis)
     h, gdt = 1.0 / N1, 0.5 / N1
@@ -54,20 +55,44 @@ def test_weno_sweep_solve(cuda, axis):
     nsys = N2 if axis == 0 else N1
     rng = np.random.default_rng(5)
     rhs = rng.uniform(-1, 1, (nsys, n))  # line s, cell i
-    lay = int(pb.Layout.TILED)
-    d_nat = rhs.reshape(nsys, n // 3, 3).transpose(1, 0, 2)  # (row, sys, p): cell 3 row + p
+    F = rhs if axis == 0 else rhs.T  # the field F(n1, n2) as [j, i]
     with pb.BtdmaGpuPlan().create(3, nsys, n // 3, 1, pb.Layout.TILED) as plan:
         a, b, c = (torch.zeros(plan.matrix_elems(), dtype=torch.float64, device=cuda) for _ in range(3))
         pb.weno_assemble(plan, axis, tU, tv, N1, N2, h, gdt, a, b, c)
-        d = torch.from_numpy(nat_to_layout(np.ascontiguousarray(d_nat), lay)).to(cuda)
+        tf = torch.from_numpy(np.ascontiguousarray(F).ravel()).to(cuda)
+        d = torch.empty(plan.vector_elems(), dtype=torch.float64, device=cuda)
+        pb.weno_gather(plan, axis, tf, N1, N2, d)
         pb.solve_cyclic(plan, a, b, c, d, 3, nsys, n // 3)
+        pb.weno_scatter(plan, axis, d, N1, N2, tf)
         torch.cuda.synchronize()
         assert not plan.check()
-    x = layout_to_vec(d.cpu().numpy(), lay, n // 3, nsys, 3)  # (row, sys, p)
-    x = x.transpose(1, 0, 2).reshape(nsys, n)
+    xf = tf.cpu().numpy().reshape(N2, N1)
+    x = xf if axis == 0 else xf.T  # [line, cell]
     T = wa.dense_operator(U, vel, axis, N1, N2, h, gdt)
     xo = np.linalg.solve(T, rhs[..., None])[..., 0]
     assert rel_err(x.T[:, :, None], xo.T[:, :, None]).max() <= 1e-12
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_weno_gather_scatter(cuda, axis, layout):
+    """Field F(n1, n2) -> the sweep's vector (cell 3r + p of line s at element p of block row
+    r) and back, bit for bit."""
+    import torch
+    rng = np.random.default_rng(40 + axis)
+    F = rng.uniform(-1, 1, (N2, N1))  # [j, i] = Fortran F(n1, n2)
+    lines = F if axis == 0 else F.T   # [line, cell]
+    nsys, n = lines.shape
+    nat = np.ascontiguousarray(lines.reshape(nsys, n // 3, 3).transpose(1, 0, 2))  # (row, sys, p)
+    with pb.BtdmaGpuPlan().create(3, nsys, n // 3, 1, layout) as plan:
+        tf = torch.from_numpy(F.ravel().copy()).to(cuda)
+        d = torch.zeros(plan.vector_elems(), dtype=torch.float64, device=cuda)
+        pb.weno_gather(plan, axis, tf, N1, N2, d)
+        back = torch.zeros_like(tf)
+        pb.weno_scatter(plan, axis, d, N1, N2, back)
+        torch.cuda.synchronize()
+    np.testing.assert_array_equal(d.cpu().numpy(), nat_to_layout(nat, int(layout)))
+    np.testing.assert_array_equal(back.cpu().numpy(), F.ravel())
 
 
 def test_weno_argument_errors(cuda):
