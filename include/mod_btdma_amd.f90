@@ -66,11 +66,28 @@ module mod_btdma_amd
             integer(c_int), value :: axis, n1, n2, n3
             real(c_double), value :: h, dt, eps
         end function
+        integer(c_int) function btd_weno_assemble(plan, axis, U, vel, n1, n2, h, gdt, a, b, c, stream) bind(C)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: plan, U, vel, a, b, c, stream
+            integer(c_int), value :: axis, n1, n2
+            real(c_double), value :: h, gdt
+        end function
+        integer(c_int) function btd_weno_gather(plan, axis, field, n1, n2, d, stream) bind(C)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: plan, field, d, stream
+            integer(c_int), value :: axis, n1, n2
+        end function
+        integer(c_int) function btd_weno_scatter(plan, axis, d, n1, n2, field, stream) bind(C)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: plan, d, field, stream
+            integer(c_int), value :: axis, n1, n2
+        end function
     end interface
 
     public :: btdma_makeplan_amd, btdma_many_amd, btdma_many_amd_profiled
     public :: btdma_many_cycl_amd, btdma_cleanplan_amd
     public :: btdma_euler_assemble_amd, btdma_sweep_scatter_amd, btdma_euler_sweep_cyclic_amd
+    public :: btdma_weno_assemble_amd, btdma_weno_gather_amd, btdma_weno_scatter_amd
 
 contains
 
@@ -143,6 +160,32 @@ contains
         real(c_double), intent(in) :: h, dt, eps
         call check(btd_euler_sweep_cyclic(plan, axis, U, dU, n1, n2, n3, h, dt, eps, work, out, c_null_ptr), &
                    'btdma_euler_sweep_cyclic_amd')
+    end subroutine
+
+    ! compute_weno_coeffs_{x,y}_kernel + build_matrix_{x,y}_kernel_weno5 (weno_mpi_gpu.f90:430-719):
+    ! the weno2d sweep's m = 3 blocks A, B, C in the plan's layout from U(1-3:n1+3, 1-3:n2+3) and
+    ! each line's speed vel (c_x(j) for axis 0, c_y(i) for axis 1); gdt = gamma*dt
+    subroutine btdma_weno_assemble_amd(plan, axis, U, vel, n1, n2, h, gdt, A, B, C)
+        type(c_ptr), intent(in) :: plan, U, vel, A, B, C
+        integer, intent(in) :: axis, n1, n2
+        real(c_double), intent(in) :: h, gdt
+        call check(btd_weno_assemble(plan, axis, U, vel, n1, n2, h, gdt, A, B, C, c_null_ptr), &
+                   'btdma_weno_assemble_amd')
+    end subroutine
+
+    ! pack_scalar_to_xblock_kernel / the transpose before the y sweep (weno_mpi_gpu.f90:279-289,
+    ! 332-343): the field F(n1, n2) as the sweep's vector D
+    subroutine btdma_weno_gather_amd(plan, axis, F, n1, n2, D)
+        type(c_ptr), intent(in) :: plan, F, D
+        integer, intent(in) :: axis, n1, n2
+        call check(btd_weno_gather(plan, axis, F, n1, n2, D, c_null_ptr), 'btdma_weno_gather_amd')
+    end subroutine
+
+    ! unpack_xblock_to_scalar_kernel / the transpose after the y sweep (:291-301, 345-356)
+    subroutine btdma_weno_scatter_amd(plan, axis, D, n1, n2, F)
+        type(c_ptr), intent(in) :: plan, D, F
+        integer, intent(in) :: axis, n1, n2
+        call check(btd_weno_scatter(plan, axis, D, n1, n2, F, c_null_ptr), 'btdma_weno_scatter_amd')
     end subroutine
 
     ! btdma_cleanplan_gpu_v2(plan) (mod_btdma_gpu_v2.f90:111)
