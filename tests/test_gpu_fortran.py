@@ -1,7 +1,8 @@
 """Fortran callers of the library through the ISO_C_BINDING module include/mod_btdma_amd.f90,
 run on the device: tests/fortran/sample_btdma_amd.f90 against the reference's published Study
 signature, tests/fortran/sample_euler_amd.f90 through an euler3d ADI step (periodic residual of
-every assembled system, assembled vs matrix-free sweeps)."""
+every assembled system, assembled vs matrix-free sweeps), tests/fortran/sample_weno_amd.f90
+through a weno2d stage's x and y sweeps (row sums, periodic residuals, conserved mass)."""
 import os
 import subprocess
 
@@ -11,7 +12,7 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran", "bin")
 
 
-@pytest.mark.parametrize("name", ["sample_btdma_amd", "sample_euler_amd"])
+@pytest.mark.parametrize("name", ["sample_btdma_amd", "sample_euler_amd", "sample_weno_amd"])
 def test_fortran_sample(cuda, name):
     exe = os.path.join(BIN, name)
     if not os.path.exists(exe):
