@@ -715,7 +715,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(independent systems shard across GPUs with no communication; see DESIGN.md)",
+              "(independent systems shard across GPUs with no communication)",
               file=sys.stderr)
         return 2
 
