@@ -655,9 +655,10 @@ def extra_euler(dev, steps: int, n: int = 256) -> dict:
 
 
 def extra_weno(dev, steps: int, n: int = 3072) -> dict:
-    """The weno2d example's implicit sweep assembly (btd_weno_assemble) on an n x n grid: one x
-    and one y sweep's m = 3 blocks from the haloed scalar field, per-line speeds of both signs,
-    tiled layout; device time per sweep and the rate of the block bytes written."""
+    """The weno2d example's implicit sweeps on an n x n grid, tiled layout, per-line speeds of
+    both signs: the assembly of one x and one y sweep's m = 3 blocks from the haloed scalar
+    field (btd_weno_assemble: device time and the rate of the block bytes written), and the
+    whole sweep (gather, assemble, periodic solve over the plan's partitions, scatter)."""
     import torch
     import pascal_btdma_amd as pb
     g = torch.Generator(device=dev)
@@ -676,7 +677,21 @@ def extra_weno(dev, steps: int, n: int = 3072) -> dict:
             written = 3 * 9 * 8 * n * (n // 3)
             out[f"sweep_{name}_ms"] = ms
             out[f"sweep_{name}_blocks_written_GBs"] = written / (ms * 1e-3) / 1e9
-            del a, b, c
+            # the whole sweep: field -> vector, blocks, periodic solve, vector -> field
+            d = torch.empty(plan.vector_elems(), dtype=torch.float64, device=dev)
+            F = torch.rand(n * n, dtype=torch.float64, device=dev, generator=g)
+
+            def sweep():
+                pb.weno_gather(plan, axis, F, n, n, d)
+                pb.weno_assemble(plan, axis, U, vel, n, n, 1.0 / n, 0.5 / n, a, b, c)
+                pb.solve_cyclic(plan, a, b, c, d, 3, n, n // 3)
+                pb.weno_scatter(plan, axis, d, n, n, F)
+            sweep()
+            t = event_time(sweep, lambda: None, steps, s)
+            out[f"sweep_{name}_pipeline_ms"] = float(np.median(t))
+            if plan.check():
+                out[f"sweep_{name}_singular"] = True
+            del a, b, c, d, F
     return out
 
 
