@@ -1,7 +1,7 @@
 // The reference's C++ example (CUDA_CXX_Port/examples/ex_btdma_noncyclic.cu) ported to the
 // C++ mirror header: the Study's diagonal workload (A = 0.25 I, B = 2 I, C = 0.25 I, d = 1),
-// 32 x 32 systems of 128 block rows, solved with npart = 1, npart = 4 and the partition count
-// the library chooses (npart = 0), checked against
+// 32 x 32 systems of 128 block rows, solved with npart = 1, npart = 4, the partition count
+// the library chooses (npart = 0) and npart = 1 in the twisted form (plan.set_form), checked against
 // the reference's published signature (sum, L2, end and middle samples) and against the
 // error behaviour of the reference (invalid sizes, dimension mismatch, uncreated plan).
 // Usage: ex_btdma_noncyclic <m> <expected_sum> <expected_l2>
@@ -45,13 +45,20 @@ int main(int argc, char** argv) {
     HIPCK(hipMalloc(&c, ms * 8));
     HIPCK(hipMalloc(&d, vs * 8));
     int failures = 0;
-    for (int npart : {1, 4, 0}) {
+    for (int npart : {1, 4, 0, -1}) {  // -1: npart = 1, always twisted (plan.set_form)
         HIPCK(hipMemcpy(a, ha.data(), ms * 8, hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(b, hb.data(), ms * 8, hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(c, hc.data(), ms * 8, hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(d, hd.data(), vs * 8, hipMemcpyHostToDevice));
         pascal_btdma::BtdmaGpuPlan plan;
-        plan.create(m, nsys, nrow, npart);
+        plan.create(m, nsys, nrow, npart < 0 ? 1 : npart);
+        if (npart < 0) {
+            plan.set_form(BTD_FORM_TWISTED);
+            if (plan.form() != BTD_FORM_TWISTED) {
+                std::printf("set_form: the plan's form did not change\n");
+                ++failures;
+            }
+        }
         if (npart == 0 && plan.npart != btd_auto_npart(m, nsys, nrow)) {
             std::printf("npart=0: plan.npart %d, btd_auto_npart %d\n", plan.npart, btd_auto_npart(m, nsys, nrow));
             ++failures;
@@ -70,8 +77,8 @@ int main(int argc, char** argv) {
         const bool ok = std::fabs(s / want_sum - 1) < 1e-9 && std::fabs(std::sqrt(s2) / want_l2 - 1) < 1e-9 &&
                         std::fabs(x0 - 0.4508066615170332) < 1e-14 && std::fabs(xm - 0.4) < 1e-14 &&
                         t.communication() == 0.0 && t.computation() > 0.0;
-        std::printf("m=%d npart=%d sum=%.16g l2=%.16g x0=%.16g xmid=%.16g total=%.3es %s\n", m, plan.npart, s,
-                    std::sqrt(s2), x0, xm, t.total, ok ? "OK" : "MISMATCH");
+        std::printf("m=%d npart=%d%s sum=%.16g l2=%.16g x0=%.16g xmid=%.16g total=%.3es %s\n", m, plan.npart,
+                    npart < 0 ? " (twisted)" : "", s, std::sqrt(s2), x0, xm, t.total, ok ? "OK" : "MISMATCH");
         failures += ok ? 0 : 1;
     }
     // error behaviour of the reference (pascal_btdma_cuda.cu:865-870,962-967,1063-1075)
