@@ -24,6 +24,7 @@ namespace {
 constexpr double kW0 = 1.0 / 10.0, kW1 = 6.0 / 10.0, kW2 = 3.0 / 10.0, kWenoEps = 1.0e-6;
 constexpr int kWenoHalo = 3;
 constexpr int kWenoRows = 16;  // block rows per lane
+constexpr int kWenoTile = 3 * kWenoRows + 7;  // cells of a staged line (+1: fewer LDS bank conflicts)
 
 // Left-biased WENO5 reconstruction at the interface to the right of g2 from the cells
 // g0..g4: the coefficient of each cell (weno5_coeffs_left, :381-398). The right-biased form
@@ -55,14 +56,27 @@ __global__ __launch_bounds__(64) void k_weno_assemble(const double* __restrict__
                                                       double* __restrict__ C, int64_t nsys, int64_t nrow) {
     // the reference's operation order, unfused, so that the blocks are the oracle's bit for bit
 #pragma clang fp contract(off)
-    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (sys >= nsys) return;
+    const int64_t s0 = (int64_t)blockIdx.x * 64, sys = s0 + threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.y * kWenoRows;
     const int64_t r1 = r0 + kWenoRows < nrow ? r0 + kWenoRows : nrow;
     const int64_t ld = (int64_t)n1 + 2 * kWenoHalo;  // leading dimension of U
+    // x sweep: the lines are rows of U, so the wavefront's 64 lines are staged through LDS,
+    // each row's cells 3 r0 - 2 .. 3 r1 + 3 loaded by consecutive lanes (whole lines of U)
+    __shared__ double tile[AXIS == 0 ? 64 : 1][kWenoTile];
+    if constexpr (AXIS == 0) {
+        const int ncell = 3 * (int)(r1 - r0) + 6;  // <= 54 < 64: one cell per lane and row
+        const int t = threadIdx.x;
+        const double* row = U + 3 * r0 + t + ld * (s0 + kWenoHalo);
+        const int rows = nsys - s0 < 64 ? (int)(nsys - s0) : 64;
+        if (t < ncell)
+            for (int jj = 0; jj < rows; ++jj) tile[jj][t] = row[ld * jj];
+        __syncthreads();
+    }
+    if (sys >= nsys) return;
     // cell c (1-based, -2..n+3) of this line
     auto cell = [&](int64_t c) {
-        return AXIS == 0 ? U[(c + 2) + ld * (sys + kWenoHalo)] : U[(sys + kWenoHalo) + ld * (c + 2)];
+        if constexpr (AXIS == 0) return tile[threadIdx.x][c - (3 * r0 - 2)];
+        return U[(sys + kWenoHalo) + ld * (c + 2)];
     };
     const double v = vel[sys];
     const bool left = v >= 0.0;
