@@ -82,10 +82,14 @@ __global__ __launch_bounds__(64) void k_weno_assemble(const double* __restrict__
     const bool left = v >= 0.0;
     const double scale = (gdt * v) / h;
     const Addr<LAYOUT> am(sys, nsys, nrow, 9);
-    for (int64_t r = r0; r < r1; ++r) {
-        double f[9];  // cells 3r-2 .. 3r+6 (1-based)
+    double f[9];  // cells 3r-2 .. 3r+6 (1-based) of block row r: a window sliding by three cells
 #pragma unroll
-        for (int t = 0; t < 9; ++t) f[t] = cell(3 * r - 2 + t);
+    for (int t = 0; t < 6; ++t) f[t + 3] = cell(3 * r0 - 2 + t);
+    for (int64_t r = r0; r < r1; ++r) {
+#pragma unroll
+        for (int t = 0; t < 6; ++t) f[t] = f[t + 3];
+#pragma unroll
+        for (int t = 6; t < 9; ++t) f[t] = cell(3 * r - 2 + t);
         double cf[4][5];  // interfaces 3r .. 3r+3 (interface k to the right of cell k)
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
