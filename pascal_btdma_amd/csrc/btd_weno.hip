@@ -167,15 +167,41 @@ hipError_t weno_l(int axis, const double* U, const double* vel, int n1, int n2, 
 // Field F(n1, n2) (Fortran order, no halo) <-> the sweep's vector d in the solve's layout:
 // element p of block row r of line sys is cell 3r + p + 1 of that line (pack_scalar_to_xblock
 // / unpack_xblock_to_scalar, weno_mpi_gpu.f90:279-301, and for the y sweep the transposes
-// around it, :332-356). One lane per (line, block row); TO_D gathers, else scatters.
+// around it, :332-356). One lane per line and kWenoRows block rows; TO_D gathers, else scatters.
 template <int LAYOUT, int AXIS, bool TO_D>
 __global__ __launch_bounds__(64) void k_weno_move(double* __restrict__ F, double* __restrict__ D, int n1,
                                                   int64_t nsys, int64_t nrow) {
-    const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (sys >= nsys) return;
+    const int64_t s0 = (int64_t)blockIdx.x * 64, sys = s0 + threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.y * kWenoRows;
     const int64_t r1 = r0 + kWenoRows < nrow ? r0 + kWenoRows : nrow;
     const Addr<LAYOUT> av(sys, nsys, nrow, 3);
+    if constexpr (AXIS == 0) {
+        // the lines are rows of F: staged through LDS so that F is read or written in whole
+        // runs of cells by consecutive lanes
+        __shared__ double tile[64][3 * kWenoRows + 1];
+        const int nc = 3 * (int)(r1 - r0), t = threadIdx.x;
+        const int rows = nsys - s0 < 64 ? (int)(nsys - s0) : 64;
+        double* row = F + 3 * r0 + t + (int64_t)n1 * s0;
+        if constexpr (TO_D) {
+            if (t < nc)
+                for (int jj = 0; jj < rows; ++jj) tile[jj][t] = row[(int64_t)n1 * jj];
+            __syncthreads();
+            if (sys < nsys)
+                for (int64_t r = r0; r < r1; ++r)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p) D[av.at(r, p)] = tile[t][3 * (r - r0) + p];
+        } else {
+            if (sys < nsys)
+                for (int64_t r = r0; r < r1; ++r)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p) tile[t][3 * (r - r0) + p] = D[av.at(r, p)];
+            __syncthreads();
+            if (t < nc)
+                for (int jj = 0; jj < rows; ++jj) row[(int64_t)n1 * jj] = tile[jj][t];
+        }
+        return;
+    }
+    if (sys >= nsys) return;
     for (int64_t r = r0; r < r1; ++r) {
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
