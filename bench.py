@@ -658,7 +658,8 @@ def extra_weno(dev, steps: int, n: int = 3072) -> dict:
     """The weno2d example's implicit sweeps on an n x n grid, tiled layout, per-line speeds of
     both signs: the assembly of one x and one y sweep's m = 3 blocks from the haloed scalar
     field (btd_weno_assemble: device time and the rate of the block bytes written), and the
-    whole sweep (gather, assemble, periodic solve over the plan's partitions, scatter)."""
+    whole sweep (gather, assemble, periodic solve over the plan's partitions, scatter), assembled
+    and matrix-free (btd_weno_sweep_cyclic: the blocks computed inside the solve's stage 1)."""
     import torch
     import pascal_btdma_amd as pb
     g = torch.Generator(device=dev)
@@ -689,9 +690,19 @@ def extra_weno(dev, steps: int, n: int = 3072) -> dict:
             sweep()
             t = event_time(sweep, lambda: None, steps, s)
             out[f"sweep_{name}_pipeline_ms"] = float(np.median(t))
+            # the same sweep matrix-free: the blocks computed inside stage 1, never stored
+            work = torch.empty(2 * plan.matrix_elems(), dtype=torch.float64, device=dev)
+
+            def sweep_mf():
+                pb.weno_gather(plan, axis, F, n, n, d)
+                pb.weno_sweep_cyclic(plan, axis, U, vel, n, n, 1.0 / n, 0.5 / n, work, d)
+                pb.weno_scatter(plan, axis, d, n, n, F)
+            sweep_mf()
+            t = event_time(sweep_mf, lambda: None, steps, s)
+            out[f"sweep_{name}_matrix_free_ms"] = float(np.median(t))
             if plan.check():
                 out[f"sweep_{name}_singular"] = True
-            del a, b, c, d, F
+            del a, b, c, d, F, work
     return out
 
 

@@ -241,6 +241,19 @@ btd_status btd_euler_sweep_cyclic(btd_plan_t plan, int axis, const double *U, co
 btd_status btd_weno_assemble(btd_plan_t plan, int axis, const double *U, const double *vel, int n1, int n2,
                              double h, double gdt, double *a, double *b, double *c, void *stream);
 
+/* Matrix-free periodic weno2d sweep (no reference counterpart; it does what
+ * build_matrix_{x,y}_kernel_weno5 and btdma_many_cycl_mpi_gpu_v2 do at weno_mpi_gpu.f90:1020,
+ * :1042/:1058, without storing A, B, C): solves (I - gdt L) x = d for the sweep along `axis`,
+ * the two-stage periodic solve with stage 1 computing each block row from U and vel as
+ * btd_weno_assemble would write it (so the result equals btd_weno_assemble followed by
+ * btd_solve_cyclic bit for bit). d: the right-hand side in the plan's layout
+ * (btd_weno_gather), replaced by x. work: device scratch of 2 * btd_layout_elems(layout,
+ * nsys, nrow, 9) doubles. npart > 1 plans use their partitions, npart = 1 plans those of
+ * btd_solve_cyclic (allocated by the first periodic call; needs nrow >= 6). Zero pivots are
+ * recorded in the plan. Asynchronous. */
+btd_status btd_weno_sweep_cyclic(btd_plan_t plan, int axis, const double *U, const double *vel, int n1, int n2,
+                                 double h, double gdt, double *work, double *d, void *stream);
+
 /* Scalar field F(n1, n2) (Fortran order, no halo) -> the weno2d sweep's vector d (m = 3, cell
  * 3r + p + 1 of line s at element p of block row r of system s), in the plan's layout; and
  * back. Replace pack_scalar_to_xblock_kernel / unpack_xblock_to_scalar_kernel

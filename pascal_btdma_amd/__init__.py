@@ -23,7 +23,7 @@ __all__ = [
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
     "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
     "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "solve_transposed_f32r", "solve_noncyclic_f32s",
-    "outer_grads_f32", "euler_assemble", "euler_sweep_cyclic", "weno_assemble", "weno_gather", "weno_scatter",
+    "outer_grads_f32", "euler_assemble", "euler_sweep_cyclic", "weno_assemble", "weno_sweep_cyclic", "weno_gather", "weno_scatter",
     "euler_sweep_work_elems", "sweep_gather", "sweep_scatter", "sweep_shape", "BtdmaBatch", "Graph",
 ]
 
@@ -397,6 +397,21 @@ def weno_assemble(plan: BtdmaGpuPlan, axis: int, U, vel, n1: int, n2: int, h: fl
             _check_tensor(name, t, nm, torch.float64)
     _check(_capi.lib().btd_weno_assemble(plan._h, int(axis), _ptr(U), _ptr(vel), int(n1), int(n2), float(h),
                                          float(gdt), _ptr(a), _ptr(b), _ptr(c), _stream(stream)))
+
+
+def weno_sweep_cyclic(plan: BtdmaGpuPlan, axis: int, U, vel, n1: int, n2: int, h: float, gdt: float,
+                      work, d, stream=None) -> None:
+    """Matrix-free periodic weno2d sweep: solves (I - gdt L) x = d along `axis` with the blocks
+    computed inside stage 1 of the two-stage periodic solve (equal, bit for bit, to
+    weno_assemble + solve_cyclic); `work` holds 2 * plan.matrix_elems() doubles."""
+    import torch
+    if plan.created:
+        _check_tensor("U", U, (n1 + 6) * (n2 + 6), torch.float64)
+        _check_tensor("vel", vel, n2 if axis == 0 else n1, torch.float64)
+        _check_tensor("work", work, 2 * plan.matrix_elems(), torch.float64)
+        _check_tensor("d", d, plan.vector_elems(), torch.float64)
+    _check(_capi.lib().btd_weno_sweep_cyclic(plan._h, int(axis), _ptr(U), _ptr(vel), int(n1), int(n2), float(h),
+                                             float(gdt), _ptr(work), _ptr(d), _stream(stream)))
 
 
 def weno_gather(plan: BtdmaGpuPlan, axis: int, field, n1: int, n2: int, d, stream=None) -> None:

@@ -42,6 +42,7 @@ SIGNATURES = {
     "btd_outer_grads_f32": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "btd_euler_assemble": (_i, [_vp, _i, _dp, _dp, _i, _i, _i, _d, _d, _d, _dp, _dp, _dp, _dp, _vp]),
     "btd_weno_assemble": (_i, [_vp, _i, _dp, _dp, _i, _i, _d, _d, _dp, _dp, _dp, _vp]),
+    "btd_weno_sweep_cyclic": (_i, [_vp, _i, _dp, _dp, _i, _i, _d, _d, _dp, _dp, _vp]),
     "btd_weno_gather": (_i, [_vp, _i, _dp, _i, _i, _dp, _vp]),
     "btd_weno_scatter": (_i, [_vp, _i, _dp, _i, _i, _dp, _vp]),
     "btd_euler_sweep_cyclic": (_i, [_vp, _i, _dp, _dp, _i, _i, _i, _d, _d, _d, _dp, _dp, _vp]),

@@ -256,27 +256,32 @@ static btd_status check_solve(btd_plan_t plan, const void* a, const void* b, con
     return BTD_OK;
 }
 
+// The periodic partitions of an npart = 1 plan, allocated by its first periodic solve.
+static btd_status ensure_cyclic_partitions(btd_plan_t plan, const char* what) {
+    if (plan->cyc_npart != 0) return BTD_OK;
+    const int pc = cyclic_npart(plan->m, plan->nsys, plan->nrow);
+    if (pc > 1) {
+        hipError_t e = alloc_partitions(plan->m, plan->nsys, plan->nrow, pc, nullptr, nullptr, &plan->cyc_part,
+                                        &plan->cyc_redA, &plan->cyc_redC, &plan->cyc_redD);
+        if (e != hipSuccess) {  // free what was allocated; the next call tries again
+            for (void** q : {(void**)&plan->cyc_part, (void**)&plan->cyc_redA, (void**)&plan->cyc_redC,
+                             (void**)&plan->cyc_redD}) {
+                if (*q) (void)hipFree(*q);
+                *q = nullptr;
+            }
+            return hip_fail(e, (std::string(what) + ": partition scratch").c_str());
+        }
+    }
+    plan->cyc_npart = pc;
+    return BTD_OK;
+}
+
 static btd_status solve_impl(btd_plan_t plan, double* a, const double* b, double* c, double* d,
                              hipStream_t s, bool timed, bool cyclic = false) {
     plan->last_stream = s;
     if (plan->npart == 1 && cyclic) {
-        if (plan->cyc_npart == 0) {
-            const int pc = cyclic_npart(plan->m, plan->nsys, plan->nrow);
-            if (pc > 1) {
-                hipError_t e = alloc_partitions(plan->m, plan->nsys, plan->nrow, pc, nullptr, nullptr,
-                                                &plan->cyc_part, &plan->cyc_redA, &plan->cyc_redC,
-                                                &plan->cyc_redD);
-                if (e != hipSuccess) {  // free what was allocated; the next call tries again
-                    for (void** q : {(void**)&plan->cyc_part, (void**)&plan->cyc_redA, (void**)&plan->cyc_redC,
-                                     (void**)&plan->cyc_redD}) {
-                        if (*q) (void)hipFree(*q);
-                        *q = nullptr;
-                    }
-                    return hip_fail(e, "btd_solve_cyclic: partition scratch");
-                }
-            }
-            plan->cyc_npart = pc;
-        }
+        btd_status st = ensure_cyclic_partitions(plan, "btd_solve_cyclic");
+        if (st != BTD_OK) return st;
         if (plan->cyc_npart > 1)
             BTD_HIP(btd::launch_two_stage(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                           plan->cyc_part, plan->cyc_part + plan->cyc_npart, plan->cyc_npart,
@@ -392,6 +397,35 @@ btd_status btd_weno_assemble(btd_plan_t plan, int axis, const double* U, const d
     plan->last_stream = (hipStream_t)stream;
     BTD_HIP(btd::launch_weno_assemble(plan->layout, axis, U, vel, n1, n2, h, gdt, a, b, c, plan->nsys, plan->nrow,
                                       (hipStream_t)stream));
+    return BTD_OK;
+}
+
+btd_status btd_weno_sweep_cyclic(btd_plan_t plan, int axis, const double* U, const double* vel, int n1, int n2,
+                                 double h, double gdt, double* work, double* d, void* stream) {
+    g_err.clear();
+    btd_status st = check_weno(plan, axis, n1, n2, "btd_weno_sweep_cyclic");
+    if (st != BTD_OK) return st;
+    if (!U || !vel || !work || !d) return fail(BTD_ERR_INVALID_ARG, "btd_weno_sweep_cyclic: NULL array");
+    if (!(h > 0.0)) return fail(BTD_ERR_INVALID_ARG, "btd_weno_sweep_cyclic: h must be > 0");
+    const int* part = plan->d_part;
+    double *ra = plan->redA, *rc = plan->redC, *rd = plan->redD;
+    int np = plan->npart;
+    if (np == 1) {
+        st = ensure_cyclic_partitions(plan, "btd_weno_sweep_cyclic");
+        if (st != BTD_OK) return st;
+        if (plan->cyc_npart < 2)
+            return fail(BTD_ERR_INVALID_ARG, "btd_weno_sweep_cyclic: needs at least 6 block rows (18 cells)");
+        np = plan->cyc_npart;
+        part = plan->cyc_part;
+        ra = plan->cyc_redA;
+        rc = plan->cyc_redC;
+        rd = plan->cyc_redD;
+    }
+    const size_t nm = btd_layout_elems(plan->layout, plan->nsys, plan->nrow, 9);
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_weno_sweep_cyclic(plan->layout, axis, U, vel, n1, n2, h, gdt, work, work + nm, d, plan->nsys,
+                                          plan->nrow, part, part + np, np, ra, rc, rd, plan->d_flag,
+                                          (hipStream_t)stream));
     return BTD_OK;
 }
 

@@ -38,6 +38,13 @@ hipError_t launch_euler_assemble(int layout, int axis, const double* U, const do
 hipError_t launch_weno_assemble(int layout, int axis, const double* U, const double* vel, int n1, int n2,
                                 double h, double gdt, double* A, double* B, double* C, int nsys, int nrow,
                                 hipStream_t s);
+// Matrix-free periodic weno2d sweep: the two-stage periodic solve with stage 1 computing the
+// blocks from U (btd_weno.hip); Aw, Cw hold stage 1's output over the plan's matrix size.
+hipError_t launch_weno_sweep_cyclic(int layout, int axis, const double* U, const double* vel, int n1, int n2,
+                                    double h, double gdt, double* Aw, double* Cw, double* D, int nsys, int nrow,
+                                    const int* pf, const int* pl, int npart, double* redA, double* redC,
+                                    double* redD, int* flag, hipStream_t s);
+
 // weno2d scalar field F(n1, n2) <-> sweep vector (m = 3) in the plan's layout (btd_weno.hip).
 hipError_t launch_weno_move(int layout, int axis, bool to_d, double* F, double* D, int n1, int nsys, int nrow,
                             hipStream_t s);

@@ -604,13 +604,20 @@ __global__ __launch_bounds__(64) void k_f32r(const float* __restrict__ A, const 
 // (tiled, 2P rows per system):
 //   row 2*part   : (Â_1, I, Ĉ_1, d̂_1) after the close, couples x_1 to x_0 and x_N
 //   row 2*part+1 : (Â_N, I, Ĉ_N, d̂_N) forward values, couples x_N to x_1 and x_{N+1}
-template <int M, int LAYOUT>
+// Where stage 1 takes the blocks of its rows: the arrays A, B, C (MemBlocks, the default), or a
+// block source that computes them for the lane's line as the rows advance (the matrix-free
+// weno2d sweep, btd_weno.hip): BS::lane(sys) gives an object with row(g), which makes row g's
+// blocks current, and b(R), a_col(k, a), a_mat(X), c_col(j, c), which read them.
+struct MemBlocks {};
+
+template <int M, int LAYOUT, class BS = MemBlocks>
 __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict__ B, double* C,
                                              double* D, int nsys, int nrow,
                                              const int* __restrict__ part_first,
                                              const int* __restrict__ part_len, int npart,
                                              double* __restrict__ redA, double* __restrict__ redC,
-                                             double* __restrict__ redD, int* __restrict__ flag) {
+                                             double* __restrict__ redD, int* __restrict__ flag, BS bs) {
+    constexpr bool kMem = std::is_same<BS, MemBlocks>::value;
     const int ntiles = (nsys + 63) / 64;
     const int tile = blockIdx.x % ntiles, part = blockIdx.x / ntiles;
     const int64_t sys = (int64_t)tile * 64 + threadIdx.x;
@@ -626,10 +633,19 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
         auto SG = [&](int i, int j) -> double& { return sp[(M + M * M + i * M + j) * 64 + lane]; };
         bool ok = true;
         double Ah[M][M], Ch[M][M], dh[M];
+        auto src = [&] {
+            if constexpr (kMem) return 0;
+            else return bs.lane(sys);
+        }();
         for (int q = 0; q < N; ++q) {
             const int64_t g = r0 + q;
             double R[M][M], v[M], X[M][M];
-            load_mat<M>(B, am, g, R);
+            if constexpr (kMem) {
+                load_mat<M>(B, am, g, R);
+            } else {
+                src.row(g);
+                src.b(R);
+            }
             load_vec<M>(D, av, g, v);
             if (q >= 2) {
     #pragma unroll
@@ -639,7 +655,10 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
     #pragma unroll
                 for (int k = 0; k < M; ++k) {
                     double ak[M];
-                    load_col<M>(A, am, g, k, ak);
+                    if constexpr (kMem)
+                        load_col<M>(A, am, g, k, ak);
+                    else
+                        src.a_col(k, ak);
     #pragma unroll
                     for (int i = 0; i < M; ++i) {
                         v[i] = fma(-ak[i], dh[k], v[i]);
@@ -651,7 +670,10 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
                     }
                 }
             } else {
-                load_mat_c<M>(A, am, g, X);
+                if constexpr (kMem)
+                    load_mat_c<M>(A, am, g, X);
+                else
+                    src.a_mat(X);
             }
             int piv[M];
             ok = lu_factor<M>(R, piv) && ok;
@@ -661,7 +683,10 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
     #pragma unroll
             for (int j = 0; j < M; ++j) {
                 double c[M];
-                load_col<M>(C, am, g, j, c);
+                if constexpr (kMem)
+                    load_col<M>(C, am, g, j, c);
+                else
+                    src.c_col(j, c);
                 lu_solve<M>(R, piv, c);
     #pragma unroll
                 for (int i = 0; i < M; ++i) Ch[i][j] = c[i];
@@ -1647,20 +1672,20 @@ hipError_t launch_cyclic_m(int layout, double* A, const double* B, double* C, do
     return hipGetLastError();
 }
 
-template <int M>
+template <int M, class BS = MemBlocks>
 hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C, double* D,
                               int nsys, int nrow, const int* pf, const int* pl, int npart,
                               double* redA, double* redC, double* redD, int* flag,
-                              hipStream_t s, hipEvent_t* ev, bool cyclic) {
+                              hipStream_t s, hipEvent_t* ev, bool cyclic, BS bs = BS{}) {
     const int ntiles = (nsys + 63) / 64;
     const dim3 block(64), gridp((unsigned)(ntiles * npart)), gridr((unsigned)ntiles);
     if (ev) (void)hipEventRecord(ev[0], s);
     const size_t lds = sizeof(double) * (2 * M * M + M) * 64;
     if constexpr (M < kModiCoopFrom) {
         if (lds > 65536) {  // above the default dynamic-LDS limit (m >= 8)
-            (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_TILED>,
+            (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_TILED, BS>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_REF_GPU>,
+            (void)hipFuncSetAttribute((const void*)k_modi<M, LAYOUT_REF_GPU, BS>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         }
     }
@@ -1677,11 +1702,11 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
             hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_REF_GPU, WV>), gridc, blockc, 0, s, A, B, C, D, nsys,
                                nrow, pf, pl, npart, redA, redC, redD, flag);
     } else if (layout == LAYOUT_TILED) {
-        hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
-                           npart, redA, redC, redD, flag);
+        hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED, BS>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
+                           npart, redA, redC, redD, flag, bs);
     } else {
-        hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
-                           npart, redA, redC, redD, flag);
+        hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU, BS>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf,
+                           pl, npart, redA, redC, redD, flag, bs);
     }
     if (ev) (void)hipEventRecord(ev[1], s);
     // the twisted reduced solve when its wavefronts all fit on the device at once (one per

@@ -49,6 +49,65 @@ __device__ __forceinline__ void weno5_left(double g0, double g1, double g2, doub
     c[4] = -w2 / 6.0;
 }
 
+// The three blocks of block row r of a line from its cells 3r-2 .. 3r+6 (f[0..8]): the WENO5
+// coefficients of the row's four interfaces, then each cell's 7-point stencil of I - gdt L
+// (scale = gdt c / h) spread over the block columns (build_matrix_{x,y}_kernel_weno5).
+__device__ __forceinline__ void weno_row_blocks(const double (&f)[9], bool left, double scale,
+                                                double (&Ab)[3][3], double (&Bb)[3][3], double (&Cb)[3][3]) {
+#pragma clang fp contract(off)
+    double cf[4][5];  // interfaces 3r .. 3r+3 (interface k to the right of cell k)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (left) {
+            weno5_left(f[u], f[u + 1], f[u + 2], f[u + 3], f[u + 4], cf[u]);
+        } else {
+            double c[5];
+            weno5_left(f[u + 5], f[u + 4], f[u + 3], f[u + 2], f[u + 1], c);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) cf[u][k] = c[4 - k];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Ab[i][j] = 0.0;
+            Bb[i][j] = i == j ? 1.0 : 0.0;
+            Cb[i][j] = 0.0;
+        }
+#pragma unroll
+    for (int irow = 0; irow < 3; ++irow) {
+        // cell 3r + irow + 1: f_hat to its right (interface irow + 1) minus f_hat to its left
+        double L[7];
+#pragma unroll
+        for (int o = 0; o < 7; ++o) L[o] = 0.0;
+        // offsets -3..3 of L[0..6]: the left-biased stencils start two cells back, the
+        // right-biased one (indices kept constant so that L stays in registers)
+        if (left) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) L[1 + k] += cf[irow + 1][k];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) L[k] -= cf[irow][k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) L[2 + k] += cf[irow + 1][k];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) L[1 + k] -= cf[irow][k];
+        }
+#pragma unroll
+        for (int o = 0; o < 7; ++o) {
+            const double val = scale * L[o];
+            const int col = irow + o - 3;  // column relative to this block row's first cell
+            if (col < 0)
+                Ab[irow][col + 3] += val;
+            else if (col < 3)
+                Bb[irow][col] += val;
+            else
+                Cb[irow][col - 3] += val;
+        }
+    }
+}
+
 template <int LAYOUT, int AXIS>
 __global__ __launch_bounds__(64) void k_weno_assemble(const double* __restrict__ U, const double* __restrict__ vel,
                                                       int n1, int n2, double h, double gdt,
@@ -90,63 +149,76 @@ __global__ __launch_bounds__(64) void k_weno_assemble(const double* __restrict__
         for (int t = 0; t < 6; ++t) f[t] = f[t + 3];
 #pragma unroll
         for (int t = 6; t < 9; ++t) f[t] = cell(3 * r - 2 + t);
-        double cf[4][5];  // interfaces 3r .. 3r+3 (interface k to the right of cell k)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (left) {
-                weno5_left(f[u], f[u + 1], f[u + 2], f[u + 3], f[u + 4], cf[u]);
-            } else {
-                double c[5];
-                weno5_left(f[u + 5], f[u + 4], f[u + 3], f[u + 2], f[u + 1], c);
-#pragma unroll
-                for (int k = 0; k < 5; ++k) cf[u][k] = c[4 - k];
-            }
-        }
         double Ab[3][3], Bb[3][3], Cb[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                Ab[i][j] = 0.0;
-                Bb[i][j] = i == j ? 1.0 : 0.0;
-                Cb[i][j] = 0.0;
-            }
-#pragma unroll
-        for (int irow = 0; irow < 3; ++irow) {
-            // cell 3r + irow + 1: f_hat to its right (interface irow + 1) minus f_hat to its left
-            double L[7];
-#pragma unroll
-            for (int o = 0; o < 7; ++o) L[o] = 0.0;
-            // offsets -3..3 of L[0..6]: the left-biased stencils start two cells back, the
-            // right-biased one (indices kept constant so that L stays in registers)
-            if (left) {
-#pragma unroll
-                for (int k = 0; k < 5; ++k) L[1 + k] += cf[irow + 1][k];
-#pragma unroll
-                for (int k = 0; k < 5; ++k) L[k] -= cf[irow][k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < 5; ++k) L[2 + k] += cf[irow + 1][k];
-#pragma unroll
-                for (int k = 0; k < 5; ++k) L[1 + k] -= cf[irow][k];
-            }
-#pragma unroll
-            for (int o = 0; o < 7; ++o) {
-                const double val = scale * L[o];
-                const int col = irow + o - 3;  // column relative to this block row's first cell
-                if (col < 0)
-                    Ab[irow][col + 3] += val;
-                else if (col < 3)
-                    Bb[irow][col] += val;
-                else
-                    Cb[irow][col - 3] += val;
-            }
-        }
+        weno_row_blocks(f, left, scale, Ab, Bb, Cb);
         store_mat<3>(A, am, r, Ab);
         store_mat<3>(B, am, r, Bb);
         store_mat<3>(C, am, r, Cb);
     }
 }
+
+// Block source of stage 1 for the matrix-free periodic sweep: a lane's blocks are computed from
+// U as its rows advance (a sliding nine-cell window), never stored.
+template <int AXIS>
+struct WenoBlocks {
+    const double* U;
+    const double* vel;
+    int n1;
+    double h, gdt;
+    struct Lane {
+        const double* U;
+        int64_t ld, sys, cur;
+        bool left;
+        double scale, f[9], A[3][3], B[3][3], C[3][3];
+        __device__ double cell(int64_t c) const {
+            return AXIS == 0 ? U[(c + 2) + ld * (sys + kWenoHalo)] : U[(sys + kWenoHalo) + ld * (c + 2)];
+        }
+        __device__ void row(int64_t g) {
+            if (g == cur + 1) {
+#pragma unroll
+                for (int t = 0; t < 6; ++t) f[t] = f[t + 3];
+#pragma unroll
+                for (int t = 6; t < 9; ++t) f[t] = cell(3 * g - 2 + t);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 9; ++t) f[t] = cell(3 * g - 2 + t);
+            }
+            cur = g;
+            weno_row_blocks(f, left, scale, A, B, C);
+        }
+        __device__ void b(double (&R)[3][3]) const {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) R[i][j] = B[i][j];
+        }
+        __device__ void a_mat(double (&X)[3][3]) const {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) X[i][j] = A[i][j];
+        }
+        __device__ void a_col(int k, double (&a)[3]) const {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) a[i] = A[i][k];
+        }
+        __device__ void c_col(int j, double (&c)[3]) const {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) c[i] = C[i][j];
+        }
+    };
+    __device__ Lane lane(int64_t sys) const {
+        Lane l;
+        l.U = U;
+        l.ld = (int64_t)n1 + 2 * kWenoHalo;
+        l.sys = sys;
+        l.cur = -2;
+        const double v = vel[sys];
+        l.left = v >= 0.0;
+        l.scale = (gdt * v) / h;
+        return l;
+    }
+};
 
 template <int LAYOUT, int AXIS>
 hipError_t weno_la(const double* U, const double* vel, int n1, int n2, double h, double gdt, double* A,
@@ -234,6 +306,19 @@ hipError_t launch_weno_move(int layout, int axis, bool to_d, double* F, double* 
                          : move_la<LAYOUT_TILED, 1>(to_d, F, D, n1, nsys, nrow, s);
     return axis == 0 ? move_la<LAYOUT_REF_GPU, 0>(to_d, F, D, n1, nsys, nrow, s)
                      : move_la<LAYOUT_REF_GPU, 1>(to_d, F, D, n1, nsys, nrow, s);
+}
+
+hipError_t launch_weno_sweep_cyclic(int layout, int axis, const double* U, const double* vel, int n1, int n2,
+                                    double h, double gdt, double* Aw, double* Cw, double* D, int nsys, int nrow,
+                                    const int* pf, const int* pl, int npart, double* redA, double* redC,
+                                    double* redD, int* flag, hipStream_t s) {
+    (void)n2;
+    if (axis == 0)
+        return launch_two_stage_m<3, WenoBlocks<0>>(layout, Aw, nullptr, Cw, D, nsys, nrow, pf, pl, npart, redA,
+                                                    redC, redD, flag, s, nullptr, true,
+                                                    WenoBlocks<0>{U, vel, n1, h, gdt});
+    return launch_two_stage_m<3, WenoBlocks<1>>(layout, Aw, nullptr, Cw, D, nsys, nrow, pf, pl, npart, redA, redC,
+                                                redD, flag, s, nullptr, true, WenoBlocks<1>{U, vel, n1, h, gdt});
 }
 
 hipError_t launch_weno_assemble(int layout, int axis, const double* U, const double* vel, int n1, int n2,

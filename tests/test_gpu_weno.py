@@ -95,6 +95,42 @@ def test_weno_gather_scatter(cuda, axis, layout):
     np.testing.assert_array_equal(back.cpu().numpy(), F.ravel())
 
 
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("layout,npart", [(pb.Layout.TILED, 1), (pb.Layout.REF_GPU, 1), (pb.Layout.TILED, 3)])
+def test_weno_matrix_free_sweep(cuda, axis, layout, npart):
+    """The matrix-free sweep equals, bit for bit, assembling the blocks and solving them with the
+    same partitions (npart = 1: the plan's periodic partitions; npart = 3: the plan's own), and
+    matches a dense solve of every line."""
+    import torch
+    U, vel, tU, tv = setup(cuda, axis, 31 + axis)
+    h, gdt = 1.0 / N1, 0.5 / N1
+    n = N1 if axis == 0 else N2
+    nsys = N2 if axis == 0 else N1
+    rng = np.random.default_rng(9)
+    F = rng.uniform(-1, 1, (N2, N1))
+    with pb.BtdmaGpuPlan().create(3, nsys, n // 3, npart, layout) as plan:
+        tf = torch.from_numpy(F.ravel().copy()).to(cuda)
+        d1 = torch.empty(plan.vector_elems(), dtype=torch.float64, device=cuda)
+        pb.weno_gather(plan, axis, tf, N1, N2, d1)
+        d2 = d1.clone()
+        work = torch.empty(2 * plan.matrix_elems(), dtype=torch.float64, device=cuda)
+        pb.weno_sweep_cyclic(plan, axis, tU, tv, N1, N2, h, gdt, work, d1)
+        a, b, c = (torch.zeros(plan.matrix_elems(), dtype=torch.float64, device=cuda) for _ in range(3))
+        pb.weno_assemble(plan, axis, tU, tv, N1, N2, h, gdt, a, b, c)
+        pb.solve_cyclic(plan, a, b, c, d2, 3, nsys, n // 3)
+        x = torch.empty_like(tf)
+        pb.weno_scatter(plan, axis, d1, N1, N2, x)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    np.testing.assert_array_equal(d1.cpu().numpy(), d2.cpu().numpy())
+    lines = F if axis == 0 else F.T
+    xl = x.cpu().numpy().reshape(N2, N1)
+    xl = xl if axis == 0 else xl.T
+    T = wa.dense_operator(U, vel, axis, N1, N2, h, gdt)
+    xo = np.linalg.solve(T, lines[..., None])[..., 0]
+    assert rel_err(xl.T[:, :, None], xo.T[:, :, None]).max() <= 1e-12
+
+
 def test_weno_argument_errors(cuda):
     import torch
     t = torch.zeros(10, dtype=torch.float64, device=cuda)
