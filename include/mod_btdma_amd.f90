@@ -72,6 +72,13 @@ module mod_btdma_amd
             integer(c_int), value :: axis, n1, n2
             real(c_double), value :: h, gdt
         end function
+        integer(c_int) function btd_weno_sweep_cyclic(plan, axis, U, vel, n1, n2, h, gdt, work, d, stream) &
+                bind(C)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: plan, U, vel, work, d, stream
+            integer(c_int), value :: axis, n1, n2
+            real(c_double), value :: h, gdt
+        end function
         integer(c_int) function btd_weno_gather(plan, axis, field, n1, n2, d, stream) bind(C)
             import :: c_int, c_ptr
             type(c_ptr), value :: plan, field, d, stream
@@ -88,6 +95,7 @@ module mod_btdma_amd
     public :: btdma_many_cycl_amd, btdma_cleanplan_amd
     public :: btdma_euler_assemble_amd, btdma_sweep_scatter_amd, btdma_euler_sweep_cyclic_amd
     public :: btdma_weno_assemble_amd, btdma_weno_gather_amd, btdma_weno_scatter_amd
+    public :: btdma_weno_sweep_cyclic_amd
 
 contains
 
@@ -171,6 +179,16 @@ contains
         real(c_double), intent(in) :: h, gdt
         call check(btd_weno_assemble(plan, axis, U, vel, n1, n2, h, gdt, A, B, C, c_null_ptr), &
                    'btdma_weno_assemble_amd')
+    end subroutine
+
+    ! the weno2d sweep's periodic solve with the blocks computed inside it (never stored): D, the
+    ! gathered right-hand side, becomes the solution; work: 2*nsys*nrow*9 device doubles
+    subroutine btdma_weno_sweep_cyclic_amd(plan, axis, U, vel, n1, n2, h, gdt, work, D)
+        type(c_ptr), intent(in) :: plan, U, vel, work, D
+        integer, intent(in) :: axis, n1, n2
+        real(c_double), intent(in) :: h, gdt
+        call check(btd_weno_sweep_cyclic(plan, axis, U, vel, n1, n2, h, gdt, work, D, c_null_ptr), &
+                   'btdma_weno_sweep_cyclic_amd')
     end subroutine
 
     ! pack_scalar_to_xblock_kernel / the transpose before the y sweep (weno_mpi_gpu.f90:279-289,
