@@ -48,15 +48,20 @@ __device__ __forceinline__ void sweep_cell(const Grid& g, int64_t sys, int64_t r
     }
 }
 
-__device__ __forceinline__ Prim load_prim(const double* __restrict__ U, int64_t off, int64_t sv) {
+// primitive variables of one cell from its five conserved ones
+__device__ __forceinline__ Prim prim_of(double rho, double ru, double rv, double rw, double rE) {
 #pragma clang fp contract(off)
     Prim p;
-    p.rho = U[off];
-    p.u = U[off + sv] / p.rho;
-    p.v = U[off + 2 * sv] / p.rho;
-    p.w = U[off + 3 * sv] / p.rho;
-    p.E = U[off + 4 * sv] / p.rho;
+    p.rho = rho;
+    p.u = ru / rho;
+    p.v = rv / rho;
+    p.w = rw / rho;
+    p.E = rE / rho;
     return p;
+}
+
+__device__ __forceinline__ Prim load_prim(const double* __restrict__ U, int64_t off, int64_t sv) {
+    return prim_of(U[off], U[off + sv], U[off + 2 * sv], U[off + 3 * sv], U[off + 4 * sv]);
 }
 
 // Flux Jacobian along AXIS at one cell: compute_Amat (:749-765), compute_Bmat (:767-783),
@@ -159,6 +164,78 @@ __global__ __launch_bounds__(64) void k_euler_assemble(const double* __restrict_
         cur = next;
         off += step;
         foff += fstep;
+    }
+}
+
+// The x sweep's assembly: its lines are the field's contiguous direction, so a lane walking its
+// own line would touch a separate cache line per lane and load. The workgroup's 64 lines are
+// staged through LDS instead, kXSub rows at a time (U cells r-1 .. r+kXSub, F cells r ..
+// r+kXSub-1 of each line), consecutive lanes loading consecutive cells of a line; each lane then
+// builds its rows from the tiles exactly as k_euler_assemble does (same operations, same result).
+constexpr int kXSub = 8;
+template <int LAYOUT>
+__global__ __launch_bounds__(64) void k_euler_assemble_x(const double* __restrict__ U,
+                                                         const double* __restrict__ F, Grid g,
+                                                         double cc, double cd, double* __restrict__ A,
+                                                         double* __restrict__ B, double* __restrict__ C,
+                                                         double* __restrict__ D, int64_t nsys,
+                                                         int64_t nrow, int R) {
+    constexpr int kUc = kXSub + 2;  // U cells per line and sub-chunk
+    __shared__ double tu[5][64][kUc + 1];
+    __shared__ double tf[5][64][kXSub + 1];
+    const int t = threadIdx.x;
+    const int64_t s0 = (int64_t)blockIdx.x * 64, sys = s0 + t;
+    const int lines = nsys - s0 < 64 ? (int)(nsys - s0) : 64;
+    const int64_t r0 = (int64_t)blockIdx.y * R;
+    const int64_t r1 = r0 + R < nrow ? r0 + R : nrow;
+    const Addr<LAYOUT> am(sys, nsys, nrow, 25), av(sys, nsys, nrow, 5);
+    const int64_t sv = g.sv(), fsv = g.n1 * g.n2 * g.n3;
+    const unsigned n2 = (unsigned)g.n2;
+    for (int64_t rs = r0; rs < r1; rs += kXSub) {
+        const int cnt = r1 - rs < kXSub ? (int)(r1 - rs) : kXSub;
+        __syncthreads();  // the previous sub-chunk's tiles have been read
+        for (int idx = t; idx < lines * kUc; idx += 64) {
+            const int l = idx / kUc, c = idx - l * kUc;
+            if (c > cnt + 1) continue;
+            const unsigned ls = (unsigned)(s0 + l);
+            const int64_t j = ls % n2, k = ls / n2;
+            const int64_t off = (rs + c) + (j + 1) * g.sj() + (k + 1) * g.sk();  // interior cell rs-1+c
+#pragma unroll
+            for (int e = 0; e < 5; ++e) tu[e][l][c] = U[off + e * sv];
+        }
+        if (F) {
+            for (int idx = t; idx < lines * kXSub; idx += 64) {
+                const int l = idx / kXSub, c = idx - l * kXSub;
+                if (c >= cnt) continue;
+                const unsigned ls = (unsigned)(s0 + l);
+                const int64_t foff = (rs + c) + g.n1 * (int64_t)ls;  // i + n1 (j + n2 k), ls = j + n2 k
+#pragma unroll
+                for (int e = 0; e < 5; ++e) tf[e][l][c] = F[foff + e * fsv];
+            }
+        }
+        __syncthreads();
+        if (sys < nsys) {
+            auto cell = [&](int c) { return prim_of(tu[0][t][c], tu[1][t][c], tu[2][t][c], tu[3][t][c], tu[4][t][c]); };
+            Prim prev = cell(0), cur = cell(1);
+            for (int q = 0; q < cnt; ++q) {
+                const Prim next = cell(q + 2);
+                double Jl[5][5], Jr[5][5], Ab[5][5], Bb[5][5], Cb[5][5];
+                euler_jacobian<0>(prev, Jl);
+                euler_jacobian<0>(next, Jr);
+                euler_blocks(Jl, Jr, cc, cd, Ab, Bb, Cb);
+                store_mat<5>(A, am, rs + q, Ab);
+                store_mat<5>(B, am, rs + q, Bb);
+                store_mat<5>(C, am, rs + q, Cb);
+                if (F) {
+                    double v[5];
+#pragma unroll
+                    for (int e = 0; e < 5; ++e) v[e] = tf[e][t][q];
+                    store_vec<5>(D, av, rs + q, v);
+                }
+                prev = cur;
+                cur = next;
+            }
+        }
     }
 }
 

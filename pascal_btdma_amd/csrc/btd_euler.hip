@@ -11,8 +11,12 @@ hipError_t assemble_la(const double* U, const double* F, const Grid& g, double c
                        double* A, double* B, double* C, double* D, int64_t nsys, int64_t nrow,
                        hipStream_t s) {
     const dim3 grid((unsigned)((nsys + 63) / 64), (unsigned)((nrow + kRowsPerBlock - 1) / kRowsPerBlock));
-    hipLaunchKernelGGL((k_euler_assemble<LAYOUT, AXIS>), grid, dim3(64), 0, s, U, F, g, cc, cd, A, B, C,
-                       D, nsys, nrow, kRowsPerBlock);
+    if constexpr (AXIS == 0)
+        hipLaunchKernelGGL((k_euler_assemble_x<LAYOUT>), grid, dim3(64), 0, s, U, F, g, cc, cd, A, B, C, D,
+                           nsys, nrow, kRowsPerBlock);
+    else
+        hipLaunchKernelGGL((k_euler_assemble<LAYOUT, AXIS>), grid, dim3(64), 0, s, U, F, g, cc, cd, A, B,
+                           C, D, nsys, nrow, kRowsPerBlock);
     return hipGetLastError();
 }
 
