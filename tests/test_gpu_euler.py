@@ -37,10 +37,13 @@ def test_assembly_is_bit_exact(cuda, grid, axis, layout):
         d = torch.full((plan.vector_elems(),), np.nan, dtype=torch.float64, device=cuda)
         pb.euler_assemble(plan, axis, upload(cuda, U), upload(cuda, rhs), n1, n2, n3, H, DT, EPS,
                           a, b, c, d)
+        # blocks alone (no right-hand side): the same blocks
+        a2, b2, c2 = (torch.full_like(a, np.nan) for _ in range(3))
+        pb.euler_assemble(plan, axis, upload(cuda, U), None, n1, n2, n3, H, DT, EPS, a2, b2, c2)
         torch.cuda.synchronize()
     A, B, C, dref = ea.assemble(U, rhs, axis, H, DT, EPS)
     lay = int(layout)
-    for got, ref in ((a, A), (b, B), (c, C)):
+    for got, ref in ((a, A), (b, B), (c, C), (a2, A), (b2, B), (c2, C)):
         np.testing.assert_array_equal(layout_to_mat(got.cpu().numpy(), lay, nrow, nsys, 5), ref)
     np.testing.assert_array_equal(layout_to_vec(d.cpu().numpy(), lay, nrow, nsys, 5), dref)
 
