@@ -1311,6 +1311,13 @@ __device__ __forceinline__ bool cyclic_solve(double* A, const double* __restrict
                 }
             }
         }
+        // C_q with the row's other loads (small m): C is written in place, so a column loaded
+        // after the previous column's store would cost one more memory round trip per column
+        constexpr bool kCEarly = M <= 5;
+        double Cq[kCEarly ? M : 1][kCEarly ? M : 1];
+        if constexpr (kCEarly) {
+            if (!last) load_mat_c<M>(C, am, q, Cq);
+        }
         int piv[M];
         ok = lu_factor<M>(R, piv) && ok;
         lu_solve<M>(R, piv, v);
@@ -1319,7 +1326,12 @@ __device__ __forceinline__ bool cyclic_solve(double* A, const double* __restrict
 #pragma unroll
             for (int j = 0; j < M; ++j) {
                 double c[M];
-                load_col<M>(C, am, q, j, c);
+                if constexpr (kCEarly) {
+#pragma unroll
+                    for (int i = 0; i < M; ++i) c[i] = Cq[i][j];
+                } else {
+                    load_col<M>(C, am, q, j, c);
+                }
                 lu_solve<M>(R, piv, c);
 #pragma unroll
                 for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
