@@ -59,10 +59,16 @@ typedef enum btd_layout {
 /* How a plan's single-partition solve runs (btd_plan_set_form). */
 typedef enum btd_form {
     BTD_FORM_PLAIN = 0,   /* the library's choice: one lane per system (m = 1, or m <= 7 in
-                           * large batches; blocks in VGPRs), m lanes per system (m >= 8; one
-                           * block row each, pivoting through LDS) or, for smaller batches,
-                           * twisted */
-    BTD_FORM_TWISTED = 1  /* two groups of m lanes per system, one eliminating from each end */
+                           * large batches; blocks in VGPRs; every block operation in the
+                           * order and rounding of the reference CPU path's BLAS/LAPACK calls,
+                           * so x is that path's result bit for bit), m lanes per system
+                           * (m >= 8; one block row each, pivoting through LDS) or, for smaller
+                           * batches, twisted (these two agree with it to rounding) */
+    BTD_FORM_TWISTED = 1, /* two groups of m lanes per system, one eliminating from each end */
+    BTD_FORM_LAPACK_ORDER = 2 /* the one-lane solve at every batch size and m: each block
+                               * operation in the order and rounding of the reference CPU path's
+                               * BLAS/LAPACK calls (dgemm/dgemv sums from the first term, dgetf2
+                               * + dgetrs, IEEE division, no fused multiply-add) */
 } btd_form;
 
 /* Phase times in seconds; the fields of BtdmaSolveTimings (pascal_btdma_cuda.hpp:24-39)
@@ -108,7 +114,12 @@ btd_status btd_plan_npart(btd_plan_t plan, int *npart);
  * meeting in the middle: half the row latency per system (on large batches its wavefronts
  * share cache lines that leave L2 between their reads, so it moves more bytes). Same
  * zero-pivot reporting; results agree with the one-lane solve to rounding; C is left
- * overwritten. Cyclic and the other solve variants are unaffected. */
+ * overwritten. BTD_FORM_LAPACK_ORDER (npart = 1, any m) runs the one-lane solve at every batch
+ * size: every block operation of mod_btdma_cpu.f90:15-78 in the order of the reference's dgemm,
+ * dgemv and dgesv as reference (unblocked dgetf2/dgetrs) BLAS/LAPACK performs them, so x is the
+ * FP64 CPU path's result bit for bit, not merely within rounding of it (slower than the
+ * library's choice on batches that do not fill the device one lane per system). Cyclic and the
+ * other solve variants are unaffected. */
 btd_status btd_plan_set_form(btd_plan_t plan, int form);
 btd_status btd_plan_form(btd_plan_t plan, int *form);
 
@@ -139,9 +150,10 @@ btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double *a, const double
  * 2*npart rows, stage 3 (btdma_many_cyclic_mpi, mod_btdma_cpu.f90:273-302). npart = 1:
  * the same over partitions the library chooses (btd_auto_npart's count where that is >= 2,
  * else 4, or 2 below 32 rows; faster than one periodic sweep per system at every size
- * measured), with their scratch allocated by the plan's first periodic solve (make that call
- * before capturing one in a graph); below 6 rows one periodic block-Thomas sweep per system
- * (needs nrow >= 3). A, C, D are overwritten. */
+ * measured), with their scratch allocated by the plan's first periodic solve: that first call
+ * allocates and copies synchronously, and is refused (BTD_ERR_INVALID_ARG) when its stream is
+ * being captured, so make one periodic call before capturing one in a graph; below 6 rows one
+ * periodic block-Thomas sweep per system (needs nrow >= 3). A, C, D are overwritten. */
 btd_status btd_solve_cyclic(btd_plan_t plan, double *a, const double *b, double *c, double *d,
                             int m, int nsys, int nrow, void *stream);
 
@@ -175,8 +187,9 @@ btd_status btd_solve_noncyclic_f32r(btd_plan_t plan, const float *a, const float
                                     void *stream);
 
 /* FP32-stored blocks, FP64 arithmetic (no counterpart in the reference): the FP64 solve of the
- * blocks' values, bit for bit what btd_solve_noncyclic computes on FP64 copies of a, b, c,
- * reading 4 bytes per block element. a, b, c are only read; d is in place (x returned in d);
+ * blocks' values, one lane per system in the reference CPU path's BLAS/LAPACK operation order,
+ * so bit for bit what that path (and btd_solve_noncyclic's one-lane solve) computes on FP64
+ * copies of a, b, c, reading 4 bytes per block element. a, b, c are only read; d is in place (x returned in d);
  * work: btd_layout_elems(layout, nsys, nrow, m*m) doubles of device scratch (holds C'). npart
  * must be 1. */
 btd_status btd_solve_noncyclic_f32s(btd_plan_t plan, const float *a, const float *b, const float *c,
@@ -249,8 +262,9 @@ btd_status btd_weno_assemble(btd_plan_t plan, int axis, const double *U, const d
  * btd_solve_cyclic bit for bit). d: the right-hand side in the plan's layout
  * (btd_weno_gather), replaced by x. work: device scratch of 2 * btd_layout_elems(layout,
  * nsys, nrow, 9) doubles. npart > 1 plans use their partitions, npart = 1 plans those of
- * btd_solve_cyclic (allocated by the first periodic call; needs nrow >= 6). Zero pivots are
- * recorded in the plan. Asynchronous. */
+ * btd_solve_cyclic (allocated by the plan's first periodic call, which is synchronous and
+ * refused while its stream is being captured; needs nrow >= 6). Zero pivots are recorded in the
+ * plan. Asynchronous once the partitions exist. */
 btd_status btd_weno_sweep_cyclic(btd_plan_t plan, int axis, const double *U, const double *vel, int n1, int n2,
                                  double h, double gdt, double *work, double *d, void *stream);
 
@@ -342,7 +356,10 @@ btd_status btd_batch_locate(btd_batch_t batch, int sys, size_t *mat_off, size_t 
  * 5 = as 4, with the items whose cost exceeds total / grid solved twisted (two groups of m
  * lanes per system eliminating from both ends, half the rows each; results agree with the
  * other placements to rounding, not bit for bit); 6 = as 4 with every cooperative item
- * twisted; grid = persistent workgroups of 64 lanes. */
+ * twisted; grid = persistent workgroups of 64 lanes. Synchronous: waits for all work queued on
+ * the device (hipDeviceSynchronize) before it rewrites the batch's work-item table, so a solve or
+ * graph replay of the batch in flight is never handed a half-written table; not to be called
+ * while a stream is being captured. */
 btd_status btd_batch_set_params(btd_batch_t batch, int placement, int grid);
 /* The seeded generators of btd_generate, per system (system index s, its own n). */
 btd_status btd_batch_generate(btd_batch_t batch, int kind, uint64_t seed, double *a, double *b,

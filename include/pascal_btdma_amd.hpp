@@ -118,7 +118,8 @@ struct BtdmaGpuPlan {
     void singular_systems(int* info) { check_status(btd_plan_info(handle, info)); }
 
     // How an npart = 1 plan's non-cyclic solve runs (btd_plan_set_form): BTD_FORM_PLAIN, the
-    // library's choice (the default), or BTD_FORM_TWISTED, always from both ends.
+    // library's choice (the default), BTD_FORM_TWISTED, always from both ends, or
+    // BTD_FORM_LAPACK_ORDER, one lane per system in the CPU path's BLAS/LAPACK operation order.
     void set_form(btd_form f) { check_status(btd_plan_set_form(handle, (int)f)); }
     btd_form form() {
         int f = 0;

@@ -22,6 +22,9 @@
  *   ora_btdma_many_cycl  pinned: dense periodic solve (n >= 3)
  *   ora_btdma_many_cyclic_mpi  pinned: dense periodic solve, P invariance
  *   ora_btdma_many_t     pinned: dense transposed solve
+ *   ora_btdma_many_q     pinned: exact rational solves (Python fractions) of small systems,
+ *                        incl. blocks of condition number up to 1e6; equality with
+ *                        ora_btdma_many on well-conditioned systems
  */
 #ifndef BTDMA_ORACLE_H
 #define BTDMA_ORACLE_H
@@ -73,6 +76,14 @@ int ora_btdma_many_cyclic_mpi(int m, int nsys, int n, int npart, double *A, doub
  * modified; w is overwritten by lambda. */
 int ora_btdma_many_t(int n, int nsys, int m, const double *a, const double *b, const double *c,
                      double *w);
+
+/* Truth solve (btdma_truth.c): the block-Thomas solve of the same FP64 inputs carried out in
+ * IEEE binary128 and rounded to FP64 at the end, i.e. the exact solution to about an ulp for
+ * block condition numbers far beyond 1e6. a, b, c, d are read only (CPU layout; C of the last
+ * row and A of the first are ignored); x (m,nsys,n) receives the solution. Returns the first
+ * zero-pivot column + 1 (0 if none). */
+int ora_btdma_many_q(int n, int nsys, int m, const double *a, const double *b, const double *c,
+                     const double *d, double *x);
 
 /* Threads used by the last call (OpenMP over systems). */
 int ora_num_threads(void);

@@ -42,6 +42,7 @@ def lib():
         L.ora_btdma_many_cycl.argtypes = [ctypes.c_int] * 3 + [_dp] * 4
         L.ora_btdma_many_cyclic_mpi.argtypes = [ctypes.c_int] * 4 + [_dp] * 4
         L.ora_btdma_many_t.argtypes = [ctypes.c_int] * 3 + [_dp] * 4
+        L.ora_btdma_many_q.argtypes = [ctypes.c_int] * 3 + [_dp] * 5
         L.ora_set_num_threads.argtypes = [ctypes.c_int]
         L.ora_num_threads.restype = ctypes.c_int
         _lib = L
@@ -113,6 +114,17 @@ def solve_cyclic_two_stage(A, B, C, D, npart: int):
     n, nsys, m = D.shape
     info = lib().ora_btdma_many_cyclic_mpi(m, nsys, n, npart, _p(A), _p(B), _p(C), _p(D))
     return D, info
+
+
+def solve_truth(A, B, C, D):
+    """The same block-Thomas solve carried out in binary128 (btdma_truth.c) and rounded to
+    FP64: the exact solution of the given FP64 inputs to about an ulp. Inputs untouched;
+    returns x (n, nsys, m) and info."""
+    A, B, C, D = _prep(A, B, C, D)
+    n, nsys, m = D.shape
+    X = np.empty_like(D)
+    info = lib().ora_btdma_many_q(n, nsys, m, _p(A), _p(B), _p(C), _p(D), _p(X))
+    return X, info
 
 
 def solve_transposed(A, B, C, W):

@@ -40,6 +40,7 @@ class Form(enum.IntEnum):
     """How a plan's single-partition solve runs (btd_form)."""
     PLAIN = 0
     TWISTED = 1
+    LAPACK_ORDER = 2
 
 
 class GenKind(enum.IntEnum):
@@ -169,7 +170,7 @@ class BtdmaGpuPlan:
 
     @property
     def form(self) -> int:
-        """How the single-partition solve runs: Form.PLAIN or Form.TWISTED (btd_plan_form)."""
+        """How the single-partition solve runs: Form.PLAIN, TWISTED or LAPACK_ORDER (btd_plan_form)."""
         f = ctypes.c_int(0)
         _check(_capi.lib().btd_plan_form(self._h, ctypes.byref(f)))
         return Form(f.value)

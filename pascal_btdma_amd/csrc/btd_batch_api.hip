@@ -110,7 +110,11 @@ static btd_status build_items(btd_batch_s* b, int placement, int grid) {
         host[1 + b->max_items + k] = items[k].npack;
         host[1 + 2 * b->max_items + k] = items[k].mode;
     }
-    hipError_t e = hipMemcpy(b->d_items - 1, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice);
+    // a solve or graph replay of this batch still in flight (on any stream) reads the table:
+    // let every queued kernel finish before the synchronous copy rewrites it
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_items - 1, host.data(), host.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bfail(BTD_ERR_HIP, std::string("batch items: ") + hipGetErrorString(e));
     b->placement = placement;
     return BTD_OK;

@@ -225,6 +225,12 @@ btd_status btd_plan_set_form(btd_plan_t plan, int form) {
         plan->form = form;
         return BTD_OK;
     }
+    if (form == BTD_FORM_LAPACK_ORDER) {
+        if (plan->npart != 1)
+            return fail(BTD_ERR_INVALID_ARG, "btd_plan_set_form: the LAPACK-order form needs npart == 1");
+        plan->form = form;
+        return BTD_OK;
+    }
     if (form != BTD_FORM_TWISTED) return fail(BTD_ERR_INVALID_ARG, "btd_plan_set_form: unknown form");
     if (plan->npart != 1 || plan->m < 2)
         return fail(BTD_ERR_INVALID_ARG, "btd_plan_set_form: the twisted form needs npart == 1 and m >= 2");
@@ -256,9 +262,17 @@ static btd_status check_solve(btd_plan_t plan, const void* a, const void* b, con
     return BTD_OK;
 }
 
-// The periodic partitions of an npart = 1 plan, allocated by its first periodic solve.
-static btd_status ensure_cyclic_partitions(btd_plan_t plan, const char* what) {
+// The periodic partitions of an npart = 1 plan, allocated by its first periodic solve
+// (hipMalloc and a synchronous table copy). That first call cannot be captured into a graph:
+// it is refused with a clear error when `s` is capturing.
+static btd_status ensure_cyclic_partitions(btd_plan_t plan, const char* what, hipStream_t s) {
     if (plan->cyc_npart != 0) return BTD_OK;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    BTD_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+        return fail(BTD_ERR_INVALID_ARG,
+                    std::string(what) + ": the plan's first periodic solve allocates its partition scratch and "
+                                        "cannot be captured; make one periodic call before capturing");
     const int pc = cyclic_npart(plan->m, plan->nsys, plan->nrow);
     if (pc > 1) {
         hipError_t e = alloc_partitions(plan->m, plan->nsys, plan->nrow, pc, nullptr, nullptr, &plan->cyc_part,
@@ -280,7 +294,7 @@ static btd_status solve_impl(btd_plan_t plan, double* a, const double* b, double
                              hipStream_t s, bool timed, bool cyclic = false) {
     plan->last_stream = s;
     if (plan->npart == 1 && cyclic) {
-        btd_status st = ensure_cyclic_partitions(plan, "btd_solve_cyclic");
+        btd_status st = ensure_cyclic_partitions(plan, "btd_solve_cyclic", s);
         if (st != BTD_OK) return st;
         if (plan->cyc_npart > 1)
             BTD_HIP(btd::launch_two_stage(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
@@ -295,6 +309,9 @@ static btd_status solve_impl(btd_plan_t plan, double* a, const double* b, double
         if (plan->form == BTD_FORM_TWISTED)
             BTD_HIP(btd::launch_plain_twisted(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                               plan->d_flag, s));
+        else if (plan->form == BTD_FORM_LAPACK_ORDER)
+            BTD_HIP(btd::launch_plain_ref(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
+                                          plan->d_flag, s));
         else
             BTD_HIP(btd::launch_plain(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                       plan->d_flag, s));
@@ -411,7 +428,7 @@ btd_status btd_weno_sweep_cyclic(btd_plan_t plan, int axis, const double* U, con
     double *ra = plan->redA, *rc = plan->redC, *rd = plan->redD;
     int np = plan->npart;
     if (np == 1) {
-        st = ensure_cyclic_partitions(plan, "btd_weno_sweep_cyclic");
+        st = ensure_cyclic_partitions(plan, "btd_weno_sweep_cyclic", (hipStream_t)stream);
         if (st != BTD_OK) return st;
         if (plan->cyc_npart < 2)
             return fail(BTD_ERR_INVALID_ARG, "btd_weno_sweep_cyclic: needs at least 6 block rows (18 cells)");

@@ -105,4 +105,80 @@ __device__ __forceinline__ void lu_solve_t(const double (&a)[M][M], const int (&
     }
 }
 
+// ---- the oracle's operation order --------------------------------------------------------
+// lu_factor_ref / lu_solve_ref compute what lu_factor / lu_solve compute, in the order and with
+// the roundings of LAPACK's unblocked dgetf2 + dgetrs as the reference CPU path calls them
+// through dgesv (Fortran_Original/src/mod_btdma_cpu.f90:38,58): multipliers a(i,k) * (1/pivot)
+// (a(i,k) / pivot when |pivot| < DBL_MIN, as dgetf2 does below sfmin), the trailing update
+// a(i,j) - l * a(k,j) unfused, the pivot itself kept on the diagonal, unit-lower substitution
+// x(i) - x(k) * l(i,k) for k ascending, and the upper solve column by column from the last:
+// x(k) / u(k,k) (IEEE division), then x(i) - x(k) * u(i,k) for every i < k. No fused
+// multiply-add anywhere, so each result is the oracle's bit for bit.
+template <int M>
+__device__ __forceinline__ bool lu_factor_ref(double (&a)[M][M], int (&piv)[M]) {
+#pragma clang fp contract(off)
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        double best = fabs(a[k][k]);
+        int p = k;
+#pragma unroll
+        for (int i = k + 1; i < M; ++i) {
+            const double v = fabs(a[i][k]);
+            const bool c = v > best;
+            best = c ? v : best;
+            p = c ? i : p;
+        }
+        piv[k] = p;
+#pragma unroll
+        for (int i = k + 1; i < M; ++i) {
+            const bool c = (p == i);
+#pragma unroll
+            for (int j = k; j < M; ++j) {
+                const double t = a[k][j];
+                a[k][j] = c ? a[i][j] : t;
+                a[i][j] = c ? t : a[i][j];
+            }
+        }
+        const double pv = a[k][k];
+        ok = ok && (pv != 0.0);
+        if (fabs(pv) >= 2.2250738585072014e-308) {  // DBL_MIN, dgetf2's sfmin
+            const double r = 1.0 / pv;
+#pragma unroll
+            for (int i = k + 1; i < M; ++i) a[i][k] = a[i][k] * r;
+        } else {
+#pragma unroll
+            for (int i = k + 1; i < M; ++i) a[i][k] = a[i][k] / pv;
+        }
+#pragma unroll
+        for (int i = k + 1; i < M; ++i)
+#pragma unroll
+            for (int j = k + 1; j < M; ++j) a[i][j] = a[i][j] - a[i][k] * a[k][j];
+    }
+    return ok;
+}
+
+template <int M>
+__device__ __forceinline__ void lu_solve_ref(const double (&a)[M][M], const int (&piv)[M], double (&x)[M]) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+#pragma unroll
+        for (int i = k + 1; i < M; ++i) {
+            const bool c = (piv[k] == i);
+            const double t = x[k];
+            x[k] = c ? x[i] : t;
+            x[i] = c ? t : x[i];
+        }
+#pragma unroll
+        for (int i = k + 1; i < M; ++i) x[i] = x[i] - x[k] * a[i][k];
+    }
+#pragma unroll
+    for (int k = M - 1; k >= 0; --k) {
+        x[k] = x[k] / a[k][k];
+#pragma unroll
+        for (int i = 0; i < k; ++i) x[i] = x[i] - x[k] * a[i][k];
+    }
+}
+
 }  // namespace btd

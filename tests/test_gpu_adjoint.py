@@ -193,3 +193,47 @@ def test_f32r_autograd_function_and_finite_differences(cuda, oracle):
     print(f"f32r finite differences: FP64 d.grad {worst_d:.3e}, FP32 block grads {worst:.3e}")
     assert worst_d <= 1e-10
     assert worst <= 1e-6  # FP32-stored block gradients: one FP32 rounding of each entry
+
+
+@pytest.mark.parametrize("m", [3, 5])
+@pytest.mark.parametrize("variant", ["fp64", "f32r"])
+def test_autograd_tiled_padding_is_zero(cuda, oracle, m, variant):
+    """nsys = 100 in the tiled layout leaves 28 padding lanes in the last tile. Every
+    gradient (and the FP32 path's x) must be finite over the whole flat tensor, with zeros
+    in the padding, even when the allocator hands back memory that held NaN."""
+    import torch
+    from pascal_btdma_amd.autograd import block_tridiagonal_solve, block_tridiagonal_solve_f32r
+    nsys, n = 100, 16
+    lay = int(pb.Layout.TILED)
+    A, B, C, d = pb.generate_host(pb.GenKind.EULER, 90 + m, m, nsys, n)
+    if variant == "f32r":
+        A, B, C = (X.astype(np.float32).astype(np.float64) for X in (A, B, C))
+    w = np.random.default_rng(m).uniform(-1, 1, d.shape)
+    bdt = torch.float32 if variant == "f32r" else torch.float64
+    up = lambda X, dt: torch.from_numpy(nat_to_layout(X, lay)).to(cuda).to(dt).requires_grad_()
+    ta, tb, tc = up(A, bdt), up(B, bdt), up(C, bdt)
+    td = up(d, torch.float64)
+    tw = torch.from_numpy(nat_to_layout(w, lay)).to(cuda)
+    # leave NaN in freed blocks of every size the backward allocates
+    for numel, dt in ((ta.numel(), bdt), (td.numel(), torch.float64)):
+        junk = [torch.full((numel,), float("nan"), dtype=dt, device=cuda) for _ in range(8)]
+        del junk
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, pb.Layout.TILED) as plan:
+        if variant == "f32r":
+            x = block_tridiagonal_solve_f32r(plan, ta, tb, tc, td, 2)
+        else:
+            x = block_tridiagonal_solve(plan, ta, tb, tc, td)
+        (x * tw).sum().backward()
+        torch.cuda.synchronize()
+    pad_mask_v = torch.from_numpy(nat_to_layout(np.ones((n, nsys, m)), lay)).to(cuda) == 0
+    pad_mask_m = torch.from_numpy(nat_to_layout(np.ones((n, nsys, m, m)), lay)).to(cuda) == 0
+    assert int(pad_mask_v.sum()) == n * 28 * m
+    for name, g, mask in (("dA", ta.grad, pad_mask_m), ("dB", tb.grad, pad_mask_m),
+                          ("dC", tc.grad, pad_mask_m), ("dd", td.grad, pad_mask_v)):
+        assert torch.isfinite(g).all(), name
+        assert (g[mask] == 0).all(), name
+        assert torch.isfinite(g.double().norm()), name
+    if variant == "f32r":
+        assert torch.isfinite(x).all() and (x.detach()[pad_mask_v] == 0).all()
+    x_, lo, _, _, _ = oracle_grads(oracle, A, B, C, d, w)
+    assert rel_err(layout_to_vec(td.grad.cpu().numpy(), lay, n, nsys, m), lo).max() <= 1e-12

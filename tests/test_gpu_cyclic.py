@@ -116,3 +116,51 @@ def test_cyclic_and_plain_on_one_plan(cuda, oracle):
                 C[-1] = 0.0
                 xo, _ = oracle.solve_plain(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
             assert rel_err(x, xo).max() <= TOL
+
+
+def test_cyclic_first_call_refused_under_capture_then_graph_replays(cuda, oracle):
+    """The first periodic solve of an npart = 1 plan allocates its partition scratch: under
+    graph capture it is refused with an error instead of breaking the capture. After one eager
+    periodic call the solve captures, and every replay equals the eager result bit for bit."""
+    import torch
+    m, nsys, n = 3, 130, 40
+    lay = int(pb.Layout.TILED)
+    rng = np.random.default_rng(77)
+    A, B, C, d = periodic_system(rng, n, nsys, m)
+    t0 = [torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d)]
+    t = [x.clone() for x in t0]
+    s1 = torch.cuda.Stream()
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, pb.Layout.TILED) as plan:
+        torch.cuda.synchronize()
+        g = pb.Graph()
+        g.begin(s1)
+        with pytest.raises(ValueError, match="cannot be captured"):
+            pb.solve_cyclic(plan, *t, m, nsys, n, stream=s1)
+        try:
+            g.end(s1)  # the capture recorded nothing
+        except pb.BtdmaError:
+            pass
+        g.destroy()
+        pb.solve_cyclic(plan, *t, m, nsys, n)  # eager: sets up the partitions
+        torch.cuda.synchronize()
+        assert not plan.check()
+        eager = t[3].clone()
+        xo, info = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+        assert info == 0
+        assert rel_err(layout_to_vec(eager.cpu().numpy(), lay, n, nsys, m), xo).max() <= TOL
+        for x, x0 in zip(t, t0):
+            x.copy_(x0)
+        torch.cuda.synchronize()
+        g = pb.Graph()
+        g.begin(s1)
+        pb.solve_cyclic(plan, *t, m, nsys, n, stream=s1)
+        g.end(s1)
+        for _ in range(3):
+            for x, x0 in zip(t, t0):
+                x.copy_(x0)
+            torch.cuda.synchronize()
+            g.launch(s1)
+            s1.synchronize()
+            assert torch.equal(t[3], eager)
+        g.destroy()
+        assert not plan.check()

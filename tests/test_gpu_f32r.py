@@ -46,23 +46,28 @@ def test_refinement_reaches_fp64_on_well_conditioned(cuda, oracle, m, layout):
 
 
 def test_refinement_on_conditioned_blocks(cuda, oracle):
-    """cond(B_q) log-uniform in [1e1, 1e6]: the error vs the oracle falls by at least 10x
-    per sweep (the contraction factor of FP32-factor refinement is about cond * 2^-24, so
-    the worst systems, cond ~ 1e6, need more than 3 sweeps to reach the FP64 floor)."""
+    """cond(B_q) log-uniform in [1e1, 1e6]. Accuracy is measured against the binary128 truth
+    (the exact solution of the FP32-rounded blocks to about an ulp), since the FP64 oracle is
+    itself an FP64 solve about cond * 1e-16 away from it. The contraction factor of FP32-factor
+    refinement is about cond * 2^-24, so the error falls by at least 10x per sweep until it
+    reaches the FP64 floor, and with the 8 sweeps allowed f32r is within 10x of the FP64 device
+    solve's own error against the truth (the bench reports 0-8 sweeps)."""
     m, nsys, n = 5, 128, 64
     A, B, C, d = pb.generate_host(pb.GenKind.COND, 9, m, nsys, n)
     xo, info = oracle.solve_plain(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
     assert info == 0
-    x64, _ = gpu_solve(A, B, C, d, int(pb.Layout.TILED))
-    floor = rel_err(x64, xo).max()
-    errs = [rel_err(run_f32r(cuda, A, B, C, d, pb.Layout.TILED, s), xo) for s in (0, 1, 2, 3)]
+    xt, info = oracle.solve_truth(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    x64, _ = gpu_solve(A, B, C, d, int(pb.Layout.TILED), form=int(pb.Form.LAPACK_ORDER))
+    np.testing.assert_array_equal(x64, xo)  # the one-lane FP64 solve is the oracle bit for bit
+    floor = rel_err(x64, xt).max()
+    errs = [rel_err(run_f32r(cuda, A, B, C, d, pb.Layout.TILED, s), xt) for s in range(9)]
     mx = [e.max() for e in errs]
-    frac = (errs[3] <= 1e-12).mean()
-    print(f"cond blocks: fp64 floor {floor:.2e}; f32r by sweeps {['%.2e' % e for e in mx]}; "
-          f"median after 3 sweeps {np.median(errs[3]):.2e}; systems <= 1e-12: {frac:.3f}")
-    for k in range(3):
-        assert mx[k + 1] <= 0.1 * mx[k]
-    assert mx[3] <= 1e-7
+    print(f"cond blocks: FP64 solve vs truth {floor:.2e}; f32r vs truth by sweeps "
+          f"{['%.2e' % e for e in mx]}")
+    for k in range(8):
+        assert mx[k + 1] <= 0.1 * mx[k] or mx[k + 1] <= 10 * floor
+    assert mx[8] <= 10 * floor
 
 
 def test_f32r_rejects_bad_arguments(cuda):
@@ -86,9 +91,9 @@ def test_fp32_storage_is_the_fp64_solve_of_the_stored_values(cuda, m, layout):
     """btd_solve_noncyclic_f32s (FP32 blocks, FP64 arithmetic, one lane per system) equals,
     bit for bit, the FP64 solve btd_solve_noncyclic of the same blocks converted to FP64 where
     that solve does the same arithmetic: one lane per system (m = 1; m <= 7 from 32768
-    systems) or m lanes per system (m >= 8, batches that fill the device that way); the blocks
-    are left as they were. Smaller batches (m >= 2) are solved twisted (another elimination
-    order) and agree to rounding."""
+    systems); the blocks are left as they were. The FP64 solve of m >= 8 batches that fill the
+    device runs m lanes per system, and smaller batches (m >= 2) are solved twisted: other
+    elimination orders, which agree to rounding."""
     import torch
     n = 13
     if m == 1:
@@ -102,7 +107,10 @@ def test_fp32_storage_is_the_fp64_solve_of_the_stored_values(cuda, m, layout):
         x32, x64 = _f32s_and_fp64(cuda, m, 70, n, layout)
         assert rel_err(x32, x64).max() <= 1e-12
     x32, x64 = _f32s_and_fp64(cuda, m, nsys, n, layout)
-    assert np.array_equal(x32, x64)
+    if m <= 7:
+        assert np.array_equal(x32, x64)
+    else:  # the FP64 solve runs m lanes per system here (its own elimination order)
+        assert rel_err(x32, x64).max() <= 1e-12
 
 
 def _f32s_and_fp64(cuda, m, nsys, n, layout):
@@ -131,9 +139,9 @@ def _f32s_and_fp64(cuda, m, nsys, n, layout):
 
 
 def test_fp32_storage_on_conditioned_blocks(cuda, oracle):
-    """cond(B_q) log-uniform in [1e1, 1e6]: the FP32-storage solve needs no refinement; it agrees
-    with the FP64 solve of the stored values and with the oracle on those values to the FP64
-    floor at this conditioning."""
+    """cond(B_q) log-uniform in [1e1, 1e6]: the FP32-storage solve needs no refinement; it is the
+    oracle's FP64 solve of the stored values bit for bit, and the twisted FP64 device solve of
+    those values is as close to the binary128 truth as the oracle, within a factor 10."""
     import torch
     m, nsys, n = 5, 128, 64
     A, B, C, d = pb.generate_host(pb.GenKind.COND, 9, m, nsys, n)
@@ -149,12 +157,15 @@ def test_fp32_storage_on_conditioned_blocks(cuda, oracle):
         pb.solve_noncyclic_f32s(plan, ta, tb, tc, td, work)
         torch.cuda.synchronize()
     x = layout_to_vec(td.cpu().numpy(), lay, n, nsys, m)
-    # 128 systems: the FP64 solve runs twisted (another elimination order), so the two agree to
-    # the FP64 floor at this conditioning rather than bit for bit
-    assert rel_err(x, x64).max() <= 1e-9
-    err = rel_err(x, xo)
-    print(f"FP32 storage, FP64 arithmetic, cond up to 1e6: max rel err vs oracle {err.max():.2e}")
-    assert err.max() <= 1e-9  # the two FP64 solvers' floor at cond 1e6 (6.5e-11 measured)
+    # the FP32-storage solve computes in the oracle's operation order: its solve of the stored
+    # values bit for bit
+    np.testing.assert_array_equal(x, xo)
+    # 128 systems: the FP64 solve runs twisted (another elimination order); against the
+    # binary128 truth it must be as accurate as the oracle, within a factor 10
+    xt, _ = oracle.solve_truth(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    e_tw, e_or = rel_err(x64, xt).max(), rel_err(xo, xt).max()
+    print(f"cond up to 1e6: oracle vs truth {e_or:.2e}, twisted FP64 vs truth {e_tw:.2e}")
+    assert e_tw <= 10 * e_or
 
 
 def test_fp32_storage_rejects_partitions(cuda):

@@ -303,3 +303,71 @@ def test_api_errors_on_device(cuda):
         v = torch.zeros(64 * 2 * 3, dtype=torch.float64, device=cuda)
         with pytest.raises(ValueError):
             pb.solve_cyclic(plan, t, t, t, v, 3, 64, 2)
+
+
+def _solve_form(cuda, A, B, C, d, layout, form):
+    x, sing = gpu_solve(A, B, C, d, int(layout), form=int(form))
+    assert not sing
+    return x
+
+
+@pytest.mark.parametrize("m", list(range(1, 11)))
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_lapack_order_form_is_the_oracle_bit_for_bit(cuda, oracle, m, layout):
+    """BTD_FORM_LAPACK_ORDER: every block operation in the oracle's order and rounding, so x
+    equals the oracle's btdma_many (mod_btdma_cpu.f90:15-78) bit for bit, every m, both
+    layouts, on row-permuted blocks (pivoting in every LU)."""
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 600 + m, m, 70, 13)
+    x = _solve_form(cuda, A, B, C, d, layout, pb.Form.LAPACK_ORDER)
+    np.testing.assert_array_equal(x, oracle_plain(oracle, A, B, C, d))
+
+
+@pytest.mark.parametrize("kind", [pb.GenKind.COND, pb.GenKind.EULER])
+def test_lapack_order_form_conditioned_blocks(cuda, oracle, kind):
+    """On the conditioned blocks (cond(B) log-uniform in 1e1..1e6, m = 5, n = 512) the LAPACK-order
+    form equals the oracle bit for bit. The library's choice for 128 systems is the twisted
+    solve (another elimination order), which differs from the oracle by rounding (up to ~1e-10
+    there); against the binary128 truth it must be as accurate as the oracle within 10x."""
+    m, nsys, n = 5, 128, 512
+    A, B, C, d = pb.generate_host(kind, 33, m, nsys, n)
+    xo = oracle_plain(oracle, A, B, C, d)
+    x = _solve_form(cuda, A, B, C, d, pb.Layout.TILED, pb.Form.LAPACK_ORDER)
+    np.testing.assert_array_equal(x, xo)
+    xt, info = oracle.solve_truth(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    xd = _solve_form(cuda, A, B, C, d, pb.Layout.TILED, pb.Form.PLAIN)
+    e_ora, e_dev = rel_err(xo, xt).max(), rel_err(xd, xt).max()
+    print(f"{kind.name}: vs truth oracle {e_ora:.2e}, default device {e_dev:.2e}; "
+          f"device vs oracle {rel_err(xd, xo).max():.2e}")
+    assert e_dev <= 10 * max(e_ora, 1e-15)
+
+
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_lapack_order_form_f32_storage(cuda, oracle, layout):
+    """btd_solve_noncyclic_f32s (one lane per system in the oracle's operation order, whatever
+    the plan's form): the FP64 solve of the FP32-stored blocks equals the oracle's solve of
+    those values bit for bit."""
+    import torch
+    m, nsys, n = 5, 70, 64
+    lay = int(layout)
+    A, B, C, d = pb.generate_host(pb.GenKind.COND, 35, m, nsys, n)  # blocks are FP32 values
+    f32 = lambda X: torch.from_numpy(nat_to_layout(X, lay).astype(np.float32)).to(cuda)
+    ta, tb, tc = f32(A), f32(B), f32(C)
+    td = torch.from_numpy(nat_to_layout(d, lay)).to(cuda)
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
+        plan.set_form(pb.Form.LAPACK_ORDER)
+        work = torch.empty(pb.layout_elems(layout, nsys, n, m * m), dtype=torch.float64, device=cuda)
+        pb.solve_noncyclic_f32s(plan, ta, tb, tc, td, work)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    x = layout_to_vec(td.cpu().numpy(), lay, n, nsys, m)
+    np.testing.assert_array_equal(x, oracle_plain(oracle, A, B, C, d))
+
+
+def test_lapack_order_form_rules(cuda):
+    with pb.BtdmaGpuPlan().create(3, 70, 30, 2, pb.Layout.TILED) as plan:
+        with pytest.raises(ValueError, match="npart == 1"):
+            plan.set_form(pb.Form.LAPACK_ORDER)
+    with pb.BtdmaGpuPlan().create(1, 70, 30, 1, pb.Layout.TILED) as plan:
+        plan.set_form(pb.Form.LAPACK_ORDER)
+        assert plan.form == pb.Form.LAPACK_ORDER

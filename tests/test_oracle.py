@@ -224,3 +224,63 @@ def test_cyclic_two_stage_matches_dense_periodic(oracle, m, n, npart):
     np.testing.assert_allclose(x, dense_x(oracle, A, B, C, d, cyclic=True), rtol=0, atol=1e-12)
     x1, _ = oracle.solve_cyclic(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
     np.testing.assert_allclose(x, x1, rtol=0, atol=1e-13)
+
+
+# ------------------------------------------------------------------ truth (binary128) --
+
+def _exact_dense_solve(T, rhs):
+    """Gaussian elimination with partial pivoting in exact rational arithmetic
+    (fractions.Fraction reads each double exactly); returns the exact solution as Fractions."""
+    from fractions import Fraction
+    N = T.shape[0]
+    M = [[Fraction(float(v)) for v in row] + [Fraction(float(r))] for row, r in zip(T, rhs)]
+    for k in range(N):
+        p = max(range(k, N), key=lambda i: abs(M[i][k]))
+        M[k], M[p] = M[p], M[k]
+        for i in range(k + 1, N):
+            if M[i][k] != 0:
+                f = M[i][k] / M[k][k]
+                M[i] = [a - f * b for a, b in zip(M[i], M[k])]
+    x = [Fraction(0)] * N
+    for k in range(N - 1, -1, -1):
+        x[k] = (M[k][N] - sum(M[k][j] * x[j] for j in range(k + 1, N))) / M[k][k]
+    return x
+
+
+@pytest.mark.parametrize("kind,m,n,nsys", [(1, 3, 4, 3),   # DDPERM: permuted dominant rows
+                                           (3, 5, 3, 3),   # COND: cond(B) log-uniform 1e1..1e6
+                                           (3, 2, 6, 3),
+                                           (2, 5, 3, 2)])  # EULER-like
+def test_truth_equals_exact_rational_solution(oracle, kind, m, n, nsys):
+    """The binary128 solve rounded to FP64 equals the exact solution of the FP64 inputs
+    (rational arithmetic on the assembled dense matrix) to 2 ulp in every component."""
+    import pascal_btdma_amd as pb
+    A, B, C, d = pb.generate_host(pb.GenKind(kind), 4242 + m, m, nsys, n)
+    xq, info = oracle.solve_truth(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    for s in range(nsys):
+        T = oracle.assemble_dense(A, B, C, s)
+        exact = _exact_dense_solve(T, d[:, s, :].reshape(-1))
+        xe = np.array([float(v) for v in exact])  # correctly rounded
+        got = xq[:, s, :].reshape(-1)
+        ulp = np.spacing(np.abs(xe))
+        assert np.all(np.abs(got - xe) <= 2 * ulp), (s, np.max(np.abs(got - xe) / ulp))
+
+
+def test_truth_matches_fp64_oracle_when_well_conditioned(oracle):
+    """On diagonally dominant rows the FP64 oracle is accurate to a few ulps, so the two agree
+    to 1e-15; on blocks of condition number up to 1e6 the FP64 oracle's distance from the
+    truth grows to about cond * 1e-16 (here above 1e-13 for some systems), which a correct
+    truth must show and a truth that merely repeated the FP64 arithmetic would not."""
+    import pascal_btdma_amd as pb
+    from helpers import rel_err
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 17, 5, 64, 96)
+    x, _ = oracle.solve_plain(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    xq, info = oracle.solve_truth(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    assert rel_err(x, xq).max() <= 1e-15
+    A, B, C, d = pb.generate_host(pb.GenKind.COND, 17, 5, 64, 96)
+    x, _ = oracle.solve_plain(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    xq, _ = oracle.solve_truth(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    e = rel_err(x, xq)
+    assert e.max() > 1e-13 and e.max() < 1e-9
