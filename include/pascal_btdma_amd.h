@@ -378,6 +378,24 @@ btd_status btd_graph_end(void *stream, btd_graph_t *graph);
 btd_status btd_graph_launch(btd_graph_t graph, void *stream);
 btd_status btd_graph_destroy(btd_graph_t graph);
 
+/* ---- Allocation-free solve ------------------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The non-periodic
+ * solve of btd_solve_noncyclic (same kernels, same in-place contract and layouts; npart = 0
+ * lets the library choose as btd_plan_create does, npart = 1 the single-partition solve) with
+ * no plan: the scratch a plan would allocate (the two-stage solve's reduced system of
+ * 2*npart rows per system, its partition table, the zero-pivot words) lives in one static
+ * device array of BTD_NOALLOC_SCRATCH_BYTES per device, held until the process ends. A solve
+ * needing more (about pad64(nsys) * 2*npart * (2m^2 + m) * 8 bytes plus 4*(nsys + 1)) is
+ * refused with BTD_ERR_INVALID_ARG before anything runs. The call allocates nothing (no
+ * hipMalloc, hipFree or allocating library routine); its first call on a device looks up the
+ * array's address there. Every call on a device shares that device's array, so calls that
+ * overlap on one device (other streams, other threads) race; calls in one stream are ordered.
+ * Zero pivots are not reported (the words are cleared by every call). Asynchronous. */
+#define BTD_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+btd_status btd_solve_noncyclic_noalloc(double *a, const double *b, double *c, double *d, int m, int nsys,
+                                       int nrow, int npart, int layout, void *stream);
+
 /* Library version string. */
 const char *btd_version(void);
 

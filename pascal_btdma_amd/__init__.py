@@ -256,6 +256,20 @@ def solve_cyclic(plan: BtdmaGpuPlan, a, b, c, d, m: int, nsys: int, nrow: int,
                                         int(nsys), int(nrow), _stream(stream)))
 
 
+def solve_noncyclic_noalloc(a, b, c, d, m: int, nsys: int, nrow: int, npart: int = 1,
+                            layout: Layout = Layout.TILED, stream=None) -> None:
+    """The non-periodic solve without a plan and without device allocation
+    (btd_solve_noncyclic_noalloc): scratch in the library's static device array, at most
+    BTD_NOALLOC_SCRATCH_BYTES (64 MiB); calls that overlap on one device race."""
+    import torch
+    nm = layout_elems(layout, nsys, nrow, m * m)
+    nv = layout_elems(layout, nsys, nrow, m)
+    for name, t, n in (("a", a, nm), ("b", b, nm), ("c", c, nm), ("d", d, nv)):
+        _check_tensor(name, t, n, torch.float64)
+    _check(_capi.lib().btd_solve_noncyclic_noalloc(_ptr(a), _ptr(b), _ptr(c), _ptr(d), int(m), int(nsys),
+                                                   int(nrow), int(npart), int(layout), _stream(stream)))
+
+
 def solve_noncyclic_save(plan: BtdmaGpuPlan, a, b, c, d, factors, pivots, stream=None) -> None:
     """Forward solve that keeps the block factorization (LU of R_q and pivots) for the
     adjoint. In place like solve_noncyclic: c returns C', d returns x."""

@@ -78,6 +78,7 @@ SIGNATURES = {
     "btd_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
     "btd_graph_launch": (_i, [_vp, _vp]),
     "btd_graph_destroy": (_i, [_vp]),
+    "btd_solve_noncyclic_noalloc": (_i, [_dp, _dp, _dp, _dp, _i, _i, _i, _i, _i, _vp]),
 }
 
 _lib = None

@@ -37,6 +37,7 @@ UNITS = [("btd_inst.hip", [f"-DBTD_M={m}"], f"btd_inst_m{m}.o") for m in range(1
     ("btd_batch_api.hip", [], "btd_batch_api.o"),
     ("btd_euler.hip", [], "btd_euler.o"),
     ("btd_weno.hip", [], "btd_weno.o"),
+    ("btd_noalloc.hip", [], "btd_noalloc.o"),
 ]
 
 
