@@ -101,8 +101,8 @@ def build_cpp_examples(verbose: bool = False) -> None:
 
 
 def build_fortran_examples(verbose: bool = False) -> None:
-    """The Fortran binding module (include/mod_btdma_amd.f90) and the Fortran programs under
-    tests/fortran, compiled with the image's amdflang (skipped if it is absent)."""
+    """The Fortran binding modules (include/mod_btdma_amd.f90, include/mod_btdma_gpu_v2.f90) and the
+    Fortran programs under tests/fortran, compiled with the image's amdflang (skipped if it is absent)."""
     root = os.path.dirname(HERE)
     src_dir = os.path.join(root, "tests", "fortran")
     fc = shutil.which("amdflang") or "/opt/rocm/lib/llvm/bin/amdflang"
@@ -110,18 +110,21 @@ def build_fortran_examples(verbose: bool = False) -> None:
         return
     out_dir = os.path.join(src_dir, "bin")
     os.makedirs(out_dir, exist_ok=True)
-    mod_obj = os.path.join(out_dir, "mod_btdma_amd.o")
-    cmd = [fc, "-O2", "-c", os.path.join(root, "include", "mod_btdma_amd.f90"), "-module-dir", out_dir,
-           "-o", mod_obj]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"building the Fortran module failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    mod_objs = []
+    for mod in ("mod_btdma_amd", "mod_btdma_gpu_v2"):
+        mod_obj = os.path.join(out_dir, mod + ".o")
+        cmd = [fc, "-O2", "-c", os.path.join(root, "include", mod + ".f90"), "-module-dir", out_dir,
+               "-o", mod_obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building the Fortran module failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        mod_objs.append(mod_obj)
     rocm_lib = os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "lib")
     for f in sorted(os.listdir(src_dir)):
         if not f.endswith(".f90"):
             continue
         exe = os.path.join(out_dir, os.path.splitext(f)[0])
-        cmd = [fc, "-O2", os.path.join(src_dir, f), mod_obj, "-I", out_dir, "-L", HERE, "-lpascal_btdma_amd",
+        cmd = [fc, "-O2", os.path.join(src_dir, f), *mod_objs, "-I", out_dir, "-L", HERE, "-lpascal_btdma_amd",
                "-L", rocm_lib, "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../../pascal_btdma_amd", "-o", exe]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
