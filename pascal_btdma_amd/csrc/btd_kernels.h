@@ -19,6 +19,8 @@
 //   k_update     stage 3, btdma_many_update (mod_btdma_cpu.f90:511-535).
 //   k_generate   seeded input generator (bench and tests).
 #pragma once
+#include <type_traits>
+
 #include "btd_coop.h"
 #include "btd_internal.h"
 
@@ -886,7 +888,17 @@ constexpr int kXcds = 8;
 // one partition go to the workgroups of one XCD in order, so neighbouring systems' shared
 // lines meet in one L2. Used for m >= 7, where the one-lane form spills its blocks to scratch
 // memory.
-template <int M, int LAYOUT, int WV>
+// Rows k_modi_coop loads ahead: 4 for m <= 5 (a deeper ring costs the occupancy of a second
+// wavefront per SIMD; 2 rows measured no faster at m = 5), 1 from m = 6.
+template <int M>
+constexpr int kModiAhead = M <= 5 ? 4 : 1;
+
+// LOCK: the WV wavefronts of a workgroup (consecutive systems of one partition) meet at a
+// barrier (s_barrier, no memory wait) before each row's loads, so the cache lines their
+// systems share (floor(64/m) systems a wavefront do not fill whole 128-byte lines) are
+// requested together and come from HBM once; every wavefront then runs every row step, those
+// past the last system on a real one, storing nothing.
+template <int M, int LAYOUT, int WV, bool LOCK = false>
 __global__ __launch_bounds__(64 * WV) void k_modi_coop(double* A, const double* __restrict__ B, double* C,
                                                        double* D, int nsys, int nrow,
                                                        const int* __restrict__ part_first,
@@ -903,15 +915,17 @@ __global__ __launch_bounds__(64 * WV) void k_modi_coop(double* A, const double* 
     const int u = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     if (u >= nitems) return;
     const int w = threadIdx.x / kTile, lane = threadIdx.x % kTile;
-    const int part = u / ngroups, chunk = (u % ngroups) * WV + w;
-    if (chunk >= nchunks) return;  // the wavefronts share no barrier
+    const int part = u / ngroups, chunk0 = (u % ngroups) * WV + w;
+    if (!LOCK && chunk0 >= nchunks) return;  // the wavefronts share no barrier
+    const bool live = chunk0 < nchunks;
+    const int chunk = live ? chunk0 : nchunks - 1;
     double* sh = shw[w];
     const int g = lane / M, i = lane % M;
     const bool grouped = g < G;
     const int64_t s0 = (int64_t)chunk * G;
     const int64_t sys = s0 + (grouped ? g : 0);
-    const bool mine = grouped && sys < nsys;
-    const int64_t se = mine ? sys : s0;  // a real system either way
+    const bool mine = live && grouped && sys < nsys;
+    const int64_t se = grouped && sys < nsys ? sys : s0;  // a real system either way
     // lanes past the last whole group read group 0's posts and write only their own slots
     const int base = grouped ? g * M : 0;
     const Addr<LAYOUT> am(se, nsys, nrow, E), av(se, nsys, nrow, M);
@@ -921,35 +935,43 @@ __global__ __launch_bounds__(64 * WV) void k_modi_coop(double* A, const double* 
     double* carry = sh;                     // slots [0, M) Ĉ, [M, 2M) Â (slot k, lane base+j: (k, j))
     double* xb = sh + (2 * M + 1) * kTile;  // the elimination exchange, 3M + 2 slots
     bool ok = true;
-    double pa[M], pb[M], pc[M], pd;
-    auto fetch = [&](int q) {
+    // rows loaded F ahead (one wavefront per item has few others to hide HBM latency behind);
+    // a row past the partition's end repeats its last row (loaded, never used)
+    constexpr int F = kModiAhead<M>;
+    double pa[F][M], pb[F][M], pc[F][M], pd[F];
+    auto fetch = [&](double (&fa)[M], double (&fb)[M], double (&fc)[M], double& fd, int q) {
         const int64_t gr = r0 + (q < N ? q : N - 1);
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-            pa[k] = A[am.at(gr, i + M * k)];
-            pb[k] = __builtin_nontemporal_load(B + am.at(gr, i + M * k));
-            pc[k] = C[am.at(gr, i + M * k)];
+            fa[k] = A[am.at(gr, i + M * k)];
+            fb[k] = __builtin_nontemporal_load(B + am.at(gr, i + M * k));
+            fc[k] = C[am.at(gr, i + M * k)];
         }
-        pd = D[av.at(gr, i)];
+        fd = D[av.at(gr, i)];
     };
-    fetch(0);
+#pragma unroll
+    for (int t = 0; t < F; ++t) fetch(pa[t], pb[t], pc[t], pd[t], t);
     double dp[M], beta[M], gamma[M], alpha = 0.0;
 #pragma unroll
     for (int k = 0; k < M; ++k) dp[k] = beta[k] = gamma[k] = 0.0;
-    for (int q = 0; q < N; ++q) {
+    for (int q0 = 0; q0 < N; q0 += F)
+#pragma unroll
+    for (int t = 0; t < F; ++t) {
+        const int q = q0 + t;
+        if (q >= N) break;  // uniform: every lane of the wavefront is in this partition
         const int64_t gr = r0 + q;
         double r[M], rhs[2 * M + 1];
-        rhs[0] = pd;
+        rhs[0] = pd[t];
         if (q >= 2) {
 #pragma unroll
             for (int j = 0; j < M; ++j) {
-                r[j] = pb[j];
+                r[j] = pb[t][j];
                 rhs[1 + j] = 0.0;
-                rhs[1 + M + j] = pc[j];
+                rhs[1 + M + j] = pc[t][j];
             }
 #pragma unroll
             for (int k = 0; k < M; ++k) {
-                const double a = pa[k];
+                const double a = pa[t][k];
                 rhs[0] = fma(-a, dp[k], rhs[0]);
 #pragma unroll
                 for (int j = 0; j < M; ++j) {
@@ -960,12 +982,13 @@ __global__ __launch_bounds__(64 * WV) void k_modi_coop(double* A, const double* 
         } else {
 #pragma unroll
             for (int j = 0; j < M; ++j) {
-                r[j] = pb[j];
-                rhs[1 + j] = pa[j];
-                rhs[1 + M + j] = pc[j];
+                r[j] = pb[t][j];
+                rhs[1 + j] = pa[t][j];
+                rhs[1 + M + j] = pc[t][j];
             }
         }
-        fetch(q + 1);
+        if constexpr (LOCK) __builtin_amdgcn_s_barrier();
+        fetch(pa[t], pb[t], pc[t], pd[t], q + F);
         double xc[2 * M], xd[M];
         ok = !coop_eliminate<M, 2>(r, rhs, xb, i, base, xc, xd) && ok;
         const bool keep = q + 1 < N;  // the last row only feeds the reduced system
@@ -1138,6 +1161,13 @@ constexpr int modi_coop_waves() { return M >= 9 ? 2 : 1; }
 #define BTD_MODI_COOP_FROM 7
 #endif
 constexpr int kModiCoopFrom = BTD_MODI_COOP_FROM;
+
+// Below kModiCoopFrom, stage 1 runs one lane per partition from this many partitions in the
+// batch (nsys * npart) and m lanes per partition below it.
+#ifndef BTD_MODI_LANE_FROM
+#define BTD_MODI_LANE_FROM 65536
+#endif
+constexpr int64_t kModiLaneFrom = BTD_MODI_LANE_FROM;
 
 // Largest number of wavefronts per CU for which the reduced system is solved twisted.
 #ifndef BTD_TWISTED_REDUCED_WAVES_PER_CU
@@ -1578,6 +1608,87 @@ __global__ __launch_bounds__(64) void k_update(const double* __restrict__ A,
     }
 }
 
+// Stage 3 with a partition's block rows over m lanes (lane i owns row i), for the batches whose
+// stage 1 runs k_modi_coop: the same x_q = d̂_q - Â_q x_1 - Ĉ_q x_{q+1} (the same operation
+// order as k_update, so the same bits), with x_{q+1} exchanged through LDS and the rows loaded
+// kAhead ahead; floor(64/m) consecutive systems of one partition per wavefront, WV wavefronts
+// (LOCK: meeting before each row's loads, as in k_modi_coop) per workgroup, the items of a
+// partition on one XCD.
+template <int M>
+struct UpdateDepth {
+    static constexpr int kAhead = M <= 2 ? 12 : M <= 3 ? 8 : M <= 5 ? 5 : 3;
+};
+
+template <int M, int LAYOUT, int WV = 1, bool LOCK = false>
+__global__ __launch_bounds__(64 * WV) void k_update_coop(const double* __restrict__ A,
+                                                         const double* __restrict__ C, double* D, int nsys,
+                                                         int nrow, const int* __restrict__ part_first,
+                                                         const int* __restrict__ part_len, int npart,
+                                                         const double* __restrict__ redD) {
+    constexpr int G = kTile / M;  // systems per wavefront
+    constexpr int K = UpdateDepth<M>::kAhead;
+    __shared__ double shw[WV][kTile];
+    const int nchunks = (nsys + G - 1) / G;
+    const int ngroups = (nchunks + WV - 1) / WV;
+    const int nitems = npart * ngroups;
+    const int per = (nitems + kXcds - 1) / kXcds;
+    const int u = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (u >= nitems) return;
+    const int w = threadIdx.x / kTile, lane = threadIdx.x % kTile;
+    const int part = u / ngroups, chunk0 = (u % ngroups) * WV + w;
+    if (!LOCK && chunk0 >= nchunks) return;  // the wavefronts share no barrier
+    const bool live = chunk0 < nchunks;
+    const int chunk = live ? chunk0 : nchunks - 1;
+    double* sh = shw[w];
+    const int g = lane / M, i = lane % M;
+    const bool grouped = g < G;
+    const int64_t s0 = (int64_t)chunk * G;
+    const int64_t sys = s0 + (grouped ? g : 0);
+    const bool mine = live && grouped && sys < nsys;
+    const int64_t se = grouped && sys < nsys ? sys : s0;  // a real system either way
+    const int base = grouped ? g * M : 0;
+    const Addr<LAYOUT> am(se, nsys, nrow, M * M), av(se, nsys, nrow, M);
+    const Addr<LAYOUT_TILED> rv(se, 0, 2 * npart, M);
+    const int64_t r0 = part_first[part];
+    const int N = part_len[part];  // >= 3
+    double x1[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) x1[j] = redD[rv.at(2 * part, j)];
+    double xn = redD[rv.at(2 * part + 1, i)];  // element i of x_{q+1}
+    if (mine) {
+        D[av.at(r0, i)] = lane_elem<M>(x1, i);
+        D[av.at(r0 + N - 1, i)] = xn;
+    }
+    double pa[K][M], pc[K][M], pd[K];
+    auto fetch = [&](double (&fa)[M], double (&fc)[M], double& fd, int q) {
+        const int64_t gr = r0 + (q < 1 ? 1 : q);  // rows 1 .. N-2
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            fa[j] = A[am.at(gr, i + M * j)];
+            fc[j] = C[am.at(gr, i + M * j)];
+        }
+        fd = D[av.at(gr, i)];
+    };
+#pragma unroll
+    for (int t = 0; t < K; ++t) fetch(pa[t], pc[t], pd[t], N - 2 - t);
+    for (int q0 = N - 2; q0 >= 1; q0 -= K)
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+            const int q = q0 - t;
+            if (q < 1) break;  // uniform: every lane of the wavefront is in this partition
+            sh[lane] = xn;
+            wave_lds_sync();
+            double x = pd[t];
+#pragma unroll
+            for (int j = 0; j < M; ++j) x = fma(-pa[t][j], x1[j], fma(-pc[t][j], sh[base + j], x));
+            wave_lds_sync();
+            if constexpr (LOCK) __builtin_amdgcn_s_barrier();
+            fetch(pa[t], pc[t], pd[t], q - K);
+            if (mine) D[av.at(r0 + q, i)] = x;
+            xn = x;
+        }
+}
+
 // --------------------------------------------------------------------------- generate --
 template <int M>
 __global__ __launch_bounds__(256) void k_generate(int kind, uint64_t seed, int layout, int nsys,
@@ -1804,24 +1915,43 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         }
     }
-    if constexpr (M >= kModiCoopFrom) {
-        // m lanes per partition (no spills); the items of a partition on one XCD
-        constexpr int WV = modi_coop_waves<M>();
-        const int64_t chunks = (nsys + kTile / M - 1) / (kTile / M);
-        const int64_t items = (int64_t)npart * ((chunks + WV - 1) / WV);
-        const dim3 gridc((unsigned)(kXcds * ((items + kXcds - 1) / kXcds))), blockc(64 * WV);
-        if (layout == LAYOUT_TILED)
-            hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_TILED, WV>), gridc, blockc, 0, s, A, B, C, D, nsys,
-                               nrow, pf, pl, npart, redA, redC, redD, flag);
-        else
-            hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_REF_GPU, WV>), gridc, blockc, 0, s, A, B, C, D, nsys,
-                               nrow, pf, pl, npart, redA, redC, redD, flag);
-    } else if (layout == LAYOUT_TILED) {
-        hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED, BS>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf, pl,
-                           npart, redA, redC, redD, flag, bs);
+    // Stage 1: m lanes per partition (k_modi_coop) for m >= kModiCoopFrom, and for smaller m
+    // (non-periodic, blocks in memory) when the partitions are too few to fill the device one
+    // lane each (fewer than kModiLaneFrom of them); otherwise one lane per partition (k_modi).
+    // Periodic solves keep k_modi below kModiCoopFrom, so that a matrix-free sweep (k_modi with
+    // its own block source) and the assembled periodic solve run the same arithmetic.
+    constexpr bool kCanCoop = M >= 2 && std::is_same<BS, MemBlocks>::value;
+    bool coop = M >= kModiCoopFrom;
+    if constexpr (kCanCoop && M < kModiCoopFrom) coop = !cyclic && (int64_t)nsys * npart < kModiLaneFrom;
+    if (coop) {
+        if constexpr (kCanCoop) {
+            // the items of a partition on one XCD. floor(64/m) systems a wavefront leave partial
+            // lines, which neighbouring wavefronts fetch again (m = 5, 512 x 65536, P = 32: 42 GB
+            // read vs 21.5 GB one lane per partition); four wavefronts locked per workgroup read
+            // them once but were slower at m = 5 (P = 8 / 16 / 32 / 64: 25.0 / 15.1 / 16.0 / 12.7 ->
+            // 38.2 / 19.3 / 16.6 / 12.8 ms): with so few partitions the chain of rows, not HBM,
+            // bounds stage 1, and locked workgroups occupy fewer CUs
+            constexpr bool kLock = false;
+            constexpr int WV = modi_coop_waves<M>();
+            const int64_t chunks = (nsys + kTile / M - 1) / (kTile / M);
+            const int64_t items = (int64_t)npart * ((chunks + WV - 1) / WV);
+            const dim3 gridc((unsigned)(kXcds * ((items + kXcds - 1) / kXcds))), blockc(64 * WV);
+            if (layout == LAYOUT_TILED)
+                hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_TILED, WV, kLock>), gridc, blockc, 0, s, A, B, C, D,
+                                   nsys, nrow, pf, pl, npart, redA, redC, redD, flag);
+            else
+                hipLaunchKernelGGL((k_modi_coop<M, LAYOUT_REF_GPU, WV, kLock>), gridc, blockc, 0, s, A, B, C, D,
+                                   nsys, nrow, pf, pl, npart, redA, redC, redD, flag);
+        }
     } else {
-        hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU, BS>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf,
-                           pl, npart, redA, redC, redD, flag, bs);
+        if constexpr (M < kModiCoopFrom) {
+            if (layout == LAYOUT_TILED)
+                hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED, BS>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf,
+                                   pl, npart, redA, redC, redD, flag, bs);
+            else
+                hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU, BS>), gridp, block, lds, s, A, B, C, D, nsys, nrow,
+                                   pf, pl, npart, redA, redC, redD, flag, bs);
+        }
     }
     if (ev) (void)hipEventRecord(ev[1], s);
     // the twisted reduced solve when its wavefronts all fit on the device at once (one per
@@ -1840,12 +1970,43 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     else if (!twisted)
         hipLaunchKernelGGL((k_reduced<M>), gridr, block, 0, s, redA, redC, redD, nsys, 2 * npart, flag);
     if (ev) (void)hipEventRecord(ev[2], s);
-    if (layout == LAYOUT_TILED)
+    // stage 3 m lanes per partition wherever stage 1 ran that way (M >= 2)
+    bool coop3 = false;
+    if constexpr (M >= 2) coop3 = M >= kModiCoopFrom || (!cyclic && (int64_t)nsys * npart < kModiLaneFrom);
+    if (coop3) {
+        if constexpr (M >= 2) {
+            // four wavefronts locked per workgroup (their shared lines read once: m = 5, 512 x 65536,
+            // P = 16 / 32 / 64: 6.5 / 5.6 / 5.4 -> 4.6 / 4.6 / 3.6 ms) once there are two
+            // wavefronts of work per CU; below that one wavefront per workgroup keeps more CUs busy
+            // (P = 8: 7.4 ms unlocked, 9.1 locked)
+            const int64_t chunks = (nsys + kTile / M - 1) / (kTile / M);
+            const bool lock = (int64_t)npart * chunks >= 2 * (int64_t)device_cus();
+            const int WV = lock ? 4 : 1;
+            const int64_t items = (int64_t)npart * ((chunks + WV - 1) / WV);
+            const dim3 gridu((unsigned)(kXcds * ((items + kXcds - 1) / kXcds))), blocku(64 * WV);
+            if (lock) {
+                if (layout == LAYOUT_TILED)
+                    hipLaunchKernelGGL((k_update_coop<M, LAYOUT_TILED, 4, true>), gridu, blocku, 0, s, A, C, D,
+                                       nsys, nrow, pf, pl, npart, redD);
+                else
+                    hipLaunchKernelGGL((k_update_coop<M, LAYOUT_REF_GPU, 4, true>), gridu, blocku, 0, s, A, C, D,
+                                       nsys, nrow, pf, pl, npart, redD);
+            } else {
+                if (layout == LAYOUT_TILED)
+                    hipLaunchKernelGGL((k_update_coop<M, LAYOUT_TILED, 1, false>), gridu, blocku, 0, s, A, C, D,
+                                       nsys, nrow, pf, pl, npart, redD);
+                else
+                    hipLaunchKernelGGL((k_update_coop<M, LAYOUT_REF_GPU, 1, false>), gridu, blocku, 0, s, A, C, D,
+                                       nsys, nrow, pf, pl, npart, redD);
+            }
+        }
+    } else if (layout == LAYOUT_TILED) {
         hipLaunchKernelGGL((k_update<M, LAYOUT_TILED>), gridp, block, 0, s, A, C, D, nsys, nrow, pf, pl,
                            npart, redD);
-    else
+    } else {
         hipLaunchKernelGGL((k_update<M, LAYOUT_REF_GPU>), gridp, block, 0, s, A, C, D, nsys, nrow, pf,
                            pl, npart, redD);
+    }
     if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
 }

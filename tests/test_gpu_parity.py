@@ -371,3 +371,31 @@ def test_lapack_order_form_rules(cuda):
     with pb.BtdmaGpuPlan().create(1, 70, 30, 1, pb.Layout.TILED) as plan:
         plan.set_form(pb.Form.LAPACK_ORDER)
         assert plan.form == pb.Form.LAPACK_ORDER
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 5, 6])
+@pytest.mark.parametrize("layout", [pb.Layout.TILED, pb.Layout.REF_GPU])
+def test_two_stage_one_lane_per_partition(cuda, oracle, m, layout):
+    """From 65536 partitions in the batch (nsys * npart) stage 1 and stage 3 run one lane per
+    partition (k_modi, k_update); below that, m lanes per partition (k_modi_coop,
+    k_update_coop), which the smaller two-stage tests cover. 16387 systems x 4 partitions of
+    n = 24 rows; sampled systems (first tile, a middle one, the partial last) vs the oracle's
+    two-stage solve."""
+    import torch
+    nsys, n, npart = 16387, 24, 4
+    nm = pb.layout_elems(layout, nsys, n, m * m)
+    nv = pb.layout_elems(layout, nsys, n, m)
+    t = [torch.empty(nm, dtype=torch.float64, device=cuda) for _ in range(3)]
+    td = torch.empty(nv, dtype=torch.float64, device=cuda)
+    pb.generate(pb.GenKind.DDPERM, 70 + m, m, nsys, n, layout, *t, td)
+    with pb.BtdmaGpuPlan().create(m, nsys, n, npart, layout) as plan:
+        pb.solve_noncyclic(plan, t[0], t[1], t[2], td, m, nsys, n)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    flat = td.cpu().numpy()
+    for s0, cnt in ((0, 64), (64 * 128, 64), (16384, 3)):
+        A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 70 + m, m, nsys, n, s0, cnt)
+        xo, info = oracle.solve_two_stage(cpu_layout(A), cpu_layout(B), cpu_layout(C), d, npart)
+        assert info == 0
+        full = layout_to_vec(flat, int(layout), n, nsys, m)
+        assert rel_err(full[:, s0:s0 + cnt], xo).max() <= TOL
