@@ -260,6 +260,28 @@ def extra_config0(dev, checks: ParityChecks) -> dict:
            "max_rel_err_vs_oracle_all_systems": None}
     checks.add(lambda: pb.generate_host(w["kind"], w["seed"], m, nsys, n), x,
                max_sink(out, "max_rel_err_vs_oracle_all_systems"))
+    # the same shape with the configuration's generator taken word for word (only B's rows
+    # permuted; pb.GenKind.BPERM), solved by the same plan form
+    pb.generate(pb.GenKind.BPERM, w["seed"], m, nsys, n, lay, a, b, c, d)
+    c0, d0 = c.clone(), d.clone()
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, lay) as plan:
+        def restore_b():
+            c.copy_(c0)
+            d.copy_(d0)
+        restore_b()
+        pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n)
+        tb = event_time(lambda: pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n), restore_b, 10,
+                        torch.cuda.current_stream())
+    xb = tiled_vec_subset(d.cpu().numpy(), n, m, list(range(nsys // 64)))
+    r = out["literal_generator"] = {
+        "workload": "the same shape, B = U[-1,1] + diag(row sums |A|+|C| + 1) with only B's rows permuted",
+        "ms": float(np.median(tb)), "max_rel_err_vs_oracle_all_systems": None,
+        "max_rel_err_vs_truth_all_systems": None}
+
+    def truth_sink(e):
+        r["max_rel_err_vs_truth_all_systems"] = float(np.max(e))
+    checks.add(lambda: pb.generate_host(pb.GenKind.BPERM, w["seed"], m, nsys, n), xb,
+               max_sink(r, "max_rel_err_vs_oracle_all_systems"), truth_sink)
     return out
 
 

@@ -320,7 +320,9 @@ btd_status btd_convert_layout(int src_layout, int dst_layout, int elems_per_bloc
 /* Seeded synthetic inputs written on the device in `layout`. kind: 0 = Study diagonal
  * workload (A=0.25I, B=2I, C=0.25I, d=1), 1 = diagonally dominant rows with each whole
  * block row permuted, 2 = Euler-like (J based, sigma 0.4), 3 = conditioned B blocks
- * (cond log-uniform in [1e1,1e6]) with A, B, C rounded to FP32. Any pointer may be NULL. */
+ * (cond log-uniform in [1e1,1e6]) with A, B, C rounded to FP32, 4 = A, C uniform in [-1,1],
+ * B uniform in [-1,1] plus (row sum |A| + row sum |C| + 1) on the diagonal with only B's rows
+ * then permuted. Any pointer may be NULL. */
 btd_status btd_generate(int kind, uint64_t seed, int m, int nsys, int nrow, int layout, double *a,
                         double *b, double *c, double *d, void *stream);
 

@@ -48,6 +48,7 @@ class GenKind(enum.IntEnum):
     DDPERM = 1
     EULER = 2
     COND = 3
+    BPERM = 4
 
 
 class BtdmaError(RuntimeError):

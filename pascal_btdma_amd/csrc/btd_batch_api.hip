@@ -256,7 +256,7 @@ btd_status btd_batch_set_params(btd_batch_t b, int placement, int grid) {
 btd_status btd_batch_generate(btd_batch_t b, int kind, uint64_t seed, double* a, double* bb,
                               double* c, double* d, void* stream) {
     if (!b) return bfail(BTD_ERR_NOT_CREATED, "btd_batch_generate: NULL batch");
-    if (!a || !bb || !c || !d || kind < 0 || kind > 3)
+    if (!a || !bb || !c || !d || kind < 0 || kind >= btd::kGenKinds)
         return bfail(BTD_ERR_INVALID_ARG, "btd_batch_generate: bad argument");
     hipError_t e = btd::launch_batch_generate(b->bd, b->d_slot_sys, b->max_rows, kind, seed, a, bb, c,
                                               d, (hipStream_t)stream);

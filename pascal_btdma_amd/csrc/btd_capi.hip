@@ -675,7 +675,7 @@ btd_status btd_convert_layout(int src_layout, int dst_layout, int E, int nsys, i
 btd_status btd_generate(int kind, uint64_t seed, int m, int nsys, int nrow, int layout, double* a,
                         double* b, double* c, double* d, void* stream) {
     g_err.clear();
-    if (kind < 0 || kind > 3 || m < 1 || m > BTD_MAX_M || nsys < 1 || nrow < 1 || layout < 0 ||
+    if (kind < 0 || kind >= btd::kGenKinds || m < 1 || m > BTD_MAX_M || nsys < 1 || nrow < 1 || layout < 0 ||
         layout > 2)
         return fail(BTD_ERR_INVALID_ARG, "btd_generate: bad argument");
     BTD_HIP(btd::launch_generate(m, kind, seed, layout, nsys, nrow, a, b, c, d, nullptr, nullptr,
@@ -686,7 +686,7 @@ btd_status btd_generate(int kind, uint64_t seed, int m, int nsys, int nrow, int 
 btd_status btd_generate_f32(int kind, uint64_t seed, int m, int nsys, int nrow, int layout,
                             float* a, float* b, float* c, double* d, void* stream) {
     g_err.clear();
-    if (kind < 0 || kind > 3 || m < 1 || m > BTD_MAX_M || nsys < 1 || nrow < 1 || layout < 0 ||
+    if (kind < 0 || kind >= btd::kGenKinds || m < 1 || m > BTD_MAX_M || nsys < 1 || nrow < 1 || layout < 0 ||
         layout > 2)
         return fail(BTD_ERR_INVALID_ARG, "btd_generate_f32: bad argument");
     BTD_HIP(btd::launch_generate(m, kind, seed, layout, nsys, nrow, nullptr, nullptr, nullptr, d, a,
@@ -719,7 +719,7 @@ extern "C" btd_status btd_generate_host(int kind, uint64_t seed, int m, int nsys
                                         int sys0, int count, double* a, double* b, double* c,
                                         double* d) {
     g_err.clear();
-    if (kind < 0 || kind > 3 || m < 1 || m > BTD_MAX_M || nsys < 1 || nrow < 1 || sys0 < 0 ||
+    if (kind < 0 || kind >= btd::kGenKinds || m < 1 || m > BTD_MAX_M || nsys < 1 || nrow < 1 || sys0 < 0 ||
         count < 0 || sys0 + count > nsys)
         return fail(BTD_ERR_INVALID_ARG, "btd_generate_host: bad argument");
     switch (m) {

@@ -55,7 +55,10 @@ enum GenKind : int {
     GEN_DDPERM = 1,  // strictly diagonally dominant rows, whole block row permuted
     GEN_EULER = 2,   // A = -0.5 s J_{q-1}, C = 0.5 s J_{q+1}, B = I + s diag|J_q| + E
     GEN_COND = 3,    // B = U diag(sv) V^T with cond log-uniform [1e1,1e6]; FP32-rounded
+    GEN_BPERM = 4,   // the small CPU-runnable configuration word for word: B = U + diag(row sums
+                     // |A| + |C| + 1), then only B's rows permuted
 };
+constexpr int kGenKinds = 5;
 
 enum GenStream : uint32_t { S_A = 1, S_B = 2, S_C = 3, S_D = 4, S_PERM = 5, S_J = 6, S_E = 7,
                             S_U = 8, S_V = 9, S_K = 10 };
@@ -136,6 +139,38 @@ BTD_HD void gen_row(int kind, uint64_t seed, int64_t sys, int64_t row, int64_t n
             }
             d[i] = 1.0;
         }
+        return;
+    }
+    if (kind == GEN_BPERM) {
+        // A, C ~ U[-1,1] (A of the first row and C of the last are 0), B = U[-1,1] + (row sum of
+        // |A| + row sum of |C| + 1) on the diagonal, then the rows of B alone permuted (A, C, d
+        // keep theirs), d ~ U[-1,1]
+        const uint64_t ka = row_key(seed, S_A, sys, row), kb = row_key(seed, S_B, sys, row);
+        const uint64_t kc = row_key(seed, S_C, sys, row), kd = row_key(seed, S_D, sys, row);
+        const uint64_t kp = row_key(seed, S_PERM, sys, row);
+        double b0[M][M];
+        for (int i = 0; i < M; ++i) {
+            double s = 1.0;
+            for (int j = 0; j < M; ++j) {
+                A[i][j] = first ? 0.0 : u11(ka, (uint32_t)(i * M + j));
+                C[i][j] = last ? 0.0 : u11(kc, (uint32_t)(i * M + j));
+                b0[i][j] = u11(kb, (uint32_t)(i * M + j));
+            }
+            for (int j = 0; j < M; ++j) s = s + det_abs(A[i][j]);
+            for (int j = 0; j < M; ++j) s = s + det_abs(C[i][j]);
+            b0[i][i] = b0[i][i] + s;
+            d[i] = u11(kd, (uint32_t)i);
+        }
+        int perm[M];
+        for (int i = 0; i < M; ++i) perm[i] = i;
+        for (int k = M - 1; k > 0; --k) {
+            const int j = (int)(mix64(kp + (uint64_t)k) % (uint64_t)(k + 1));
+            const int t = perm[k];
+            perm[k] = perm[j];
+            perm[j] = t;
+        }
+        for (int i = 0; i < M; ++i)
+            for (int j = 0; j < M; ++j) B[i][j] = b0[perm[i]][j];
         return;
     }
     if (kind == GEN_DDPERM) {

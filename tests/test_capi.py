@@ -166,3 +166,31 @@ def test_new_entry_points_reject_an_uncreated_plan():
     for name, args in calls.items():
         assert getattr(lib, name)(*args) == NOT_CREATED, name
         assert lib.btd_last_error() != b"", name
+
+
+def test_generator_bperm_is_the_literal_small_configuration():
+    """BPERM: A, C uniform in [-1, 1] (A of the first row and C of the last zero), d uniform;
+    B's rows are a permutation of U + diag(1 + row sums of |A| and |C|) with U in [-1, 1): some
+    row permutation of B minus that diagonal lies in [-1, 1), and A, C, d are not permuted (their
+    rows line up with the diagonal shift). Permutations other than the identity occur."""
+    import itertools
+    m = 3
+    A, B, C, d = pb.generate_host(pb.GenKind.BPERM, 21, m, 40, 6)
+    assert np.all(A[0] == 0) and np.all(C[-1] == 0)
+    for X in (A, C, d):
+        assert np.all(np.abs(X) <= 1.0)
+    s = 1.0 + np.abs(A).sum(-1) + np.abs(C).sum(-1)  # (n, nsys, m)
+    nonid = 0
+    for q in range(B.shape[0]):
+        for k in range(B.shape[1]):
+            found = None
+            for perm in itertools.permutations(range(m)):
+                b0 = B[q, k][np.argsort(perm)]  # undo: B[i] = b0[perm[i]]
+                u = b0 - np.diag(s[q, k])
+                if np.all(u >= -1.0) and np.all(u < 1.0):
+                    found = perm
+                    break
+            assert found is not None, (q, k)
+            nonid += found != tuple(range(m))
+    assert nonid > 0
+

@@ -73,7 +73,7 @@ def test_two_stage_matches_oracle(cuda, oracle, m, n, npart, layout):
 
 
 @pytest.mark.parametrize("kind", [pb.GenKind.DDPERM, pb.GenKind.EULER, pb.GenKind.COND,
-                                  pb.GenKind.DIAG])
+                                  pb.GenKind.DIAG, pb.GenKind.BPERM])
 @pytest.mark.parametrize("m", [2, 3, 5, 8, 10])
 def test_device_generator_matches_host(cuda, kind, m):
     import torch
@@ -399,3 +399,22 @@ def test_two_stage_one_lane_per_partition(cuda, oracle, m, layout):
         assert info == 0
         full = layout_to_vec(flat, int(layout), n, nsys, m)
         assert rel_err(full[:, s0:s0 + cnt], xo).max() <= TOL
+
+
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_config0_literal_generator_full_batch(cuda, oracle, layout):
+    """The small configuration as its text states it (GenKind.BPERM: only B's rows permuted):
+    1024 systems, m = 3, n = 64, every system against the oracle (<= 1e-12) and, since these rows
+    are no longer diagonally dominant, both against the binary128 truth (the device solve as
+    accurate as the oracle within 10x)."""
+    A, B, C, d = pb.generate_host(pb.GenKind.BPERM, 2024, 3, 1024, 64)
+    x, sing = gpu_solve(A, B, C, d, int(layout))
+    assert not sing
+    xo = oracle_plain(oracle, A, B, C, d)
+    xt, info = oracle.solve_truth(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
+    assert info == 0
+    e_o, e_t, e_ot = rel_err(x, xo).max(), rel_err(x, xt).max(), rel_err(xo, xt).max()
+    print(f"literal config0 {layout.name}: vs oracle {e_o:.2e}, vs truth {e_t:.2e}, oracle vs truth {e_ot:.2e}")
+    assert e_o <= TOL
+    assert e_t <= 10 * max(e_ot, 1e-16)
+
