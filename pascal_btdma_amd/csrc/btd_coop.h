@@ -65,10 +65,31 @@ constexpr int kCoopSlots = kCoopSlotsFor<kMaxM>;  // any m (the mixed-size batch
 // lane, the whole xd = R^{-1} d. Returns 1 if a pivot was zero. The code is free of
 // lane-dependent branches so that LDS reads issue in batches. LDS: xb holds (NS+1)*M + 2
 // slots: the row [0, M), d at M, S at [M+1, M+1+NS*M), the candidate after them.
+//
+// Up to kGatherMaxM the same result comes from coop_eliminate_gather: one exchange instead of
+// M + 1 (each lane reads the group's whole block once and repeats the elimination in its
+// registers, bit for bit the operations of the exchanged form; outputs compared equal on every
+// solve form). Measured against the exchanged form: the latency-bound small batches 7-25% faster
+// (the reference Study's 1024 x 128 case at m = 2 / 3 / 5: 0.061 / 0.090 / 0.157 -> 0.051 /
+// 0.068 / 0.146 ms; 1024 x 64 at m = 3: 0.050 -> 0.040 ms; twisted m = 5 16384 x 2048 13.0 ->
+// 12.7 ms), the mixed-size batch unchanged; at m = 8, where a lane then holds 80 more doubles,
+// 7-15% slower (cooperative 16384 x 1024 12.8 -> 14.5 ms, Study m = 8 0.31 -> 0.36 ms), so m >= 7
+// keep the exchanged form.
+#ifndef BTD_COOP_GATHER_MAX_M
+#define BTD_COOP_GATHER_MAX_M 6
+#endif
+constexpr int kGatherMaxM = BTD_COOP_GATHER_MAX_M;
+
+template <int M, int NS>
+__device__ __forceinline__ int coop_eliminate_gather(const double (&r)[M], const double (&rhs)[NS * M + 1],
+                                                     double* __restrict__ xb, int i, int base,
+                                                     double (&xc)[NS * M], double (&xd)[M]);
+
 template <int M, int NS = 1>
 __device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[NS * M + 1],
                                               double* __restrict__ xb, int i, int base,
                                               double (&xc)[NS * M], double (&xd)[M]) {
+    if constexpr (M <= kGatherMaxM) return coop_eliminate_gather<M, NS>(r, rhs, xb, i, base, xc, xd);
     constexpr int NR = NS * M + 1;     // right-hand-side entries of a row
     constexpr int CAND = M + NR;       // the candidate's slot
     const int lane = threadIdx.x & (kTile - 1);  // the lane in its wavefront
@@ -130,6 +151,102 @@ __device__ __forceinline__ int coop_eliminate(double (&r)[M], double (&rhs)[NS *
             sd = fma(-u, xd[j], sd);
         }
         const double rk = fast_rcp(xb[k * kTile + pl]);
+#pragma unroll
+        for (int c = 0; c < NS; ++c) xc[c * M + k] = sc[c] * rk;
+        xd[k] = sd * rk;
+    }
+    return zero_pivot;
+}
+
+// coop_eliminate with one exchange: every lane posts its row of R and of the right-hand sides
+// once; every lane then reads its group's M rows (R, the d column and, of each S block, column
+// i) and eliminates them in registers exactly as the exchanged form does across lanes: at step k
+// the pivot is the row not yet chosen with the largest |R(t, k)|, the lowest row on a tie; each
+// row not yet chosen is updated with l = R(t, k) * fast_rcp(pivot), R(t, j) = fma(-l, R(p, j),
+// R(t, j)) and the same on its right-hand sides; back substitution follows the pivots' order.
+// The rows are kept in pivot order by branch-free exchanges (the original row numbers ride
+// along for the tie rule), so every value is computed by the same operations on the same
+// operands as in the exchanged form: the same bits (zeros aside, whose sign may differ).
+template <int M, int NS>
+__device__ __forceinline__ int coop_eliminate_gather(const double (&r)[M], const double (&rhs)[NS * M + 1],
+                                                     double* __restrict__ xb, int i, int base,
+                                                     double (&xc)[NS * M], double (&xd)[M]) {
+    constexpr int NR = NS * M + 1;
+    const int lane = threadIdx.x & (kTile - 1);  // the lane in its wavefront
+#pragma unroll
+    for (int j = 0; j < M; ++j) xb[j * kTile + lane] = r[j];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) xb[(M + j) * kTile + lane] = rhs[j];
+    wave_lds_sync();
+    double R[M][M], X[M][NS + 1];  // row t: R(t, :), d(t), S(t, c*M + i) for each block c
+    int org[M];
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) R[t][j] = xb[j * kTile + base + t];
+        X[t][0] = xb[M * kTile + base + t];
+#pragma unroll
+        for (int c = 0; c < NS; ++c) X[t][1 + c] = xb[(M + 1 + c * M + i) * kTile + base + t];
+        org[t] = t;
+    }
+    wave_lds_sync();  // every read of this exchange before the next one's writes
+    int zero_pivot = 0;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        double best = fabs(R[k][k]);
+        int p = k, po = org[k];
+#pragma unroll
+        for (int t = k + 1; t < M; ++t) {
+            const double v = fabs(R[t][k]);
+            const bool c = v > best || (v == best && org[t] < po);
+            best = c ? v : best;
+            p = c ? t : p;
+            po = c ? org[t] : po;
+        }
+#pragma unroll
+        for (int t = k + 1; t < M; ++t) {
+            const bool c = p == t;
+#pragma unroll
+            for (int j = k; j < M; ++j) {
+                const double u = R[k][j];
+                R[k][j] = c ? R[t][j] : u;
+                R[t][j] = c ? u : R[t][j];
+            }
+#pragma unroll
+            for (int j = 0; j < NS + 1; ++j) {
+                const double u = X[k][j];
+                X[k][j] = c ? X[t][j] : u;
+                X[t][j] = c ? u : X[t][j];
+            }
+            const int o = org[k];
+            org[k] = c ? org[t] : o;
+            org[t] = c ? o : org[t];
+        }
+        zero_pivot |= R[k][k] == 0.0 ? 1 : 0;
+        const double rk = fast_rcp(R[k][k]);
+#pragma unroll
+        for (int t = k + 1; t < M; ++t) {
+            const double l = R[t][k] * rk;
+#pragma unroll
+            for (int j = k + 1; j < M; ++j) R[t][j] = fma(-l, R[k][j], R[t][j]);
+#pragma unroll
+            for (int j = 0; j < NS + 1; ++j) X[t][j] = fma(-l, X[k][j], X[t][j]);
+        }
+    }
+#pragma unroll
+    for (int k = M - 1; k >= 0; --k) {
+        double sc[NS];
+#pragma unroll
+        for (int c = 0; c < NS; ++c) sc[c] = X[k][1 + c];
+        double sd = X[k][0];
+#pragma unroll
+        for (int j = k + 1; j < M; ++j) {
+            const double u = R[k][j];
+#pragma unroll
+            for (int c = 0; c < NS; ++c) sc[c] = fma(-u, xc[c * M + j], sc[c]);
+            sd = fma(-u, xd[j], sd);
+        }
+        const double rk = fast_rcp(R[k][k]);
 #pragma unroll
         for (int c = 0; c < NS; ++c) xc[c * M + k] = sc[c] * rk;
         xd[k] = sd * rk;
