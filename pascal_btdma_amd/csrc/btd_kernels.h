@@ -702,14 +702,18 @@ __global__ __launch_bounds__(64) void k_f32r(const float* __restrict__ A, const 
 // blocks current, and b(R), a_col(k, a), a_mat(X), c_col(j, c), which read them.
 struct MemBlocks {};
 
-template <int M, int LAYOUT, class BS = MemBlocks>
-__global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict__ B, double* C,
+// AHEAD (blocks in memory only): row q+1's A, B, C, d are loaded while row q is eliminated, for
+// batches with too few partitions to hide HBM latency behind other wavefronts; the extra 80
+// doubles (m = 5) a lane holds cost occupancy, so such launches ask for one wavefront per SIMD.
+template <int M, int LAYOUT, class BS = MemBlocks, bool AHEAD = false>
+__global__ __launch_bounds__(64, AHEAD ? 1 : 2) void k_modi(double* A, const double* __restrict__ B, double* C,
                                              double* D, int nsys, int nrow,
                                              const int* __restrict__ part_first,
                                              const int* __restrict__ part_len, int npart,
                                              double* __restrict__ redA, double* __restrict__ redC,
                                              double* __restrict__ redD, int* __restrict__ flag, BS bs) {
     constexpr bool kMem = std::is_same<BS, MemBlocks>::value;
+    static_assert(kMem || !AHEAD, "rows are loaded ahead only from memory");
     const int ntiles = (nsys + 63) / 64;
     const int tile = blockIdx.x % ntiles, part = blockIdx.x / ntiles;
     const int64_t sys = (int64_t)tile * 64 + threadIdx.x;
@@ -729,16 +733,43 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
             if constexpr (kMem) return 0;
             else return bs.lane(sys);
         }();
+        // AHEAD: the next row's blocks (natural form) and right-hand side
+        double nA[AHEAD ? M : 1][AHEAD ? M : 1], nB[AHEAD ? M : 1][AHEAD ? M : 1];
+        double nC[AHEAD ? M : 1][AHEAD ? M : 1], nd[AHEAD ? M : 1];
+        auto fetch_next = [&](int64_t g) {
+            if constexpr (AHEAD) {
+                load_mat_c<M>(A, am, g, nA);
+                load_mat_c<M>(B, am, g, nB);
+                load_mat_c<M>(C, am, g, nC);
+                load_vec<M>(D, av, g, nd);
+            }
+        };
+        fetch_next(r0);
         for (int q = 0; q < N; ++q) {
             const int64_t g = r0 + q;
             double R[M][M], v[M], X[M][M];
-            if constexpr (kMem) {
-                load_mat<M>(B, am, g, R);
+            double cA[AHEAD ? M : 1][AHEAD ? M : 1], cC[AHEAD ? M : 1][AHEAD ? M : 1];
+            if constexpr (AHEAD) {
+    #pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    v[i] = nd[i];
+    #pragma unroll
+                    for (int j = 0; j < M; ++j) {
+                        R[i][j] = nB[i][j];
+                        cA[i][j] = nA[i][j];
+                        cC[i][j] = nC[i][j];
+                    }
+                }
+                fetch_next(q + 1 < N ? g + 1 : g);
             } else {
-                src.row(g);
-                src.b(R);
+                if constexpr (kMem) {
+                    load_mat<M>(B, am, g, R);
+                } else {
+                    src.row(g);
+                    src.b(R);
+                }
+                load_vec<M>(D, av, g, v);
             }
-            load_vec<M>(D, av, g, v);
             if (q >= 2) {
     #pragma unroll
                 for (int i = 0; i < M; ++i)
@@ -747,10 +778,14 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
     #pragma unroll
                 for (int k = 0; k < M; ++k) {
                     double ak[M];
-                    if constexpr (kMem)
+                    if constexpr (AHEAD) {
+    #pragma unroll
+                        for (int i = 0; i < M; ++i) ak[i] = cA[i][k];
+                    } else if constexpr (kMem) {
                         load_col<M>(A, am, g, k, ak);
-                    else
+                    } else {
                         src.a_col(k, ak);
+                    }
     #pragma unroll
                     for (int i = 0; i < M; ++i) {
                         v[i] = fma(-ak[i], dh[k], v[i]);
@@ -762,10 +797,16 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
                     }
                 }
             } else {
-                if constexpr (kMem)
+                if constexpr (AHEAD) {
+    #pragma unroll
+                    for (int i = 0; i < M; ++i)
+    #pragma unroll
+                        for (int j = 0; j < M; ++j) X[i][j] = cA[i][j];
+                } else if constexpr (kMem) {
                     load_mat_c<M>(A, am, g, X);
-                else
+                } else {
                     src.a_mat(X);
+                }
             }
             int piv[M];
             ok = lu_factor<M>(R, piv) && ok;
@@ -775,10 +816,14 @@ __global__ __launch_bounds__(64) void k_modi(double* A, const double* __restrict
     #pragma unroll
             for (int j = 0; j < M; ++j) {
                 double c[M];
-                if constexpr (kMem)
+                if constexpr (AHEAD) {
+    #pragma unroll
+                    for (int i = 0; i < M; ++i) c[i] = cC[i][j];
+                } else if constexpr (kMem) {
                     load_col<M>(C, am, g, j, c);
-                else
+                } else {
                     src.c_col(j, c);
+                }
                 lu_solve<M>(R, piv, c);
     #pragma unroll
                 for (int i = 0; i < M; ++i) Ch[i][j] = c[i];
@@ -1163,11 +1208,25 @@ constexpr int modi_coop_waves() { return M >= 9 ? 2 : 1; }
 constexpr int kModiCoopFrom = BTD_MODI_COOP_FROM;
 
 // Below kModiCoopFrom, stage 1 runs one lane per partition from this many partitions in the
-// batch (nsys * npart) and m lanes per partition below it.
+// batch (nsys * npart), with the next row loaded ahead from kModiCoopBelow, and m lanes per
+// partition below that (launch_two_stage_m).
 #ifndef BTD_MODI_LANE_FROM
 #define BTD_MODI_LANE_FROM 65536
 #endif
 constexpr int64_t kModiLaneFrom = BTD_MODI_LANE_FROM;
+#ifndef BTD_MODI_COOP_BELOW
+#define BTD_MODI_COOP_BELOW 16384
+#endif
+constexpr int64_t kModiCoopBelow = BTD_MODI_COOP_BELOW;
+
+// Largest m for which the few-partition forms run (m lanes per partition in stage 1 and 3, the
+// next row loaded ahead in stage 1): they were verified at m = 2-5 in both layouts. At m = 6 a
+// solve with the reference layout, 8195 systems x 4 partitions (stage 1 loading ahead, whose m = 6
+// build spills 1.2-1.3 KB per lane to scratch; stage 3 m lanes per partition, run for the first
+// time at m = 6) ended in an illegal memory access whose cause reading the code has not shown, so
+// from m = 6 both stages keep one lane per partition without loading ahead, as before these
+// forms existed.
+constexpr int kFewFormsMaxM = 5;
 
 // Largest number of wavefronts per CU for which the reduced system is solved twisted.
 #ifndef BTD_TWISTED_REDUCED_WAVES_PER_CU
@@ -1920,9 +1979,20 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     // lane each (fewer than kModiLaneFrom of them); otherwise one lane per partition (k_modi).
     // Periodic solves keep k_modi below kModiCoopFrom, so that a matrix-free sweep (k_modi with
     // its own block source) and the assembled periodic solve run the same arithmetic.
+    // Below kModiCoopFrom, by the partitions in the batch (nsys * npart; m = 5, 512 x 65536,
+    // stage 1 at P = 8 / 16 / 32 / 64): from kModiLaneFrom one lane per partition (k_modi); from
+    // kModiCoopBelow one lane per partition with the next row loaded during the current one
+    // (47.6 / 24.4 / 12.7 / 7.5 ms); below that m lanes per partition (k_modi_coop: 25.0 / 15.6 /
+    // 16.9 / 13.0 ms), non-periodic solves only. Loading ahead changes no arithmetic, so periodic
+    // solves with few partitions use it too and still equal the matrix-free sweeps bit for bit.
+    // These few-partition forms are enabled for m <= kFewFormsMaxM only (see kFewFormsMaxM).
     constexpr bool kCanCoop = M >= 2 && std::is_same<BS, MemBlocks>::value;
-    bool coop = M >= kModiCoopFrom;
-    if constexpr (kCanCoop && M < kModiCoopFrom) coop = !cyclic && (int64_t)nsys * npart < kModiLaneFrom;
+    bool coop = M >= kModiCoopFrom, ahead = false;
+    if constexpr (kCanCoop && M <= kFewFormsMaxM) {
+        const int64_t lanes = (int64_t)nsys * npart;
+        coop = !cyclic && lanes < kModiCoopBelow;
+        ahead = !coop && lanes < kModiLaneFrom;
+    }
     if (coop) {
         if constexpr (kCanCoop) {
             // the items of a partition on one XCD. floor(64/m) systems a wavefront leave partial
@@ -1945,12 +2015,26 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
         }
     } else {
         if constexpr (M < kModiCoopFrom) {
-            if (layout == LAYOUT_TILED)
-                hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED, BS>), gridp, block, lds, s, A, B, C, D, nsys, nrow, pf,
-                                   pl, npart, redA, redC, redD, flag, bs);
-            else
-                hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU, BS>), gridp, block, lds, s, A, B, C, D, nsys, nrow,
-                                   pf, pl, npart, redA, redC, redD, flag, bs);
+            bool launched = false;
+            if constexpr (std::is_same<BS, MemBlocks>::value) {
+                if (ahead) {
+                    if (layout == LAYOUT_TILED)
+                        hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED, BS, true>), gridp, block, lds, s, A, B, C, D,
+                                           nsys, nrow, pf, pl, npart, redA, redC, redD, flag, bs);
+                    else
+                        hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU, BS, true>), gridp, block, lds, s, A, B, C, D,
+                                           nsys, nrow, pf, pl, npart, redA, redC, redD, flag, bs);
+                    launched = true;
+                }
+            }
+            if (!launched) {
+                if (layout == LAYOUT_TILED)
+                    hipLaunchKernelGGL((k_modi<M, LAYOUT_TILED, BS>), gridp, block, lds, s, A, B, C, D, nsys, nrow,
+                                       pf, pl, npart, redA, redC, redD, flag, bs);
+                else
+                    hipLaunchKernelGGL((k_modi<M, LAYOUT_REF_GPU, BS>), gridp, block, lds, s, A, B, C, D, nsys,
+                                       nrow, pf, pl, npart, redA, redC, redD, flag, bs);
+            }
         }
     }
     if (ev) (void)hipEventRecord(ev[1], s);
@@ -1972,7 +2056,7 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     if (ev) (void)hipEventRecord(ev[2], s);
     // stage 3 m lanes per partition wherever stage 1 ran that way (M >= 2)
     bool coop3 = false;
-    if constexpr (M >= 2) coop3 = M >= kModiCoopFrom || (!cyclic && (int64_t)nsys * npart < kModiLaneFrom);
+    if constexpr (M >= 2 && M <= kFewFormsMaxM) coop3 = !cyclic && (int64_t)nsys * npart < kModiLaneFrom;
     if (coop3) {
         if constexpr (M >= 2) {
             // four wavefronts locked per workgroup (their shared lines read once: m = 5, 512 x 65536,
