@@ -919,6 +919,282 @@ __global__ __launch_bounds__(64, AHEAD ? 1 : 2) void k_modi(double* A, const dou
     }
 }
 
+// ------------------------------------------------------------------------ modi, twisted --
+// Stage 1 with each partition swept from both ends, by lanes l (top) and l + 32 (bottom) of
+// one wavefront for system 32 * (blockIdx.x mod ceil(nsys/32)) + l, blocks in memory,
+// partitions of at least 4 rows: each lane's chain of dependent rows is half the partition's,
+// for batches with too few partitions to fill the device. The top lane runs k_modi's forward
+// sweep (rows loaded ahead) over rows 0..k, k = ceil(N/2) - 1; the bottom lane runs the same
+// sweep mirrored, over rows N-1 down to k+1, with A and C exchanging roles. A lane's rows then
+// satisfy, with its anchor x_a (x_0' top, x_{N-1} bottom) and the next row of its sweep x_s
+// (x_{q+1} top, x_{q-1} bottom),
+//   x_q + Â_q x_a + Ĉ_q x_s = d̂_q   (in place: d̂ over D, Â over the array of A's role, Ĉ over
+//                                    the array of C's role: A and C top, C and A bottom),
+// and row 1 (top) / row N-2 (bottom) is composed on the way, as in k_modi, as
+// x = alpha - beta x_a - gamma x_s(last row). The lanes' last rows p (k and k+1) meet: with
+// the partner's last row x_n + H x_f + K x_p = h (x_f the partner's anchor), each lane solves
+//   x_p = a - b x_a - c x_f,  S = I - Ĉ_p K,  a = S^{-1}(d̂_p - Ĉ_p h),  b = S^{-1} Â_p,
+//   c = -S^{-1} Ĉ_p H,
+// and stores it as row p (its Ĉ now the coefficient of x_f); the partner's solution
+// x_n = a' - b' x_f - c' x_a finishes the composition:
+//   alpha -= gamma a',  beta -= gamma c',  gamma <- -gamma b'.
+// The close of row 0 (top) / row N-1 (bottom) is k_modi's; the top lane writes reduced row
+// 2*part, the bottom lane row 2*part+1 with its Â (the coefficient of x_N) as the reduced C and
+// its Ĉ (of x_0') as the reduced A: the reduced system of k_modi, to rounding. Stage 3 is
+// k_update_twisted. The composition and the two exchanges of the meeting share the LDS slots
+// ([element][lane], sp).
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64, 1) void k_modi_twisted(double* A, const double* __restrict__ B, double* C,
+                                                        double* D, int nsys, int nrow,
+                                                        const int* __restrict__ part_first,
+                                                        const int* __restrict__ part_len, int npart,
+                                                        double* __restrict__ redA, double* __restrict__ redC,
+                                                        double* __restrict__ redD, int* __restrict__ flag) {
+    constexpr int H = kTile / 2;
+    const int nh = (nsys + H - 1) / H;
+    const int half = blockIdx.x % nh, part = blockIdx.x / nh;
+    const int lane = threadIdx.x;
+    const bool top = lane < H;
+    const int64_t sys = (int64_t)half * H + (lane % H);
+    if (sys >= nsys) return;  // the two lanes of a system leave together
+    const int64_t r0 = part_first[part];
+    const int N = part_len[part];  // >= 4
+    const int k = (N + 1) / 2 - 1;
+    const int L = top ? k + 1 : N - 1 - k;  // rows of this lane's sweep, >= 2
+    const int64_t g0 = top ? r0 : r0 + N - 1, dg = top ? 1 : -1;
+    double* Ar = top ? A : C;  // couples row q to the previous row of the sweep
+    double* Cr = top ? C : A;  // to the next one
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    extern __shared__ double sp[];
+    const int pl = lane ^ H;  // the partner lane
+    auto SA = [&](int i) -> double& { return sp[i * 64 + lane]; };
+    auto SB = [&](int i, int j) -> double& { return sp[(M + i * M + j) * 64 + lane]; };
+    auto SG = [&](int i, int j) -> double& { return sp[(M + M * M + i * M + j) * 64 + lane]; };
+    auto PA = [&](int i) { return sp[i * 64 + pl]; };
+    auto PB = [&](int i, int j) { return sp[(M + i * M + j) * 64 + pl]; };
+    auto PG = [&](int i, int j) { return sp[(M + M * M + i * M + j) * 64 + pl]; };
+    bool ok = true;
+    double Ah[M][M], Ch[M][M], dh[M];
+    double nA[M][M], nB[M][M], nC[M][M], nd[M];
+    auto fetch_next = [&](int64_t g) {
+        load_mat_c<M>(Ar, am, g, nA);
+        load_mat_c<M>(B, am, g, nB);
+        load_mat_c<M>(Cr, am, g, nC);
+        load_vec<M>(D, av, g, nd);
+    };
+    fetch_next(g0);
+    for (int q = 0; q < L; ++q) {
+        const int64_t g = g0 + q * dg;
+        double R[M][M], v[M], X[M][M], cA[M][M], cC[M][M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            v[i] = nd[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                R[i][j] = nB[i][j];
+                cA[i][j] = nA[i][j];
+                cC[i][j] = nC[i][j];
+            }
+        }
+        fetch_next(q + 1 < L ? g + dg : g);
+        if (q >= 2) {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) X[i][j] = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < M; ++kk)
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    v[i] = fma(-cA[i][kk], dh[kk], v[i]);
+#pragma unroll
+                    for (int j = 0; j < M; ++j) {
+                        R[i][j] = fma(-cA[i][kk], Ch[kk][j], R[i][j]);
+                        X[i][j] = fma(-cA[i][kk], Ah[kk][j], X[i][j]);
+                    }
+                }
+        } else {
+#pragma unroll
+            for (int i = 0; i < M; ++i)
+#pragma unroll
+                for (int j = 0; j < M; ++j) X[i][j] = cA[i][j];
+        }
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, v);
+        solve_rows<M>(R, piv, X);
+        solve_rows<M>(R, piv, cC);
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            dh[i] = v[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                Ah[i][j] = X[i][j];
+                Ch[i][j] = cC[i][j];
+            }
+        }
+        if (q + 1 < L) {  // the last row is stored after the meeting
+            store_mat<M>(Ar, am, g, Ah);
+            store_mat<M>(Cr, am, g, Ch);
+            store_vec<M>(D, av, g, dh);
+        }
+        if (q == 1) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                SA(i) = dh[i];
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    SB(i, j) = Ah[i][j];
+                    SG(i, j) = Ch[i][j];
+                }
+            }
+        } else if (q >= 2) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                double gi[M];
+#pragma unroll
+                for (int kk = 0; kk < M; ++kk) gi[kk] = SG(i, kk);
+                double a = SA(i);
+#pragma unroll
+                for (int kk = 0; kk < M; ++kk) a = fma(-gi[kk], dh[kk], a);
+                SA(i) = a;
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    double b = SB(i, j), c = 0.0;
+#pragma unroll
+                    for (int kk = 0; kk < M; ++kk) {
+                        b = fma(-gi[kk], Ah[kk][j], b);
+                        c = fma(-gi[kk], Ch[kk][j], c);
+                    }
+                    SB(i, j) = b;
+                    SG(i, j) = c;
+                }
+            }
+        }
+    }
+    // the composition to registers: its slots carry the meeting's exchanges
+    double al[M], be[M][M], ga[M][M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        al[i] = SA(i);
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            be[i][j] = SB(i, j);
+            ga[i][j] = SG(i, j);
+        }
+    }
+    // exchange 1: the last rows (d̂, Â, Ĉ)_p; S = I - Ĉ_p K, [a | b | c] = S^{-1}[d̂_p - Ĉ_p h | Â_p | -Ĉ_p H]
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        SA(i) = dh[i];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            SB(i, j) = Ah[i][j];
+            SG(i, j) = Ch[i][j];
+        }
+    }
+    wave_lds_sync();
+    double S[M][M], ma[M], mc[M][M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        double t = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < M; ++kk) t = fma(Ch[i][kk], PA(kk), t);
+        ma[i] = dh[i] - t;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double s = 0.0, h = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < M; ++kk) {
+                s = fma(Ch[i][kk], PG(kk, j), s);
+                h = fma(Ch[i][kk], PB(kk, j), h);
+            }
+            S[i][j] = (i == j ? 1.0 : 0.0) - s;
+            mc[i][j] = -h;
+        }
+    }
+    wave_lds_sync();  // the partner has read this lane's row before it is overwritten
+    {
+        int piv[M];
+        ok = lu_factor<M>(S, piv) && ok;
+        lu_solve<M>(S, piv, ma);
+        solve_rows<M>(S, piv, Ah);
+        solve_rows<M>(S, piv, mc);
+    }
+    const int64_t gp = g0 + (L - 1) * dg;
+    store_vec<M>(D, av, gp, ma);
+    store_mat<M>(Ar, am, gp, Ah);
+    store_mat<M>(Cr, am, gp, mc);
+    // exchange 2: (a, b, c); the partner's finishes the composition
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        SA(i) = ma[i];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            SB(i, j) = Ah[i][j];
+            SG(i, j) = mc[i][j];
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        double gi[M];
+#pragma unroll
+        for (int kk = 0; kk < M; ++kk) gi[kk] = ga[i][kk];
+        double a = al[i];
+#pragma unroll
+        for (int kk = 0; kk < M; ++kk) a = fma(-gi[kk], PA(kk), a);
+        dh[i] = a;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double b = be[i][j], c = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < M; ++kk) {
+                b = fma(-gi[kk], PG(kk, j), b);
+                c = fma(-gi[kk], PB(kk, j), c);
+            }
+            Ah[i][j] = b;
+            Ch[i][j] = c;
+        }
+    }
+    // close, as in k_modi, with (dh, Ah, Ch) = (alpha, beta, gamma) through the whole partition
+    {
+        double A0[M][M], C0[M][M], d0[M], R[M][M], X[M][M], T[M][M], t[M];
+        load_mat_c<M>(Ar, am, g0, A0);
+        load_mat_c<M>(Cr, am, g0, C0);
+        load_vec<M>(D, av, g0, d0);
+        mv<M>(C0, dh, t);
+#pragma unroll
+        for (int i = 0; i < M; ++i) d0[i] -= t[i];
+        mm<M>(C0, Ah, T);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) R[i][j] = (i == j ? 1.0 : 0.0) - T[i][j];
+        mm<M>(C0, Ch, X);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) X[i][j] = -X[i][j];
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, d0);
+        solve_rows<M>(R, piv, A0);
+        solve_rows<M>(R, piv, X);
+        const Addr<LAYOUT_TILED> rm(sys, 0, 2 * npart, M * M), rv(sys, 0, 2 * npart, M);
+        const int rr = 2 * part + (top ? 0 : 1);
+        if (top) {
+            store_mat<M>(redA, rm, rr, A0);
+            store_mat<M>(redC, rm, rr, X);
+        } else {
+            store_mat<M>(redA, rm, rr, X);
+            store_mat<M>(redC, rm, rr, A0);
+        }
+        store_vec<M>(redD, rv, rr, d0);
+    }
+    if (!ok) report_singular(flag, sys);
+}
+
 // XCDs of the device: workgroups are dispatched to them round robin (workgroup b runs on XCD
 // b mod kXcds), each XCD with its own L2.
 constexpr int kXcds = 8;
@@ -1218,9 +1494,16 @@ constexpr int64_t kModiLaneFrom = BTD_MODI_LANE_FROM;
 #define BTD_MODI_COOP_BELOW 16384
 #endif
 constexpr int64_t kModiCoopBelow = BTD_MODI_COOP_BELOW;
+// Below this many partitions in the batch, non-periodic solves whose partitions have at least
+// 4 rows run both stages twisted (k_modi_twisted, k_update_twisted) instead.
+#ifndef BTD_MODI_TWISTED_BELOW
+#define BTD_MODI_TWISTED_BELOW 65536
+#endif
+constexpr int64_t kModiTwistedBelow = BTD_MODI_TWISTED_BELOW;
 
-// Largest m for which the few-partition forms run (m lanes per partition in stage 1 and 3, the
-// next row loaded ahead in stage 1): they were verified at m = 2-5 in both layouts. At m = 6 a
+// Largest m for which the few-partition forms run (two lanes per partition from both ends, m
+// lanes per partition in stage 1 and 3, the next row loaded ahead in stage 1): they were
+// verified at m = 2-5 in both layouts. At m = 6 a
 // solve with the reference layout, 8195 systems x 4 partitions (stage 1 loading ahead, whose m = 6
 // build spills 1.2-1.3 KB per lane to scratch; stage 3 m lanes per partition, run for the first
 // time at m = 6) ended in an illegal memory access whose cause reading the code has not shown, so
@@ -1667,6 +1950,66 @@ __global__ __launch_bounds__(64) void k_update(const double* __restrict__ A,
     }
 }
 
+// Stage 3 after k_modi_twisted, the same two lanes per system: x_0' and x_{N-1} come from the
+// reduced solution; each lane walks its half back toward its anchor x_a,
+//   x_q = d̂_q - Â_q x_a - Ĉ_q x_s,   q = p .. 1 of its sweep,
+// x_s starting at the partner's anchor (row p's Ĉ is its coefficient) and then the row just
+// computed; the next row's blocks are loaded while the current one is formed.
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(64) void k_update_twisted(const double* __restrict__ A,
+                                                       const double* __restrict__ C, double* D, int nsys,
+                                                       int nrow, const int* __restrict__ part_first,
+                                                       const int* __restrict__ part_len, int npart,
+                                                       const double* __restrict__ redD) {
+    constexpr int H = kTile / 2;
+    const int nh = (nsys + H - 1) / H;
+    const int half = blockIdx.x % nh, part = blockIdx.x / nh;
+    const bool top = threadIdx.x < H;
+    const int64_t sys = (int64_t)half * H + (threadIdx.x % H);
+    if (sys >= nsys) return;
+    const int64_t r0 = part_first[part];
+    const int N = part_len[part];  // >= 4
+    const int k = (N + 1) / 2 - 1;
+    const int L = top ? k + 1 : N - 1 - k;
+    const int64_t g0 = top ? r0 : r0 + N - 1, dg = top ? 1 : -1;
+    const double* Ar = top ? A : C;
+    const double* Cr = top ? C : A;
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    const Addr<LAYOUT_TILED> rv(sys, 0, 2 * npart, M);
+    double xa[M], xs[M];
+    load_vec<M>(redD, rv, 2 * part + (top ? 0 : 1), xa);
+    load_vec<M>(redD, rv, 2 * part + (top ? 1 : 0), xs);
+    store_vec<M>(D, av, g0, xa);
+    double na[M][M], nc[M][M], nx[M];
+    auto fetch = [&](int q) {
+        const int64_t g = g0 + q * dg;
+        load_mat<M>(Ar, am, g, na);
+        load_mat<M>(Cr, am, g, nc);
+        load_vec<M>(D, av, g, nx);
+    };
+    fetch(L - 1);
+    for (int q = L - 1; q >= 1; --q) {
+        double a[M][M], c[M][M], x[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            x[i] = nx[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                a[i][j] = na[i][j];
+                c[i][j] = nc[i][j];
+            }
+        }
+        fetch(q > 1 ? q - 1 : q);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) x[i] = fma(-a[i][j], xa[j], fma(-c[i][j], xs[j], x[i]));
+#pragma unroll
+        for (int i = 0; i < M; ++i) xs[i] = x[i];
+        store_vec<M>(D, av, g0 + q * dg, x);
+    }
+}
+
 // Stage 3 with a partition's block rows over m lanes (lane i owns row i), for the batches whose
 // stage 1 runs k_modi_coop: the same x_q = d̂_q - Â_q x_1 - Ĉ_q x_{q+1} (the same operation
 // order as k_update, so the same bits), with x_{q+1} exchanged through LDS and the rows loaded
@@ -1980,20 +2323,34 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     // Periodic solves keep k_modi below kModiCoopFrom, so that a matrix-free sweep (k_modi with
     // its own block source) and the assembled periodic solve run the same arithmetic.
     // Below kModiCoopFrom, by the partitions in the batch (nsys * npart; m = 5, 512 x 65536,
-    // stage 1 at P = 8 / 16 / 32 / 64): from kModiLaneFrom one lane per partition (k_modi); from
-    // kModiCoopBelow one lane per partition with the next row loaded during the current one
-    // (47.6 / 24.4 / 12.7 / 7.5 ms); below that m lanes per partition (k_modi_coop: 25.0 / 15.6 /
-    // 16.9 / 13.0 ms), non-periodic solves only. Loading ahead changes no arithmetic, so periodic
-    // solves with few partitions use it too and still equal the matrix-free sweeps bit for bit.
-    // These few-partition forms are enabled for m <= kFewFormsMaxM only (see kFewFormsMaxM).
+    // stage 1 at P = 8 / 16 / 32 / 64): from kModiLaneFrom one lane per partition (k_modi); below
+    // kModiTwistedBelow, non-periodic with partitions of at least 4 rows, two lanes per partition
+    // from both ends (k_modi_twisted: 21.1 / 10.8 / 7.0-7.4 / 6.2-7.4 ms, then k_update_twisted);
+    // otherwise from kModiCoopBelow one lane per partition with the next row loaded during the
+    // current one (47.6 / 24.4 / 12.7 / 7.5 ms) and below that m lanes per partition (k_modi_coop:
+    // 25.0 / 15.6 / 16.9 / 13.0 ms), non-periodic solves only. Loading ahead changes no
+    // arithmetic, so periodic solves with few partitions use it too and still equal the
+    // matrix-free sweeps bit for bit. These few-partition forms are enabled for m <= kFewFormsMaxM
+    // only (see kFewFormsMaxM).
     constexpr bool kCanCoop = M >= 2 && std::is_same<BS, MemBlocks>::value;
-    bool coop = M >= kModiCoopFrom, ahead = false;
+    bool coop = M >= kModiCoopFrom, ahead = false, twisted12 = false;
     if constexpr (kCanCoop && M <= kFewFormsMaxM) {
         const int64_t lanes = (int64_t)nsys * npart;
-        coop = !cyclic && lanes < kModiCoopBelow;
-        ahead = !coop && lanes < kModiLaneFrom;
+        twisted12 = !cyclic && lanes < kModiTwistedBelow && nrow >= 4 * npart;
+        coop = !twisted12 && !cyclic && lanes < kModiCoopBelow;
+        ahead = !twisted12 && !coop && lanes < kModiLaneFrom;
     }
-    if (coop) {
+    if (twisted12) {
+        if constexpr (kCanCoop && M <= kFewFormsMaxM) {
+            const dim3 gridt((unsigned)(((nsys + kTile / 2 - 1) / (kTile / 2)) * npart));
+            if (layout == LAYOUT_TILED)
+                hipLaunchKernelGGL((k_modi_twisted<M, LAYOUT_TILED>), gridt, block, lds, s, A, B, C, D, nsys, nrow,
+                                   pf, pl, npart, redA, redC, redD, flag);
+            else
+                hipLaunchKernelGGL((k_modi_twisted<M, LAYOUT_REF_GPU>), gridt, block, lds, s, A, B, C, D, nsys,
+                                   nrow, pf, pl, npart, redA, redC, redD, flag);
+        }
+    } else if (coop) {
         if constexpr (kCanCoop) {
             // the items of a partition on one XCD. floor(64/m) systems a wavefront leave partial
             // lines, which neighbouring wavefronts fetch again (m = 5, 512 x 65536, P = 32: 42 GB
@@ -2054,10 +2411,21 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     else if (!twisted)
         hipLaunchKernelGGL((k_reduced<M>), gridr, block, 0, s, redA, redC, redD, nsys, 2 * npart, flag);
     if (ev) (void)hipEventRecord(ev[2], s);
-    // stage 3 m lanes per partition wherever stage 1 ran that way (M >= 2)
+    // stage 3: twisted after the twisted stage 1; otherwise m lanes per partition below
+    // kModiLaneFrom partitions (M >= 2)
     bool coop3 = false;
     if constexpr (M >= 2 && M <= kFewFormsMaxM) coop3 = !cyclic && (int64_t)nsys * npart < kModiLaneFrom;
-    if (coop3) {
+    if (twisted12) {
+        if constexpr (kCanCoop && M <= kFewFormsMaxM) {
+            const dim3 gridt((unsigned)(((nsys + kTile / 2 - 1) / (kTile / 2)) * npart));
+            if (layout == LAYOUT_TILED)
+                hipLaunchKernelGGL((k_update_twisted<M, LAYOUT_TILED>), gridt, block, 0, s, A, C, D, nsys, nrow,
+                                   pf, pl, npart, redD);
+            else
+                hipLaunchKernelGGL((k_update_twisted<M, LAYOUT_REF_GPU>), gridt, block, 0, s, A, C, D, nsys, nrow,
+                                   pf, pl, npart, redD);
+        }
+    } else if (coop3) {
         if constexpr (M >= 2) {
             // four wavefronts locked per workgroup (their shared lines read once: m = 5, 512 x 65536,
             // P = 16 / 32 / 64: 6.5 / 5.6 / 5.4 -> 4.6 / 4.6 / 3.6 ms) once there are two

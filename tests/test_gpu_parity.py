@@ -378,12 +378,11 @@ def test_lapack_order_form_rules(cuda):
 @pytest.mark.parametrize("nsys", [16387, 8195])
 def test_two_stage_one_lane_per_partition(cuda, oracle, m, layout, nsys):
     """From 65536 partitions in the batch (nsys * npart) stage 1 and stage 3 run one lane per
-    partition (k_modi, k_update); from 16384, stage 1 one lane per partition with the next row
-    loaded ahead and stage 3 m lanes per partition (k_update_coop); below that, m lanes per
-    partition in both (k_modi_coop), which the smaller two-stage tests cover (m <= 5; from m = 6
-    both stages one lane per partition at every count). 16387 or 8195 systems x 4 partitions of
-    n = 24 rows; sampled systems (first tile, a middle one, the partial last) vs the oracle's
-    two-stage solve."""
+    partition (k_modi, k_update); below that, for m <= 5, two lanes per partition from both
+    ends (k_modi_twisted, k_update_twisted; partitions of 3 rows keep the forms of
+    test_two_stage_three_row_partitions); from m = 6 one lane per partition at every count.
+    16387 or 8195 systems x 4 partitions of n = 24 rows; sampled systems (first tile, a middle
+    one, the partial last) vs the oracle's two-stage solve."""
     import torch
     if m == 6 and nsys == 8195:
         pytest.skip("m = 6, 8195 x 4 partitions: this case ended in an illegal memory access when the "
@@ -425,3 +424,57 @@ def test_config0_literal_generator_full_batch(cuda, oracle, layout):
     assert e_o <= TOL
     assert e_t <= 10 * max(e_ot, 1e-16)
 
+
+
+@pytest.mark.parametrize("m", [2, 3, 4, 5])
+@pytest.mark.parametrize("layout", [pb.Layout.TILED, pb.Layout.REF_GPU])
+@pytest.mark.parametrize("nsys,n,npart", [(100, 16, 4), (33, 21, 5), (31, 200, 3), (1000, 40, 8)])
+def test_two_stage_twisted(cuda, oracle, m, layout, nsys, n, npart):
+    """Below 65536 partitions in the batch, non-periodic solves whose partitions have at least 4
+    rows run both stages twisted (k_modi_twisted, k_update_twisted: two lanes per partition,
+    from both ends, meeting in the middle). Partitions of exactly 4 rows, of 4 and 5, of 66-67
+    and of 5; system counts that leave a half-filled or partial last wavefront (32 systems per
+    wavefront). Every system vs the oracle's two-stage solve."""
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 300 + m + n, m, nsys, n)
+    x, sing = gpu_solve(A, B, C, d, int(layout), npart=npart)
+    assert not sing
+    xo, info = oracle.solve_two_stage(cpu_layout(A), cpu_layout(B), cpu_layout(C), d, npart)
+    assert info == 0
+    assert rel_err(x, xo).max() <= TOL
+
+
+@pytest.mark.parametrize("m,nsys,n,npart", [(3, 200, 11, 3), (5, 8195, 11, 3)])
+@pytest.mark.parametrize("layout", [pb.Layout.TILED, pb.Layout.REF_GPU])
+def test_two_stage_three_row_partitions(cuda, oracle, m, nsys, n, npart, layout):
+    """Partitions of 3 rows (nrow < 4 npart) keep the forms without a meeting: m lanes per
+    partition below 16384 partitions in the batch, one lane per partition with the next row
+    loaded ahead from there. Every system vs the oracle's two-stage solve."""
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 400 + m, m, nsys, n)
+    x, sing = gpu_solve(A, B, C, d, int(layout), npart=npart)
+    assert not sing
+    xo, info = oracle.solve_two_stage(cpu_layout(A), cpu_layout(B), cpu_layout(C), d, npart)
+    assert info == 0
+    assert rel_err(x, xo).max() <= TOL
+
+
+@pytest.mark.parametrize("m", [2, 5])
+def test_per_system_info_twisted_stage1(cuda, m):
+    """Zero block rows met by each lane of the twisted stage 1: a partition's first row (the top
+    lane's close), its last row (the bottom lane's), the two rows where the lanes meet, and an
+    interior row of each half; named per system in both layouts."""
+    import torch
+    npart, nsys = 3, 90
+    n = 4 * npart + 5  # partitions of 6, 6, 5 rows
+    firsts = [pb.partition_1d(0, n - 1, npart, r)[0] for r in range(npart)]
+    lens = [pb.partition_1d(0, n - 1, npart, r)[1] - firsts[r] + 1 for r in range(npart)]
+    k1 = (lens[1] + 1) // 2 - 1
+    bad = [(1, firsts[0]), (17, firsts[1] + lens[1] - 1), (33, firsts[1] + k1), (34, firsts[1] + k1 + 1),
+           (64, firsts[2] + 1), (89, firsts[2] + lens[2] - 2)]
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 13, m, nsys, n)
+    for s_, q in bad:
+        A[q, s_] = B[q, s_] = C[q, s_] = 0.0
+    for lay in (pb.Layout.TILED, pb.Layout.REF_GPU):
+        ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, int(lay))).to(cuda) for X in (A, B, C, d))
+        with pb.BtdmaGpuPlan().create(m, nsys, n, npart, lay) as plan:
+            pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
+            assert plan.info().nonzero()[0].tolist() == sorted(s_ for s_, _ in bad)
