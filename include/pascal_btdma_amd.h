@@ -65,11 +65,16 @@ typedef enum btd_form {
                            * (m >= 8; one block row each, pivoting through LDS) or, for smaller
                            * batches, twisted (these two agree with it to rounding) */
     BTD_FORM_TWISTED = 1, /* two groups of m lanes per system, one eliminating from each end */
-    BTD_FORM_LAPACK_ORDER = 2 /* the one-lane solve at every batch size and m: each block
-                               * operation in the order and rounding of the reference CPU path's
-                               * BLAS/LAPACK calls (dgemm/dgemv sums from the first term, dgetf2
-                               * + dgetrs, IEEE division, no fused multiply-add) */
+    BTD_FORM_LAPACK_ORDER = 2, /* the one-lane solve at every batch size and m: each block
+                                * operation in the order and rounding of the reference CPU path's
+                                * BLAS/LAPACK calls (dgemm/dgemv sums from the first term, dgetf2
+                                * + dgetrs, IEEE division, no fused multiply-add) */
+    BTD_FORM_WORKSPACE = 3     /* the solve of BTD_FORM_LAPACK_ORDER with C' written to a
+                                * workspace the plan owns instead of over C: C is only read */
 } btd_form;
+
+/* Most workspaces btd_plan_place_workspace tries in one call. */
+#define BTD_MAX_WORKSPACE_CANDIDATES 16
 
 /* Phase times in seconds; the fields of BtdmaSolveTimings (pascal_btdma_cuda.hpp:24-39)
  * and BTDMA_TIMING_gpu_v2 (mod_btdma_gpu_v2.f90:38-45). On one GPU the two exchange
@@ -118,10 +123,31 @@ btd_status btd_plan_npart(btd_plan_t plan, int *npart);
  * size: every block operation of mod_btdma_cpu.f90:15-78 in the order of the reference's dgemm,
  * dgemv and dgesv as reference (unblocked dgetf2/dgetrs) BLAS/LAPACK performs them, so x is the
  * FP64 CPU path's result bit for bit, not merely within rounding of it (slower than the
- * library's choice on batches that do not fill the device one lane per system). Cyclic and the
- * other solve variants are unaffected. */
+ * library's choice on batches that do not fill the device one lane per system).
+ * BTD_FORM_WORKSPACE (npart = 1, any m) is that solve with C'_q written to a workspace of the
+ * layout's size of an m*m block array, allocated by this call and owned by the plan, instead of
+ * over C: C is only read, so a caller that solves the same blocks again need not restore C, and
+ * the library, not the caller, decides where the written array lies (btd_plan_place_workspace).
+ * Same x, bit for bit, as BTD_FORM_LAPACK_ORDER. Setting another form frees the workspace.
+ * Cyclic and the other solve variants are unaffected. */
 btd_status btd_plan_set_form(btd_plan_t plan, int form);
 btd_status btd_plan_form(btd_plan_t plan, int *form);
+
+/* New (no counterpart in the reference), for plans in BTD_FORM_WORKSPACE: picks where in HBM the
+ * workspace lies. The time of the one-lane solve depends on the physical pages of the two arrays
+ * it writes (the workspace and D), which no allocation call chooses: on MI355X, m = 5, n = 512,
+ * 131072 systems, the same solve of the same arrays took 13.2 ms or 14.9 ms depending on the
+ * allocation, each allocation keeping its time. This call allocates up to `candidates` - 1 more
+ * workspaces (as many as fit), times two solves of the caller's arrays with each (D restored
+ * from a copy before each solve and at the end, so D returns unchanged), keeps the fastest
+ * workspace and frees the others. ms (NULL or `candidates` entries) receives each candidate's
+ * time in milliseconds, -1 for those not allocated; candidate 0 is the plan's workspace before
+ * the call. The solves raise the plan's zero-pivot words as a solve would. Synchronizes `stream`
+ * and allocates: not capturable. The choice holds for these arrays; another D may prefer
+ * another workspace. 1 <= candidates <= BTD_MAX_WORKSPACE_CANDIDATES. */
+btd_status btd_plan_place_workspace(btd_plan_t plan, const double *a, const double *b, const double *c,
+                                    double *d, int m, int nsys, int nrow, int candidates, double *ms,
+                                    void *stream);
 
 /* Replaces BtdmaGpuPlan::destroy (pascal_btdma_cuda.hpp:134, pascal_btdma_cuda.cu:1007-1033)
  * and btdma_cleanplan_gpu_v2 (mod_btdma_gpu_v2.f90:111-129). NULL is accepted. */

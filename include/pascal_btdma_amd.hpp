@@ -118,9 +118,17 @@ struct BtdmaGpuPlan {
     void singular_systems(int* info) { check_status(btd_plan_info(handle, info)); }
 
     // How an npart = 1 plan's non-cyclic solve runs (btd_plan_set_form): BTD_FORM_PLAIN, the
-    // library's choice (the default), BTD_FORM_TWISTED, always from both ends, or
-    // BTD_FORM_LAPACK_ORDER, one lane per system in the CPU path's BLAS/LAPACK operation order.
+    // library's choice (the default), BTD_FORM_TWISTED, always from both ends,
+    // BTD_FORM_LAPACK_ORDER, one lane per system in the CPU path's BLAS/LAPACK operation order,
+    // or BTD_FORM_WORKSPACE, that solve with C' in a plan-owned workspace (C only read).
     void set_form(btd_form f) { check_status(btd_plan_set_form(handle, (int)f)); }
+
+    // BTD_FORM_WORKSPACE only: keep the fastest of up to `candidates` workspace allocations for
+    // these arrays (btd_plan_place_workspace); ms: NULL or `candidates` entries.
+    void place_workspace(const double* a, const double* b, const double* c, double* d, int candidates,
+                         double* ms = nullptr, hipStream_t stream = nullptr) {
+        check_status(btd_plan_place_workspace(handle, a, b, c, d, m, nsys, nrow, candidates, ms, stream));
+    }
     btd_form form() {
         int f = 0;
         check_status(btd_plan_form(handle, &f));

@@ -41,6 +41,7 @@ class Form(enum.IntEnum):
     PLAIN = 0
     TWISTED = 1
     LAPACK_ORDER = 2
+    WORKSPACE = 3
 
 
 class GenKind(enum.IntEnum):
@@ -171,15 +172,28 @@ class BtdmaGpuPlan:
 
     @property
     def form(self) -> int:
-        """How the single-partition solve runs: Form.PLAIN, TWISTED or LAPACK_ORDER (btd_plan_form)."""
+        """How the single-partition solve runs: Form.PLAIN, TWISTED, LAPACK_ORDER or WORKSPACE
+        (btd_plan_form)."""
         f = ctypes.c_int(0)
         _check(_capi.lib().btd_plan_form(self._h, ctypes.byref(f)))
         return Form(f.value)
 
     def set_form(self, form: "Form") -> "BtdmaGpuPlan":
-        """btd_plan_set_form: Form.TWISTED needs npart == 1 and m >= 2 (either layout)."""
+        """btd_plan_set_form: Form.TWISTED needs npart == 1 and m >= 2 (either layout);
+        LAPACK_ORDER and WORKSPACE need npart == 1 (WORKSPACE allocates the plan's C' array)."""
         _check(_capi.lib().btd_plan_set_form(self._h, int(form)))
         return self
+
+    def place_workspace(self, a, b, c, d, candidates: int = 8, stream=None) -> list[float]:
+        """btd_plan_place_workspace: with Form.WORKSPACE set, time the solve of these arrays with
+        up to `candidates` workspace allocations and keep the fastest; d returns unchanged.
+        Returns each candidate's time in ms (-1: not allocated; candidate 0 was the plan's)."""
+        _validate(self, a, b, c, d, self.m, self.nsys, self.nrow)
+        ms = (ctypes.c_double * int(candidates))()
+        _check(_capi.lib().btd_plan_place_workspace(self._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d), self.m,
+                                                     self.nsys, self.nrow, int(candidates), ms,
+                                                     _stream(stream)))
+        return [float(x) for x in ms]
 
     def check(self) -> bool:
         """Synchronise; True if any block LU met a zero pivot since the last check."""

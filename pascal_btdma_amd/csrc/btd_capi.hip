@@ -71,12 +71,15 @@ struct btd_plan_s {
     int cyc_npart = 0;
     int* cyc_part = nullptr;
     double *cyc_redA = nullptr, *cyc_redC = nullptr, *cyc_redD = nullptr;
+    // BTD_FORM_WORKSPACE: C' of the single-partition solve goes here (the layout's elements of an
+    // m*m block array), allocated when the form is set, placed by btd_plan_place_workspace
+    double* ws = nullptr;
 };
 
 static void plan_free(btd_plan_s* p) {
     if (!p) return;
     for (void* q : {(void*)p->d_part, (void*)p->redA, (void*)p->redC, (void*)p->redD, (void*)p->cyc_part,
-                    (void*)p->cyc_redA, (void*)p->cyc_redC, (void*)p->cyc_redD})
+                    (void*)p->cyc_redA, (void*)p->cyc_redC, (void*)p->cyc_redD, (void*)p->ws})
         if (q) (void)hipFree(q);
     if (p->d_flag) (void)hipFree(p->d_flag);
     for (auto& e : p->ev)
@@ -218,22 +221,46 @@ btd_status btd_plan_npart(btd_plan_t plan, int* npart) {
     return BTD_OK;
 }
 
+static size_t ws_bytes(const btd_plan_s* p) {
+    return sizeof(double) * btd_layout_elems(p->layout, p->nsys, p->nrow, p->m * p->m);
+}
+
+// Leaving BTD_FORM_WORKSPACE frees the workspace (after the plan's last solve stream is done).
+static void drop_workspace(btd_plan_s* p) {
+    if (!p->ws) return;
+    if (p->last_stream) (void)hipStreamSynchronize(p->last_stream);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(p->ws);
+    p->ws = nullptr;
+}
+
 btd_status btd_plan_set_form(btd_plan_t plan, int form) {
     g_err.clear();
     if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_plan_set_form: uncreated plan");
     if (form == BTD_FORM_PLAIN) {
+        drop_workspace(plan);
         plan->form = form;
         return BTD_OK;
     }
-    if (form == BTD_FORM_LAPACK_ORDER) {
+    if (form == BTD_FORM_LAPACK_ORDER || form == BTD_FORM_WORKSPACE) {
         if (plan->npart != 1)
-            return fail(BTD_ERR_INVALID_ARG, "btd_plan_set_form: the LAPACK-order form needs npart == 1");
+            return fail(BTD_ERR_INVALID_ARG, form == BTD_FORM_WORKSPACE
+                                                 ? "btd_plan_set_form: the workspace form needs npart == 1"
+                                                 : "btd_plan_set_form: the LAPACK-order form needs npart == 1");
+        if (form == BTD_FORM_WORKSPACE && !plan->ws) {
+            void* p = nullptr;
+            hipError_t e = hipMalloc(&p, ws_bytes(plan));
+            if (e != hipSuccess) return hip_fail(e, "btd_plan_set_form: workspace");
+            plan->ws = static_cast<double*>(p);
+        }
+        if (form == BTD_FORM_LAPACK_ORDER) drop_workspace(plan);
         plan->form = form;
         return BTD_OK;
     }
     if (form != BTD_FORM_TWISTED) return fail(BTD_ERR_INVALID_ARG, "btd_plan_set_form: unknown form");
     if (plan->npart != 1 || plan->m < 2)
         return fail(BTD_ERR_INVALID_ARG, "btd_plan_set_form: the twisted form needs npart == 1 and m >= 2");
+    drop_workspace(plan);
     plan->form = form;
     return BTD_OK;
 }
@@ -309,9 +336,10 @@ static btd_status solve_impl(btd_plan_t plan, double* a, const double* b, double
         if (plan->form == BTD_FORM_TWISTED)
             BTD_HIP(btd::launch_plain_twisted(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                               plan->d_flag, s));
-        else if (plan->form == BTD_FORM_LAPACK_ORDER)
-            BTD_HIP(btd::launch_plain_ref(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
-                                          plan->d_flag, s));
+        else if (plan->form == BTD_FORM_LAPACK_ORDER || plan->form == BTD_FORM_WORKSPACE)
+            BTD_HIP(btd::launch_plain_ref(plan->m, plan->layout, a, b, c,
+                                          plan->form == BTD_FORM_WORKSPACE ? plan->ws : c, d, plan->nsys,
+                                          plan->nrow, plan->d_flag, s));
         else
             BTD_HIP(btd::launch_plain(plan->m, plan->layout, a, b, c, d, plan->nsys, plan->nrow,
                                       plan->d_flag, s));
@@ -523,6 +551,76 @@ btd_status btd_plan_info(btd_plan_t plan, int* info) {
     if (!plan || !plan->created) return fail(BTD_ERR_NOT_CREATED, "btd_plan_info: uncreated plan");
     if (!info) return fail(BTD_ERR_INVALID_ARG, "btd_plan_info: NULL info");
     return read_status(plan, nullptr, info);
+}
+
+btd_status btd_plan_place_workspace(btd_plan_t plan, const double* a, const double* b, const double* c,
+                                    double* d, int m, int nsys, int nrow, int candidates, double* ms,
+                                    void* stream) {
+    g_err.clear();
+    const char* what = "btd_plan_place_workspace";
+    btd_status st = check_solve(plan, a, b, c, d, m, nsys, nrow);
+    if (st != BTD_OK) return st;
+    if (plan->form != BTD_FORM_WORKSPACE || !plan->ws)
+        return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": the plan's form is not BTD_FORM_WORKSPACE");
+    if (candidates < 1 || candidates > BTD_MAX_WORKSPACE_CANDIDATES)
+        return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": candidates must be in 1.." +
+                                             std::to_string(BTD_MAX_WORKSPACE_CANDIDATES));
+    hipStream_t s = (hipStream_t)stream;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    BTD_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+        return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": allocates and synchronizes; cannot be captured");
+    if (ms)
+        for (int k = 0; k < candidates; ++k) ms[k] = -1.0;
+    const size_t wb = ws_bytes(plan);
+    const size_t db = sizeof(double) * btd_layout_elems(plan->layout, plan->nsys, plan->nrow, plan->m);
+    BTD_HIP(hipStreamSynchronize(s));
+    void* dsave = nullptr;
+    BTD_HIP(hipMalloc(&dsave, db));
+    std::vector<double*> cand{plan->ws};
+    for (int k = 1; k < candidates; ++k) {  // as many as fit; the plan's own is the first
+        void* p = nullptr;
+        if (hipMalloc(&p, wb) != hipSuccess) {
+            (void)hipGetLastError();
+            break;
+        }
+        cand.push_back(static_cast<double*>(p));
+    }
+    std::vector<double> t(cand.size(), 0.0);
+    hipError_t e = hipMemcpyAsync(dsave, d, db, hipMemcpyDeviceToDevice, s);
+    // two timed solves on each candidate (the first of a fresh allocation maps its pages): the
+    // faster counts
+    for (size_t k = 0; k < cand.size() && e == hipSuccess; ++k) {
+        double best = 0.0;
+        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+            e = hipMemcpyAsync(d, dsave, db, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess) e = hipEventRecord(plan->ev[0], s);
+            if (e == hipSuccess)
+                e = btd::launch_plain_ref(plan->m, plan->layout, a, b, c, cand[k], d, plan->nsys, plan->nrow,
+                                          plan->d_flag, s);
+            if (e == hipSuccess) e = hipEventRecord(plan->ev[1], s);
+            if (e == hipSuccess) e = hipEventSynchronize(plan->ev[1]);
+            if (e == hipSuccess) {
+                const double v = ev_seconds(plan->ev[0], plan->ev[1]) * 1e3;
+                best = rep == 0 || v < best ? v : best;
+            }
+        }
+        t[k] = best;
+    }
+    size_t pick = 0;
+    for (size_t k = 1; k < cand.size(); ++k)
+        if (t[k] < t[pick]) pick = k;
+    if (e == hipSuccess) e = hipMemcpyAsync(d, dsave, db, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) plan->ws = cand[pick];  // on failure the plan keeps its own
+    for (size_t k = 0; k < cand.size(); ++k)
+        if (cand[k] != plan->ws) (void)hipFree(cand[k]);
+    (void)hipFree(dsave);
+    if (e != hipSuccess) return hip_fail(e, what);
+    if (ms)
+        for (size_t k = 0; k < cand.size(); ++k) ms[k] = t[k];
+    plan->last_stream = s;
+    return BTD_OK;
 }
 
 btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double* a, const double* b, double* c,

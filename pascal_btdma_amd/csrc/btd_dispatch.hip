@@ -119,7 +119,8 @@ hipError_t launch_outer_f32_1(int, const double*, const double*, float*, float*,
                               hipStream_t);
 hipError_t launch_plain_f32s_1(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_1(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_1(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 hipError_t launch_plain_save_2(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -132,7 +133,8 @@ hipError_t launch_outer_f32_2(int, const double*, const double*, float*, float*,
                               hipStream_t);
 hipError_t launch_plain_f32s_2(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_2(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_2(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 hipError_t launch_plain_save_3(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -145,7 +147,8 @@ hipError_t launch_outer_f32_3(int, const double*, const double*, float*, float*,
                               hipStream_t);
 hipError_t launch_plain_f32s_3(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_3(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_3(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 hipError_t launch_plain_save_4(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -158,7 +161,8 @@ hipError_t launch_outer_f32_4(int, const double*, const double*, float*, float*,
                               hipStream_t);
 hipError_t launch_plain_f32s_4(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_4(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_4(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 hipError_t launch_plain_save_5(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -171,7 +175,8 @@ hipError_t launch_outer_f32_5(int, const double*, const double*, float*, float*,
                               hipStream_t);
 hipError_t launch_plain_f32s_5(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_5(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_5(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 hipError_t launch_plain_save_6(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -184,7 +189,8 @@ hipError_t launch_outer_f32_6(int, const double*, const double*, float*, float*,
                               hipStream_t);
 hipError_t launch_plain_f32s_6(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_6(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_6(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 hipError_t launch_plain_save_7(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -197,7 +203,8 @@ hipError_t launch_outer_f32_7(int, const double*, const double*, float*, float*,
                               hipStream_t);
 hipError_t launch_plain_f32s_7(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_7(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_7(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 hipError_t launch_plain_save_8(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -210,7 +217,8 @@ hipError_t launch_outer_f32_8(int, const double*, const double*, float*, float*,
                               hipStream_t);
 hipError_t launch_plain_f32s_8(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_8(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_8(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 hipError_t launch_plain_save_9(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -223,7 +231,8 @@ hipError_t launch_outer_f32_9(int, const double*, const double*, float*, float*,
                               hipStream_t);
 hipError_t launch_plain_f32s_9(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_9(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_9(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 hipError_t launch_plain_save_10(int, const double*, const double*, double*, double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
@@ -236,7 +245,8 @@ hipError_t launch_outer_f32_10(int, const double*, const double*, float*, float*
                               hipStream_t);
 hipError_t launch_plain_f32s_10(int, const float*, const float*, const float*, double*, double*, int,
                                int, int*, hipStream_t);
-hipError_t launch_plain_ref_10(int, const double*, const double*, double*, double*, int, int, int*, hipStream_t);
+hipError_t launch_plain_ref_10(int, const double*, const double*, const double*, double*, double*, int, int, int*,
+                              hipStream_t);
 
 // ---------------------------------------------------------------------------- convert --
 __global__ __launch_bounds__(256) void k_convert(int src_layout, int dst_layout, int E, int nsys,
@@ -337,9 +347,9 @@ hipError_t launch_plain_f32s(int m, int layout, const float* A, const float* B, 
 #undef CALL
 }
 
-hipError_t launch_plain_ref(int m, int layout, const double* A, const double* B, double* C, double* D,
-                            int nsys, int nrow, int* flag, hipStream_t s) {
-#define CALL(M) launch_plain_ref_##M(layout, A, B, C, D, nsys, nrow, flag, s)
+hipError_t launch_plain_ref(int m, int layout, const double* A, const double* B, const double* C, double* Cw,
+                            double* D, int nsys, int nrow, int* flag, hipStream_t s) {
+#define CALL(M) launch_plain_ref_##M(layout, A, B, C, Cw, D, nsys, nrow, flag, s)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }

@@ -67,9 +67,9 @@ hipError_t BTD_NAME(launch_plain_f32s_)(int layout, const float* A, const float*
     return launch_plain_f32s_m<BTD_M>(layout, A, B, C, Cw, D, nsys, nrow, flag, s);
 }
 
-hipError_t BTD_NAME(launch_plain_ref_)(int layout, const double* A, const double* B, double* C,
-                                       double* D, int nsys, int nrow, int* flag, hipStream_t s) {
-    return launch_plain_ref_m<BTD_M>(layout, A, B, C, D, nsys, nrow, flag, s);
+hipError_t BTD_NAME(launch_plain_ref_)(int layout, const double* A, const double* B, const double* C,
+                                       double* Cw, double* D, int nsys, int nrow, int* flag, hipStream_t s) {
+    return launch_plain_ref_m<BTD_M>(layout, A, B, C, Cw, D, nsys, nrow, flag, s);
 }
 
 hipError_t BTD_NAME(launch_outer_f32_)(int layout, const double* Lam, const double* X, float* dA,

@@ -59,9 +59,10 @@ hipError_t launch_sweep_move(int layout, int axis, double* F, double* D, int n1,
 hipError_t launch_plain_f32s(int m, int layout, const float* A, const float* B, const float* C,
                              double* Cw, double* D, int nsys, int nrow, int* flag, hipStream_t s);
 
-// The single-partition solve one lane per system (the oracle's operation order) at any size.
-hipError_t launch_plain_ref(int m, int layout, const double* A, const double* B, double* C, double* D,
-                            int nsys, int nrow, int* flag, hipStream_t s);
+// The single-partition solve one lane per system (the oracle's operation order) at any size,
+// C' written to Cw (Cw = C: in place).
+hipError_t launch_plain_ref(int m, int layout, const double* A, const double* B, const double* C, double* Cw,
+                            double* D, int nsys, int nrow, int* flag, hipStream_t s);
 
 hipError_t launch_outer_f32(int m, int layout, const double* Lam, const double* X, float* dA,
                             float* dB, float* dC, int nsys, int nrow, hipStream_t s);

@@ -347,11 +347,13 @@ __device__ __forceinline__ void plain_solve(const TB* __restrict__ A, const TB* 
 // The one-lane solve computes in the oracle's operation order (ORD): measured at the same time
 // as the fused order on the m = 5, n = 512, 131072-system batch (14.90 / 14.90 ms Euler-like,
 // 14.70 / 14.71 ms on conditioned blocks), it returns the oracle's x bit for bit.
+// C'_q goes to Cw: C itself for the in-place solve, a plan's workspace for BTD_FORM_WORKSPACE
+// (C then only read).
 template <int M, int LAYOUT>
 __global__ __launch_bounds__(64) void k_plain(const double* __restrict__ A,
-                                              const double* __restrict__ B, double* C, double* D,
-                                              int nsys, int nrow, int* __restrict__ flag) {
-    plain_solve<M, LAYOUT, false, double, true>(A, B, C, C, D, nullptr, nullptr, nsys, nrow, flag);
+                                              const double* __restrict__ B, const double* C, double* Cw,
+                                              double* D, int nsys, int nrow, int* __restrict__ flag) {
+    plain_solve<M, LAYOUT, false, double, true>(A, B, C, Cw, D, nullptr, nullptr, nsys, nrow, flag);
 }
 
 // FP32-stored blocks, FP64 arithmetic: the FP64 solve of the blocks' values (bit for bit what
@@ -2176,13 +2178,13 @@ constexpr int kPlainTwistedFrom = BTD_PLAIN_TWISTED_FROM;
 constexpr int kPlainLaneFrom = 32768;
 
 template <int M>
-void launch_plain_lane(int layout, const double* A, const double* B, double* C, double* D, int nsys, int nrow,
-                       int* flag, hipStream_t s) {
+void launch_plain_lane(int layout, const double* A, const double* B, const double* C, double* Cw, double* D,
+                       int nsys, int nrow, int* flag, hipStream_t s) {
     const dim3 grid((nsys + 63) / 64), block(64);
     if (layout == LAYOUT_TILED)
-        hipLaunchKernelGGL((k_plain<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+        hipLaunchKernelGGL((k_plain<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, Cw, D, nsys, nrow, flag);
     else
-        hipLaunchKernelGGL((k_plain<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+        hipLaunchKernelGGL((k_plain<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, Cw, D, nsys, nrow, flag);
 }
 
 template <int M>
@@ -2195,13 +2197,13 @@ hipError_t launch_plain_m(int layout, const double* A, const double* B, double* 
             launch_plain_coop<M, LAYOUT_REF_GPU>(A, B, C, D, nsys, nrow, flag, s);
     } else if constexpr (M >= 2 && M >= kPlainTwistedFrom) {
         if (nsys >= kPlainLaneFrom)
-            launch_plain_lane<M>(layout, A, B, C, D, nsys, nrow, flag, s);
+            launch_plain_lane<M>(layout, A, B, C, C, D, nsys, nrow, flag, s);
         else if (layout == LAYOUT_TILED)
             launch_plain_twisted_auto<M, LAYOUT_TILED>(A, B, C, D, nsys, nrow, flag, s);
         else
             launch_plain_twisted_auto<M, LAYOUT_REF_GPU>(A, B, C, D, nsys, nrow, flag, s);
     } else {
-        launch_plain_lane<M>(layout, A, B, C, D, nsys, nrow, flag, s);
+        launch_plain_lane<M>(layout, A, B, C, C, D, nsys, nrow, flag, s);
     }
     return hipGetLastError();
 }
@@ -2246,11 +2248,12 @@ hipError_t launch_plain_f32s_m(int layout, const float* A, const float* B, const
     return hipGetLastError();
 }
 
-// BTD_FORM_LAPACK_ORDER: the one-lane solve (the oracle's operation order) at any batch size and m.
+// BTD_FORM_LAPACK_ORDER: the one-lane solve (the oracle's operation order) at any batch size and
+// m; BTD_FORM_WORKSPACE: the same with C' written to the workspace Cw (Cw = C: in place).
 template <int M>
-hipError_t launch_plain_ref_m(int layout, const double* A, const double* B, double* C, double* D, int nsys,
-                              int nrow, int* flag, hipStream_t s) {
-    launch_plain_lane<M>(layout, A, B, C, D, nsys, nrow, flag, s);
+hipError_t launch_plain_ref_m(int layout, const double* A, const double* B, const double* C, double* Cw,
+                              double* D, int nsys, int nrow, int* flag, hipStream_t s) {
+    launch_plain_lane<M>(layout, A, B, C, Cw, D, nsys, nrow, flag, s);
     return hipGetLastError();
 }
 

@@ -478,3 +478,57 @@ def test_per_system_info_twisted_stage1(cuda, m):
         with pb.BtdmaGpuPlan().create(m, nsys, n, npart, lay) as plan:
             pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
             assert plan.info().nonzero()[0].tolist() == sorted(s_ for s_, _ in bad)
+
+
+@pytest.mark.parametrize("m", [1, 3, 5, 8])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_workspace_form(cuda, oracle, m, layout):
+    """BTD_FORM_WORKSPACE: the LAPACK-order one-lane solve with C' written to the plan's own
+    workspace. x equals the oracle's btdma_many bit for bit; C is only read; after
+    btd_plan_place_workspace (3 candidates: D restored, one workspace kept) the next solve gives
+    the same bits again, and so does a second placement."""
+    import torch
+    nsys, n = 130, 24
+    lay = int(layout)
+    A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 640 + m, m, nsys, n)
+    xo = oracle_plain(oracle, A, B, C, d)
+    ta, tb, tc, td = (torch.from_numpy(nat_to_layout(X, lay)).to(cuda) for X in (A, B, C, d))
+    c0, d0 = tc.clone(), td.clone()
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
+        plan.set_form(pb.Form.WORKSPACE)
+        assert plan.form == pb.Form.WORKSPACE
+        pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
+        torch.cuda.synchronize()
+        assert not plan.check()
+        assert torch.equal(tc, c0)
+        np.testing.assert_array_equal(layout_to_vec(td.cpu().numpy(), lay, n, nsys, m), xo)
+        for _ in range(2):
+            td.copy_(d0)
+            ms = plan.place_workspace(ta, tb, tc, td, candidates=3)
+            assert len(ms) == 3 and all(t > 0 for t in ms)
+            assert torch.equal(td, d0) and torch.equal(tc, c0)
+            pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
+            torch.cuda.synchronize()
+            assert torch.equal(tc, c0)
+            np.testing.assert_array_equal(layout_to_vec(td.cpu().numpy(), lay, n, nsys, m), xo)
+        plan.set_form(pb.Form.PLAIN)  # frees the workspace
+        td.copy_(d0)
+        pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
+        torch.cuda.synchronize()
+        assert rel_err(layout_to_vec(td.cpu().numpy(), lay, n, nsys, m), xo).max() <= TOL
+
+
+def test_workspace_form_rules(cuda):
+    import torch
+    with pb.BtdmaGpuPlan().create(3, 70, 30, 2, pb.Layout.TILED) as plan:
+        with pytest.raises(ValueError, match="npart == 1"):
+            plan.set_form(pb.Form.WORKSPACE)
+    with pb.BtdmaGpuPlan().create(3, 70, 30, 1, pb.Layout.TILED) as plan:
+        t = [torch.zeros(plan.matrix_elems(), dtype=torch.float64, device=cuda) for _ in range(3)]
+        v = torch.zeros(plan.vector_elems(), dtype=torch.float64, device=cuda)
+        with pytest.raises(ValueError, match="not BTD_FORM_WORKSPACE"):
+            plan.place_workspace(*t, v)
+        plan.set_form(pb.Form.WORKSPACE)
+        for bad in (0, 17):
+            with pytest.raises(ValueError, match="candidates"):
+                plan.place_workspace(*t, v, candidates=bad)
