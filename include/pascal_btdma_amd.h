@@ -100,10 +100,13 @@ btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npar
 
 /* New: the partition count a plan created with npart = 0 gets on one GPU, where partitions are
  * an occupancy choice rather than a rank count: 1 (the single-kernel solve) for nsys >= 32768
- * (nsys >= 4096 for m >= 7 and 8192 for m = 2, 4, whose single-partition solve then runs m or
- * 2m lanes per system; and for m >= 5 systems of fewer than 128 rows), otherwise
- * about 65536 / nsys (one wave per SIMD), at most nrow / 8, and 1 when that is < 2.
- * Pure function (no device use). */
+ * (nsys >= 8192 for m = 2, 4 and 4096 for m >= 6, whose single-partition solve then runs m or 2m
+ * lanes per system). Otherwise, for m <= 5 (both stages two lanes per partition): 1 below 128
+ * rows when nsys < 8192, else about 16384 / nsys partitions (at least 2 from 8192 systems) of at
+ * least 16 rows, rounded down to a power of two; for m = 6: 1 up to 128 rows, else about
+ * 65536 / nsys of at least 8 rows, a power of two; for m >= 7: 1 below 128 rows, else about
+ * 65536 / nsys (one wave per SIMD) of at least 8 rows. 1 whenever that is < 2. Calibrated on
+ * MI355X over m = 2-8, 256-16384 systems, 64-2048 rows. Pure function (no device use). */
 int btd_auto_npart(int m, int nsys, int nrow);
 
 /* The plan's partition count (the one chosen when it was created with npart = 0). */
@@ -174,8 +177,9 @@ btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double *a, const double
  * C++ port. Row 0 couples to x_{n-1} through A of row 0 and row n-1 to x_0 through C of
  * row n-1. npart > 1: stage 1 as in btd_solve_noncyclic, a periodic reduced system of
  * 2*npart rows, stage 3 (btdma_many_cyclic_mpi, mod_btdma_cpu.f90:273-302). npart = 1:
- * the same over partitions the library chooses (btd_auto_npart's count where that is >= 2,
- * else 4, or 2 below 32 rows; faster than one periodic sweep per system at every size
+ * the same over partitions the library chooses (about 65536 / nsys of at least 8 rows, 1 from
+ * 32768 systems (8192 for m = 2, 4, 4096 for m >= 7) and below 128 rows at m >= 5; where that is
+ * < 2, 4, or 2 below 32 rows; faster than one periodic sweep per system at every size
  * measured), with their scratch allocated by the plan's first periodic solve: that first call
  * allocates and copies synchronously, and is refused (BTD_ERR_INVALID_ARG) when its stream is
  * being captured, so make one periodic call before capturing one in a graph; below 6 rows one

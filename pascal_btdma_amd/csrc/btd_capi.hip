@@ -112,18 +112,13 @@ size_t btd_layout_elems(int layout, int nsys, int nrow, int elems_per_block) {
     return (size_t)btd::layout_elems(layout, nsys, nrow, elems_per_block);
 }
 
-int btd_auto_npart(int m, int nsys, int nrow) {
-    // One GPU: partitions are an occupancy choice. The single-kernel solve takes a batch that
-    // fills the device on its own: at least half a wave per SIMD (32768 systems) with one lane
-    // per system; 4096 systems for m >= 7, whose single-partition solve gives every system m
-    // or 2m lanes (cooperative or twisted); 8192 for m = 2 and 4, where the twisted solve's
-    // wavefronts read whole halves or quarters of lines (floor(32/m) systems) and measured up
-    // to 21% faster than two-stage from 8192 systems (m = 3, 5, 6 were not). A smaller batch
-    // gets enough partitions for about one wave per SIMD (65536 lanes), keeping >= 8 rows per
-    // partition so that the reduced system of 2P rows stays small against the partition work;
-    // except systems of fewer than 128 rows at m >= 5, where the three launches and the reduced
-    // solve of the two-stage form cost more than the twisted solve's n/2 row steps (64 rows,
-    // 256-2048 systems: m = 5 1.2-1.65x, m = 8 0.96-1.43x faster single-partition).
+// The partition count for stage-1 kernels with one lane per partition (the periodic solve's,
+// and m >= 7): about one wave per SIMD (65536 lanes), >= 8 rows per partition; 1 from 32768
+// systems (4096 for m >= 7, 8192 for m = 2, 4), and for systems of fewer than 128 rows at m >= 5,
+// where the three launches and the reduced solve of the two-stage form cost more than the
+// twisted solve's n/2 row steps (64 rows, 256-2048 systems: m = 5 1.2-1.65x, m = 8 0.96-1.43x
+// faster single-partition).
+static int auto_npart_one_lane(int m, int nsys, int nrow) {
     constexpr int64_t kLanes = 65536;
     if (nsys < 1 || nrow < 2) return 1;
     const int64_t one = m >= 7 ? 4096 : (m == 2 || m == 4) ? 8192 : kLanes / 2;
@@ -132,6 +127,43 @@ int btd_auto_npart(int m, int nsys, int nrow) {
     const int64_t most = nrow / 8;
     const int64_t p = want < most ? want : most;
     return p < 2 ? 1 : (int)p;
+}
+
+static int pow2_floor(int64_t p) {
+    int q = 1;
+    while ((int64_t)q * 2 <= p) q *= 2;
+    return q;
+}
+
+int btd_auto_npart(int m, int nsys, int nrow) {
+    // One GPU: partitions are an occupancy choice. The single-kernel solve takes a batch that
+    // fills the device on its own: 32768 systems with one lane per system; 8192 for m = 2 and 4,
+    // whose twisted solve's wavefronts read whole halves or quarters of lines; 4096 for m >= 6,
+    // whose single-partition solve is twisted (m = 6) or gives every system m or 2m lanes.
+    // Smaller batches, measured over m = 2-8, 256-16384 systems, 64-2048 rows (P = 1..128, the
+    // best against this rule: geometric mean 1.02x, against the one-lane rule 1.10x):
+    //   m <= 5 (stage 1 and 3 twisted, two lanes per partition): about 16384 partitions,
+    //     >= 16 rows each, a power of two; at least 2 from 8192 systems; 1 below 128 rows
+    //     for fewer than 8192 systems;
+    //   m = 6: 1 up to 128 rows, else the one-lane rule's count rounded to a power of two;
+    //   m >= 7: the one-lane rule (auto_npart_one_lane).
+    if (nsys < 1 || nrow < 2) return 1;
+    if (m >= 7) return auto_npart_one_lane(m, nsys, nrow);
+    const int64_t one = m == 6 ? 4096 : (m == 2 || m == 4) ? 8192 : 32768;
+    if ((int64_t)nsys >= one) return 1;
+    int64_t want, most;
+    if (m == 6) {
+        if (nrow <= 128) return 1;
+        want = (65536 + (int64_t)nsys - 1) / nsys;
+        most = nrow / 8;
+    } else {
+        if (nrow < 128 && nsys < 8192) return 1;
+        want = (16384 + (int64_t)nsys - 1) / nsys;
+        if (nsys >= 8192 && want < 2) want = 2;
+        most = nrow / 16;
+    }
+    const int64_t p = want < most ? want : most;
+    return p < 2 ? 1 : pow2_floor(p);
 }
 
 // Row partitions of every system (btd_partition_1d) on the host and as a device table
@@ -167,7 +199,7 @@ static hipError_t alloc_partitions(int m, int nsys, int nrow, int npart, std::ve
 // count of btd_auto_npart where that is >= 2 and otherwise 4 (2 for fewer than 32 rows; P = 2,
 // 3, 4, 8 were within 5% of each other on large batches). Fewer than 6 rows keep k_cyclic.
 static int cyclic_npart(int m, int nsys, int nrow) {
-    int p = btd_auto_npart(m, nsys, nrow);
+    int p = auto_npart_one_lane(m, nsys, nrow);
     if (p < 2) p = nrow >= 32 ? 4 : 2;
     return nrow >= 3 * p ? p : 1;
 }

@@ -922,19 +922,20 @@ __global__ __launch_bounds__(64, AHEAD ? 1 : 2) void k_modi(double* A, const dou
 }
 
 // ------------------------------------------------------------------------ modi, twisted --
-// Stage 1 with each partition swept from both ends, by lanes l (top) and l + 32 (bottom) of
-// one wavefront for system 32 * (blockIdx.x mod ceil(nsys/32)) + l, blocks in memory,
-// partitions of at least 4 rows: each lane's chain of dependent rows is half the partition's,
-// for batches with too few partitions to fill the device. The top lane runs k_modi's forward
-// sweep (rows loaded ahead) over rows 0..k, k = ceil(N/2) - 1; the bottom lane runs the same
-// sweep mirrored, over rows N-1 down to k+1, with A and C exchanging roles. A lane's rows then
-// satisfy, with its anchor x_a (x_0' top, x_{N-1} bottom) and the next row of its sweep x_s
-// (x_{q+1} top, x_{q-1} bottom),
+// Stage 1 with each partition swept from both ends, by the two wavefronts of a workgroup (the top
+// and the bottom half of partition blockIdx.x / ntiles of the 64 systems of tile
+// blockIdx.x mod ntiles, lane l on system 64 * tile + l), blocks in memory, partitions of at
+// least 4 rows: each lane's chain of dependent rows is half the partition's, for batches with
+// too few partitions to fill the device. The direction is uniform over a wavefront, so the
+// arrays it reads are too. The top wavefront runs k_modi's forward sweep (rows loaded ahead)
+// over rows 0..k, k = ceil(N/2) - 1; the bottom one runs the same sweep mirrored, over rows N-1
+// down to k+1, with A and C exchanging roles. A lane's rows then satisfy, with its anchor x_a
+// (x_0' top, x_{N-1} bottom) and the next row of its sweep x_s (x_{q+1} top, x_{q-1} bottom),
 //   x_q + Â_q x_a + Ĉ_q x_s = d̂_q   (in place: d̂ over D, Â over the array of A's role, Ĉ over
 //                                    the array of C's role: A and C top, C and A bottom),
 // and row 1 (top) / row N-2 (bottom) is composed on the way, as in k_modi, as
-// x = alpha - beta x_a - gamma x_s(last row). The lanes' last rows p (k and k+1) meet: with
-// the partner's last row x_n + H x_f + K x_p = h (x_f the partner's anchor), each lane solves
+// x = alpha - beta x_a - gamma x_s(last row). The two last rows p (k and k+1) of a system meet:
+// with the partner's last row x_n + H x_f + K x_p = h (x_f the partner's anchor), each lane solves
 //   x_p = a - b x_a - c x_f,  S = I - Ĉ_p K,  a = S^{-1}(d̂_p - Ĉ_p h),  b = S^{-1} Â_p,
 //   c = -S^{-1} Ĉ_p H,
 // and stores it as row p (its Ĉ now the coefficient of x_f); the partner's solution
@@ -943,22 +944,23 @@ __global__ __launch_bounds__(64, AHEAD ? 1 : 2) void k_modi(double* A, const dou
 // The close of row 0 (top) / row N-1 (bottom) is k_modi's; the top lane writes reduced row
 // 2*part, the bottom lane row 2*part+1 with its Â (the coefficient of x_N) as the reduced C and
 // its Ĉ (of x_0') as the reduced A: the reduced system of k_modi, to rounding. Stage 3 is
-// k_update_twisted. The composition and the two exchanges of the meeting share the LDS slots
-// ([element][lane], sp).
-template <int M, int LAYOUT>
-__global__ __launch_bounds__(64, 1) void k_modi_twisted(double* A, const double* __restrict__ B, double* C,
-                                                        double* D, int nsys, int nrow,
-                                                        const int* __restrict__ part_first,
-                                                        const int* __restrict__ part_len, int npart,
-                                                        double* __restrict__ redA, double* __restrict__ redC,
-                                                        double* __restrict__ redD, int* __restrict__ flag) {
-    constexpr int H = kTile / 2;
-    const int nh = (nsys + H - 1) / H;
-    const int half = blockIdx.x % nh, part = blockIdx.x / nh;
-    const int lane = threadIdx.x;
-    const bool top = lane < H;
-    const int64_t sys = (int64_t)half * H + (lane % H);
-    if (sys >= nsys) return;  // the two lanes of a system leave together
+// k_update_twisted. Each wavefront's composition and its side of the meeting's two exchanges
+// share its LDS slots ([wavefront][element][lane], sp). AHEAD: each row's blocks are loaded
+// while the previous one is eliminated (one wavefront per SIMD); without, at the start of their
+// own row step.
+template <int M, int LAYOUT, bool AHEAD = true>
+__global__ __launch_bounds__(2 * kTile, AHEAD ? 1 : 2) void k_modi_twisted(
+    double* A, const double* __restrict__ B, double* C, double* D, int nsys, int nrow,
+    const int* __restrict__ part_first, const int* __restrict__ part_len, int npart, double* __restrict__ redA,
+    double* __restrict__ redC, double* __restrict__ redD, int* __restrict__ flag) {
+    constexpr int E2 = 2 * M * M + M;  // LDS slots of one wavefront
+    const int ntiles = (nsys + kTile - 1) / kTile;
+    const int tile = blockIdx.x % ntiles, part = blockIdx.x / ntiles;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);  // 0 top, 1 bottom
+    const int lane = threadIdx.x % kTile;
+    const bool top = wv == 0;
+    const int64_t sys = (int64_t)tile * kTile + lane;
+    if (sys >= nsys) return;  // a system's two lanes leave together: both wavefronts map lanes alike
     const int64_t r0 = part_first[part];
     const int N = part_len[part];  // >= 4
     const int k = (N + 1) / 2 - 1;
@@ -967,38 +969,49 @@ __global__ __launch_bounds__(64, 1) void k_modi_twisted(double* A, const double*
     double* Ar = top ? A : C;  // couples row q to the previous row of the sweep
     double* Cr = top ? C : A;  // to the next one
     const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
-    extern __shared__ double sp[];
-    const int pl = lane ^ H;  // the partner lane
-    auto SA = [&](int i) -> double& { return sp[i * 64 + lane]; };
-    auto SB = [&](int i, int j) -> double& { return sp[(M + i * M + j) * 64 + lane]; };
-    auto SG = [&](int i, int j) -> double& { return sp[(M + M * M + i * M + j) * 64 + lane]; };
-    auto PA = [&](int i) { return sp[i * 64 + pl]; };
-    auto PB = [&](int i, int j) { return sp[(M + i * M + j) * 64 + pl]; };
-    auto PG = [&](int i, int j) { return sp[(M + M * M + i * M + j) * 64 + pl]; };
+    extern __shared__ double sp_all[];
+    double* sp = sp_all + wv * E2 * kTile;         // this wavefront's slots
+    double* pp = sp_all + (1 - wv) * E2 * kTile;   // the partner's
+    auto SA = [&](int i) -> double& { return sp[i * kTile + lane]; };
+    auto SB = [&](int i, int j) -> double& { return sp[(M + i * M + j) * kTile + lane]; };
+    auto SG = [&](int i, int j) -> double& { return sp[(M + M * M + i * M + j) * kTile + lane]; };
+    auto PA = [&](int i) { return pp[i * kTile + lane]; };
+    auto PB = [&](int i, int j) { return pp[(M + i * M + j) * kTile + lane]; };
+    auto PG = [&](int i, int j) { return pp[(M + M * M + i * M + j) * kTile + lane]; };
     bool ok = true;
     double Ah[M][M], Ch[M][M], dh[M];
-    double nA[M][M], nB[M][M], nC[M][M], nd[M];
+    constexpr int MA = AHEAD ? M : 1;
+    double nA[MA][MA], nB[MA][MA], nC[MA][MA], nd[MA];
     auto fetch_next = [&](int64_t g) {
-        load_mat_c<M>(Ar, am, g, nA);
-        load_mat_c<M>(B, am, g, nB);
-        load_mat_c<M>(Cr, am, g, nC);
-        load_vec<M>(D, av, g, nd);
+        if constexpr (AHEAD) {
+            load_mat_c<M>(Ar, am, g, nA);
+            load_mat_c<M>(B, am, g, nB);
+            load_mat_c<M>(Cr, am, g, nC);
+            load_vec<M>(D, av, g, nd);
+        }
     };
     fetch_next(g0);
     for (int q = 0; q < L; ++q) {
         const int64_t g = g0 + q * dg;
         double R[M][M], v[M], X[M][M], cA[M][M], cC[M][M];
+        if constexpr (AHEAD) {
 #pragma unroll
-        for (int i = 0; i < M; ++i) {
-            v[i] = nd[i];
+            for (int i = 0; i < M; ++i) {
+                v[i] = nd[i];
 #pragma unroll
-            for (int j = 0; j < M; ++j) {
-                R[i][j] = nB[i][j];
-                cA[i][j] = nA[i][j];
-                cC[i][j] = nC[i][j];
+                for (int j = 0; j < M; ++j) {
+                    R[i][j] = nB[i][j];
+                    cA[i][j] = nA[i][j];
+                    cC[i][j] = nC[i][j];
+                }
             }
+            fetch_next(q + 1 < L ? g + dg : g);
+        } else {
+            load_mat_c<M>(Ar, am, g, cA);
+            load_mat<M>(B, am, g, R);
+            load_mat_c<M>(Cr, am, g, cC);
+            load_vec<M>(D, av, g, v);
         }
-        fetch_next(q + 1 < L ? g + dg : g);
         if (q >= 2) {
 #pragma unroll
             for (int i = 0; i < M; ++i)
@@ -1095,7 +1108,7 @@ __global__ __launch_bounds__(64, 1) void k_modi_twisted(double* A, const double*
             SG(i, j) = Ch[i][j];
         }
     }
-    wave_lds_sync();
+    __syncthreads();
     double S[M][M], ma[M], mc[M][M];
 #pragma unroll
     for (int i = 0; i < M; ++i) {
@@ -1115,7 +1128,7 @@ __global__ __launch_bounds__(64, 1) void k_modi_twisted(double* A, const double*
             mc[i][j] = -h;
         }
     }
-    wave_lds_sync();  // the partner has read this lane's row before it is overwritten
+    __syncthreads();  // the partner has read this lane's row before it is overwritten
     {
         int piv[M];
         ok = lu_factor<M>(S, piv) && ok;
@@ -1137,7 +1150,7 @@ __global__ __launch_bounds__(64, 1) void k_modi_twisted(double* A, const double*
             SG(i, j) = mc[i][j];
         }
     }
-    wave_lds_sync();
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < M; ++i) {
         double gi[M];
@@ -1496,12 +1509,28 @@ constexpr int64_t kModiLaneFrom = BTD_MODI_LANE_FROM;
 #define BTD_MODI_COOP_BELOW 16384
 #endif
 constexpr int64_t kModiCoopBelow = BTD_MODI_COOP_BELOW;
-// Below this many partitions in the batch, non-periodic solves whose partitions have at least
-// 4 rows run both stages twisted (k_modi_twisted, k_update_twisted) instead.
+// From kModiTwistedFrom partitions in the batch (and below kModiTwistedBelow in the tiled layout),
+// non-periodic solves whose partitions have at least 4 rows run both stages twisted
+// (k_modi_twisted, k_update_twisted) instead; below kModiTwistedFrom the m lanes per partition of
+// k_modi_coop give more lanes than the twisted form's two. Measured (m = 5, whole solve, twisted
+// against the other forms): reference layout 16384 x 2048 P = 4 10.2 vs 20.2 ms, 512 x 65536
+// P = 128 11.3 vs 20.1, 32768 x 512 P = 2 5.4 vs 9.7 (one lane per partition in that layout reads
+// half as fast); tiled, from 65536 partitions, within -8% / +9% of one lane per partition
+// (16384 x 2048 P = 4 10.4 vs 9.6, 32768 x 512 P = 2 4.8 vs 5.3), so one lane stays there; at
+// 4096 partitions (512 x 65536 P = 8) 31.8 vs 30.2 ms cooperative, at 8192 (P = 16) 17.0 vs 20.5.
 #ifndef BTD_MODI_TWISTED_BELOW
 #define BTD_MODI_TWISTED_BELOW 65536
 #endif
 constexpr int64_t kModiTwistedBelow = BTD_MODI_TWISTED_BELOW;
+#ifndef BTD_MODI_TWISTED_FROM
+#define BTD_MODI_TWISTED_FROM 8192
+#endif
+constexpr int64_t kModiTwistedFrom = BTD_MODI_TWISTED_FROM;
+#ifndef BTD_TWISTED_REF_AHEAD_MAX_M
+#define BTD_TWISTED_REF_AHEAD_MAX_M 5
+#endif
+template <int M>
+constexpr bool kTwistedRefAhead = M <= BTD_TWISTED_REF_AHEAD_MAX_M;
 
 // Largest m for which the few-partition forms run (two lanes per partition from both ends, m
 // lanes per partition in stage 1 and 3, the next row loaded ahead in stage 1): they were
@@ -1952,22 +1981,21 @@ __global__ __launch_bounds__(64) void k_update(const double* __restrict__ A,
     }
 }
 
-// Stage 3 after k_modi_twisted, the same two lanes per system: x_0' and x_{N-1} come from the
-// reduced solution; each lane walks its half back toward its anchor x_a,
+// Stage 3 after k_modi_twisted, the same two wavefronts per tile and partition: x_0' and x_{N-1}
+// come from the reduced solution; each lane walks its half back toward its anchor x_a,
 //   x_q = d̂_q - Â_q x_a - Ĉ_q x_s,   q = p .. 1 of its sweep,
 // x_s starting at the partner's anchor (row p's Ĉ is its coefficient) and then the row just
 // computed; the next row's blocks are loaded while the current one is formed.
 template <int M, int LAYOUT>
-__global__ __launch_bounds__(64) void k_update_twisted(const double* __restrict__ A,
-                                                       const double* __restrict__ C, double* D, int nsys,
-                                                       int nrow, const int* __restrict__ part_first,
-                                                       const int* __restrict__ part_len, int npart,
-                                                       const double* __restrict__ redD) {
-    constexpr int H = kTile / 2;
-    const int nh = (nsys + H - 1) / H;
-    const int half = blockIdx.x % nh, part = blockIdx.x / nh;
-    const bool top = threadIdx.x < H;
-    const int64_t sys = (int64_t)half * H + (threadIdx.x % H);
+__global__ __launch_bounds__(2 * kTile) void k_update_twisted(const double* __restrict__ A,
+                                                              const double* __restrict__ C, double* D, int nsys,
+                                                              int nrow, const int* __restrict__ part_first,
+                                                              const int* __restrict__ part_len, int npart,
+                                                              const double* __restrict__ redD) {
+    const int ntiles = (nsys + kTile - 1) / kTile;
+    const int tile = blockIdx.x % ntiles, part = blockIdx.x / ntiles;
+    const bool top = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile) == 0;
+    const int64_t sys = (int64_t)tile * kTile + threadIdx.x % kTile;
     if (sys >= nsys) return;
     const int64_t r0 = part_first[part];
     const int N = part_len[part];  // >= 4
@@ -2327,11 +2355,12 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     // its own block source) and the assembled periodic solve run the same arithmetic.
     // Below kModiCoopFrom, by the partitions in the batch (nsys * npart; m = 5, 512 x 65536,
     // stage 1 at P = 8 / 16 / 32 / 64): from kModiLaneFrom one lane per partition (k_modi); below
-    // kModiTwistedBelow, non-periodic with partitions of at least 4 rows, two lanes per partition
-    // from both ends (k_modi_twisted: 21.1 / 10.8 / 7.0-7.4 / 6.2-7.4 ms, then k_update_twisted);
-    // otherwise from kModiCoopBelow one lane per partition with the next row loaded during the
-    // current one (47.6 / 24.4 / 12.7 / 7.5 ms) and below that m lanes per partition (k_modi_coop:
-    // 25.0 / 15.6 / 16.9 / 13.0 ms), non-periodic solves only. Loading ahead changes no
+    // kModiTwistedBelow (tiled layout; any count in the reference layout) down to kModiTwistedFrom,
+    // non-periodic with partitions of at least 4 rows, two lanes per partition from both ends
+    // (k_modi_twisted: 21.1 / 10.8 / 7.0-7.4 / 6.2-7.4 ms, then k_update_twisted); otherwise from
+    // kModiCoopBelow one lane per partition with the next row loaded during the current one (47.6 /
+    // 24.4 / 12.7 / 7.5 ms) and below that m lanes per partition (k_modi_coop: 25.0 / 15.6 / 16.9 /
+    // 13.0 ms), non-periodic solves only. Loading ahead changes no
     // arithmetic, so periodic solves with few partitions use it too and still equal the
     // matrix-free sweeps bit for bit. These few-partition forms are enabled for m <= kFewFormsMaxM
     // only (see kFewFormsMaxM).
@@ -2339,19 +2368,21 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     bool coop = M >= kModiCoopFrom, ahead = false, twisted12 = false;
     if constexpr (kCanCoop && M <= kFewFormsMaxM) {
         const int64_t lanes = (int64_t)nsys * npart;
-        twisted12 = !cyclic && lanes < kModiTwistedBelow && nrow >= 4 * npart;
+        twisted12 = !cyclic && lanes >= kModiTwistedFrom && (layout != LAYOUT_TILED || lanes < kModiTwistedBelow) &&
+                    nrow >= 4 * npart;
         coop = !twisted12 && !cyclic && lanes < kModiCoopBelow;
         ahead = !twisted12 && !coop && lanes < kModiLaneFrom;
     }
     if (twisted12) {
         if constexpr (kCanCoop && M <= kFewFormsMaxM) {
-            const dim3 gridt((unsigned)(((nsys + kTile / 2 - 1) / (kTile / 2)) * npart));
+            const dim3 block2(2 * kTile);
+            const size_t lds2 = 2 * lds;  // each wavefront's slots
             if (layout == LAYOUT_TILED)
-                hipLaunchKernelGGL((k_modi_twisted<M, LAYOUT_TILED>), gridt, block, lds, s, A, B, C, D, nsys, nrow,
-                                   pf, pl, npart, redA, redC, redD, flag);
-            else
-                hipLaunchKernelGGL((k_modi_twisted<M, LAYOUT_REF_GPU>), gridt, block, lds, s, A, B, C, D, nsys,
+                hipLaunchKernelGGL((k_modi_twisted<M, LAYOUT_TILED>), gridp, block2, lds2, s, A, B, C, D, nsys,
                                    nrow, pf, pl, npart, redA, redC, redD, flag);
+            else
+                hipLaunchKernelGGL((k_modi_twisted<M, LAYOUT_REF_GPU, kTwistedRefAhead<M>>), gridp, block2, lds2,
+                                   s, A, B, C, D, nsys, nrow, pf, pl, npart, redA, redC, redD, flag);
         }
     } else if (coop) {
         if constexpr (kCanCoop) {
@@ -2420,12 +2451,12 @@ hipError_t launch_two_stage_m(int layout, double* A, const double* B, double* C,
     if constexpr (M >= 2 && M <= kFewFormsMaxM) coop3 = !cyclic && (int64_t)nsys * npart < kModiLaneFrom;
     if (twisted12) {
         if constexpr (kCanCoop && M <= kFewFormsMaxM) {
-            const dim3 gridt((unsigned)(((nsys + kTile / 2 - 1) / (kTile / 2)) * npart));
+            const dim3 block2(2 * kTile);
             if (layout == LAYOUT_TILED)
-                hipLaunchKernelGGL((k_update_twisted<M, LAYOUT_TILED>), gridt, block, 0, s, A, C, D, nsys, nrow,
+                hipLaunchKernelGGL((k_update_twisted<M, LAYOUT_TILED>), gridp, block2, 0, s, A, C, D, nsys, nrow,
                                    pf, pl, npart, redD);
             else
-                hipLaunchKernelGGL((k_update_twisted<M, LAYOUT_REF_GPU>), gridt, block, 0, s, A, C, D, nsys, nrow,
+                hipLaunchKernelGGL((k_update_twisted<M, LAYOUT_REF_GPU>), gridp, block2, 0, s, A, C, D, nsys, nrow,
                                    pf, pl, npart, redD);
         }
     } else if (coop3) {

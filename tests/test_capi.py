@@ -72,22 +72,28 @@ def test_plan_create_rejects_bad_arguments(args, exc):
 
 
 def test_auto_npart_rule():
-    """npart = 0: the single-kernel solve for large batches (4096 systems suffice for m >= 7),
-    about one wave per SIMD (65536 lanes) for small ones, at least 8 rows per partition."""
+    """npart = 0: the single-kernel solve for large batches (8192 systems suffice for m = 2, 4,
+    4096 for m >= 6); for smaller m <= 5 batches about 16384 partitions of >= 16 rows (a power of
+    two; at least 2 from 8192 systems; none below 128 rows under 8192 systems); for m = 6 the
+    65536-lane count as a power of two (none up to 128 rows); for m >= 7 about 65536 lanes of
+    >= 8 rows."""
     auto = lambda m, nsys, nrow: pb._capi.lib().btd_auto_npart(m, nsys, nrow)
     assert auto(5, 131072, 512) == 1          # the headline batch
     assert auto(5, 32768, 512) == 1
-    assert auto(5, 1024, 128) == 16           # the Study's single-GPU case: nrow / 8
-    assert auto(5, 512, 65536) == 128         # long systems: 65536 lanes
+    assert auto(5, 1024, 128) == 8            # the Study's single-GPU case: nrow / 16
+    assert auto(5, 512, 65536) == 32          # long systems: 16384 partitions
+    assert auto(5, 700, 200) == 8             # min(24, 12) -> power of two
     assert auto(8, 4096, 2048) == 1           # m >= 7: m or 2m lanes per system fill the device
     assert auto(8, 2048, 2048) == 32
-    assert auto(7, 4096, 2048) == 1 and auto(7, 2048, 2048) == 32 and auto(6, 4096, 2048) == 16
-    assert auto(2, 8192, 512) == 1 and auto(4, 8192, 512) == 1 and auto(3, 8192, 512) == 8
-    assert auto(4, 4096, 2048) == 16
-    assert auto(5, 1024, 64) == 1 and auto(8, 256, 127) == 1 and auto(4, 1024, 64) == 8  # short systems
-    assert auto(3, 1024, 64) == 8
-    assert auto(3, 1024, 15) == 1             # fewer than 2 partitions of 8 rows
-    for m, nsys, nrow in [(2, 7, 40), (10, 3000, 4096), (5, 1, 2)]:
+    assert auto(7, 4096, 2048) == 1 and auto(7, 2048, 2048) == 32 and auto(7, 700, 200) == 25
+    assert auto(6, 4096, 2048) == 1 and auto(6, 2048, 2048) == 32 and auto(6, 700, 300) == 32
+    assert auto(6, 1024, 128) == 1
+    assert auto(2, 8192, 512) == 1 and auto(4, 8192, 512) == 1
+    assert auto(3, 8192, 512) == 2 and auto(3, 16384, 64) == 2 and auto(5, 16384, 128) == 2
+    assert auto(4, 4096, 2048) == 4 and auto(2, 256, 512) == 32 and auto(3, 256, 2048) == 64
+    assert auto(5, 1024, 64) == 1 and auto(8, 256, 127) == 1 and auto(4, 1024, 64) == 1  # short systems
+    assert auto(3, 1024, 15) == 1
+    for m, nsys, nrow in [(2, 7, 40), (10, 3000, 4096), (5, 1, 2), (3, 9000, 20), (6, 100, 129)]:
         p = auto(m, nsys, nrow)
         assert p == 1 or (p >= 2 and nrow >= 8 * p)
 

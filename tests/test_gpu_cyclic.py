@@ -84,8 +84,8 @@ def test_cyclic_single_partition_row_counts(cuda, oracle, m, n):
 
 @pytest.mark.parametrize("m,nsys,n", [(3, 3000, 64), (8, 4100, 40), (3, 200, 33)])
 def test_cyclic_single_partition_partition_counts(cuda, oracle, m, nsys, n):
-    """npart = 1 periodic solves whose library-chosen partition count is 8 (btd_auto_npart), 4
-    (a batch the single-partition solve fills the device with) and 4 (33 rows)."""
+    """npart = 1 periodic solves whose library-chosen partition count is 8 (64 rows / 8), 4 (a
+    batch the single-partition solve fills the device with) and 4 (33 rows)."""
     rng = np.random.default_rng(m + n)
     A, B, C, d = periodic_system(rng, n, nsys, m)
     x = gpu_cyclic(cuda, A, B, C, d, pb.Layout.REF_GPU, 1)

@@ -138,7 +138,8 @@ def test_study_signatures_on_device(cuda, row):
         assert abs(x[pb.vector_index(0, z, 0, nsys, n)] - float(row[key])) < 1e-14
 
 
-@pytest.mark.parametrize("m,nsys,n,expect", [(3, 1024, 64, 8), (5, 700, 200, 25), (2, 40000, 6, 1), (5, 700, 64, 1)])
+@pytest.mark.parametrize("m,nsys,n,expect", [(3, 1024, 256, 16), (5, 700, 200, 8), (2, 40000, 6, 1), (5, 700, 64, 1),
+                                              (6, 700, 300, 32), (7, 700, 200, 25), (3, 9000, 200, 2)])
 def test_auto_npart_plan(cuda, oracle, m, nsys, n, expect):
     """A plan created with npart = 0 takes btd_auto_npart's count and still matches the oracle."""
     import torch
@@ -378,8 +379,9 @@ def test_lapack_order_form_rules(cuda):
 @pytest.mark.parametrize("nsys", [16387, 8195])
 def test_two_stage_one_lane_per_partition(cuda, oracle, m, layout, nsys):
     """From 65536 partitions in the batch (nsys * npart) stage 1 and stage 3 run one lane per
-    partition (k_modi, k_update); below that, for m <= 5, two lanes per partition from both
-    ends (k_modi_twisted, k_update_twisted; partitions of 3 rows keep the forms of
+    partition (k_modi, k_update) in the tiled layout; below that, and at every count from 8192 in
+    the reference layout, for m <= 5, two lanes per partition from both ends (k_modi_twisted,
+    k_update_twisted; partitions of 3 rows keep the forms of
     test_two_stage_three_row_partitions); from m = 6 one lane per partition at every count.
     16387 or 8195 systems x 4 partitions of n = 24 rows; sampled systems (first tile, a middle
     one, the partial last) vs the oracle's two-stage solve."""
@@ -428,13 +430,13 @@ def test_config0_literal_generator_full_batch(cuda, oracle, layout):
 
 @pytest.mark.parametrize("m", [2, 3, 4, 5])
 @pytest.mark.parametrize("layout", [pb.Layout.TILED, pb.Layout.REF_GPU])
-@pytest.mark.parametrize("nsys,n,npart", [(100, 16, 4), (33, 21, 5), (31, 200, 3), (1000, 40, 8)])
+@pytest.mark.parametrize("nsys,n,npart", [(2050, 16, 4), (1700, 21, 5), (2731, 200, 3), (1030, 40, 8)])
 def test_two_stage_twisted(cuda, oracle, m, layout, nsys, n, npart):
-    """Below 65536 partitions in the batch, non-periodic solves whose partitions have at least 4
-    rows run both stages twisted (k_modi_twisted, k_update_twisted: two lanes per partition,
-    from both ends, meeting in the middle). Partitions of exactly 4 rows, of 4 and 5, of 66-67
-    and of 5; system counts that leave a half-filled or partial last wavefront (32 systems per
-    wavefront). Every system vs the oracle's two-stage solve."""
+    """From 8192 partitions in the batch (below 65536 in the tiled layout), non-periodic solves
+    whose partitions have at least 4 rows run both stages twisted (k_modi_twisted,
+    k_update_twisted: the two wavefronts of a workgroup sweep each partition from both ends and
+    meet in the middle). Partitions of exactly 4 rows, of 4 and 5, of 66-67 and of 5; system
+    counts that leave a partial last tile. Every system vs the oracle's two-stage solve."""
     A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 300 + m + n, m, nsys, n)
     x, sing = gpu_solve(A, B, C, d, int(layout), npart=npart)
     assert not sing
@@ -463,13 +465,13 @@ def test_per_system_info_twisted_stage1(cuda, m):
     lane's close), its last row (the bottom lane's), the two rows where the lanes meet, and an
     interior row of each half; named per system in both layouts."""
     import torch
-    npart, nsys = 3, 90
+    npart, nsys = 3, 2800  # 8400 partitions: the twisted stages
     n = 4 * npart + 5  # partitions of 6, 6, 5 rows
     firsts = [pb.partition_1d(0, n - 1, npart, r)[0] for r in range(npart)]
     lens = [pb.partition_1d(0, n - 1, npart, r)[1] - firsts[r] + 1 for r in range(npart)]
     k1 = (lens[1] + 1) // 2 - 1
     bad = [(1, firsts[0]), (17, firsts[1] + lens[1] - 1), (33, firsts[1] + k1), (34, firsts[1] + k1 + 1),
-           (64, firsts[2] + 1), (89, firsts[2] + lens[2] - 2)]
+           (64, firsts[2] + 1), (2799, firsts[2] + lens[2] - 2)]
     A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 13, m, nsys, n)
     for s_, q in bad:
         A[q, s_] = B[q, s_] = C[q, s_] = 0.0
