@@ -235,7 +235,9 @@ __device__ __forceinline__ bool coop_item(const BatchDev& b, const double* A, co
     // slots, which no group reads
     const StridedAddr<M * M * W, W> am{mo};
     const StridedAddr<M * W, W> av{vo};
-    return coop_rows<M>(A, B, C, D, am, av, i, grouped ? g * M : 0, n, nmax, b.pack_rows[pk], sh);
+    // the row form at every m: the batch kernel also holds the one-lane blocks, and the
+    // column form's registers on top of them spill (the batch ran 2x slower with it)
+    return coop_rows<M, false, false>(A, B, C, D, am, av, i, grouped ? g * M : 0, n, nmax, b.pack_rows[pk], sh);
 }
 
 // A twisted item: the systems of pack p, two groups of m lanes each.
@@ -259,7 +261,7 @@ __device__ __forceinline__ bool twisted_item(const BatchDev& b, const double* A,
     const int pbase = grouped ? (top ? base + M : base - M) : M;
     const StridedAddr<M * M * W, W> am{mo};
     const StridedAddr<M * W, W> av{vo};
-    return coop_twisted<M>(A, B, C, D, am, av, i, base, pbase, top, n, steps, nmax, sh);
+    return coop_twisted<M, false, false, false>(A, B, C, D, am, av, i, base, pbase, top, n, steps, nmax, sh);
 }
 
 // Persistent solve of a packed batch. KIND selects the code a placement needs, so that the
@@ -366,14 +368,23 @@ __global__ __launch_bounds__(64) void k_batch(BatchDev b, const double* __restri
 // Work-counter reset as a kernel (a memset node would not replay reliably in a graph).
 __global__ void k_batch_reset(int* counter) { counter[0] = 0; }
 
-// Generator for a packed batch: one thread per (pack, row, slot).
+// Generator for a packed batch: one thread per (pack, row, slot). Storage of a pack that holds
+// no system row (slots past the pack's systems, rows past a shorter system's end) is written
+// with zeros, so every element of a generated batch is defined.
 template <int M>
 __device__ __forceinline__ void gen_batch_elem(int kind, uint64_t seed, int64_t sys, int row, int n,
                                                int64_t mo, int64_t vo, double* A, double* B,
-                                               double* C, double* D) {
+                                               double* C, double* D, bool real) {
     constexpr int64_t es = batch_width(M);
     double a[M][M], bb[M][M], c[M][M], d[M];
-    gen_row<M>(kind, seed, sys, row, n, a, bb, c, d);
+    if (real) {
+        gen_row<M>(kind, seed, sys, row, n, a, bb, c, d);
+    } else {
+        for (int j = 0; j < M; ++j) {
+            d[j] = 0.0;
+            for (int i = 0; i < M; ++i) a[i][j] = bb[i][j] = c[i][j] = 0.0;
+        }
+    }
     const int64_t rm = mo + (int64_t)row * M * M * es, rv = vo + (int64_t)row * M * es;
     for (int j = 0; j < M; ++j)
         for (int i = 0; i < M; ++i) {
@@ -389,22 +400,23 @@ __global__ __launch_bounds__(256) void k_batch_generate(BatchDev b, const int* _
                                                         double* C, double* D) {
     const int row = blockIdx.x * 4 + threadIdx.x / kTile, slot = threadIdx.x % kTile;
     for (int p = blockIdx.y; p < b.npacks; p += gridDim.y) {
-        if (slot >= b.pack_count[p]) continue;
-        const int n = b.slot_rows[b.pack_slot0[p] + slot];
-        if (row >= n) continue;
-        const int64_t sys = slot_sys[b.pack_slot0[p] + slot];
+        if (slot >= batch_width(b.pack_m[p]) || row >= b.pack_rows[p]) continue;  // not storage of pack p
+        const bool used = slot < b.pack_count[p];
+        const int n = used ? b.slot_rows[b.pack_slot0[p] + slot] : 0;
+        const bool real = used && row < n;
+        const int64_t sys = used ? slot_sys[b.pack_slot0[p] + slot] : 0;
         const int64_t mo = b.mat_off[p] + slot, vo = b.vec_off[p] + slot;
         switch (b.pack_m[p]) {
-            case 1: gen_batch_elem<1>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-            case 2: gen_batch_elem<2>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-            case 3: gen_batch_elem<3>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-            case 4: gen_batch_elem<4>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-            case 5: gen_batch_elem<5>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-            case 6: gen_batch_elem<6>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-            case 7: gen_batch_elem<7>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-            case 8: gen_batch_elem<8>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-            case 9: gen_batch_elem<9>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
-            case 10: gen_batch_elem<10>(kind, seed, sys, row, n, mo, vo, A, B, C, D); break;
+            case 1: gen_batch_elem<1>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
+            case 2: gen_batch_elem<2>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
+            case 3: gen_batch_elem<3>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
+            case 4: gen_batch_elem<4>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
+            case 5: gen_batch_elem<5>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
+            case 6: gen_batch_elem<6>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
+            case 7: gen_batch_elem<7>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
+            case 8: gen_batch_elem<8>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
+            case 9: gen_batch_elem<9>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
+            case 10: gen_batch_elem<10>(kind, seed, sys, row, n, mo, vo, A, B, C, D, real); break;
             default: break;
         }
     }

@@ -1,5 +1,6 @@
 // Cooperative block-Thomas rows: a system's block rows spread over m lanes of a wavefront
-// (lane i owns row i of every block), pivot search and row exchange through LDS. Used by the
+// (lane i owns row i of every block; from kColFromM, lane j owns column j), pivot search and
+// row exchange through LDS. Used by the
 // mixed-size batch (btd_batch.h), the single-partition cooperative and twisted solves, stage 1
 // of the two-stage solve for m >= 7 and the twisted solve of its reduced system (btd_kernels.h). Storage: element e (column-major within the block) of row q of a lane's
 // system is at am.at(q, e) for blocks and av.at(q, e) for vectors; StridedAddr gives the
@@ -266,6 +267,335 @@ __device__ __forceinline__ double lane_elem(const double (&v)[M], int i) {
     return __longlong_as_double((long long)bits);
 }
 
+// ---------------------------------------------------------------------------------------
+// Column form of the cooperative row step (m >= kColFromM). Lane j of a group owns column j
+// of R_q, of the block R_q is solved against (C_q; A_q in the bottom half of a twisted solve)
+// and of the result C'_q; every lane carries the whole right-hand side. The carried column of
+// C'_{q-1} then never leaves its lane: the only block exchanged is A_q (each lane posts its
+// column, every lane reads all of it for R_q(:, j) = B_q(:, j) - A_q C'_{q-1}(:, j) and
+// d_q - A_q d'_{q-1}), and the pivot search is local to the lane that owns the column. A row
+// takes m + 2 LDS round trips (the row form: 2m + 2) and about half the LDS operations. Every
+// value is computed by the same operations on the same operands as in the row form (the same
+// multipliers r(t, k) * fast_rcp(pivot), the same fused updates and back substitution), so the
+// results are the row form's bit for bit, except where two candidates tie exactly for a pivot:
+// the row form then takes the lowest original row, this form the first row in dgetf2's swapped
+// order. Measured against the row form (which waits on its exchanges most of its cycles at
+// m >= 7: m = 8, 64% of wave cycles waiting, VALU busy 10%): the Study's m = 8 16384 x 2048
+// case 34.2 -> 31.0 ms, m = 8 tiled 16384 x 1024 12.9 -> 12.0 ms, twisted m = 8 1024 x 2048
+// 5.6 -> 5.0 ms and 4096 x 4096 25.4 -> 23.8 ms, m = 7 16384 x 1024 13.0 -> 11.4 ms, m = 10
+// 16384 x 512 16.6 -> 15.9 ms, m = 9 8192 x 512 unchanged (12.2 / 12.4 ms). The mixed-size batch
+// keeps the row form: its kernel also holds the one-lane blocks, and with the column form on
+// top it spilled 1.3 KB per lane and ran 2x slower.
+#ifndef BTD_COOP_COL_FROM_M
+#define BTD_COOP_COL_FROM_M 7
+#endif
+constexpr int kColFromM = BTD_COOP_COL_FROM_M;
+
+// Rows of the column form's forward sweep loaded ahead.
+template <int M>
+struct ColDepth {
+#ifdef BTD_COL_FWD_AHEAD
+    static constexpr int kFwdAhead = BTD_COL_FWD_AHEAD;
+#else
+    static constexpr int kFwdAhead = 1;
+#endif
+};
+
+// One block row in the column form. r: column j of R (destroyed); c: column j of the block
+// solved against, replaced by column j of R^{-1} C (WITH_C); dv: the whole right-hand side,
+// replaced by R^{-1} d. Pivot step k: lane k of the group (the owner of column k) picks the
+// first largest |r| among rows k..M-1 (dgetf2's choice) and posts the pivot row index, the
+// pivot's reciprocal (hardware reciprocal and two Newton steps) and the multipliers of the
+// rows below it after the exchange of rows k and p; every lane exchanges rows k and p of its
+// columns and eliminates. Back substitution: every lane posts its column of U and solves its
+// own column and the whole d. xb: the group's exchange, [slot][lane], M + 2 slots. Returns 1
+// if this lane's column met a zero pivot (lane k of a group sees the pivot of step k).
+template <int M, bool WITH_C>
+__device__ __forceinline__ int coop_eliminate_col(double (&r)[M], double (&c)[M], double (&dv)[M],
+                                                  double* __restrict__ xb, int j, int base) {
+    const int lane = threadIdx.x & (kTile - 1);  // the lane in its wavefront
+    int zero_pivot = 0;
+    double rk[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        double best = fabs(r[k]), pv = r[k];
+        int p = k;
+#pragma unroll
+        for (int t = k + 1; t < M; ++t) {
+            const double v = fabs(r[t]);
+            const bool b = v > best;
+            best = b ? v : best;
+            pv = b ? r[t] : pv;
+            p = b ? t : p;
+        }
+        zero_pivot |= (j == k && pv == 0.0) ? 1 : 0;
+        const double rp = fast_rcp(pv);
+#pragma unroll
+        for (int t = k + 1; t < M; ++t) xb[t * kTile + lane] = (p == t ? r[k] : r[t]) * rp;
+        xb[M * kTile + lane] = (double)p;
+        xb[(M + 1) * kTile + lane] = rp;
+        wave_lds_sync();
+        const int src = base + k;
+        const int pk = (int)xb[M * kTile + src];
+        rk[k] = xb[(M + 1) * kTile + src];
+        double l[M];
+#pragma unroll
+        for (int t = k + 1; t < M; ++t) l[t] = xb[t * kTile + src];
+#pragma unroll
+        for (int t = k + 1; t < M; ++t) {
+            const bool s = pk == t;
+            const double u = r[k], w = dv[k];
+            r[k] = s ? r[t] : u;
+            r[t] = s ? u : r[t];
+            dv[k] = s ? dv[t] : w;
+            dv[t] = s ? w : dv[t];
+            if constexpr (WITH_C) {
+                const double z = c[k];
+                c[k] = s ? c[t] : z;
+                c[t] = s ? z : c[t];
+            }
+        }
+#pragma unroll
+        for (int t = k + 1; t < M; ++t) {
+            r[t] = fma(-l[t], r[k], r[t]);
+            dv[t] = fma(-l[t], dv[k], dv[t]);
+            if constexpr (WITH_C) c[t] = fma(-l[t], c[k], c[t]);
+        }
+    }
+    // U(k, j') = r[k] of lane j' (j' > k)
+#pragma unroll
+    for (int t = 0; t < M - 1; ++t) xb[t * kTile + lane] = r[t];
+    wave_lds_sync();
+#pragma unroll
+    for (int k = M - 1; k >= 0; --k) {  // in place: c[t], dv[t] (t > k) already solved
+        double sc = c[k], sd = dv[k];
+#pragma unroll
+        for (int t = k + 1; t < M; ++t) {
+            const double u = xb[k * kTile + base + t];
+            if constexpr (WITH_C) sc = fma(-u, c[t], sc);
+            sd = fma(-u, dv[t], sd);
+        }
+        if constexpr (WITH_C) c[k] = sc * rk[k];
+        dv[k] = sd * rk[k];
+    }
+    return zero_pivot;
+}
+
+// Forward sweep of coop_rows in the column form (lane j owns column j); the arguments are
+// coop_rows'. d'_q(j) goes over d_q and C'_q(:, j) over C_q(:, j) as in the row form, so the
+// backward sweep is shared. xlast: this lane's element of its system's last d'.
+template <int M, bool LOCK, class AM, class AV>
+__device__ __forceinline__ bool coop_fwd_col(const double* __restrict__ A, const double* __restrict__ B,
+                                             double* C, double* D, const AM& am, const AV& av, int j,
+                                             int base, int n, int nmax, int rows, double* __restrict__ sh,
+                                             double& xlast) {
+    constexpr int F = ColDepth<M>::kFwdAhead;
+    const int lane = threadIdx.x & (kTile - 1);
+    bool ok = true;
+    double pa[F][M], pb[F][M], pc[F][M], pd[F];
+    auto fetch = [&](double (&fa)[M], double (&fb)[M], double (&fc)[M], double& fd, int q) {
+        q = q < rows ? q : rows - 1;
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            fa[t] = __builtin_nontemporal_load(A + am.at(q, t + M * j));
+            fb[t] = __builtin_nontemporal_load(B + am.at(q, t + M * j));
+            fc[t] = C[am.at(q, t + M * j)];
+        }
+        fd = D[av.at(q, j)];
+    };
+#pragma unroll
+    for (int s = 0; s < F; ++s) fetch(pa[s], pb[s], pc[s], pd[s], s);
+    double cc[M], dp[M];  // column j of C'_{q-1}; d'_{q-1}, whole
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+        cc[t] = 0.0;
+        dp[t] = 0.0;
+    }
+    for (int q0 = 0; q0 < nmax; q0 += F) {
+#pragma unroll
+        for (int s = 0; s < F; ++s) {
+            const int q = q0 + s;
+            if (q < nmax) {  // uniform: the last chunk may be partial
+                const bool act = q < n;
+                const bool has_next = q + 1 < n;
+#pragma unroll
+                for (int t = 0; t < M; ++t) sh[t * kTile + lane] = pa[s][t];  // A_q(t, j)
+                sh[M * kTile + lane] = pd[s];                                   // d_q(j)
+                wave_lds_sync();
+                double r[M], c[M], dv[M];
+#pragma unroll
+                for (int t = 0; t < M; ++t) {
+                    r[t] = pb[s][t];
+                    c[t] = pc[s][t];
+                    dv[t] = sh[M * kTile + base + t];
+                }
+                if (q > 0) {
+#pragma unroll
+                    for (int k = 0; k < M; ++k)
+#pragma unroll
+                        for (int t = 0; t < M; ++t) {
+                            const double a = sh[t * kTile + base + k];
+                            r[t] = fma(-a, cc[k], r[t]);
+                            dv[t] = fma(-a, dp[k], dv[t]);
+                        }
+                }
+                wave_lds_sync();
+                if constexpr (LOCK) __builtin_amdgcn_s_barrier();
+                fetch(pa[s], pb[s], pc[s], pd[s], q + F);
+                const int zero_pivot = coop_eliminate_col<M, true>(r, c, dv, sh, j, base);
+                ok = ok && !(act && zero_pivot);
+                if (act) {
+                    const double dj = lane_elem<M>(dv, j);
+                    D[av.at(q, j)] = dj;
+                    xlast = dj;
+                }
+                if (has_next) {
+#pragma unroll
+                    for (int t = 0; t < M; ++t) C[am.at(q, t + M * j)] = c[t];
+                }
+#pragma unroll
+                for (int t = 0; t < M; ++t) {
+                    cc[t] = c[t];
+                    dp[t] = dv[t];
+                }
+                wave_lds_sync();
+            }
+        }
+    }
+    return ok;
+}
+
+// Both halves of coop_twisted up to and including the meeting row, in the column form (lane j
+// owns column j; sub/sup as in coop_twisted). Stores d'_q(j) and the solved block's column j
+// for every row a group eliminates, and x_h(j) from the top group; xh: the whole x_h.
+template <int M, bool UNIT_B, bool LOCK, class AM, class AV>
+__device__ __forceinline__ bool coop_twisted_fwd_col(const double* A, const double* __restrict__ B, double* C,
+                                                     double* D, const AM& am, const AV& av, int j, int base,
+                                                     int pbase, bool top, int n, int h, int cnt, int steps,
+                                                     int rows, double* __restrict__ sh, double (&xh)[M]) {
+    constexpr int F = ColDepth<M>::kFwdAhead;
+    const int lane = threadIdx.x & (kTile - 1);
+    const double* sub = top ? A : C;
+    const double* sup = top ? C : A;
+    auto clamp_row = [&](int q) { return q < 0 ? 0 : (q >= rows ? rows - 1 : q); };
+    bool ok = true;
+    double ps[F][M], pg[F][M], pu[F][M], pd[F];
+    auto fetch = [&](double (&fs)[M], double (&fg)[M], double (&fu)[M], double& fd, int t) {
+        const int q = clamp_row(top ? t : n - 1 - t);
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            fs[k] = sub[am.at(q, k + M * j)];
+            if constexpr (UNIT_B)
+                fg[k] = j == k ? 1.0 : 0.0;
+            else
+                fg[k] = __builtin_nontemporal_load(B + am.at(q, k + M * j));
+            fu[k] = sup[am.at(q, k + M * j)];
+        }
+        fd = D[av.at(q, j)];
+    };
+#pragma unroll
+    for (int s = 0; s < F; ++s) fetch(ps[s], pg[s], pu[s], pd[s], s);
+    double cc[M], dp[M];  // column j of the carried block; the carried d, whole
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+        cc[t] = 0.0;
+        dp[t] = 0.0;
+    }
+    for (int t0 = 0; t0 < steps; t0 += F) {
+#pragma unroll
+        for (int s = 0; s < F; ++s) {
+            const int t = t0 + s;
+            if (t < steps) {  // uniform: the last chunk may be partial
+                const bool act = t < cnt;
+                const int q = clamp_row(top ? t : n - 1 - t);
+#pragma unroll
+                for (int k = 0; k < M; ++k) sh[k * kTile + lane] = ps[s][k];
+                sh[M * kTile + lane] = pd[s];
+                wave_lds_sync();
+                double r[M], c[M], dv[M];
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    r[k] = pg[s][k];
+                    c[k] = pu[s][k];
+                    dv[k] = sh[M * kTile + base + k];
+                }
+                if (t > 0) {
+#pragma unroll
+                    for (int k = 0; k < M; ++k)
+#pragma unroll
+                        for (int u = 0; u < M; ++u) {
+                            const double a = sh[u * kTile + base + k];
+                            r[u] = fma(-a, cc[k], r[u]);
+                            dv[u] = fma(-a, dp[k], dv[u]);
+                        }
+                }
+                wave_lds_sync();
+                if constexpr (LOCK) __builtin_amdgcn_s_barrier();
+                fetch(ps[s], pg[s], pu[s], pd[s], t + F);
+                const int zero_pivot = coop_eliminate_col<M, true>(r, c, dv, sh, j, base);
+                ok = ok && !(act && zero_pivot);
+                if (act) {
+                    D[av.at(q, j)] = lane_elem<M>(dv, j);
+#pragma unroll
+                    for (int k = 0; k < M; ++k) C[am.at(q, k + M * j)] = c[k];
+                }
+#pragma unroll
+                for (int k = 0; k < M; ++k) {
+                    cc[k] = act ? c[k] : cc[k];
+                    dp[k] = act ? dv[k] : dp[k];
+                }
+                wave_lds_sync();
+            }
+        }
+    }
+    // meeting row h: (B_h - A_h C'_{h-1} - C_h A''_{h+1}) x_h = d_h - A_h d'_{h-1} - C_h d''_{h+1},
+    // solved by both groups of a system from the same operands in the same order
+    const int q = h;
+    double r[M], dv[M], dummy[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        sh[k * kTile + lane] = cc[k];                                                   // carry column j
+        sh[(M + 1 + k) * kTile + lane] = h > 0 ? A[am.at(q, k + M * j)] : 0.0;          // A_h(k, j)
+        sh[(2 * M + 1 + k) * kTile + lane] = C[am.at(q, k + M * j)];                     // C_h(k, j)
+        if constexpr (UNIT_B)
+            r[k] = j == k ? 1.0 : 0.0;
+        else
+            r[k] = B[am.at(q, k + M * j)];
+        dummy[k] = 0.0;
+    }
+    sh[M * kTile + lane] = lane_elem<M>(dp, j);       // element j of the carried d
+    sh[(3 * M + 1) * kTile + lane] = D[av.at(q, j)];  // d_h(j)
+    wave_lds_sync();
+    const int tb = top ? base : pbase, bb = top ? pbase : base;
+    double ct[M], cb[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        ct[k] = sh[k * kTile + tb + j];  // C'_{h-1}(k, j)
+        cb[k] = sh[k * kTile + bb + j];  // A''_{h+1}(k, j)
+        dv[k] = sh[(3 * M + 1) * kTile + base + k];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double dt = sh[M * kTile + tb + k], db = sh[M * kTile + bb + k];
+#pragma unroll
+        for (int u = 0; u < M; ++u) {
+            const double a = sh[(M + 1 + u) * kTile + base + k];      // A_h(u, k)
+            const double cz = sh[(2 * M + 1 + u) * kTile + base + k];  // C_h(u, k)
+            dv[u] = fma(-a, dt, dv[u]);
+            dv[u] = fma(-cz, db, dv[u]);
+            r[u] = fma(-a, ct[k], r[u]);
+            r[u] = fma(-cz, cb[k], r[u]);
+        }
+    }
+    wave_lds_sync();
+    const int zero_pivot = coop_eliminate_col<M, false>(r, dummy, dv, sh, j, base);
+    ok = ok && !(n > 0 && zero_pivot);
+    if (n > 0 && top) D[av.at(q, j)] = lane_elem<M>(dv, j);
+#pragma unroll
+    for (int k = 0; k < M; ++k) xh[k] = dv[k];
+    return ok;
+}
+
 // Cooperative solve: M lanes per system, lane i owns block row i of every block; row q is
 // R_q = B_q - A_q C'_{q-1} eliminated with rhs (d_q - A_q d'_{q-1}, C_q) (coop_eliminate),
 // C'_q stored over C_q and carried in LDS, d'_q over d_q and carried whole in registers;
@@ -277,16 +607,20 @@ __device__ __forceinline__ double lane_elem(const double (&v)[M], int i) {
 // system's n compute on what the rows hold and store nothing.
 // With LOCK each row step's loads are issued after a workgroup barrier (s_barrier, no memory
 // wait), so the wavefronts of a workgroup, which must all run nmax steps, load each row together.
-template <int M, bool LOCK = false, class AM, class AV>
+template <int M, bool LOCK = false, bool COL = (M >= kColFromM), class AM, class AV>
 __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict__ B, double* C,
                           double* D, const AM& am, const AV& av, int i, int base, int n, int nmax, int rows,
                           double* __restrict__ sh) {
-    constexpr int F = CoopDepth<M>::kFwdAhead;
     constexpr int G = CoopDepth<M>::kBwdAhead;
-    double* carry = sh;
-    double* xb = sh + (M + 1) * kTile;
     const int lane = threadIdx.x & (kTile - 1);  // the lane in its wavefront
     bool ok = true;
+    double xlast = 0.0;
+    if constexpr (COL) {
+        ok = coop_fwd_col<M, LOCK>(A, B, C, D, am, av, i, base, n, nmax, rows, sh, xlast);
+    } else {
+    constexpr int F = CoopDepth<M>::kFwdAhead;
+    double* carry = sh;
+    double* xb = sh + (M + 1) * kTile;
     double pa[F][M], pb[F][M], pc[F][M], pd[F];
     auto fetch = [&](double (&fa)[M], double (&fb)[M], double (&fc)[M], double& fd, int q) {
         q = q < rows ? q : rows - 1;
@@ -303,7 +637,6 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
     double dp[M];  // d'_{q-1}, whole, in every lane of the system
 #pragma unroll
     for (int k = 0; k < M; ++k) dp[k] = 0.0;
-    double xlast = 0.0;
     for (int q0 = 0; q0 < nmax; q0 += F) {
 #pragma unroll
         for (int s = 0; s < F; ++s) {
@@ -349,6 +682,7 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
                 wave_lds_sync();
             }
         }
+    }
     }
     // back substitution over the rows: x_q(i) = d'_q(i) - sum_j C'_q(i,j) x_{q+1}(j), with
     // C'_q(i, :) and d'_q(i) loaded kBwdAhead rows ahead (rows are clamped to the lane's pack).
@@ -403,21 +737,26 @@ __device__ bool coop_rows(const double* __restrict__ A, const double* __restrict
 // groups finish. ES is the element stride of the storage; with UNIT_B every B_q is the
 // identity and B is not read (the reduced system of the two-stage solve).
 // LOCK: as in coop_rows (every wavefront of the workgroup runs the same `steps`).
-template <int M, bool UNIT_B = false, bool LOCK = false, class AM, class AV>
+template <int M, bool UNIT_B = false, bool LOCK = false, bool COL = (M >= kColFromM), class AM, class AV>
 __device__ bool coop_twisted(const double* A, const double* __restrict__ B, double* C, double* D,
                              const AM& am, const AV& av, int i, int base, int pbase, bool top, int n,
                              int steps, int rows, double* __restrict__ sh) {
-    constexpr int F = CoopDepth<M>::kFwdAhead;
     constexpr int G = CoopDepth<M>::kBwdAhead;
-    double* carry = sh;
-    double* xb = sh + (M + 1) * kTile;
     const int lane = threadIdx.x & (kTile - 1);  // the lane in its wavefront
     const int h = n > 0 ? (n - 1) / 2 : 0;
     const int cnt = n > 0 ? (top ? h : n - 1 - h) : 0;
-    const double* sub = top ? A : C;  // the block coupling row q to the row eliminated before it
-    const double* sup = top ? C : A;  // the block coupling row q to the row eliminated after it
     auto clamp_row = [&](int q) { return q < 0 ? 0 : (q >= rows ? rows - 1 : q); };
     bool ok = true;
+    double xh[M];
+    if constexpr (COL) {
+        ok = coop_twisted_fwd_col<M, UNIT_B, LOCK>(A, B, C, D, am, av, i, base, pbase, top, n, h, cnt, steps,
+                                                   rows, sh, xh);
+    } else {
+    constexpr int F = CoopDepth<M>::kFwdAhead;
+    double* carry = sh;
+    double* xb = sh + (M + 1) * kTile;
+    const double* sub = top ? A : C;  // the block coupling row q to the row eliminated before it
+    const double* sup = top ? C : A;  // the block coupling row q to the row eliminated after it
     double ps[F][M], pg[F][M], pu[F][M], pd[F];
     auto fetch = [&](double (&fs)[M], double (&fg)[M], double (&fu)[M], double& fd, int t) {
         const int q = clamp_row(top ? t : n - 1 - t);
@@ -486,7 +825,6 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
     carry[M * kTile + lane] = lane_elem<M>(dp, i);
     wave_lds_sync();
     const int tb = top ? base : pbase, bb = top ? pbase : base;
-    double xh[M];
     {
         const int q = h;  // the meeting row
         double r[M], rhs[M + 1];
@@ -516,6 +854,7 @@ __device__ bool coop_twisted(const double* A, const double* __restrict__ B, doub
         const int zero_pivot = coop_eliminate<M>(r, rhs, xb, i, base, xc, xh);
         ok = ok && !(n > 0 && zero_pivot);
         if (n > 0 && top) D[av.at(q, i)] = lane_elem<M>(xh, i);
+    }
     }
     // outward substitution, kBwdAhead rows ahead
     double xprev = lane_elem<M>(xh, i);

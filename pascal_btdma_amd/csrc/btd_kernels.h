@@ -2130,9 +2130,16 @@ __global__ __launch_bounds__(256) void k_generate(int kind, uint64_t seed, int l
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= npad * nrow) return;
     const int64_t row = idx / npad, sys = idx % npad;
-    if (sys >= nsys) return;
+    if (sys >= nsys && layout != LAYOUT_TILED) return;  // only the tiled layout has padding lanes
     double a[M][M], b[M][M], c[M][M], d[M];
-    gen_row<M>(kind, seed, sys, row, nrow, a, b, c, d);
+    if (sys < nsys) {
+        gen_row<M>(kind, seed, sys, row, nrow, a, b, c, d);
+    } else {  // the tiled layout's padding lanes: zeros, so every element of the arrays is defined
+        for (int j = 0; j < M; ++j) {
+            d[j] = 0.0;
+            for (int i = 0; i < M; ++i) a[i][j] = b[i][j] = c[i][j] = 0.0;
+        }
+    }
     for (int j = 0; j < M; ++j)
         for (int i = 0; i < M; ++i) {
             const int64_t o = elem_offset(layout, sys, row, i + M * j, M * M, nsys, nrow);

@@ -187,3 +187,33 @@ def test_tuner_and_graph_replay(cuda, oracle):
     x = np.transpose(flat, (0, 2, 1))[:, :4]
     assert (np.abs(x - xo).max(axis=(0, 2)) / np.abs(xo).max(axis=(0, 2))).max() <= TOL
     plan.destroy()
+
+
+@pytest.mark.parametrize("m,nsys,n,placement,form", [
+    (7, 600, 40, 6, pb.Form.TWISTED), (8, 600, 41, 6, pb.Form.TWISTED), (10, 300, 40, 6, pb.Form.TWISTED),
+    (8, 8200, 24, 4, pb.Form.PLAIN), (10, 3100, 25, 4, pb.Form.PLAIN)])
+def test_column_form_equals_row_form(cuda, m, nsys, n, placement, form):
+    """From m = 7 the single-partition cooperative and twisted solves run the column form
+    (lane j owns column j of every block); the mixed-size batch runs the row form (lane i owns
+    row i). Both perform the same operations on the same operands, so on the same systems
+    (equal m and n, so both generators give the same values) the plain plan's x equals the
+    batch's bit for bit: twisted (placement 6, every item twisted) and cooperative
+    (placement 4; PLAIN picks the cooperative kernel at these batch sizes for m >= 8)."""
+    import torch
+    bt, a, b, c, d = make_batch(cuda, np.full(nsys, m), np.full(nsys, n), pb.GenKind.DDPERM, 9)
+    bt.set_params(placement, 512)
+    bt.solve(a, b, c, d)
+    lay = pb.Layout.REF_GPU
+    t = [torch.empty(pb.layout_elems(lay, nsys, n, m * m if k < 3 else m), dtype=torch.float64, device=cuda)
+         for k in range(4)]
+    pb.generate(pb.GenKind.DDPERM, 9, m, nsys, n, lay, *t)
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, lay) as plan:
+        plan.set_form(form)
+        pb.solve_noncyclic(plan, *t, m, nsys, n)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    assert not bt.check()
+    hd = d.cpu().numpy()
+    x = np.transpose(t[3].cpu().numpy().reshape(m, n, nsys), (1, 2, 0))  # (n, nsys, m)
+    for s in range(nsys):
+        np.testing.assert_array_equal(x[:, s, :], bt.extract(hd, s), err_msg=f"system {s}")
