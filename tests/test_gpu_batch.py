@@ -191,7 +191,8 @@ def test_tuner_and_graph_replay(cuda, oracle):
 
 @pytest.mark.parametrize("m,nsys,n,placement,form", [
     (7, 600, 40, 6, pb.Form.TWISTED), (8, 600, 41, 6, pb.Form.TWISTED), (10, 300, 40, 6, pb.Form.TWISTED),
-    (8, 8200, 24, 4, pb.Form.PLAIN), (10, 3100, 25, 4, pb.Form.PLAIN)])
+    (9, 130, 2, 6, pb.Form.TWISTED), (7, 129, 3, 6, pb.Form.TWISTED), (10, 97, 4, 6, pb.Form.TWISTED),
+    (8, 8200, 24, 4, pb.Form.PLAIN), (9, 4100, 30, 4, pb.Form.PLAIN), (10, 3100, 25, 4, pb.Form.PLAIN)])
 def test_column_form_equals_row_form(cuda, m, nsys, n, placement, form):
     """From m = 7 the single-partition cooperative and twisted solves run the column form
     (lane j owns column j of every block); the mixed-size batch runs the row form (lane i owns
