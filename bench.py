@@ -506,7 +506,21 @@ def extra_f32r(dev, steps: int, fp64_ms: float, checks: ParityChecks) -> dict:
             "bytes_per_row_model": 28 * m * m + 32 * m}
         xs = tiled_vec_subset(xs_[: 2 * n * m * 64].cpu().numpy(), n, m, [0, 1])
         checks.add(sample, xs, *error_sinks(r))
-        del work64, xs_
+        # the same solve with C' in the plan's workspace, placed for this solve by timing
+        # candidates (btd_plan_place_workspace_f32s), as the headline's FP64 workspace is
+        x_caller = xs_.clone()
+        del work64
+        plan.set_form(pb.Form.WORKSPACE)
+        cand = plan.place_workspace_f32s(a, b, c, d, candidates=8)
+        t = event_time(lambda: pb.solve_noncyclic_f32s(plan, a, b, c, xs_),
+                       lambda: xs_.copy_(d), steps, torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        ms = float(np.median(t))
+        out["fp32_storage_fp64_arithmetic_placed"] = {
+            "ms": ms, "block_rows_per_s": nsys * n / (ms * 1e-3), "speedup_vs_fp64_plain": fp64_ms / ms,
+            "placement_candidates_ms": cand, "x_bit_identical_to_caller_workspace": bool(torch.equal(xs_, x_caller))}
+        plan.set_form(pb.Form.PLAIN)
+        del x_caller, xs_
         # the same mode on the headline's well-conditioned Euler-like blocks rounded to FP32
         pb.generate_f32(pb.GenKind.EULER, 1, m, nsys, n, lay, a, b, c, d)
         work64 = torch.empty(plan.matrix_elems(), dtype=torch.float64, device=dev)

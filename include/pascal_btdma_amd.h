@@ -152,6 +152,13 @@ btd_status btd_plan_place_workspace(btd_plan_t plan, const double *a, const doub
                                     double *d, int m, int nsys, int nrow, int candidates, double *ms,
                                     void *stream);
 
+/* New (no counterpart in the reference): btd_plan_place_workspace for the FP32-storage solve
+ * (btd_solve_noncyclic_f32s with work == NULL, which then writes C' to the plan's workspace):
+ * the candidates are timed with that solve of the FP32 blocks a, b, c and the FP64 d, which is
+ * restored and returned unchanged. Same rules and limits as btd_plan_place_workspace. */
+btd_status btd_plan_place_workspace_f32s(btd_plan_t plan, const float *a, const float *b, const float *c,
+                                         double *d, int candidates, double *ms, void *stream);
+
 /* Replaces BtdmaGpuPlan::destroy (pascal_btdma_cuda.hpp:134, pascal_btdma_cuda.cu:1007-1033)
  * and btdma_cleanplan_gpu_v2 (mod_btdma_gpu_v2.f90:111-129). NULL is accepted. */
 btd_status btd_plan_destroy(btd_plan_t plan);
@@ -220,8 +227,9 @@ btd_status btd_solve_noncyclic_f32r(btd_plan_t plan, const float *a, const float
  * blocks' values, one lane per system in the reference CPU path's BLAS/LAPACK operation order,
  * so bit for bit what that path (and btd_solve_noncyclic's one-lane solve) computes on FP64
  * copies of a, b, c, reading 4 bytes per block element. a, b, c are only read; d is in place (x returned in d);
- * work: btd_layout_elems(layout, nsys, nrow, m*m) doubles of device scratch (holds C'). npart
- * must be 1. */
+ * work: btd_layout_elems(layout, nsys, nrow, m*m) doubles of device scratch (holds C'), or NULL
+ * for the plan's workspace (a plan in BTD_FORM_WORKSPACE; its placement for this solve is picked
+ * by btd_plan_place_workspace_f32s). npart must be 1. */
 btd_status btd_solve_noncyclic_f32s(btd_plan_t plan, const float *a, const float *b, const float *c,
                                     double *d, double *work, void *stream);
 

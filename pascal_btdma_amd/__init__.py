@@ -195,6 +195,19 @@ class BtdmaGpuPlan:
                                                      _stream(stream)))
         return [float(x) for x in ms]
 
+    def place_workspace_f32s(self, a, b, c, d, candidates: int = 8, stream=None) -> list[float]:
+        """btd_plan_place_workspace_f32s: place_workspace for the FP32-storage solve
+        (solve_noncyclic_f32s with work=None): a, b, c float32 blocks, d float64, unchanged."""
+        import torch
+        nm, nv = self.matrix_elems(), self.vector_elems()
+        for name, t, n, dt in (("a", a, nm, torch.float32), ("b", b, nm, torch.float32),
+                               ("c", c, nm, torch.float32), ("d", d, nv, torch.float64)):
+            _check_tensor(name, t, n, dt)
+        ms = (ctypes.c_double * int(candidates))()
+        _check(_capi.lib().btd_plan_place_workspace_f32s(self._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
+                                                          int(candidates), ms, _stream(stream)))
+        return [float(x) for x in ms]
+
     def check(self) -> bool:
         """Synchronise; True if any block LU met a zero pivot since the last check."""
         s = ctypes.c_int(0)
@@ -334,16 +347,19 @@ def solve_noncyclic_f32r(plan: BtdmaGpuPlan, a, b, c, d, x, work, sweeps: int = 
                                                 _ptr(x), _ptr(work), int(sweeps), _stream(stream)))
 
 
-def solve_noncyclic_f32s(plan: BtdmaGpuPlan, a, b, c, d, work, stream=None) -> None:
+def solve_noncyclic_f32s(plan: BtdmaGpuPlan, a, b, c, d, work=None, stream=None) -> None:
     """FP32-stored blocks (a, b, c float32, only read), FP64 arithmetic: d (float64) is
     replaced by x, the FP64 solve of the blocks' values; work holds C' (plan.matrix_elems()
-    float64)."""
+    float64), or None for the workspace of a plan in Form.WORKSPACE (placed for this solve by
+    plan.place_workspace_f32s)."""
     import torch
     if plan.created:
         nm, nv = plan.matrix_elems(), plan.vector_elems()
-        for name, t, n, dt in (("a", a, nm, torch.float32), ("b", b, nm, torch.float32),
-                               ("c", c, nm, torch.float32), ("d", d, nv, torch.float64),
-                               ("work", work, nm, torch.float64)):
+        checks = [("a", a, nm, torch.float32), ("b", b, nm, torch.float32),
+                  ("c", c, nm, torch.float32), ("d", d, nv, torch.float64)]
+        if work is not None:
+            checks.append(("work", work, nm, torch.float64))
+        for name, t, n, dt in checks:
             _check_tensor(name, t, n, dt)
     _check(_capi.lib().btd_solve_noncyclic_f32s(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
                                                 _ptr(work), _stream(stream)))

@@ -54,6 +54,7 @@ SIGNATURES = {
     "btd_plan_set_form": (_i, [_vp, _i]),
     "btd_plan_form": (_i, [_vp, ctypes.POINTER(_i)]),
     "btd_plan_place_workspace": (_i, [_vp, _dp, _dp, _dp, _dp, _i, _i, _i, _i, ctypes.POINTER(_d), _vp]),
+    "btd_plan_place_workspace_f32s": (_i, [_vp, _dp, _dp, _dp, _dp, _i, ctypes.POINTER(_d), _vp]),
     "btd_plan_info": (_i, [_vp, _vp]),
     "btd_last_error": (ctypes.c_char_p, []),
     "btd_matrix_index": (ctypes.c_size_t, [_i, _i, _i, _i, _i, _i, _i]),

@@ -585,19 +585,20 @@ btd_status btd_plan_info(btd_plan_t plan, int* info) {
     return read_status(plan, nullptr, info);
 }
 
-btd_status btd_plan_place_workspace(btd_plan_t plan, const double* a, const double* b, const double* c,
-                                    double* d, int m, int nsys, int nrow, int candidates, double* ms,
-                                    void* stream) {
-    g_err.clear();
-    const char* what = "btd_plan_place_workspace";
-    btd_status st = check_solve(plan, a, b, c, d, m, nsys, nrow);
-    if (st != BTD_OK) return st;
+static btd_status check_single_partition(btd_plan_t plan, const char* what);
+
+// Placement of a plan's workspace by timing: `solve(ws)` runs the workspace form's solve of the
+// caller's arrays with candidate workspace ws on stream s, d (db bytes) restored from a copy
+// before each solve and at the end. Shared by the FP64 and the FP32-storage solves.
+extern "C++" {
+template <class Solve>
+static btd_status place_workspace_impl(btd_plan_t plan, double* d, int candidates, double* ms, hipStream_t s,
+                                       const char* what, Solve solve) {
     if (plan->form != BTD_FORM_WORKSPACE || !plan->ws)
         return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": the plan's form is not BTD_FORM_WORKSPACE");
     if (candidates < 1 || candidates > BTD_MAX_WORKSPACE_CANDIDATES)
         return fail(BTD_ERR_INVALID_ARG, std::string(what) + ": candidates must be in 1.." +
                                              std::to_string(BTD_MAX_WORKSPACE_CANDIDATES));
-    hipStream_t s = (hipStream_t)stream;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     BTD_HIP(hipStreamIsCapturing(s, &cap));
     if (cap != hipStreamCaptureStatusNone)
@@ -627,9 +628,7 @@ btd_status btd_plan_place_workspace(btd_plan_t plan, const double* a, const doub
         for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
             e = hipMemcpyAsync(d, dsave, db, hipMemcpyDeviceToDevice, s);
             if (e == hipSuccess) e = hipEventRecord(plan->ev[0], s);
-            if (e == hipSuccess)
-                e = btd::launch_plain_ref(plan->m, plan->layout, a, b, c, cand[k], d, plan->nsys, plan->nrow,
-                                          plan->d_flag, s);
+            if (e == hipSuccess) e = solve(cand[k]);
             if (e == hipSuccess) e = hipEventRecord(plan->ev[1], s);
             if (e == hipSuccess) e = hipEventSynchronize(plan->ev[1]);
             if (e == hipSuccess) {
@@ -653,6 +652,31 @@ btd_status btd_plan_place_workspace(btd_plan_t plan, const double* a, const doub
         for (size_t k = 0; k < cand.size(); ++k) ms[k] = t[k];
     plan->last_stream = s;
     return BTD_OK;
+}
+}  // extern "C++"
+
+btd_status btd_plan_place_workspace(btd_plan_t plan, const double* a, const double* b, const double* c,
+                                    double* d, int m, int nsys, int nrow, int candidates, double* ms,
+                                    void* stream) {
+    g_err.clear();
+    btd_status st = check_solve(plan, a, b, c, d, m, nsys, nrow);
+    if (st != BTD_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    return place_workspace_impl(plan, d, candidates, ms, s, "btd_plan_place_workspace", [&](double* ws) {
+        return btd::launch_plain_ref(plan->m, plan->layout, a, b, c, ws, d, plan->nsys, plan->nrow, plan->d_flag, s);
+    });
+}
+
+btd_status btd_plan_place_workspace_f32s(btd_plan_t plan, const float* a, const float* b, const float* c,
+                                         double* d, int candidates, double* ms, void* stream) {
+    g_err.clear();
+    btd_status st = check_single_partition(plan, "btd_plan_place_workspace_f32s");
+    if (st != BTD_OK) return st;
+    if (!a || !b || !c || !d) return fail(BTD_ERR_INVALID_ARG, "btd_plan_place_workspace_f32s: NULL array");
+    hipStream_t s = (hipStream_t)stream;
+    return place_workspace_impl(plan, d, candidates, ms, s, "btd_plan_place_workspace_f32s", [&](double* ws) {
+        return btd::launch_plain_f32s(plan->m, plan->layout, a, b, c, ws, d, plan->nsys, plan->nrow, plan->d_flag, s);
+    });
 }
 
 btd_status btd_solve_noncyclic_profiled(btd_plan_t plan, double* a, const double* b, double* c,
@@ -744,8 +768,14 @@ btd_status btd_solve_noncyclic_f32s(btd_plan_t plan, const float* a, const float
     g_err.clear();
     btd_status st = check_single_partition(plan, "btd_solve_noncyclic_f32s");
     if (st != BTD_OK) return st;
-    if (!a || !b || !c || !d || !work)
+    if (!a || !b || !c || !d)
         return fail(BTD_ERR_INVALID_ARG, "btd_solve_noncyclic_f32s: NULL array");
+    if (!work) {  // the plan's workspace (BTD_FORM_WORKSPACE), placed by btd_plan_place_workspace_f32s
+        if (plan->form != BTD_FORM_WORKSPACE || !plan->ws)
+            return fail(BTD_ERR_INVALID_ARG,
+                        "btd_solve_noncyclic_f32s: NULL work needs a plan in BTD_FORM_WORKSPACE");
+        work = plan->ws;
+    }
     plan->last_stream = (hipStream_t)stream;
     BTD_HIP(btd::launch_plain_f32s(plan->m, plan->layout, a, b, c, work, d, plan->nsys, plan->nrow,
                                    plan->d_flag, (hipStream_t)stream));

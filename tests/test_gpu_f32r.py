@@ -168,6 +168,40 @@ def test_fp32_storage_on_conditioned_blocks(cuda, oracle):
     assert e_tw <= 10 * e_or
 
 
+@pytest.mark.parametrize("layout", [pb.Layout.TILED, pb.Layout.REF_GPU])
+def test_fp32_storage_in_the_placed_workspace(cuda, layout):
+    """work=None solves into the plan's workspace (Form.WORKSPACE): after
+    place_workspace_f32s, whose timed solves return d unchanged and leave the blocks as they
+    were, x equals the solve with a caller's work array bit for bit; without the workspace form
+    work=None is refused."""
+    import torch
+    m, nsys, n = 5, 700, 40
+    A, B, C, d = pb.generate_host(pb.GenKind.COND, 31, m, nsys, n)
+    A, B, C = fp32_round(A, B, C)
+    lay = int(layout)
+    f32 = lambda X: torch.from_numpy(nat_to_layout(X, lay).astype(np.float32)).to(cuda)
+    ta, tb, tc = f32(A), f32(B), f32(C)
+    td = torch.from_numpy(nat_to_layout(d, lay)).to(cuda)
+    keep = [t.clone() for t in (ta, tb, tc, td)]
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
+        with pytest.raises(ValueError, match="BTD_FORM_WORKSPACE"):
+            pb.solve_noncyclic_f32s(plan, ta, tb, tc, td.clone())
+        ref = td.clone()
+        pb.solve_noncyclic_f32s(plan, ta, tb, tc, ref, torch.empty(plan.matrix_elems(), dtype=torch.float64,
+                                                                   device=cuda))
+        plan.set_form(pb.Form.WORKSPACE)
+        ms = plan.place_workspace_f32s(ta, tb, tc, td, candidates=3)
+        assert len(ms) == 3 and ms[0] > 0
+        torch.cuda.synchronize()
+        for t, k in zip((ta, tb, tc, td), keep):
+            assert torch.equal(t, k)
+        x = td.clone()
+        pb.solve_noncyclic_f32s(plan, ta, tb, tc, x)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    assert torch.equal(x, ref)
+
+
 def test_fp32_storage_rejects_partitions(cuda):
     import torch
     t = torch.zeros(10, dtype=torch.float32, device=cuda)
