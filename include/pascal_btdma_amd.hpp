@@ -129,6 +129,11 @@ struct BtdmaGpuPlan {
                          double* ms = nullptr, hipStream_t stream = nullptr) {
         check_status(btd_plan_place_workspace(handle, a, b, c, d, m, nsys, nrow, candidates, ms, stream));
     }
+    // The same for the FP32-storage solve (solve_noncyclic_f32s with work_dev == nullptr).
+    void place_workspace_f32s(const float* a, const float* b, const float* c, double* d, int candidates,
+                              double* ms = nullptr, hipStream_t stream = nullptr) {
+        check_status(btd_plan_place_workspace_f32s(handle, a, b, c, d, candidates, ms, stream));
+    }
     btd_form form() {
         int f = 0;
         check_status(btd_plan_form(handle, &f));
@@ -177,7 +182,8 @@ inline void solve_cyclic(BtdmaGpuPlan& plan, double* a_dev, double* b_dev, doubl
 
 // Entry points with no counterpart in the reference (see the C header for their contracts).
 
-// FP32-stored blocks, FP64 arithmetic; work: btd_layout_elems(layout, nsys, nrow, m*m) doubles.
+// FP32-stored blocks, FP64 arithmetic; work: btd_layout_elems(layout, nsys, nrow, m*m) doubles, or
+// nullptr for the workspace of a plan in BTD_FORM_WORKSPACE.
 inline void solve_noncyclic_f32s(BtdmaGpuPlan& plan, const float* a_dev, const float* b_dev,
                                  const float* c_dev, double* d_dev, double* work_dev,
                                  hipStream_t stream = nullptr) {
