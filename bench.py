@@ -224,8 +224,8 @@ def extra_study(dev) -> dict:
     return out
 
 
-def extra_config0(dev, checks: ParityChecks) -> dict:
-    """Config 0: 1024 systems, m = 3, n = 64 (the CPU-runnable case), FP64 plain; all
+def extra_small(dev, checks: ParityChecks) -> dict:
+    """The small case: 1024 systems, m = 3, n = 64 (the CPU-runnable size), FP64 plain; all
     systems are checked against the oracle in the CPU leg."""
     import torch
     import pascal_btdma_amd as pb
@@ -288,8 +288,8 @@ def extra_config0(dev, checks: ParityChecks) -> dict:
     return out
 
 
-def config2_batch(dev):
-    """The 20000 mixed systems of config 2 (m uniform in {2,3,5,8,10}, n log-uniform in
+def mixed_batch(dev):
+    """The 20000 mixed-size systems (m uniform in {2,3,5,8,10}, n log-uniform in
     [32,1024], seeded), packed and generated: (batch, m, n, a, b, c0, d0)."""
     import torch
     import pascal_btdma_amd as pb
@@ -305,9 +305,9 @@ def config2_batch(dev):
 
 
 def batch_probe(dev, placement: int, grid: int, reps: int = 3) -> dict:
-    """Only the config-2 batch, `reps` solves at fixed launch parameters (for profilers)."""
+    """Only the 20000-system mixed-size batch, `reps` solves at fixed launch parameters (for profilers)."""
     import torch
-    bt, m, n, a, b, c0, d0 = config2_batch(dev)
+    bt, m, n, a, b, c0, d0 = mixed_batch(dev)
     bt.set_params(placement, grid)
     c, d = c0.clone(), d0.clone()
     t = event_time(lambda: bt.solve(a, b, c, d), lambda: (c.copy_(c0), d.copy_(d0)), reps,
@@ -317,7 +317,7 @@ def batch_probe(dev, placement: int, grid: int, reps: int = 3) -> dict:
 
 
 def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
-    """Config 2: 20000 systems with m uniform in {2,3,5,8,10}, n log-uniform in [32,1024],
+    """Mixed sizes plus long systems: 20000 systems with m uniform in {2,3,5,8,10}, n log-uniform in [32,1024],
     plus 512 long systems (m = 5, n = 65536) split into P partitions, all replayed as one
     captured graph (the batch's persistent kernel and the long systems' three kernels, one
     after the other or on forked branches). The autotuner picks the batch placement/grid and
@@ -327,7 +327,7 @@ def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
     import torch
     import pascal_btdma_amd as pb
     from pascal_btdma_amd.tuner import tune_batch, tune_partitions
-    bt, m, n, a, b, c0, d0 = config2_batch(dev)
+    bt, m, n, a, b, c0, d0 = mixed_batch(dev)
     nb = len(m)
     tb = tune_batch(bt, a, b, c0, d0, reps=2)
     lm, lnsys, ln = 5, 512, 65536
@@ -450,10 +450,10 @@ def extra_batch(dev, steps: int, checks: ParityChecks) -> dict:
 
 
 def extra_f32r(dev, steps: int, fp64_ms: float, checks: ParityChecks) -> dict:
-    """Config 3: m = 5, n = 512, 131072 systems, blocks with cond log-uniform [1e1, 1e6]
+    """FP32-stored conditioned blocks: m = 5, n = 512, 131072 systems, blocks with cond log-uniform [1e1, 1e6]
     stored in FP32. Modes: FP32 solve + s FP64-residual refinement sweeps (s = 0..8), FP32
     storage with FP64 arithmetic (no refinement), and the FP64 plain kernel on the same
-    (FP32-rounded) values; times against the FP64 plain kernel's time on config 1. Every
+    (FP32-rounded) values; times against the FP64 plain kernel's time on the headline workload. Every
     mode's error on a sample of systems is measured against the oracle on the same inputs and
     against the binary128 truth solve of those inputs (the oracle is itself an FP64 solve,
     about cond * 1e-16 from the exact solution)."""
@@ -467,7 +467,7 @@ def extra_f32r(dev, steps: int, fp64_ms: float, checks: ParityChecks) -> dict:
     pb.generate_f32(pb.GenKind.COND, 3, m, nsys, n, lay, a, b, c, d)
     x = torch.empty_like(d)
     out = {"workload": "cond-controlled B (1e1..1e6), m=5, n=512, 131072 systems, FP32 blocks",
-           "fp64_plain_ms_config1": fp64_ms}
+           "fp64_plain_ms_headline": fp64_ms}
     sample = lambda: pb.generate_host(pb.GenKind.COND, 3, m, nsys, n, 0, 128)
 
     def error_sinks(r):
@@ -555,7 +555,7 @@ def extra_f32r(dev, steps: int, fp64_ms: float, checks: ParityChecks) -> dict:
 
 
 def extra_adjoint(dev, steps: int) -> dict:
-    """Config 4: WENO-like m = 3, n = 256, 262144 systems through the autograd Functions:
+    """The differentiable solve: WENO-like m = 3, n = 256, 262144 systems through the autograd Functions:
     forward (with factors kept) and backward (adjoint kernel) times, raw kernel times, in
     FP64 and with FP32-stored blocks + FP64 refinement. The blocks are rounded to FP32
     values first, so that both variants solve the same systems."""
@@ -803,12 +803,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="headline workload only")
     ap.add_argument("--only", default="",
-                    help="comma list of extras to run (study,config0,batch,f32r,adjoint,euler,weno)")
+                    help="comma list of extras to run (study,small,batch,f32r,adjoint,euler,weno)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workspace-candidates", type=int, default=8,
                     help="workspace allocations btd_plan_place_workspace tries for the headline")
     ap.add_argument("--batch-probe", default="",
-                    help="PLACEMENT,GRID: only solve the config-2 batch 3 times at these launch "
+                    help="PLACEMENT,GRID: only solve the 20000-system mixed-size batch 3 times at these launch "
                          "parameters and print the times (for profilers)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="only copy one buffer of this many GiB to another (counter calibration)")
@@ -936,7 +936,7 @@ def main() -> int:
     if not args.no_extras:
         only = set(filter(None, args.only.split(",")))
         jobs = [("study", lambda: extra_study(dev)),
-                ("config0", lambda: extra_config0(dev, checks)),
+                ("small", lambda: extra_small(dev, checks)),
                 ("batch", lambda: extra_batch(dev, 3, checks)),
                 ("f32r", lambda: extra_f32r(dev, 3, ms, checks)),
                 ("adjoint", lambda: extra_adjoint(dev, 3)),

@@ -24,13 +24,13 @@ def oracle_plain(oracle, A, B, C, d):
 
 
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
-def test_config0_full_batch(cuda, oracle, layout):
+def test_small_case_full_batch(cuda, oracle, layout):
     """1024 systems, m = 3, n = 64, diagonally dominant rows with permuted block rows."""
     A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 2024, 3, 1024, 64)
     x, sing = gpu_solve(A, B, C, d, int(layout))
     assert not sing
     err = rel_err(x, oracle_plain(oracle, A, B, C, d))
-    print(f"config0 layout={layout.name} max rel err {err.max():.3e}")
+    print(f"small case layout={layout.name} max rel err {err.max():.3e}")
     assert err.max() <= TOL
 
 
@@ -410,7 +410,7 @@ def test_two_stage_one_lane_per_partition(cuda, oracle, m, layout, nsys):
 
 
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
-def test_config0_literal_generator_full_batch(cuda, oracle, layout):
+def test_small_case_literal_generator_full_batch(cuda, oracle, layout):
     """The small configuration as its text states it (GenKind.BPERM: only B's rows permuted):
     1024 systems, m = 3, n = 64, every system against the oracle (<= 1e-12) and, since these rows
     are no longer diagonally dominant, both against the binary128 truth (the device solve as
@@ -422,7 +422,7 @@ def test_config0_literal_generator_full_batch(cuda, oracle, layout):
     xt, info = oracle.solve_truth(cpu_layout(A), cpu_layout(B), cpu_layout(C), d)
     assert info == 0
     e_o, e_t, e_ot = rel_err(x, xo).max(), rel_err(x, xt).max(), rel_err(xo, xt).max()
-    print(f"literal config0 {layout.name}: vs oracle {e_o:.2e}, vs truth {e_t:.2e}, oracle vs truth {e_ot:.2e}")
+    print(f"literal small case {layout.name}: vs oracle {e_o:.2e}, vs truth {e_t:.2e}, oracle vs truth {e_ot:.2e}")
     assert e_o <= TOL
     assert e_t <= 10 * max(e_ot, 1e-16)
 
