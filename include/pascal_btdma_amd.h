@@ -203,6 +203,16 @@ btd_status btd_solve_cyclic(btd_plan_t plan, double *a, const double *b, double 
 btd_status btd_solve_noncyclic_save(btd_plan_t plan, const double *a, const double *b, double *c,
                                     double *d, double *factors, uint64_t *pivots, void *stream);
 
+/* btd_solve_noncyclic_save with c and d only read (no counterpart in the reference): C'_q goes to
+ * c_out and x to x (each the size of c and d), so a caller that keeps its inputs (the
+ * differentiable solve) copies nothing first. c_out may be c and x may be d (then it is
+ * btd_solve_noncyclic_save); other overlaps are not allowed. Same x and C'_q (q < nrow - 1) bit
+ * for bit; out of place, c_out's last row (no C' there: the last row's C is ignored) and the
+ * tiled layout's padding lanes of c_out and x are written with zeros. */
+btd_status btd_solve_noncyclic_save_out(btd_plan_t plan, const double *a, const double *b, const double *c,
+                                        const double *d, double *c_out, double *x, double *factors,
+                                        uint64_t *pivots, void *stream);
+
 /* Adjoint of the solve for L(x) with dL/dx = w (no counterpart in the reference):
  * lambda = T^{-T} w from the factors of btd_solve_noncyclic_save (a: the original A,
  * c_factored: C' as returned in c, x: the solution), then dd = lambda and the banded

@@ -21,7 +21,7 @@ __all__ = [
     "Layout", "Form", "GenKind", "BtdmaGpuPlan", "BtdmaSolveTimings", "BtdmaError", "HipError",
     "SingularError", "solve_noncyclic", "solve_noncyclic_profiled", "solve_cyclic",
     "matrix_index", "vector_index", "partition_1d", "layout_elems", "convert_layout",
-    "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save",
+    "generate", "generate_f32", "generate_host", "kMaxBlockSize", "solve_noncyclic_save", "solve_noncyclic_save_out",
     "solve_adjoint", "solve_noncyclic_f32r", "f32r_work_elems", "solve_transposed_f32r", "solve_noncyclic_f32s",
     "outer_grads_f32", "euler_assemble", "euler_sweep_cyclic", "weno_assemble", "weno_sweep_cyclic", "weno_gather", "weno_scatter",
     "euler_sweep_work_elems", "sweep_gather", "sweep_scatter", "sweep_shape", "BtdmaBatch", "Graph",
@@ -309,6 +309,23 @@ def solve_noncyclic_save(plan: BtdmaGpuPlan, a, b, c, d, factors, pivots, stream
         _validate(plan, a, b, c, d, plan.m, plan.nsys, plan.nrow)
     _check(_capi.lib().btd_solve_noncyclic_save(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
                                                 _ptr(factors), _ptr(pivots), _stream(stream)))
+
+
+def solve_noncyclic_save_out(plan: BtdmaGpuPlan, a, b, c, d, c_out, x, factors, pivots,
+                             stream=None) -> None:
+    """solve_noncyclic_save with c and d only read: C' goes to c_out and x to x (c_out may be
+    c and x may be d; no other overlap)."""
+    import torch
+    if plan.created:
+        nm, nv1 = plan.matrix_elems(), layout_elems(plan.layout, plan.nsys, plan.nrow, 1)
+        _check_tensor("factors", factors, nm, torch.float64)
+        _check_tensor("pivots", pivots, nv1, torch.int64)
+        _check_tensor("c_out", c_out, nm, torch.float64)
+        _check_tensor("x", x, plan.vector_elems(), torch.float64)
+        _validate(plan, a, b, c, d, plan.m, plan.nsys, plan.nrow)
+    _check(_capi.lib().btd_solve_noncyclic_save_out(plan._h, _ptr(a), _ptr(b), _ptr(c), _ptr(d),
+                                                    _ptr(c_out), _ptr(x), _ptr(factors), _ptr(pivots),
+                                                    _stream(stream)))
 
 
 def solve_adjoint(plan: BtdmaGpuPlan, a, c_factored, factors, pivots, x, w, lam, da, db, dc,

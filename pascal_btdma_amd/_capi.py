@@ -35,6 +35,7 @@ SIGNATURES = {
                                           ctypes.POINTER(BtdTimings), _vp]),
     "btd_solve_cyclic": (_i, [_vp, _dp, _dp, _dp, _dp, _i, _i, _i, _vp]),
     "btd_solve_noncyclic_save": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
+    "btd_solve_noncyclic_save_out": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "btd_solve_adjoint": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "btd_solve_noncyclic_f32r": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _i, _vp]),
     "btd_solve_noncyclic_f32s": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _vp]),

@@ -17,7 +17,7 @@ from __future__ import annotations
 import torch
 
 from . import (BtdmaGpuPlan, f32r_work_elems, layout_elems, outer_grads_f32, solve_adjoint,
-               solve_noncyclic_f32r, solve_noncyclic_save, solve_transposed_f32r)
+               solve_noncyclic_f32r, solve_noncyclic_save_out, solve_transposed_f32r)
 
 
 class BlockTridiagonalSolve(torch.autograd.Function):
@@ -26,12 +26,13 @@ class BlockTridiagonalSolve(torch.autograd.Function):
         if plan.npart != 1:
             raise ValueError("the differentiable solve needs a plan with npart == 1")
         a, b = a.contiguous(), b.contiguous()
-        cw = c.detach().clone()       # the solve overwrites C with C'
-        x = d.detach().clone()        # and d with x
+        c, d = c.detach().contiguous(), d.detach().contiguous()
+        cw = torch.empty_like(c)      # C' (c and d are only read)
+        x = torch.empty_like(d)
         factors = torch.empty_like(cw)
         pivots = torch.empty(layout_elems(plan.layout, plan.nsys, plan.nrow, 1),
                              dtype=torch.int64, device=d.device)
-        solve_noncyclic_save(plan, a.detach(), b.detach(), cw, x, factors, pivots)
+        solve_noncyclic_save_out(plan, a.detach(), b.detach(), c, d, cw, x, factors, pivots)
         ctx.plan = plan
         ctx.save_for_backward(a.detach(), cw, factors, pivots, x)
         return x

@@ -725,7 +725,21 @@ btd_status btd_solve_noncyclic_save(btd_plan_t plan, const double* a, const doub
     if (!a || !b || !c || !d || !factors || !pivots)
         return fail(BTD_ERR_INVALID_ARG, "btd_solve_noncyclic_save: NULL array");
     plan->last_stream = (hipStream_t)stream;
-    BTD_HIP(btd::launch_plain_save(plan->m, plan->layout, a, b, c, d, factors, pivots, plan->nsys,
+    BTD_HIP(btd::launch_plain_save(plan->m, plan->layout, a, b, c, c, d, d, factors, pivots, plan->nsys,
+                                   plan->nrow, plan->d_flag, (hipStream_t)stream));
+    return BTD_OK;
+}
+
+btd_status btd_solve_noncyclic_save_out(btd_plan_t plan, const double* a, const double* b, const double* c,
+                                        const double* d, double* c_out, double* x, double* factors,
+                                        uint64_t* pivots, void* stream) {
+    g_err.clear();
+    btd_status st = check_single_partition(plan, "btd_solve_noncyclic_save_out");
+    if (st != BTD_OK) return st;
+    if (!a || !b || !c || !d || !c_out || !x || !factors || !pivots)
+        return fail(BTD_ERR_INVALID_ARG, "btd_solve_noncyclic_save_out: NULL array");
+    plan->last_stream = (hipStream_t)stream;
+    BTD_HIP(btd::launch_plain_save(plan->m, plan->layout, a, b, c, c_out, d, x, factors, pivots, plan->nsys,
                                    plan->nrow, plan->d_flag, (hipStream_t)stream));
     return BTD_OK;
 }

@@ -108,7 +108,8 @@ hipError_t launch_cyclic_10(int, double*, const double*, double*, double*, int, 
 hipError_t launch_generate_10(int, uint64_t, int, int, int, double*, double*, double*, double*,
                              float*, float*, float*, hipStream_t);
 
-hipError_t launch_plain_save_1(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_1(int, const double*, const double*, const double*, double*, const double*,
+                               double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_1(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -122,7 +123,8 @@ hipError_t launch_plain_f32s_1(int, const float*, const float*, const float*, do
 hipError_t launch_plain_ref_1(int, const double*, const double*, const double*, double*, double*, int, int, int*,
                               hipStream_t);
 
-hipError_t launch_plain_save_2(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_2(int, const double*, const double*, const double*, double*, const double*,
+                               double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_2(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -136,7 +138,8 @@ hipError_t launch_plain_f32s_2(int, const float*, const float*, const float*, do
 hipError_t launch_plain_ref_2(int, const double*, const double*, const double*, double*, double*, int, int, int*,
                               hipStream_t);
 
-hipError_t launch_plain_save_3(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_3(int, const double*, const double*, const double*, double*, const double*,
+                               double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_3(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -150,7 +153,8 @@ hipError_t launch_plain_f32s_3(int, const float*, const float*, const float*, do
 hipError_t launch_plain_ref_3(int, const double*, const double*, const double*, double*, double*, int, int, int*,
                               hipStream_t);
 
-hipError_t launch_plain_save_4(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_4(int, const double*, const double*, const double*, double*, const double*,
+                               double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_4(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -164,7 +168,8 @@ hipError_t launch_plain_f32s_4(int, const float*, const float*, const float*, do
 hipError_t launch_plain_ref_4(int, const double*, const double*, const double*, double*, double*, int, int, int*,
                               hipStream_t);
 
-hipError_t launch_plain_save_5(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_5(int, const double*, const double*, const double*, double*, const double*,
+                               double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_5(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -178,7 +183,8 @@ hipError_t launch_plain_f32s_5(int, const float*, const float*, const float*, do
 hipError_t launch_plain_ref_5(int, const double*, const double*, const double*, double*, double*, int, int, int*,
                               hipStream_t);
 
-hipError_t launch_plain_save_6(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_6(int, const double*, const double*, const double*, double*, const double*,
+                               double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_6(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -192,7 +198,8 @@ hipError_t launch_plain_f32s_6(int, const float*, const float*, const float*, do
 hipError_t launch_plain_ref_6(int, const double*, const double*, const double*, double*, double*, int, int, int*,
                               hipStream_t);
 
-hipError_t launch_plain_save_7(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_7(int, const double*, const double*, const double*, double*, const double*,
+                               double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_7(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -206,7 +213,8 @@ hipError_t launch_plain_f32s_7(int, const float*, const float*, const float*, do
 hipError_t launch_plain_ref_7(int, const double*, const double*, const double*, double*, double*, int, int, int*,
                               hipStream_t);
 
-hipError_t launch_plain_save_8(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_8(int, const double*, const double*, const double*, double*, const double*,
+                               double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_8(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -220,7 +228,8 @@ hipError_t launch_plain_f32s_8(int, const float*, const float*, const float*, do
 hipError_t launch_plain_ref_8(int, const double*, const double*, const double*, double*, double*, int, int, int*,
                               hipStream_t);
 
-hipError_t launch_plain_save_9(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_9(int, const double*, const double*, const double*, double*, const double*,
+                               double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_9(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -234,7 +243,8 @@ hipError_t launch_plain_f32s_9(int, const float*, const float*, const float*, do
 hipError_t launch_plain_ref_9(int, const double*, const double*, const double*, double*, double*, int, int, int*,
                               hipStream_t);
 
-hipError_t launch_plain_save_10(int, const double*, const double*, double*, double*, double*,
+hipError_t launch_plain_save_10(int, const double*, const double*, const double*, double*, const double*,
+                                double*, double*,
                                uint64_t*, int, int, int*, hipStream_t);
 hipError_t launch_adjoint_10(int, const double*, const double*, const double*, const uint64_t*,
                             const double*, const double*, double*, double*, double*, double*, int,
@@ -309,10 +319,10 @@ hipError_t launch_generate(int m, int kind, uint64_t seed, int layout, int nsys,
 #undef CALL
 }
 
-hipError_t launch_plain_save(int m, int layout, const double* A, const double* B, double* C,
-                             double* D, double* F, uint64_t* PV, int nsys, int nrow, int* flag,
-                             hipStream_t s) {
-#define CALL(M) launch_plain_save_##M(layout, A, B, C, D, F, PV, nsys, nrow, flag, s)
+hipError_t launch_plain_save(int m, int layout, const double* A, const double* B, const double* C,
+                             double* Cw, const double* Din, double* D, double* F, uint64_t* PV, int nsys,
+                             int nrow, int* flag, hipStream_t s) {
+#define CALL(M) launch_plain_save_##M(layout, A, B, C, Cw, Din, D, F, PV, nsys, nrow, flag, s)
     BTD_DISPATCH_M(m, CALL)
 #undef CALL
 }

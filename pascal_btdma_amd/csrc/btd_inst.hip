@@ -42,10 +42,10 @@ hipError_t BTD_NAME(launch_generate_)(int kind, uint64_t seed, int layout, int n
     return launch_generate_m<BTD_M>(kind, seed, layout, nsys, nrow, A, B, C, D, Af, Bf, Cf, s);
 }
 
-hipError_t BTD_NAME(launch_plain_save_)(int layout, const double* A, const double* B, double* C,
-                                        double* D, double* F, uint64_t* PV, int nsys, int nrow,
-                                        int* flag, hipStream_t s) {
-    return launch_plain_save_m<BTD_M>(layout, A, B, C, D, F, PV, nsys, nrow, flag, s);
+hipError_t BTD_NAME(launch_plain_save_)(int layout, const double* A, const double* B, const double* C,
+                                        double* Cw, const double* Din, double* D, double* F, uint64_t* PV,
+                                        int nsys, int nrow, int* flag, hipStream_t s) {
+    return launch_plain_save_m<BTD_M>(layout, A, B, C, Cw, Din, D, F, PV, nsys, nrow, flag, s);
 }
 
 hipError_t BTD_NAME(launch_adjoint_)(int layout, const double* A, const double* Cp,

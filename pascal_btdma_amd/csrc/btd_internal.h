@@ -18,9 +18,9 @@ hipError_t launch_plain(int m, int layout, const double* A, const double* B, dou
 hipError_t launch_plain_twisted(int m, int layout, const double* A, const double* B, double* C, double* D,
                                 int nsys, int nrow, int* flag, hipStream_t s);
 
-hipError_t launch_plain_save(int m, int layout, const double* A, const double* B, double* C,
-                             double* D, double* F, uint64_t* PV, int nsys, int nrow, int* flag,
-                             hipStream_t s);
+hipError_t launch_plain_save(int m, int layout, const double* A, const double* B, const double* C,
+                             double* Cw, const double* Din, double* D, double* F, uint64_t* PV, int nsys,
+                             int nrow, int* flag, hipStream_t s);
 
 hipError_t launch_adjoint(int m, int layout, const double* A, const double* Cp, const double* F,
                           const uint64_t* PV, const double* X, const double* W, double* Lam,

@@ -264,25 +264,34 @@ __device__ __forceinline__ void report_singular(int* flag, int64_t sys) {
 }
 
 // Blocks stored as TB (double, or float for the FP32-storage solve, which computes in FP64);
-// C'_q goes to Cw (C itself for the in-place FP64 solve). ORD: every block operation in the
+// C'_q goes to Cw (C itself for the in-place FP64 solve); d is read from Din and d', then x,
+// written to D (Din == D in place). ORD: every block operation in the
 // oracle's order and rounding (gemm_sub_ref, lu_factor_ref, lu_solve_ref, and the back
 // substitution as x_q - (C'_q x_{q+1}) with the product summed from its first term, unfused),
 // so x equals the oracle's btdma_many bit for bit.
 template <int M, int LAYOUT, bool SAVE, typename TB, bool ORD = false>
 __device__ __forceinline__ void plain_solve(const TB* __restrict__ A, const TB* __restrict__ B,
-                                            const TB* C, double* Cw, double* D,
+                                            const TB* C, double* Cw, const double* Din, double* D,
                                             double* __restrict__ F, uint64_t* __restrict__ PV,
                                             int nsys, int nrow, int* __restrict__ flag) {
     static_assert(!(SAVE && ORD), "the adjoint's saved factors use the default order");
     const int64_t sys = (int64_t)blockIdx.x * 64 + threadIdx.x;
-    if (sys >= nsys) return;
+    if (sys >= nsys) {
+        // out of place (the forward of the differentiable solve), the outputs' padding lanes of
+        // the tiled layout get zeros instead of whatever their allocation held
+        if constexpr (SAVE) {
+            if ((const void*)D != (const void*)Din) zero_pad_lane<M, LAYOUT>(D, sys, nsys, nrow);
+            if ((const void*)Cw != (const void*)C) zero_pad_lane<M * M, LAYOUT>(Cw, sys, nsys, nrow);
+        }
+        return;
+    }
     const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M), ap(sys, nsys, nrow, 1);
     double Cp[M][M], dp[M];
     bool ok = true;
     for (int64_t q = 0; q < nrow; ++q) {
         double R[M][M], v[M];
         load_mat<M>(B, am, q, R);
-        load_vec<M>(D, av, q, v);
+        load_vec<M>(Din, av, q, v);
         if (q > 0) {
             if constexpr (ORD)
                 gemm_sub_ref<M>(A, am, q, Cp, dp, R, v);
@@ -322,6 +331,12 @@ __device__ __forceinline__ void plain_solve(const TB* __restrict__ A, const TB* 
                 for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
                 store_col<M>(Cw, am, q, j, c);
             }
+        } else if constexpr (SAVE) {
+            // the last row has no C' (its C is ignored); out of place its block gets zeros
+            if ((const void*)Cw != (const void*)C) {
+                const double z[M][M] = {};
+                store_mat<M>(Cw, am, q, z);
+            }
         }
     }
     // back substitution; dp holds x_{n-1}
@@ -353,7 +368,7 @@ template <int M, int LAYOUT>
 __global__ __launch_bounds__(64) void k_plain(const double* __restrict__ A,
                                               const double* __restrict__ B, const double* C, double* Cw,
                                               double* D, int nsys, int nrow, int* __restrict__ flag) {
-    plain_solve<M, LAYOUT, false, double, true>(A, B, C, Cw, D, nullptr, nullptr, nsys, nrow, flag);
+    plain_solve<M, LAYOUT, false, double, true>(A, B, C, Cw, D, D, nullptr, nullptr, nsys, nrow, flag);
 }
 
 // FP32-stored blocks, FP64 arithmetic: the FP64 solve of the blocks' values (bit for bit what
@@ -364,16 +379,18 @@ __global__ __launch_bounds__(64) void k_plain_f32s(const float* __restrict__ A,
                                                    const float* __restrict__ B, const float* C,
                                                    double* Cw, double* D, int nsys, int nrow,
                                                    int* __restrict__ flag) {
-    plain_solve<M, LAYOUT, false, float, true>(A, B, C, Cw, D, nullptr, nullptr, nsys, nrow, flag);
+    plain_solve<M, LAYOUT, false, float, true>(A, B, C, Cw, D, D, nullptr, nullptr, nsys, nrow, flag);
 }
 
+// The forward solve that keeps the block factors for the adjoint. C' goes to Cw and d', then x,
+// to D, with d read from Din: in place with Cw == C and Din == D, or with C and Din only read.
 template <int M, int LAYOUT>
 __global__ __launch_bounds__(64) void k_plain_save(const double* __restrict__ A,
-                                                   const double* __restrict__ B, double* C,
-                                                   double* D, double* __restrict__ F,
-                                                   uint64_t* __restrict__ PV, int nsys, int nrow,
-                                                   int* __restrict__ flag) {
-    plain_solve<M, LAYOUT, true, double>(A, B, C, C, D, F, PV, nsys, nrow, flag);
+                                                   const double* __restrict__ B, const double* C,
+                                                   double* Cw, const double* Din, double* D,
+                                                   double* __restrict__ F, uint64_t* __restrict__ PV,
+                                                   int nsys, int nrow, int* __restrict__ flag) {
+    plain_solve<M, LAYOUT, true, double>(A, B, C, Cw, Din, D, F, PV, nsys, nrow, flag);
 }
 
 // --------------------------------------------------------------------------- adjoint --
@@ -2244,16 +2261,16 @@ hipError_t launch_plain_m(int layout, const double* A, const double* B, double* 
 }
 
 template <int M>
-hipError_t launch_plain_save_m(int layout, const double* A, const double* B, double* C, double* D,
-                               double* F, uint64_t* PV, int nsys, int nrow, int* flag,
-                               hipStream_t s) {
+hipError_t launch_plain_save_m(int layout, const double* A, const double* B, const double* C, double* Cw,
+                               const double* Din, double* D, double* F, uint64_t* PV, int nsys, int nrow,
+                               int* flag, hipStream_t s) {
     const dim3 grid((nsys + 63) / 64), block(64);
     if (layout == LAYOUT_TILED)
-        hipLaunchKernelGGL((k_plain_save<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, F, PV, nsys,
-                           nrow, flag);
+        hipLaunchKernelGGL((k_plain_save<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, Cw, Din, D, F, PV,
+                           nsys, nrow, flag);
     else
-        hipLaunchKernelGGL((k_plain_save<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, F, PV, nsys,
-                           nrow, flag);
+        hipLaunchKernelGGL((k_plain_save<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, Cw, Din, D, F, PV,
+                           nsys, nrow, flag);
     return hipGetLastError();
 }
 

@@ -160,6 +160,7 @@ def test_new_entry_points_reject_an_uncreated_plan():
     calls = {
         "btd_solve_noncyclic_f32s": (null, 1, 1, 1, 1, 1, null),
         "btd_plan_place_workspace_f32s": (null, 1, 1, 1, 1, 2, None, null),
+        "btd_solve_noncyclic_save_out": (null, 1, 1, 1, 1, 1, 1, 1, 1, null),
         "btd_solve_transposed_f32r": (null, 1, 1, 1, 1, 1, 1, 2, null),
         "btd_outer_grads_f32": (null, 1, 1, 1, 1, 1, null),
         "btd_euler_assemble": (null, 0, 1, 1, 4, 4, 4, 0.1, 0.1, 0.1, 1, 1, 1, 1, null),

@@ -59,6 +59,47 @@ def test_adjoint_matches_transposed_oracle(cuda, oracle, m, layout):
     assert np.all(layout_to_mat(dc.cpu().numpy(), lay, n, nsys, m)[-1] == 0)
 
 
+@pytest.mark.parametrize("m", [2, 5])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_save_out_equals_the_in_place_save(cuda, m, layout):
+    """solve_noncyclic_save_out (c and d only read, C' and x to their own arrays) gives the
+    in-place solve_noncyclic_save's C', x, factors and pivots bit for bit and leaves c, d as
+    they were; c_out = c, x = d is the in-place call itself. The out-of-place outputs start
+    NaN-filled: their tiled padding lanes must come back zero (the generated inputs' are), and
+    so must c_out's last row (the generator's C of the last row is zero, and the in-place call
+    leaves it there)."""
+    import torch
+    nsys, n = 100, 17
+    t = [torch.empty(pb.layout_elems(layout, nsys, n, m * m if k < 3 else m), dtype=torch.float64, device=cuda)
+         for k in range(4)]
+    pb.generate(pb.GenKind.DDPERM, 41 + m, m, nsys, n, layout, *t)
+    a, b, c, d = t
+    c0, d0 = c.clone(), d.clone()
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
+        mk = lambda: (torch.empty_like(c), torch.empty(pb.layout_elems(layout, nsys, n, 1), dtype=torch.int64,
+                                                        device=cuda))
+        F1, P1 = mk()
+        ci, di = c.clone(), d.clone()
+        pb.solve_noncyclic_save(plan, a, b, ci, di, F1, P1)
+        F2, P2 = mk()
+        co, xo = torch.full_like(c, float("nan")), torch.full_like(d, float("nan"))
+        pb.solve_noncyclic_save_out(plan, a, b, c, d, co, xo, F2, P2)
+        F3, P3 = mk()
+        cs, ds = c.clone(), d.clone()
+        pb.solve_noncyclic_save_out(plan, a, b, cs, ds, cs, ds, F3, P3)
+        torch.cuda.synchronize()
+        assert not plan.check()
+    assert torch.equal(c, c0) and torch.equal(d, d0)
+    for got in ((co, xo, F2, P2), (cs, ds, F3, P3)):
+        for k, (g, r) in enumerate(zip(got, (ci, di, F1, P1))):
+            if k < 2:
+                assert torch.equal(g, r)
+            else:  # factors and pivots: the real systems' (their padding lanes are not written)
+                lay, E = int(layout), (m * m if k == 2 else 1)
+                view = lambda X: layout_to_vec(X.cpu().numpy(), lay, n, nsys, E)
+                np.testing.assert_array_equal(view(g), view(r))
+
+
 def fd_check(oracle, base, w, grads, count, seed, names=("A", "B", "C", "d")):
     """Largest |FD - g| / max(1, |g|) over `count` random entries (A of rows >= 1, C of rows
     <= n-2, B, d), the FD taken of the oracle's FP64 solve of that entry's system, loss
