@@ -371,6 +371,120 @@ __global__ __launch_bounds__(64) void k_plain(const double* __restrict__ A,
     plain_solve<M, LAYOUT, false, double, true>(A, B, C, Cw, D, D, nullptr, nullptr, nsys, nrow, flag);
 }
 
+// Single-partition solve with two lanes per system, one from each end, blocks in registers:
+// the two wavefronts of a 128-thread workgroup take the same 64 systems (one tile). With
+// h = (n-1)/2, the top wavefront eliminates rows 0..h-1 downwards as k_plain does (C'_q =
+// R_q^{-1} C_q over C_q, d'_q over d_q), the bottom one rows n-1..h+1 upwards (S_q = B_q -
+// C_q A''_{q+1}, A''_q = S_q^{-1} A_q over C_q, which row q has just consumed, d''_q =
+// S_q^{-1}(d_q - C_q d''_{q+1}) over d_q). They exchange their carried block and vector
+// through LDS, both solve the meeting row
+//   (B_h - A_h C'_{h-1} - C_h A''_{h+1}) x_h = d_h - A_h d'_{h-1} - C_h d''_{h+1}
+// by the same operations in the same order, and substitute outwards: x_q = d'_q - C'_q x_{q+1}
+// (q < h), x_q = d''_q - A''_q x_{q-1} (q > h). Half the dependent rows of k_plain per lane for
+// the same bytes (C' / A'' and d' / d'' written and read back once, A not written): for batches
+// too small to fill the device one lane per system. A of row 0 and C of row n-1 are not read.
+template <int M, int LAYOUT>
+__global__ __launch_bounds__(2 * kTile) void k_plain_halves(const double* __restrict__ A,
+                                                           const double* __restrict__ B, double* C,
+                                                           double* D, int nsys, int nrow,
+                                                           int* __restrict__ flag) {
+    constexpr int X = M * M + M;  // LDS slots of a wavefront's carry
+    __shared__ double xs[2][X * kTile];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);  // 0 top, 1 bottom
+    const int lane = threadIdx.x % kTile;
+    const bool top = wv == 0;
+    const int64_t sys = (int64_t)blockIdx.x * kTile + lane;
+    if (sys >= nsys) return;  // a system's two lanes leave together: both wavefronts map lanes alike
+    const Addr<LAYOUT> am(sys, nsys, nrow, M * M), av(sys, nsys, nrow, M);
+    const int h = (nrow - 1) / 2;
+    const int L = top ? h : nrow - 1 - h;  // rows this lane eliminates before the meeting
+    const int64_t g0 = top ? 0 : nrow - 1, dg = top ? 1 : -1;
+    const double* Ar = top ? A : C;  // couples row g to the row eliminated before it
+    const double* Cr = top ? C : A;  // to the row eliminated after it
+    double Cp[M][M], dp[M];          // the carry: C'_{g} / A''_{g} and d'_{g} / d''_{g}
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        dp[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) Cp[i][j] = 0.0;
+    }
+    bool ok = true;
+    for (int t = 0; t < L; ++t) {
+        const int64_t g = g0 + t * dg;
+        double R[M][M], v[M];
+        load_mat<M>(B, am, g, R);
+        load_vec<M>(D, av, g, v);
+        if (t > 0) gemm_sub<M>(Ar, am, g, Cp, dp, R, v);
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, v);
+#pragma unroll
+        for (int i = 0; i < M; ++i) dp[i] = v[i];
+        store_vec<M>(D, av, g, dp);
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double c[M];
+            load_col<M>(Cr, am, g, j, c);
+            lu_solve<M>(R, piv, c);
+#pragma unroll
+            for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
+            store_col<M>(C, am, g, j, c);
+        }
+    }
+    // the meeting row: each wavefront posts its carry and reads its partner's
+    double* mine = xs[wv];
+    const double* other = xs[1 - wv];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        mine[(M * M + i) * kTile + lane] = dp[i];
+#pragma unroll
+        for (int j = 0; j < M; ++j) mine[(i * M + j) * kTile + lane] = Cp[i][j];
+    }
+    __syncthreads();
+    double Yt[M][M], yt[M], Yb[M][M], yb[M];  // top's C'_{h-1}, d'_{h-1}; bottom's A''_{h+1}, d''_{h+1}
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        yt[i] = top ? dp[i] : other[(M * M + i) * kTile + lane];
+        yb[i] = top ? other[(M * M + i) * kTile + lane] : dp[i];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            Yt[i][j] = top ? Cp[i][j] : other[(i * M + j) * kTile + lane];
+            Yb[i][j] = top ? other[(i * M + j) * kTile + lane] : Cp[i][j];
+        }
+    }
+    double xh[M];
+    {
+        double R[M][M];
+        load_mat<M>(B, am, h, R);
+        load_vec<M>(D, av, h, xh);
+        if (h > 0) gemm_sub<M>(A, am, h, Yt, yt, R, xh);
+        if (h < nrow - 1) gemm_sub<M>(C, am, h, Yb, yb, R, xh);
+        int piv[M];
+        ok = lu_factor<M>(R, piv) && ok;
+        lu_solve<M>(R, piv, xh);
+    }
+    __syncthreads();  // both wavefronts have read D_h and C_h before the top one writes x_h
+    if (top) store_vec<M>(D, av, h, xh);
+    // outward substitution from the meeting row
+    double xn[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) xn[i] = xh[i];
+    for (int t = L - 1; t >= 0; --t) {
+        const int64_t g = g0 + t * dg;
+        double x[M], c[M][M];
+        load_vec<M>(D, av, g, x);
+        load_mat<M>(static_cast<const double*>(C), am, g, c);
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) x[i] = fma(-c[i][j], xn[j], x[i]);
+#pragma unroll
+        for (int i = 0; i < M; ++i) xn[i] = x[i];
+        store_vec<M>(D, av, g, x);
+    }
+    if (!ok) report_singular(flag, sys);
+}
+
 // FP32-stored blocks, FP64 arithmetic: the FP64 solve of the blocks' values (bit for bit what
 // k_plain computes on their FP64 copies, and so the oracle's solve of those values), reading 4
 // bytes per block element; C' (FP64) goes to the workspace Cw, x replaces D.
@@ -2227,7 +2341,10 @@ void launch_plain_coop(const double* A, const double* B, double* C, double* D, i
 #define BTD_PLAIN_TWISTED_FROM 2
 #endif
 constexpr int kPlainTwistedFrom = BTD_PLAIN_TWISTED_FROM;
-constexpr int kPlainLaneFrom = 32768;
+#ifndef BTD_PLAIN_LANE_FROM
+#define BTD_PLAIN_LANE_FROM 32768
+#endif
+constexpr int kPlainLaneFrom = BTD_PLAIN_LANE_FROM;
 
 template <int M>
 void launch_plain_lane(int layout, const double* A, const double* B, const double* C, double* Cw, double* D,
@@ -2239,6 +2356,54 @@ void launch_plain_lane(int layout, const double* A, const double* B, const doubl
         hipLaunchKernelGGL((k_plain<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, Cw, D, nsys, nrow, flag);
 }
 
+// Batches on which the single-partition solve runs two lanes per system from both ends with
+// the blocks in registers (k_plain_halves): m = 3, 5, 6, 7 from 16384 systems, m = 2, 4 from
+// 32768; below 131072 systems at m = 2-4, below 98304 at m = 5, at every size from m = 6.
+// Outside that, the cooperative twisted solve (2m lanes per system) runs below 32768 systems
+// and one lane per system (k_plain) from 32768. Measured on 1024-262144 systems of 256-2048
+// rows, both layouts:
+//   - against the twisted solve below 32768 systems: halves faster from 16384 systems at m = 3,
+//     5, 6, 7 (m = 5 16384 x 2048 12.4 -> 8.6 ms tiled, 15.1 -> 12.5 reference; m = 7 22.5 ->
+//     17.9 ms tiled), slower below 8192 (one lane per half leaves the device idle: m = 5
+//     1024 x 2048 2.1 -> 6.1 ms) and at m = 2, 4, whose twisted wavefronts read whole halves
+//     and quarters of lines (m = 2 16384 x 2048 1.4 -> 2.6 ms);
+//   - against one lane per system from 32768 systems: halves faster while one lane per system
+//     fills at most one wavefront per SIMD (m = 5 32768 x 2048 19.9 -> 13.1 ms tiled, 30.0 ->
+//     16.3 reference; m = 2 5.7 -> 3.1 ms tiled; m = 4 65536 x 1024 11.2 -> 8.9 ms); beyond,
+//     on the same placement of the arrays in HBM (four placements each), m = 5 98304 x 512 1.13x
+//     slower (tiled) and 131072 x 512 1.00-1.04x, m = 3 131072 x 512 0.95-1.00x, m = 2, 4
+//     131072 x 512 1.16-1.18x slower in the reference layout, while m = 6 stays 0.90-1.00x and
+//     m = 7 0.78-0.92x up to 262144 systems (one lane per system at m = 6, 7 runs long
+//     dependent chains at two wavefronts per SIMD). k_plain performs the oracle's operations
+//     bit for bit, so it keeps the sizes where the two are equal.
+#ifndef BTD_PLAIN_HALVES_FROM
+#define BTD_PLAIN_HALVES_FROM 16384
+#endif
+#ifndef BTD_PLAIN_HALVES_M_MASK
+#define BTD_PLAIN_HALVES_M_MASK 0xFC  // bit m set: m = 2 ... 7
+#endif
+template <int M>
+constexpr bool kPlainHalvesM = M <= 7 && ((BTD_PLAIN_HALVES_M_MASK >> M) & 1);
+template <int M>
+constexpr int kPlainHalvesFrom = (M == 2 || M == 4) ? 32768 : BTD_PLAIN_HALVES_FROM;
+#ifdef BTD_PLAIN_HALVES_BELOW
+template <int M>
+constexpr int kPlainHalvesBelow = BTD_PLAIN_HALVES_BELOW;
+#else
+template <int M>
+constexpr int kPlainHalvesBelow = M >= 6 ? 0x7fffffff : M == 5 ? 98304 : 131072;
+#endif
+
+template <int M>
+void launch_plain_halves(int layout, const double* A, const double* B, double* C, double* D, int nsys,
+                         int nrow, int* flag, hipStream_t s) {
+    const dim3 grid((nsys + kTile - 1) / kTile), block(2 * kTile);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    else
+        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+}
+
 template <int M>
 hipError_t launch_plain_m(int layout, const double* A, const double* B, double* C, double* D,
                                  int nsys, int nrow, int* flag, hipStream_t s) {
@@ -2248,7 +2413,9 @@ hipError_t launch_plain_m(int layout, const double* A, const double* B, double* 
         else
             launch_plain_coop<M, LAYOUT_REF_GPU>(A, B, C, D, nsys, nrow, flag, s);
     } else if constexpr (M >= 2 && M >= kPlainTwistedFrom) {
-        if (nsys >= kPlainLaneFrom)
+        if (kPlainHalvesM<M> && nsys >= kPlainHalvesFrom<M> && nsys < kPlainHalvesBelow<M>)
+            launch_plain_halves<M>(layout, A, B, C, D, nsys, nrow, flag, s);
+        else if (nsys >= kPlainLaneFrom)
             launch_plain_lane<M>(layout, A, B, C, C, D, nsys, nrow, flag, s);
         else if (layout == LAYOUT_TILED)
             launch_plain_twisted_auto<M, LAYOUT_TILED>(A, B, C, D, nsys, nrow, flag, s);

@@ -89,33 +89,24 @@ def test_f32r_rejects_bad_arguments(cuda):
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
 def test_fp32_storage_is_the_fp64_solve_of_the_stored_values(cuda, m, layout):
     """btd_solve_noncyclic_f32s (FP32 blocks, FP64 arithmetic, one lane per system) equals,
-    bit for bit, the FP64 solve btd_solve_noncyclic of the same blocks converted to FP64 where
-    that solve does the same arithmetic: one lane per system (m = 1; m <= 7 from 32768
-    systems); the blocks are left as they were. The FP64 solve of m >= 8 batches that fill the
-    device runs m lanes per system, and smaller batches (m >= 2) are solved twisted: other
-    elimination orders, which agree to rounding."""
-    import torch
+    bit for bit, the FP64 solve btd_solve_noncyclic of the same blocks converted to FP64 in the
+    BTD_FORM_LAPACK_ORDER form (one lane per system in the oracle's operation order, at every
+    size and m); the blocks are left as they were. The default form runs other elimination
+    orders by m and batch size (m lanes per system, 2m twisted, two lanes per system from both
+    ends), which agree to rounding."""
     n = 13
-    if m == 1:
-        nsys = 70
-    else:
-        cus, g = torch.cuda.get_device_properties(0).multi_processor_count, 64 // m
-        if m <= 7:
-            nsys = 32768 + 5
-        else:
-            nsys = ((8 if layout == pb.Layout.TILED else 4) if m == 8 else 2) * cus * g + 5
-        x32, x64 = _f32s_and_fp64(cuda, m, 70, n, layout)
-        assert rel_err(x32, x64).max() <= 1e-12
-    x32, x64 = _f32s_and_fp64(cuda, m, nsys, n, layout)
-    if m <= 7:
-        assert np.array_equal(x32, x64)
-    else:  # the FP64 solve runs m lanes per system here (its own elimination order)
-        assert rel_err(x32, x64).max() <= 1e-12
+    sizes = (70, 4096 + 5) if m > 1 else (70,)
+    for nsys in sizes:
+        x32, x64 = _f32s_and_fp64(cuda, m, nsys, n, layout, pb.Form.LAPACK_ORDER)
+        assert np.array_equal(x32, x64), f"nsys={nsys}"
+    x32, x64 = _f32s_and_fp64(cuda, m, sizes[-1], n, layout)
+    assert rel_err(x32, x64).max() <= 1e-12
 
 
-def _f32s_and_fp64(cuda, m, nsys, n, layout):
-    """x from the FP32-storage solve and from the FP64 solve of the same (FP32-rounded) blocks,
-    as natural-form arrays; asserts the FP32 blocks are unchanged."""
+def _f32s_and_fp64(cuda, m, nsys, n, layout, form=None):
+    """x from the FP32-storage solve and from the FP64 solve of the same (FP32-rounded) blocks
+    (in `form`, else the plan's default), as natural-form arrays; asserts the FP32 blocks are
+    unchanged."""
     import torch
     A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 300 + m, m, nsys, n)
     A, B, C = fp32_round(A, B, C)
@@ -129,6 +120,8 @@ def _f32s_and_fp64(cuda, m, nsys, n, layout):
         x32 = td.clone()
         pb.solve_noncyclic_f32s(plan, ta, tb, tc, x32, work)
         a64, b64, c64, x64 = ta.double(), tb.double(), tc.double(), td.clone()
+        if form is not None:
+            plan.set_form(form)
         pb.solve_noncyclic(plan, a64, b64, c64, x64, m, nsys, n)
         torch.cuda.synchronize()
         assert not plan.check()

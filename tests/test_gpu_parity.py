@@ -225,8 +225,11 @@ def _plain_regimes(m, layout, cus):
     g = 64 // m
     if m <= 7:
         lock_from = 2 if 3 <= m <= 6 else 4
-        return {"twisted, 1 wavefront": 70, "twisted, 4 locked wavefronts": lock_from * cus * (32 // m) + 3,
-                "one lane per system": 32768 + 5}
+        shapes = {"twisted, 1 wavefront": 70, "twisted, 4 locked wavefronts": lock_from * cus * (32 // m) + 3,
+                  "two lanes per system from both ends": (32768 if m in (2, 4) else 16384) + 5}
+        if m <= 5:
+            shapes["one lane per system"] = (98304 if m == 5 else 131072) + 5
+        return shapes
     if m == 8:
         lock_from = (8 if layout == pb.Layout.TILED else 4) * cus * g
         return {"twisted, 4 locked wavefronts": 4 * cus * 4 + 3, "cooperative, locked": lock_from + 5}
@@ -255,6 +258,31 @@ def test_plain_large_batch_shapes(cuda, oracle, m, layout):
         with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
             pb.solve_noncyclic(plan, ta, tb, tc, td, m, nsys, n)
             assert plan.info().nonzero()[0].tolist() == bad
+
+
+@pytest.mark.parametrize("m", [2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
+def test_plain_two_lanes_from_both_ends(cuda, oracle, m, layout):
+    """From 16384 systems (m = 3, 5, 6, 7) or 32768 (m = 2, 4) the single-partition solve gives
+    every system two lanes: one eliminates rows 0..h-1 downward, the other rows n-1..h+1 upward
+    (h = (n-1)/2), they meet at row h through LDS and substitute outward. n = 2 (no top rows),
+    3, 4 (one and two bottom rows), 7 and 64; sampled systems (the first tile, a middle one, the
+    partial last tile) against the oracle."""
+    import torch
+    nsys = (32768 if m in (2, 4) else 16384) + 5
+    for n in (2, 3, 4, 7, 64):
+        print(f"[case] m={m} {layout.name} nsys={nsys} n={n}", flush=True)
+        t = [torch.empty(pb.layout_elems(layout, nsys, n, m * m if k < 3 else m), dtype=torch.float64, device=cuda)
+             for k in range(4)]
+        pb.generate(pb.GenKind.DDPERM, 80 + m + n, m, nsys, n, layout, *t)
+        with pb.BtdmaGpuPlan().create(m, nsys, n, 1, layout) as plan:
+            pb.solve_noncyclic(plan, *t, m, nsys, n)
+            torch.cuda.synchronize()
+            assert not plan.check()
+        full = layout_to_vec(t[3].cpu().numpy(), int(layout), n, nsys, m)
+        for s0, cnt in ((0, 64), (64 * 128, 64), (nsys - 5, 5)):
+            A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 80 + m + n, m, nsys, n, s0, cnt)
+            assert rel_err(full[:, s0:s0 + cnt], oracle_plain(oracle, A, B, C, d)).max() <= TOL
 
 
 @pytest.mark.parametrize("m,nsys", [(9, 61), (10, 61), (10, 5)])
