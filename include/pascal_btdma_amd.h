@@ -100,7 +100,7 @@ btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npar
 
 /* New: the partition count a plan created with npart = 0 gets on one GPU, where partitions are
  * an occupancy choice rather than a rank count: 1 (the single-kernel solve) for nsys >= 32768
- * at m = 1, 20480 at m = 3, 5 (two lanes per system there), 8192 at m = 2, 4 and 4096 at
+ * at m = 1, 12288 at m = 3, 5 (two lanes per system there), 8192 at m = 2, 4 and 4096 at
  * m >= 6 (two, m or 2m lanes per system). Otherwise, for m <= 5 (both stages two lanes per partition): 1 below 128
  * rows when nsys < 8192, else about 16384 / nsys partitions (at least 2 from 8192 systems) of at
  * least 16 rows, rounded down to a power of two; for m = 6: 1 up to 128 rows, else about
@@ -116,12 +116,11 @@ btd_status btd_plan_npart(btd_plan_t plan, int *npart);
  * BTD_FORM_PLAIN (the default) is the library's choice: one lane per system for m = 1, for
  * m = 2-4 from 131072 systems and m = 5 from 98304; two lanes per system, one eliminating
  * downwards from row 0 and one upwards from row nrow-1 with the blocks in registers, meeting in
- * the middle, for m = 3, 5, 6, 7 from 16384 systems and m = 2, 4 from 32768 (up to those
- * one-lane sizes; at every size from 16384 for m = 6, 7); for m >= 8 m lanes per system (one
+ * the middle, from 24576 systems at m = 2, 12288 at m = 3-5, 8192 at m = 6 and 16384 at m = 7
+ * (up to those one-lane sizes; at every size from m = 6); for m >= 8 m lanes per system (one
  * block row each); and the twisted form below for batches too small to fill the device those
- * ways: 2 <= m <= 7 below 16384 / 32768 systems, m = 8 below 32 systems per CU (reference
- * layout) or 64 (tiled), i.e. 8192 / 16384 systems on 256 CUs, m = 9, 10 below 14 / 12
- * systems per CU. BTD_FORM_TWISTED (m >= 2, either layout) always gives each system two
+ * ways: 2 <= m <= 7 below those sizes, m = 8 below 32 systems per CU (reference layout) or 64
+ * (tiled), i.e. 8192 / 16384 systems on 256 CUs, m = 9, 10 below 14 / 12 systems per CU. BTD_FORM_TWISTED (m >= 2, either layout) always gives each system two
  * groups of m lanes, one eliminating downwards from row 0 and one upwards from row nrow-1,
  * meeting in the middle: half the row latency per system (on large batches its wavefronts
  * share cache lines that leave L2 between their reads, so it moves more bytes). Same

@@ -137,13 +137,12 @@ static int pow2_floor(int64_t p) {
 
 int btd_auto_npart(int m, int nsys, int nrow) {
     // One GPU: partitions are an occupancy choice. The single-kernel solve takes a batch that
-    // fills the device on its own: 32768 systems at m = 1 (one lane per system); 20480 for m = 3
-    // and 5, whose single-partition solve gives every system two lanes, one from each end, from
-    // 16384 systems (m = 3 / 5 at 20480-28672 systems of 256-2048 rows, both layouts: 0.74-0.91x
-    // the time of P = 2; at 16384 0.8-1.2x, so P = 2 stays there); 8192 for m = 2 and 4, whose
-    // twisted solve's wavefronts read whole halves or quarters of lines; 4096 for m >= 6, whose
-    // single-partition solve is twisted (m = 6; two lanes per system from 16384) or gives every
-    // system m or 2m lanes.
+    // fills the device on its own: 32768 systems at m = 1 (one lane per system); 12288 for m = 3
+    // and 5, whose single-partition solve gives every system two lanes there, one from each end
+    // (12288-24576 systems of 512 and 2048 rows, both layouts: 0.70-0.87x the time of the best
+    // P in 2-8; at 8192 0.97-1.16x, so P = 2 stays there); 8192 for m = 2 and 4, whose twisted
+    // solve's wavefronts read whole halves or quarters of lines; 4096 for m >= 6, whose
+    // single-partition solve is twisted or gives every system two, m or 2m lanes.
     // Smaller batches, measured over m = 2-8, 256-16384 systems, 64-2048 rows (P = 1..128, the
     // best against this rule: geometric mean 1.02x, against the one-lane rule 1.10x):
     //   m <= 5 (stage 1 and 3 twisted, two lanes per partition): about 16384 partitions,
@@ -153,7 +152,7 @@ int btd_auto_npart(int m, int nsys, int nrow) {
     //   m >= 7: the one-lane rule (auto_npart_one_lane).
     if (nsys < 1 || nrow < 2) return 1;
     if (m >= 7) return auto_npart_one_lane(m, nsys, nrow);
-    const int64_t one = m == 6 ? 4096 : (m == 2 || m == 4) ? 8192 : (m == 3 || m == 5) ? 20480 : 32768;
+    const int64_t one = m == 6 ? 4096 : (m == 2 || m == 4) ? 8192 : (m == 3 || m == 5) ? 12288 : 32768;
     if ((int64_t)nsys >= one) return 1;
     int64_t want, most;
     if (m == 6) {

@@ -371,6 +371,20 @@ __global__ __launch_bounds__(64) void k_plain(const double* __restrict__ A,
     plain_solve<M, LAYOUT, false, double, true>(A, B, C, Cw, D, D, nullptr, nullptr, nsys, nrow, flag);
 }
 
+// R <- R - X*Y and v <- v - X*y with X in registers: gemm_sub's operations in its order.
+template <int M>
+__device__ __forceinline__ void gemm_sub_reg(const double (&X)[M][M], const double (&Y)[M][M],
+                                             const double (&y)[M], double (&R)[M][M], double (&v)[M]) {
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            v[i] = fma(-X[i][k], y[k], v[i]);
+#pragma unroll
+            for (int j = 0; j < M; ++j) R[i][j] = fma(-X[i][k], Y[k][j], R[i][j]);
+        }
+}
+
 // Single-partition solve with two lanes per system, one from each end, blocks in registers:
 // the two wavefronts of a 128-thread workgroup take the same 64 systems (one tile). With
 // h = (n-1)/2, the top wavefront eliminates rows 0..h-1 downwards as k_plain does (C'_q =
@@ -382,12 +396,12 @@ __global__ __launch_bounds__(64) void k_plain(const double* __restrict__ A,
 // by the same operations in the same order, and substitute outwards: x_q = d'_q - C'_q x_{q+1}
 // (q < h), x_q = d''_q - A''_q x_{q-1} (q > h). Half the dependent rows of k_plain per lane for
 // the same bytes (C' / A'' and d' / d'' written and read back once, A not written): for batches
-// too small to fill the device one lane per system. A of row 0 and C of row n-1 are not read.
-template <int M, int LAYOUT>
-__global__ __launch_bounds__(2 * kTile) void k_plain_halves(const double* __restrict__ A,
-                                                           const double* __restrict__ B, double* C,
-                                                           double* D, int nsys, int nrow,
-                                                           int* __restrict__ flag) {
+// too small to fill the device one lane per system. A of row 0 and C of row n-1 are not used.
+template <int M, int LAYOUT, bool AHEAD>
+__global__ __launch_bounds__(2 * kTile, AHEAD ? 1 : 2) void k_plain_halves(const double* __restrict__ A,
+                                                                          const double* __restrict__ B,
+                                                                          double* C, double* D, int nsys,
+                                                                          int nrow, int* __restrict__ flag) {
     constexpr int X = M * M + M;  // LDS slots of a wavefront's carry
     __shared__ double xs[2][X * kTile];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);  // 0 top, 1 bottom
@@ -408,27 +422,73 @@ __global__ __launch_bounds__(2 * kTile) void k_plain_halves(const double* __rest
 #pragma unroll
         for (int j = 0; j < M; ++j) Cp[i][j] = 0.0;
     }
+    // AHEAD: the operands of the next row (after the last one, of the meeting row h) are loaded
+    // while the current row is eliminated, and the next row's d' and carried block while the
+    // current x is formed. Row h is written by neither wavefront before the meeting, and A of
+    // row 0 / C of row n-1 are loaded but not used.
+    constexpr int MA = AHEAD ? M : 1;
+    double nA[MA][MA], nB[MA][MA], nC[MA][MA], nd[MA];
+    auto fetch = [&](int64_t g) {
+        if constexpr (AHEAD) {
+            load_mat_c<M>(Ar, am, g, nA);
+            load_mat_c<M>(B, am, g, nB);
+            load_mat_c<M>(Cr, am, g, nC);
+            load_vec<M>(D, av, g, nd);
+        }
+    };
+    fetch(g0);
     bool ok = true;
     for (int t = 0; t < L; ++t) {
         const int64_t g = g0 + t * dg;
         double R[M][M], v[M];
-        load_mat<M>(B, am, g, R);
-        load_vec<M>(D, av, g, v);
-        if (t > 0) gemm_sub<M>(Ar, am, g, Cp, dp, R, v);
         int piv[M];
-        ok = lu_factor<M>(R, piv) && ok;
-        lu_solve<M>(R, piv, v);
+        if constexpr (AHEAD) {
+            double cA[M][M], cC[M][M];
 #pragma unroll
-        for (int i = 0; i < M; ++i) dp[i] = v[i];
-        store_vec<M>(D, av, g, dp);
+            for (int i = 0; i < M; ++i) {
+                v[i] = nd[i];
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-            double c[M];
-            load_col<M>(Cr, am, g, j, c);
-            lu_solve<M>(R, piv, c);
+                for (int j = 0; j < M; ++j) {
+                    R[i][j] = nB[i][j];
+                    cA[i][j] = nA[i][j];
+                    cC[i][j] = nC[i][j];
+                }
+            }
+            fetch(g + dg);
+            if (t > 0) gemm_sub_reg<M>(cA, Cp, dp, R, v);
+            ok = lu_factor<M>(R, piv) && ok;
+            lu_solve<M>(R, piv, v);
 #pragma unroll
-            for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
-            store_col<M>(C, am, g, j, c);
+            for (int i = 0; i < M; ++i) dp[i] = v[i];
+            store_vec<M>(D, av, g, dp);
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double c[M];
+#pragma unroll
+                for (int i = 0; i < M; ++i) c[i] = cC[i][j];
+                lu_solve<M>(R, piv, c);
+#pragma unroll
+                for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
+                store_col<M>(C, am, g, j, c);
+            }
+        } else {
+            load_mat<M>(B, am, g, R);
+            load_vec<M>(D, av, g, v);
+            if (t > 0) gemm_sub<M>(Ar, am, g, Cp, dp, R, v);
+            ok = lu_factor<M>(R, piv) && ok;
+            lu_solve<M>(R, piv, v);
+#pragma unroll
+            for (int i = 0; i < M; ++i) dp[i] = v[i];
+            store_vec<M>(D, av, g, dp);
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double c[M];
+                load_col<M>(Cr, am, g, j, c);
+                lu_solve<M>(R, piv, c);
+#pragma unroll
+                for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
+                store_col<M>(C, am, g, j, c);
+            }
         }
     }
     // the meeting row: each wavefront posts its carry and reads its partner's
@@ -455,10 +515,26 @@ __global__ __launch_bounds__(2 * kTile) void k_plain_halves(const double* __rest
     double xh[M];
     {
         double R[M][M];
-        load_mat<M>(B, am, h, R);
-        load_vec<M>(D, av, h, xh);
-        if (h > 0) gemm_sub<M>(A, am, h, Yt, yt, R, xh);
-        if (h < nrow - 1) gemm_sub<M>(C, am, h, Yb, yb, R, xh);
+        if constexpr (AHEAD) {
+            double Ah[M][M], Ch[M][M];  // the top wavefront fetched A_h as its Ar, the bottom one as Cr
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                xh[i] = nd[i];
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    R[i][j] = nB[i][j];
+                    Ah[i][j] = top ? nA[i][j] : nC[i][j];
+                    Ch[i][j] = top ? nC[i][j] : nA[i][j];
+                }
+            }
+            if (h > 0) gemm_sub_reg<M>(Ah, Yt, yt, R, xh);
+            if (h < nrow - 1) gemm_sub_reg<M>(Ch, Yb, yb, R, xh);
+        } else {
+            load_mat<M>(B, am, h, R);
+            load_vec<M>(D, av, h, xh);
+            if (h > 0) gemm_sub<M>(A, am, h, Yt, yt, R, xh);
+            if (h < nrow - 1) gemm_sub<M>(C, am, h, Yb, yb, R, xh);
+        }
         int piv[M];
         ok = lu_factor<M>(R, piv) && ok;
         lu_solve<M>(R, piv, xh);
@@ -469,11 +545,29 @@ __global__ __launch_bounds__(2 * kTile) void k_plain_halves(const double* __rest
     double xn[M];
 #pragma unroll
     for (int i = 0; i < M; ++i) xn[i] = xh[i];
+    double px[MA], pc[MA][MA];
+    auto fetch_back = [&](int64_t g) {
+        if constexpr (AHEAD) {
+            load_vec<M>(D, av, g, px);
+            load_mat_c<M>(static_cast<const double*>(C), am, g, pc);
+        }
+    };
+    if (L > 0) fetch_back(g0 + (L - 1) * dg);
     for (int t = L - 1; t >= 0; --t) {
         const int64_t g = g0 + t * dg;
         double x[M], c[M][M];
-        load_vec<M>(D, av, g, x);
-        load_mat<M>(static_cast<const double*>(C), am, g, c);
+        if constexpr (AHEAD) {
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                x[i] = px[i];
+#pragma unroll
+                for (int j = 0; j < M; ++j) c[i][j] = pc[i][j];
+            }
+            if (t > 0) fetch_back(g - dg);
+        } else {
+            load_vec<M>(D, av, g, x);
+            load_mat<M>(static_cast<const double*>(C), am, g, c);
+        }
 #pragma unroll
         for (int i = 0; i < M; ++i)
 #pragma unroll
@@ -2357,35 +2451,42 @@ void launch_plain_lane(int layout, const double* A, const double* B, const doubl
 }
 
 // Batches on which the single-partition solve runs two lanes per system from both ends with
-// the blocks in registers (k_plain_halves): m = 3, 5, 6, 7 from 16384 systems, m = 2, 4 from
-// 32768; below 131072 systems at m = 2-4, below 98304 at m = 5, at every size from m = 6.
-// Outside that, the cooperative twisted solve (2m lanes per system) runs below 32768 systems
-// and one lane per system (k_plain) from 32768. Measured on 1024-262144 systems of 256-2048
-// rows, both layouts:
-//   - against the twisted solve below 32768 systems: halves faster from 16384 systems at m = 3,
-//     5, 6, 7 (m = 5 16384 x 2048 12.4 -> 8.6 ms tiled, 15.1 -> 12.5 reference; m = 7 22.5 ->
-//     17.9 ms tiled), slower below 8192 (one lane per half leaves the device idle: m = 5
-//     1024 x 2048 2.1 -> 6.1 ms) and at m = 2, 4, whose twisted wavefronts read whole halves
-//     and quarters of lines (m = 2 16384 x 2048 1.4 -> 2.6 ms);
-//   - against one lane per system from 32768 systems: halves faster while one lane per system
-//     fills at most one wavefront per SIMD (m = 5 32768 x 2048 19.9 -> 13.1 ms tiled, 30.0 ->
-//     16.3 reference; m = 2 5.7 -> 3.1 ms tiled; m = 4 65536 x 1024 11.2 -> 8.9 ms); beyond,
-//     on the same placement of the arrays in HBM (four placements each), m = 5 98304 x 512 1.13x
-//     slower (tiled) and 131072 x 512 1.00-1.04x, m = 3 131072 x 512 0.95-1.00x, m = 2, 4
-//     131072 x 512 1.16-1.18x slower in the reference layout, while m = 6 stays 0.90-1.00x and
-//     m = 7 0.78-0.92x up to 262144 systems (one lane per system at m = 6, 7 runs long
-//     dependent chains at two wavefronts per SIMD). k_plain performs the oracle's operations
-//     bit for bit, so it keeps the sizes where the two are equal.
-#ifndef BTD_PLAIN_HALVES_FROM
-#define BTD_PLAIN_HALVES_FROM 16384
-#endif
+// the blocks in registers (k_plain_halves): from 24576 systems at m = 2, 12288 at m = 3-5, 8192
+// at m = 6 and 16384 at m = 7; below 131072 systems at m = 2-4 and 98304 at m = 5, at every
+// size from m = 6. Below, the cooperative twisted solve (2m lanes per system) runs; above, one
+// lane per system (k_plain). Up to 32768 systems at m <= 5 and below 20480 at m = 6, where a
+// lane's chain of dependent rows bounds the solve rather than HBM, each row's operands are
+// loaded while the previous row is eliminated (AHEAD; at m = 7 those registers spill). Measured
+// on 2048-262144 systems of 256-2048 rows, both layouts:
+//   - against the twisted solve and the two-stage solve at the library's partition count:
+//     m = 5 16384 x 2048 11.0 -> 7.8 ms reference layout (P = 2 -> halves), 12288 x 2048 7.3 ->
+//     5.2 ms tiled; m = 6 12288 x 2048 21.0 -> 13.4 ms and m = 3 16384 x 2048 4.2 -> 3.1 ms reference;
+//     slower below those sizes (the device is idle: m = 5 4096 x 2048 2.8 -> 6.1 ms) and at m = 2
+//     below 24576 (whose twisted wavefronts read whole halves of lines: 16384 x 2048 1.4 -> 1.6);
+//   - loads one row ahead against none: m = 5 16384 x 2048 14.9 -> 7.3 ms reference layout,
+//     7.9 -> 6.8 tiled, 32768 x 2048 18.2 -> 14.3 / 13.2 -> 13.2 ms; m = 3 16384 x 2048 4.3 ->
+//     3.1 ms; m = 6 16384 x 2048 19.0 -> 14.6 / 12.2 -> 11.9 ms, 24576 x 2048 1.07x slower tiled;
+//     from 65536 systems (two wavefronts per SIMD) 1.03-1.05x slower;
+//   - against one lane per system from 32768 systems: faster while one lane per system fills at
+//     most one wavefront per SIMD (m = 5 32768 x 2048 19.9 -> 13.1 ms tiled, 30.0 -> 14.3
+//     reference; m = 2 5.7 -> 3.1 ms tiled; m = 4 65536 x 1024 11.2 -> 8.9 ms); beyond, on the
+//     same placement of the arrays in HBM (four placements each), m = 5 98304 x 512 1.00-1.13x
+//     and 131072 x 512 1.00-1.04x, m = 3 131072 x 512 0.95-1.00x, m = 2, 4 131072 x 512
+//     1.16-1.18x slower in the reference layout, while m = 6 stays 0.90-1.00x and m = 7
+//     0.78-0.92x up to 262144 systems. k_plain performs the oracle's operations bit for bit, so
+//     it keeps the sizes where the two are equal.
 #ifndef BTD_PLAIN_HALVES_M_MASK
 #define BTD_PLAIN_HALVES_M_MASK 0xFC  // bit m set: m = 2 ... 7
 #endif
 template <int M>
 constexpr bool kPlainHalvesM = M <= 7 && ((BTD_PLAIN_HALVES_M_MASK >> M) & 1);
+#ifdef BTD_PLAIN_HALVES_FROM
 template <int M>
-constexpr int kPlainHalvesFrom = (M == 2 || M == 4) ? 32768 : BTD_PLAIN_HALVES_FROM;
+constexpr int kPlainHalvesFrom = BTD_PLAIN_HALVES_FROM;
+#else
+template <int M>
+constexpr int kPlainHalvesFrom = M == 2 ? 24576 : M <= 5 ? 12288 : M == 6 ? 8192 : 16384;
+#endif
 #ifdef BTD_PLAIN_HALVES_BELOW
 template <int M>
 constexpr int kPlainHalvesBelow = BTD_PLAIN_HALVES_BELOW;
@@ -2393,15 +2494,36 @@ constexpr int kPlainHalvesBelow = BTD_PLAIN_HALVES_BELOW;
 template <int M>
 constexpr int kPlainHalvesBelow = M >= 6 ? 0x7fffffff : M == 5 ? 98304 : 131072;
 #endif
+#ifdef BTD_PLAIN_HALVES_AHEAD_BELOW
+template <int M>
+constexpr int kPlainHalvesAheadBelow = M <= 6 ? BTD_PLAIN_HALVES_AHEAD_BELOW : 0;
+#else
+template <int M>
+constexpr int kPlainHalvesAheadBelow = M <= 5 ? 32769 : M == 6 ? 20480 : 0;
+#endif
+
+template <int M, bool AHEAD>
+void launch_plain_halves_as(int layout, const double* A, const double* B, double* C, double* D, int nsys,
+                            int nrow, int* flag, hipStream_t s) {
+    const dim3 grid((nsys + kTile - 1) / kTile), block(2 * kTile);
+    if (layout == LAYOUT_TILED)
+        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_TILED, AHEAD>), grid, block, 0, s, A, B, C, D, nsys, nrow,
+                           flag);
+    else
+        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_REF_GPU, AHEAD>), grid, block, 0, s, A, B, C, D, nsys,
+                           nrow, flag);
+}
 
 template <int M>
 void launch_plain_halves(int layout, const double* A, const double* B, double* C, double* D, int nsys,
                          int nrow, int* flag, hipStream_t s) {
-    const dim3 grid((nsys + kTile - 1) / kTile), block(2 * kTile);
-    if (layout == LAYOUT_TILED)
-        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_TILED>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
-    else
-        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_REF_GPU>), grid, block, 0, s, A, B, C, D, nsys, nrow, flag);
+    if constexpr (kPlainHalvesAheadBelow<M> > 0) {
+        if (nsys < kPlainHalvesAheadBelow<M>) {
+            launch_plain_halves_as<M, true>(layout, A, B, C, D, nsys, nrow, flag, s);
+            return;
+        }
+    }
+    launch_plain_halves_as<M, false>(layout, A, B, C, D, nsys, nrow, flag, s);
 }
 
 template <int M>

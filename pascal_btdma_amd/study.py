@@ -17,9 +17,13 @@ inputs, runs `iterations` profiled solves and writes
 these rows apart from its cuda-cxx / fortran-original rows. With --npart auto every case runs
 with the partition count the library chooses for one GPU (btd_auto_npart) instead of nranks;
 those rows keep the case's nranks and carry mpi_mode "in-hbm-np<P>" and the partitions' row counts.
+With --form twisted or lapack_order, cases solved in one partition run that form of the
+single-partition solve (btd_plan_set_form: two groups of m lanes per system from both ends, or
+one lane per system in the CPU path's operation order) instead of the library's choice, and
+their mpi_mode ends in "-twisted" / "-lapack_order"; cases with more partitions are refused.
 
     python -m pascal_btdma_amd.study --profile-out P.csv --signature-out S.csv [--iterations 10]
-        [--case nranks,n1,n2,n3,m ...] [--npart auto]
+        [--case nranks,n1,n2,n3,m ...] [--npart auto] [--form plain|twisted|lapack_order]
 """
 from __future__ import annotations
 
@@ -57,12 +61,22 @@ def _fmt(x: float, digits: int) -> str:
     return f"{x:.{digits}g}"
 
 
-def run_case(case, iterations: int = 10, device=None, auto_npart: bool = False):
-    """Profile one case; returns (timing rows, signature row) as lists of CSV strings."""
+FORMS = ("plain", "twisted", "lapack_order")
+
+
+def run_case(case, iterations: int = 10, device=None, auto_npart: bool = False, form: str = "plain"):
+    """Profile one case; returns (timing rows, signature row) as lists of CSV strings. `form`
+    (one of FORMS) selects the single-partition solve's form; other than "plain" it needs a
+    case solved in one partition."""
     import torch
     import pascal_btdma_amd as pb
     nranks, n1, n2, n3, m = case
     nsys, nrow = n1 * n2, n3
+    npart = pb._capi.lib().btd_auto_npart(m, nsys, nrow) if auto_npart else nranks
+    if form not in FORMS:
+        raise ValueError(f"form must be one of {FORMS}, got {form!r}")
+    if form != "plain" and npart != 1:
+        raise ValueError(f"form {form!r} needs a case solved in one partition (npart = {npart})")
     dev = device or torch.device("cuda:0")
     lay = pb.Layout.REF_GPU
     nm, nv = nsys * nrow * m * m, nsys * nrow * m
@@ -70,13 +84,16 @@ def run_case(case, iterations: int = 10, device=None, auto_npart: bool = False):
     d = torch.empty(nv, dtype=torch.float64, device=dev)
     pb.generate(pb.GenKind.DIAG, 0, m, nsys, nrow, lay, a, b, c, d)
     a0, c0, d0 = a.clone(), c.clone(), d.clone()
-    npart = pb._capi.lib().btd_auto_npart(m, nsys, nrow) if auto_npart else nranks
     mode = f"{MPI_MODE}-np{npart}" if auto_npart else MPI_MODE
+    if form != "plain":
+        mode += f"-{form}"
     sizes = [pb.partition_1d(0, nrow - 1, npart, r) for r in range(npart)]
     rows = [last - first + 1 for first, last in sizes]
     head = f"btdma,noncyclic,{IMPLEMENTATION},{nranks},{n1},{n2},{n3},{m},{nsys},{min(rows)},{max(rows)}"
     timing, signature = [], None
     with pb.BtdmaGpuPlan().create(m, nsys, nrow, npart, lay) as plan:
+        if form != "plain":
+            plan.set_form(pb.Form.TWISTED if form == "twisted" else pb.Form.LAPACK_ORDER)
         for it in range(iterations):
             a.copy_(a0)
             c.copy_(c0)
@@ -115,12 +132,14 @@ def main(argv=None) -> int:
                     help="nranks,n1,n2,n3,m (repeatable; default: the reference Study's case matrix)")
     ap.add_argument("--npart", choices=["nranks", "auto"], default="nranks",
                     help="partitions per system: the case's rank count, or the library's choice")
+    ap.add_argument("--form", choices=FORMS, default="plain",
+                    help="single-partition solve form (cases in one partition only)")
     args = ap.parse_args(argv)
     cases = [tuple(int(v) for v in c.split(",")) for c in args.case] or STUDY_CASES
     for case in cases:
         if len(case) != 5:
             raise SystemExit(f"--case needs nranks,n1,n2,n3,m, got {case}")
-        timing, signature = run_case(case, args.iterations, auto_npart=args.npart == "auto")
+        timing, signature = run_case(case, args.iterations, auto_npart=args.npart == "auto", form=args.form)
         _append(args.profile_out, PROFILE_HEADER, timing)
         _append(args.signature_out, SIGNATURE_HEADER, [signature])
         print(f"[study] case np={case[0]} {case[1]}x{case[2]}x{case[3]} m={case[4]} done", flush=True)

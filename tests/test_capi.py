@@ -73,7 +73,7 @@ def test_plan_create_rejects_bad_arguments(args, exc):
 
 def test_auto_npart_rule():
     """npart = 0: the single-kernel solve for large batches (8192 systems suffice for m = 2, 4,
-    20480 for m = 3, 5, 4096 for m >= 6); for smaller m <= 5 batches about 16384 partitions of >= 16 rows (a power of
+    12288 for m = 3, 5, 4096 for m >= 6); for smaller m <= 5 batches about 16384 partitions of >= 16 rows (a power of
     two; at least 2 from 8192 systems; none below 128 rows under 8192 systems); for m = 6 the
     65536-lane count as a power of two (none up to 128 rows); for m >= 7 about 65536 lanes of
     >= 8 rows."""
@@ -89,12 +89,13 @@ def test_auto_npart_rule():
     assert auto(6, 4096, 2048) == 1 and auto(6, 2048, 2048) == 32 and auto(6, 700, 300) == 32
     assert auto(6, 1024, 128) == 1
     assert auto(2, 8192, 512) == 1 and auto(4, 8192, 512) == 1
-    assert auto(3, 8192, 512) == 2 and auto(3, 16384, 64) == 2 and auto(5, 16384, 128) == 2
+    assert auto(3, 8192, 512) == 2 and auto(3, 8192, 64) == 2 and auto(5, 12287, 128) == 2
+    assert auto(3, 16384, 64) == 1 and auto(5, 16384, 128) == 1
     assert auto(4, 4096, 2048) == 4 and auto(2, 256, 512) == 32 and auto(3, 256, 2048) == 64
     assert auto(5, 1024, 64) == 1 and auto(8, 256, 127) == 1 and auto(4, 1024, 64) == 1  # short systems
     assert auto(3, 1024, 15) == 1
-    assert auto(3, 20480, 2048) == 1 and auto(5, 20480, 256) == 1 and auto(3, 32767, 512) == 1
-    assert auto(5, 16384, 2048) == 2 and auto(3, 20479, 2048) == 2 and auto(1, 20480, 512) == 2
+    assert auto(3, 12288, 2048) == 1 and auto(5, 16384, 256) == 1 and auto(3, 32767, 512) == 1
+    assert auto(5, 12287, 2048) == 2 and auto(3, 8192, 2048) == 2 and auto(1, 20480, 512) == 2
     for m, nsys, nrow in [(2, 7, 40), (10, 3000, 4096), (5, 1, 2), (3, 9000, 20), (6, 100, 129)]:
         p = auto(m, nsys, nrow)
         assert p == 1 or (p >= 2 and nrow >= 8 * p)

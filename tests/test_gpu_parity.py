@@ -218,6 +218,12 @@ def test_cooperative_plain_edges(cuda, oracle, m, layout):
         assert plan.info().nonzero()[0].tolist() == [0, 33, 69]
 
 
+# where the m <= 7 single-partition solve starts giving every system two lanes from both ends,
+# and the largest batch on which it loads each row's operands one row ahead (none at m = 7)
+HALVES_FROM = {2: 24576, 3: 12288, 4: 12288, 5: 12288, 6: 8192, 7: 16384}
+HALVES_AHEAD_TO = {2: 32768, 3: 32768, 4: 32768, 5: 32768, 6: 20479, 7: 0}
+
+
 def _plain_regimes(m, layout, cus):
     """Batch sizes that reach each kernel shape of the m >= 2 single-partition solve on a
     device of `cus` CUs (wavefronts of work per CU decide it), each a few systems past its
@@ -226,7 +232,10 @@ def _plain_regimes(m, layout, cus):
     if m <= 7:
         lock_from = 2 if 3 <= m <= 6 else 4
         shapes = {"twisted, 1 wavefront": 70, "twisted, 4 locked wavefronts": lock_from * cus * (32 // m) + 3,
-                  "two lanes per system from both ends": (32768 if m in (2, 4) else 16384) + 5}
+                  "two lanes per system from both ends" + (", loads a row ahead" if HALVES_AHEAD_TO[m] else ""):
+                      HALVES_FROM[m] + 5}
+        if HALVES_AHEAD_TO[m]:
+            shapes["two lanes per system from both ends"] = HALVES_AHEAD_TO[m] + 5
         if m <= 5:
             shapes["one lane per system"] = (98304 if m == 5 else 131072) + 5
         return shapes
@@ -263,14 +272,17 @@ def test_plain_large_batch_shapes(cuda, oracle, m, layout):
 @pytest.mark.parametrize("m", [2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("layout", [pb.Layout.REF_GPU, pb.Layout.TILED])
 def test_plain_two_lanes_from_both_ends(cuda, oracle, m, layout):
-    """From 16384 systems (m = 3, 5, 6, 7) or 32768 (m = 2, 4) the single-partition solve gives
-    every system two lanes: one eliminates rows 0..h-1 downward, the other rows n-1..h+1 upward
-    (h = (n-1)/2), they meet at row h through LDS and substitute outward. n = 2 (no top rows),
-    3, 4 (one and two bottom rows), 7 and 64; sampled systems (the first tile, a middle one, the
-    partial last tile) against the oracle."""
+    """From HALVES_FROM[m] systems the single-partition solve gives every system two lanes: one
+    eliminates rows 0..h-1 downward, the other rows n-1..h+1 upward (h = (n-1)/2), they meet at
+    row h through LDS and substitute outward; up to HALVES_AHEAD_TO[m] systems each row's
+    operands are loaded while the previous row is eliminated. Both forms: n = 2 (no top rows), 3,
+    4 (one and two bottom rows), 7 and 64 on the first, 2, 5 and 64 past it; sampled systems
+    (the first tile, a middle one, the partial last tile) against the oracle."""
     import torch
-    nsys = (32768 if m in (2, 4) else 16384) + 5
-    for n in (2, 3, 4, 7, 64):
+    cases = [(HALVES_FROM[m] + 5, n) for n in (2, 3, 4, 7, 64)]
+    if HALVES_AHEAD_TO[m]:
+        cases += [(HALVES_AHEAD_TO[m] + 5, n) for n in (2, 5, 64)]
+    for nsys, n in cases:
         print(f"[case] m={m} {layout.name} nsys={nsys} n={n}", flush=True)
         t = [torch.empty(pb.layout_elems(layout, nsys, n, m * m if k < 3 else m), dtype=torch.float64, device=cuda)
              for k in range(4)]
@@ -280,7 +292,7 @@ def test_plain_two_lanes_from_both_ends(cuda, oracle, m, layout):
             torch.cuda.synchronize()
             assert not plan.check()
         full = layout_to_vec(t[3].cpu().numpy(), int(layout), n, nsys, m)
-        for s0, cnt in ((0, 64), (64 * 128, 64), (nsys - 5, 5)):
+        for s0, cnt in ((0, 64), (64 * (nsys // 128), 64), (nsys - 5, 5)):
             A, B, C, d = pb.generate_host(pb.GenKind.DDPERM, 80 + m + n, m, nsys, n, s0, cnt)
             assert rel_err(full[:, s0:s0 + cnt], oracle_plain(oracle, A, B, C, d)).max() <= TOL
 
