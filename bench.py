@@ -45,6 +45,10 @@ WORKLOADS = {
 # diagonal workload 32 x 32 x 128 (nsys 1024, n 128), solve time in ms
 # (Study/result/tables/summary_by_case.csv:2-5).
 STUDY_H200_MS = {2: 2.134, 3: 3.901, 5: 11.581, 8: 28.550}
+# The Study's largest cases, 8 x H200 (one rank each), CUDA C++ port, device MPI mode, diagonal
+# workload 128 x 128 x 2048 (nsys 16384, n 2048), solve time in ms
+# (Study/result/tables/summary_by_case.csv:25,27,29).
+STUDY_LARGE_H200X8_MS = {2: 6.579, 5: 29.902, 8: 84.719}
 
 
 def algorithmic_bytes(m: int, rows: int) -> float:
@@ -221,6 +225,38 @@ def extra_study(dev) -> dict:
                         "h200_ms_published": h200, "speedup_npart1": h200 / res[1],
                         "speedup_best": h200 / res[best],
                         "block_rows_per_s_best": nsys * n / (res[best] * 1e-3)}
+    return out
+
+
+def extra_study_large(dev) -> dict:
+    """The Study's largest cases (diagonal workload, 128 x 128 x 2048, reference layout) for
+    m = 2, 5, 8 on one GPU at the library's partition count, against the published time of the
+    reference on 8 H200s."""
+    import torch
+    import pascal_btdma_amd as pb
+    nsys, n = 16384, 2048
+    lay = pb.Layout.REF_GPU
+    out = {}
+    for m, h200 in STUDY_LARGE_H200X8_MS.items():
+        a, b, c, d = (torch.empty(nsys * n * (m * m if i < 3 else m), dtype=torch.float64, device=dev)
+                      for i in range(4))
+        pb.generate(pb.GenKind.DIAG, 0, m, nsys, n, lay, a, b, c, d)
+        a0, c0, d0 = a.clone(), c.clone(), d.clone()
+
+        def restore():
+            a.copy_(a0)
+            c.copy_(c0)
+            d.copy_(d0)
+        with pb.BtdmaGpuPlan().create(m, nsys, n, 0, lay) as plan:
+            restore()
+            pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n)
+            t = event_time(lambda: pb.solve_noncyclic(plan, a, b, c, d, m, nsys, n), restore, 5,
+                           torch.cuda.current_stream())
+            ms = float(np.median(t))
+            out[f"m{m}"] = {"auto_npart": plan.npart, "ms": ms, "h200x8_ms_published": h200,
+                            "speedup_vs_h200x8": h200 / ms, "block_rows_per_s": nsys * n / (ms * 1e-3)}
+        del a, b, c, d, a0, c0, d0
+        torch.cuda.empty_cache()
     return out
 
 
@@ -936,6 +972,7 @@ def main() -> int:
     if not args.no_extras:
         only = set(filter(None, args.only.split(",")))
         jobs = [("study", lambda: extra_study(dev)),
+                ("study_large", lambda: extra_study_large(dev)),
                 ("small", lambda: extra_small(dev, checks)),
                 ("batch", lambda: extra_batch(dev, 3, checks)),
                 ("f32r", lambda: extra_f32r(dev, 3, ms, checks)),
