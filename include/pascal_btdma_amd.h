@@ -100,8 +100,8 @@ btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npar
 
 /* New: the partition count a plan created with npart = 0 gets on one GPU, where partitions are
  * an occupancy choice rather than a rank count: 1 (the single-kernel solve) for nsys >= 32768
- * at m = 1, 12288 at m = 3, 5 (two lanes per system there), 8192 at m = 2, 4 and 4096 at
- * m >= 6 (two, m or 2m lanes per system). Otherwise, for m <= 5 (both stages two lanes per partition): 1 below 128
+ * at m = 1, 12288 at m = 3, 5 (two lanes per system there), 8192 at m = 2, 4, 4096 at m = 6
+ * and 3072 at m >= 7 (two, m or 2m lanes per system). Otherwise, for m <= 5 (both stages two lanes per partition): 1 below 128
  * rows when nsys < 8192, else about 16384 / nsys partitions (at least 2 from 8192 systems) of at
  * least 16 rows, rounded down to a power of two; for m = 6: 1 up to 128 rows, else about
  * 65536 / nsys of at least 8 rows, a power of two; for m >= 7: 1 below 128 rows, else about

@@ -141,17 +141,19 @@ int btd_auto_npart(int m, int nsys, int nrow) {
     // and 5, whose single-partition solve gives every system two lanes there, one from each end
     // (12288-24576 systems of 512 and 2048 rows, both layouts: 0.70-0.87x the time of the best
     // P in 2-8; at 8192 0.97-1.16x, so P = 2 stays there); 8192 for m = 2 and 4, whose twisted
-    // solve's wavefronts read whole halves or quarters of lines; 4096 for m >= 6, whose
-    // single-partition solve is twisted or gives every system two, m or 2m lanes.
+    // solve's wavefronts read whole halves or quarters of lines; 4096 for m = 6 and 3072 for
+    // m >= 7, whose single-partition solve is twisted or gives every system two, m or 2m lanes
+    // (m = 7-10 at 3072 systems of 128-256 rows, both layouts: the one-lane rule's P = 22 took
+    // 1.08-1.50x the time of P = 1).
     // Smaller batches, measured over m = 2-8, 256-16384 systems, 64-2048 rows (P = 1..128, the
     // best against this rule: geometric mean 1.02x, against the one-lane rule 1.10x):
     //   m <= 5 (stage 1 and 3 twisted, two lanes per partition): about 16384 partitions,
     //     >= 16 rows each, a power of two; at least 2 from 8192 systems; 1 below 128 rows
     //     for fewer than 8192 systems;
     //   m = 6: 1 up to 128 rows, else the one-lane rule's count rounded to a power of two;
-    //   m >= 7: the one-lane rule (auto_npart_one_lane).
+    //   m >= 7: the one-lane rule (auto_npart_one_lane) below 3072 systems.
     if (nsys < 1 || nrow < 2) return 1;
-    if (m >= 7) return auto_npart_one_lane(m, nsys, nrow);
+    if (m >= 7) return nsys >= 3072 ? 1 : auto_npart_one_lane(m, nsys, nrow);
     const int64_t one = m == 6 ? 4096 : (m == 2 || m == 4) ? 8192 : (m == 3 || m == 5) ? 12288 : 32768;
     if ((int64_t)nsys >= one) return 1;
     int64_t want, most;

@@ -73,7 +73,7 @@ def test_plan_create_rejects_bad_arguments(args, exc):
 
 def test_auto_npart_rule():
     """npart = 0: the single-kernel solve for large batches (8192 systems suffice for m = 2, 4,
-    12288 for m = 3, 5, 4096 for m >= 6); for smaller m <= 5 batches about 16384 partitions of >= 16 rows (a power of
+    12288 for m = 3, 5, 4096 for m = 6, 3072 for m >= 7); for smaller m <= 5 batches about 16384 partitions of >= 16 rows (a power of
     two; at least 2 from 8192 systems; none below 128 rows under 8192 systems); for m = 6 the
     65536-lane count as a power of two (none up to 128 rows); for m >= 7 about 65536 lanes of
     >= 8 rows."""
@@ -86,6 +86,7 @@ def test_auto_npart_rule():
     assert auto(8, 4096, 2048) == 1           # m >= 7: m or 2m lanes per system fill the device
     assert auto(8, 2048, 2048) == 32
     assert auto(7, 4096, 2048) == 1 and auto(7, 2048, 2048) == 32 and auto(7, 700, 200) == 25
+    assert auto(7, 3072, 256) == 1 and auto(10, 3072, 2048) == 1 and auto(9, 3071, 256) == 22
     assert auto(6, 4096, 2048) == 1 and auto(6, 2048, 2048) == 32 and auto(6, 700, 300) == 32
     assert auto(6, 1024, 128) == 1
     assert auto(2, 8192, 512) == 1 and auto(4, 8192, 512) == 1
