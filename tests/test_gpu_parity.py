@@ -574,3 +574,40 @@ def test_workspace_form_rules(cuda):
         for bad in (0, 17):
             with pytest.raises(ValueError, match="candidates"):
                 plan.place_workspace(*t, v, candidates=bad)
+
+
+@pytest.mark.parametrize("m,nsys", [(5, 300), (5, 12288 + 5), (5, 32768 + 5), (3, 131072 + 5), (8, 8192 + 5)])
+def test_plain_graph_replays_equal_the_eager_solve(cuda, m, nsys):
+    """The single-partition solve captured into a graph on a side stream (twisted, two lanes
+    per system with and without loads a row ahead, one lane per system, cooperative, by m and
+    batch size) replays to the eager result bit for bit, three times, with the inputs restored
+    eagerly between replays."""
+    import torch
+    n = 24
+    lay = pb.Layout.TILED
+    t0 = [torch.empty(pb.layout_elems(lay, nsys, n, m * m if k < 3 else m), dtype=torch.float64, device=cuda)
+          for k in range(4)]
+    pb.generate(pb.GenKind.DDPERM, 5 + m, m, nsys, n, lay, *t0)
+    t = [x.clone() for x in t0]
+    s1 = torch.cuda.Stream()
+    with pb.BtdmaGpuPlan().create(m, nsys, n, 1, lay) as plan:
+        pb.solve_noncyclic(plan, *t, m, nsys, n)
+        torch.cuda.synchronize()
+        assert not plan.check()
+        eager = t[3].clone()
+        for x, x0 in zip(t, t0):
+            x.copy_(x0)
+        torch.cuda.synchronize()
+        g = pb.Graph()
+        g.begin(s1)
+        pb.solve_noncyclic(plan, *t, m, nsys, n, stream=s1)
+        g.end(s1)
+        for _ in range(3):
+            for x, x0 in zip(t, t0):
+                x.copy_(x0)
+            torch.cuda.synchronize()
+            g.launch(s1)
+            s1.synchronize()
+            assert torch.equal(t[3], eager)
+        g.destroy()
+        assert not plan.check()
