@@ -397,8 +397,8 @@ __device__ __forceinline__ void gemm_sub_reg(const double (&X)[M][M], const doub
 // (q < h), x_q = d''_q - A''_q x_{q-1} (q > h). Half the dependent rows of k_plain per lane for
 // the same bytes (C' / A'' and d' / d'' written and read back once, A not written): for batches
 // too small to fill the device one lane per system. A of row 0 and C of row n-1 are not used.
-template <int M, int LAYOUT, bool AHEAD>
-__global__ __launch_bounds__(2 * kTile, AHEAD ? 1 : 2) void k_plain_halves(const double* __restrict__ A,
+template <int M, int LAYOUT, bool AHEAD, bool LIGHT = false>
+__global__ __launch_bounds__(2 * kTile, (AHEAD || LIGHT) ? 1 : 2) void k_plain_halves(const double* __restrict__ A,
                                                                           const double* __restrict__ B,
                                                                           double* C, double* D, int nsys,
                                                                           int nrow, int* __restrict__ flag) {
@@ -424,15 +424,20 @@ __global__ __launch_bounds__(2 * kTile, AHEAD ? 1 : 2) void k_plain_halves(const
     }
     // AHEAD: the operands of the next row (after the last one, of the meeting row h) are loaded
     // while the current row is eliminated, and the next row's d' and carried block while the
-    // current x is formed. Row h is written by neither wavefront before the meeting, and A of
-    // row 0 / C of row n-1 are loaded but not used.
+    // current x is formed. LIGHT: only the next row's A (its couplings to the carried row) and
+    // d. Row h is written by neither wavefront before the meeting, and A of row 0 / C of row n-1
+    // are loaded but not used.
     constexpr int MA = AHEAD ? M : 1;
-    double nA[MA][MA], nB[MA][MA], nC[MA][MA], nd[MA];
+    constexpr int ML = (AHEAD || LIGHT) ? M : 1;
+    double nA[ML][ML], nB[MA][MA], nC[MA][MA], nd[ML];
     auto fetch = [&](int64_t g) {
         if constexpr (AHEAD) {
             load_mat_c<M>(Ar, am, g, nA);
             load_mat_c<M>(B, am, g, nB);
             load_mat_c<M>(Cr, am, g, nC);
+            load_vec<M>(D, av, g, nd);
+        } else if constexpr (LIGHT) {
+            load_mat_c<M>(Ar, am, g, nA);
             load_vec<M>(D, av, g, nd);
         }
     };
@@ -466,6 +471,49 @@ __global__ __launch_bounds__(2 * kTile, AHEAD ? 1 : 2) void k_plain_halves(const
                 double c[M];
 #pragma unroll
                 for (int i = 0; i < M; ++i) c[i] = cC[i][j];
+                lu_solve<M>(R, piv, c);
+#pragma unroll
+                for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
+                store_col<M>(C, am, g, j, c);
+            }
+        } else if constexpr (LIGHT) {
+            // A and d of this row were loaded during the previous row: A C' is formed while B
+            // arrives, then the next row's A and d are requested
+            load_mat<M>(B, am, g, R);
+            double S[M][M], sv[M];
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                v[i] = nd[i];
+                sv[i] = 0.0;
+#pragma unroll
+                for (int j = 0; j < M; ++j) S[i][j] = 0.0;
+            }
+            if (t > 0) {
+#pragma unroll
+                for (int k = 0; k < M; ++k)
+#pragma unroll
+                    for (int i = 0; i < M; ++i) {
+                        sv[i] = fma(nA[i][k], dp[k], sv[i]);
+#pragma unroll
+                        for (int j = 0; j < M; ++j) S[i][j] = fma(nA[i][k], Cp[k][j], S[i][j]);
+                    }
+            }
+            fetch(g + dg);
+#pragma unroll
+            for (int i = 0; i < M; ++i) {
+                v[i] -= sv[i];
+#pragma unroll
+                for (int j = 0; j < M; ++j) R[i][j] -= S[i][j];
+            }
+            ok = lu_factor<M>(R, piv) && ok;
+            lu_solve<M>(R, piv, v);
+#pragma unroll
+            for (int i = 0; i < M; ++i) dp[i] = v[i];
+            store_vec<M>(D, av, g, dp);
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                double c[M];
+                load_col<M>(Cr, am, g, j, c);
                 lu_solve<M>(R, piv, c);
 #pragma unroll
                 for (int i = 0; i < M; ++i) Cp[i][j] = c[i];
@@ -2456,7 +2504,10 @@ void launch_plain_lane(int layout, const double* A, const double* B, const doubl
 // size from m = 6. Below, the cooperative twisted solve (2m lanes per system) runs; above, one
 // lane per system (k_plain). Up to 32768 systems at m <= 5 and below 20480 at m = 6, where a
 // lane's chain of dependent rows bounds the solve rather than HBM, each row's operands are
-// loaded while the previous row is eliminated (AHEAD; at m = 7 those registers spill). Measured
+// loaded while the previous row is eliminated (AHEAD; at m = 7 those registers spill, so below
+// 32768 systems m = 7 loads only A and d ahead and forms A C' while B arrives: LIGHT, 16384 x
+// 2048 19.0 -> 16.0 ms reference layout, 17.4 -> 14.7 tiled; m = 6 1.2-1.3x slower with it than
+// with AHEAD; from 32768 systems within 4% either way). Measured
 // on 2048-262144 systems of 256-2048 rows, both layouts:
 //   - against the twisted solve and the two-stage solve at the library's partition count:
 //     m = 5 16384 x 2048 11.0 -> 7.8 ms reference layout (P = 2 -> halves), 12288 x 2048 7.3 ->
@@ -2501,22 +2552,31 @@ constexpr int kPlainHalvesAheadBelow = M <= 6 ? BTD_PLAIN_HALVES_AHEAD_BELOW : 0
 template <int M>
 constexpr int kPlainHalvesAheadBelow = M <= 5 ? 32769 : M == 6 ? 20480 : 0;
 #endif
+// m = 7 below 32768 systems: only A and d one row ahead (LIGHT; A C' is formed while B arrives)
+template <int M>
+constexpr int kPlainHalvesLightBelow = M == 7 ? 32768 : 0;
 
-template <int M, bool AHEAD>
+template <int M, bool AHEAD, bool LIGHT = false>
 void launch_plain_halves_as(int layout, const double* A, const double* B, double* C, double* D, int nsys,
                             int nrow, int* flag, hipStream_t s) {
     const dim3 grid((nsys + kTile - 1) / kTile), block(2 * kTile);
     if (layout == LAYOUT_TILED)
-        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_TILED, AHEAD>), grid, block, 0, s, A, B, C, D, nsys, nrow,
-                           flag);
-    else
-        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_REF_GPU, AHEAD>), grid, block, 0, s, A, B, C, D, nsys,
+        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_TILED, AHEAD, LIGHT>), grid, block, 0, s, A, B, C, D, nsys,
                            nrow, flag);
+    else
+        hipLaunchKernelGGL((k_plain_halves<M, LAYOUT_REF_GPU, AHEAD, LIGHT>), grid, block, 0, s, A, B, C, D,
+                           nsys, nrow, flag);
 }
 
 template <int M>
 void launch_plain_halves(int layout, const double* A, const double* B, double* C, double* D, int nsys,
                          int nrow, int* flag, hipStream_t s) {
+    if constexpr (kPlainHalvesLightBelow<M> > 0) {
+        if (nsys < kPlainHalvesLightBelow<M>) {
+            launch_plain_halves_as<M, false, true>(layout, A, B, C, D, nsys, nrow, flag, s);
+            return;
+        }
+    }
     if constexpr (kPlainHalvesAheadBelow<M> > 0) {
         if (nsys < kPlainHalvesAheadBelow<M>) {
             launch_plain_halves_as<M, true>(layout, A, B, C, D, nsys, nrow, flag, s);
