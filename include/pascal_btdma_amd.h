@@ -116,7 +116,7 @@ btd_status btd_plan_npart(btd_plan_t plan, int *npart);
  * BTD_FORM_PLAIN (the default) is the library's choice: one lane per system for m = 1, for
  * m = 2-4 from 131072 systems and m = 5 from 98304; two lanes per system, one eliminating
  * downwards from row 0 and one upwards from row nrow-1 with the blocks in registers, meeting in
- * the middle, from 24576 systems at m = 2, 12288 at m = 3-5 and m = 7 and 6144 at m = 6
+ * the middle, from 24576 systems at m = 2, 12288 at m = 3-5, 6144 at m = 6, above 8192 at m = 7
  * (up to those one-lane sizes; at every size from m = 6); for m >= 8 m lanes per system (one
  * block row each); and the twisted form below for batches too small to fill the device those
  * ways: 2 <= m <= 7 below those sizes, m = 8 below 32 systems per CU (reference layout) or 64

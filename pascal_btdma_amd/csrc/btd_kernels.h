@@ -2499,8 +2499,8 @@ void launch_plain_lane(int layout, const double* A, const double* B, const doubl
 }
 
 // Batches on which the single-partition solve runs two lanes per system from both ends with
-// the blocks in registers (k_plain_halves): from 24576 systems at m = 2, 12288 at m = 3-5 and
-// m = 7, 6144 at m = 6; below 131072 systems at m = 2-4 and 98304 at m = 5, at every
+// the blocks in registers (k_plain_halves): from 24576 systems at m = 2, 12288 at m = 3-5, 6144
+// at m = 6 and above 8192 at m = 7; below 131072 systems at m = 2-4 and 98304 at m = 5, at every
 // size from m = 6. Below, the cooperative twisted solve (2m lanes per system) runs; above, one
 // lane per system (k_plain). Up to 32768 systems at m <= 5 and below 20480 at m = 6, where a
 // lane's chain of dependent rows bounds the solve rather than HBM, each row's operands are
@@ -2512,8 +2512,10 @@ void launch_plain_lane(int layout, const double* A, const double* B, const doubl
 //   - against the twisted solve and the two-stage solve at the library's partition count:
 //     m = 5 16384 x 2048 11.0 -> 7.8 ms reference layout (P = 2 -> halves), 12288 x 2048 7.3 ->
 //     5.2 ms tiled; m = 6 12288 x 2048 21.0 -> 13.4 ms and m = 3 16384 x 2048 4.2 -> 3.1 ms reference;
-//     m = 6 6144 x 2048 15.3 -> 12.4 ms, m = 7 12288 x 2048 27.9 -> 15.6 ms reference (the twisted
-//     solve's wavefronts need a second round there; at 8192 systems it is 1.02-1.12x faster);
+//     m = 6 6144 x 2048 15.3 -> 12.4 ms (5120: 1.4x slower), m = 7 8448-12288 x 2048 0.56-0.76x
+//     the time (above 8192 systems the twisted solve's wavefronts, four systems each, exceed two
+//     per SIMD and need a second round: 8448 x 2048 21.4 -> 14.9 ms reference; at 8192 it is
+//     1.02-1.12x faster);
 //     slower below those sizes (the device is idle: m = 5 4096 x 2048 2.8 -> 6.1 ms) and at m = 2
 //     below 24576 (whose twisted wavefronts read whole halves of lines: 16384 x 2048 1.4 -> 1.6);
 //   - loads one row ahead against none: m = 5 16384 x 2048 14.9 -> 7.3 ms reference layout,
@@ -2538,7 +2540,7 @@ template <int M>
 constexpr int kPlainHalvesFrom = BTD_PLAIN_HALVES_FROM;
 #else
 template <int M>
-constexpr int kPlainHalvesFrom = M == 2 ? 24576 : M <= 5 ? 12288 : M == 6 ? 6144 : 12288;
+constexpr int kPlainHalvesFrom = M == 2 ? 24576 : M <= 5 ? 12288 : M == 6 ? 6144 : 8193;
 #endif
 #ifdef BTD_PLAIN_HALVES_BELOW
 template <int M>

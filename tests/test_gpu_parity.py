@@ -221,7 +221,7 @@ def test_cooperative_plain_edges(cuda, oracle, m, layout):
 # where the m <= 7 single-partition solve starts giving every system two lanes from both ends,
 # and the largest batch on which it loads each row's operands one row ahead (at m = 7 only A
 # and d)
-HALVES_FROM = {2: 24576, 3: 12288, 4: 12288, 5: 12288, 6: 6144, 7: 12288}
+HALVES_FROM = {2: 24576, 3: 12288, 4: 12288, 5: 12288, 6: 6144, 7: 8193}
 HALVES_AHEAD_TO = {2: 32768, 3: 32768, 4: 32768, 5: 32768, 6: 20479, 7: 32767}
 
 
