@@ -119,8 +119,8 @@ btd_status btd_plan_npart(btd_plan_t plan, int *npart);
  * the middle, from 24576 systems at m = 2, 12288 at m = 3-5, 6144 at m = 6, above 8192 at m = 7
  * (up to those one-lane sizes; at every size from m = 6); for m >= 8 m lanes per system (one
  * block row each); and the twisted form below for batches too small to fill the device those
- * ways: 2 <= m <= 7 below those sizes, m = 8 below 32 systems per CU (reference layout) or 64
- * (tiled), i.e. 8192 / 16384 systems on 256 CUs, m = 9, 10 below 14 / 12 systems per CU. BTD_FORM_TWISTED (m >= 2, either layout) always gives each system two
+ * ways: 2 <= m <= 7 below those sizes, m = 8 up to 16 systems per CU (reference layout) or 32
+ * (tiled), i.e. 4096 / 8192 systems on 256 CUs, m = 9, 10 below 14 / 12 systems per CU. BTD_FORM_TWISTED (m >= 2, either layout) always gives each system two
  * groups of m lanes, one eliminating downwards from row 0 and one upwards from row nrow-1,
  * meeting in the middle: half the row latency per system (on large batches its wavefronts
  * share cache lines that leave L2 between their reads, so it moves more bytes). Same

@@ -2440,17 +2440,20 @@ void launch_plain_coop_as(const double* A, const double* B, double* C, double* D
 // The single-partition solve for m >= 8, as measured (m = 8-10, 1024-32768 systems of
 // 128-4096 rows, both layouts), by the wavefronts of cooperative work per CU (floor(64/m)
 // systems a wavefront):
-// * m = 8 (two cooperative wavefronts per SIMD), from kCoopLockFrom per CU (bandwidth-bound: 8
-//   tiled, 4 reference): cooperative, locked workgroups of 2 (tiled) or 4 (reference)
-//   wavefronts, 13-33% faster than one unlocked wavefront, reading the compulsory bytes instead
-//   of twice them (16384 x 2048: 43.4 -> 34.2 ms reference, 36.4 -> 24.6 ms tiled); below
-//   that the twisted form (latency-bound: two groups per system halve the sequential rows):
-//   4096 x 4096 30.8 -> 25.2 ms (reference), 20.6 -> 18.2 ms (tiled), 1024 x 2048 10.3 -> 5.7 ms.
+// * m = 8 (two cooperative wavefronts per SIMD), above kCoopLockFrom per CU (2 reference, 4
+//   tiled: above 4096 / 8192 systems on 256 CUs): cooperative, locked workgroups of 2 (tiled)
+//   or 4 (reference) wavefronts, 13-33% faster than one unlocked wavefront, reading the
+//   compulsory bytes instead of twice them (16384 x 2048: 43.4 -> 34.2 ms reference, 36.4 ->
+//   24.6 ms tiled); up to that the twisted form (latency-bound: two groups per system halve the
+//   sequential rows): 4096 x 4096 35.8 -> 23.8 ms (reference), 1024 x 2048 10.3 -> 5.7 ms. The
+//   twisted form's time jumps past those sizes (its wavefronts of four systems need another
+//   round: 4096 -> 5120 x 2048 11.7 -> 20.6 ms reference, 8192 -> 8448 14.0 -> 20.1 ms tiled),
+//   where the cooperative one is 0.82-0.88x its time (5120-7168 reference, 8448-12288 tiled).
 // * m >= 9 (one wavefront per SIMD): cooperative in unlocked workgroups of 4 wavefronts (5-29%
 //   faster than one) from 2 wavefronts of work per CU; below, twisted (1024 x 2048 26.4 ->
 //   13.5 ms reference, 18.5 -> 10.6 ms tiled; 4096 x 1024 stays cooperative, 15.0 vs 17.6 ms).
 #ifndef BTD_COOP_LOCK_FROM
-#define BTD_COOP_LOCK_FROM 4
+#define BTD_COOP_LOCK_FROM 2
 #endif
 template <int LAYOUT>
 constexpr int kCoopLockFrom = (LAYOUT == LAYOUT_TILED ? 2 : 1) * BTD_COOP_LOCK_FROM;
@@ -2465,7 +2468,7 @@ void launch_plain_coop(const double* A, const double* B, double* C, double* D, i
         else
             launch_plain_coop_as<M, LAYOUT, 4, false>(A, B, C, D, nsys, nrow, flag, s);
     } else {
-        if (waves < (int64_t)kCoopLockFrom<LAYOUT> * cus)
+        if (waves <= (int64_t)kCoopLockFrom<LAYOUT> * cus)
             launch_plain_twisted_auto<M, LAYOUT>(A, B, C, D, nsys, nrow, flag, s);
         else if constexpr (LAYOUT == LAYOUT_TILED)
             launch_plain_coop_as<M, LAYOUT, 2, true>(A, B, C, D, nsys, nrow, flag, s);

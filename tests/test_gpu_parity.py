@@ -240,9 +240,9 @@ def _plain_regimes(m, layout, cus):
         if m <= 5:
             shapes["one lane per system"] = (98304 if m == 5 else 131072) + 5
         return shapes
-    if m == 8:
-        lock_from = (8 if layout == pb.Layout.TILED else 4) * cus * g
-        return {"twisted, 4 locked wavefronts": 4 * cus * 4 + 3, "cooperative, locked": lock_from + 5}
+    if m == 8:  # twisted up to 2 (reference layout) / 4 (tiled) cooperative wavefronts per CU
+        lock_from = (4 if layout == pb.Layout.TILED else 2) * cus * g
+        return {"twisted, 4 locked wavefronts": lock_from - 3, "cooperative, locked": lock_from + 5}
     return {"cooperative, 4 wavefronts": 2 * cus * g + 7}
 
 
