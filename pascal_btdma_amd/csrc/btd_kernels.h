@@ -2505,26 +2505,26 @@ void launch_plain_lane(int layout, const double* A, const double* B, const doubl
 // the blocks in registers (k_plain_halves): from 24576 systems at m = 2, 12288 at m = 3-5, 6144
 // at m = 6 and above 8192 at m = 7; below 131072 systems at m = 2-4 and 98304 at m = 5, at every
 // size from m = 6. Below, the cooperative twisted solve (2m lanes per system) runs; above, one
-// lane per system (k_plain). Up to 32768 systems at m <= 5 and below 20480 at m = 6, where a
-// lane's chain of dependent rows bounds the solve rather than HBM, each row's operands are
-// loaded while the previous row is eliminated (AHEAD; at m = 7 those registers spill, so below
-// 32768 systems m = 7 loads only A and d ahead and forms A C' while B arrives: LIGHT, 16384 x
-// 2048 19.0 -> 16.0 ms reference layout, 17.4 -> 14.7 tiled; m = 6 1.2-1.3x slower with it than
-// with AHEAD; from 32768 systems within 4% either way). Measured
-// on 2048-262144 systems of 256-2048 rows, both layouts:
+// lane per system (k_plain). Where a lane's chain of dependent rows bounds the solve rather than
+// HBM, each row's operands are loaded while the previous row is eliminated: up to 32768 systems
+// at m <= 5 and below 20480 at m = 6 (AHEAD); at m = 7 whole rows ahead spill, so below 32768
+// systems only A and d are loaded ahead and A C' is formed while B arrives (LIGHT). Measured on
+// 2048-262144 systems of 256-2048 rows, both layouts:
 //   - against the twisted solve and the two-stage solve at the library's partition count:
-//     m = 5 16384 x 2048 11.0 -> 7.8 ms reference layout (P = 2 -> halves), 12288 x 2048 7.3 ->
-//     5.2 ms tiled; m = 6 12288 x 2048 21.0 -> 13.4 ms and m = 3 16384 x 2048 4.2 -> 3.1 ms reference;
-//     m = 6 6144 x 2048 15.3 -> 12.4 ms (5120: 1.4x slower), m = 7 8448-12288 x 2048 0.56-0.76x
-//     the time (above 8192 systems the twisted solve's wavefronts, four systems each, exceed two
-//     per SIMD and need a second round: 8448 x 2048 21.4 -> 14.9 ms reference; at 8192 it is
-//     1.02-1.12x faster);
-//     slower below those sizes (the device is idle: m = 5 4096 x 2048 2.8 -> 6.1 ms) and at m = 2
-//     below 24576 (whose twisted wavefronts read whole halves of lines: 16384 x 2048 1.4 -> 1.6);
-//   - loads one row ahead against none: m = 5 16384 x 2048 14.9 -> 7.3 ms reference layout,
-//     7.9 -> 6.8 tiled, 32768 x 2048 18.2 -> 14.3 / 13.2 -> 13.2 ms; m = 3 16384 x 2048 4.3 ->
-//     3.1 ms; m = 6 16384 x 2048 19.0 -> 14.6 / 12.2 -> 11.9 ms, 24576 x 2048 1.07x slower tiled;
-//     from 65536 systems (two wavefronts per SIMD) 1.03-1.05x slower;
+//     m = 5 16384 x 2048 11.0 -> 7.8 ms reference layout (P = 2 before), 12288 x 2048 7.3 ->
+//     5.2 ms tiled; m = 3 16384 x 2048 4.2 -> 3.1 ms and m = 6 12288 x 2048 21.0 -> 13.4 ms,
+//     6144 x 2048 15.3 -> 12.4 ms reference (5120: 1.4x slower); m = 7 8448-12288 x 2048
+//     0.56-0.76x the time (above 8192 systems the twisted solve's wavefronts, four systems each,
+//     exceed two per SIMD and need a second round: 8448 x 2048 21.4 -> 14.9 ms reference; at
+//     8192 it is 1.02-1.12x faster); slower below those sizes, where the device is idle (m = 5
+//     4096 x 2048 2.8 -> 6.1 ms), and at m = 2 below 24576, whose twisted wavefronts read whole
+//     halves of lines (16384 x 2048 1.4 -> 1.6 ms);
+//   - loads ahead against none: m = 5 16384 x 2048 14.9 -> 7.3 ms reference layout, 7.9 -> 6.8
+//     tiled, 32768 x 2048 18.2 -> 14.3 / 13.2 -> 13.2 ms; m = 3 16384 x 2048 4.3 -> 3.1 ms; m = 6
+//     16384 x 2048 19.0 -> 14.6 / 12.2 -> 11.9 ms, 24576 x 2048 1.07x slower tiled; from 65536
+//     systems (two wavefronts per SIMD) 1.03-1.05x slower; m = 7 (LIGHT) 16384 x 2048 19.0 ->
+//     16.0 ms reference, 17.4 -> 14.7 tiled, from 32768 systems within 4% (m = 6 LIGHT: 1.2-1.3x
+//     slower than AHEAD);
 //   - against one lane per system from 32768 systems: faster while one lane per system fills at
 //     most one wavefront per SIMD (m = 5 32768 x 2048 19.9 -> 13.1 ms tiled, 30.0 -> 14.3
 //     reference; m = 2 5.7 -> 3.1 ms tiled; m = 4 65536 x 1024 11.2 -> 8.9 ms); beyond, on the
