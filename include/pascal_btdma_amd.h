@@ -120,7 +120,7 @@ btd_status btd_plan_npart(btd_plan_t plan, int *npart);
  * (up to those one-lane sizes; at every size from m = 6); for m >= 8 m lanes per system (one
  * block row each); and the twisted form below for batches too small to fill the device those
  * ways: 2 <= m <= 7 below those sizes, m = 8 up to 16 systems per CU (reference layout) or 32
- * (tiled), i.e. 4096 / 8192 systems on 256 CUs, m = 9, 10 below 14 / 12 systems per CU. BTD_FORM_TWISTED (m >= 2, either layout) always gives each system two
+ * (tiled), i.e. 4096 / 8192 systems on 256 CUs, m = 9, 10 up to 12 systems per CU (3072). BTD_FORM_TWISTED (m >= 2, either layout) always gives each system two
  * groups of m lanes, one eliminating downwards from row 0 and one upwards from row nrow-1,
  * meeting in the middle: half the row latency per system (on large batches its wavefronts
  * share cache lines that leave L2 between their reads, so it moves more bytes). Same

@@ -2449,9 +2449,12 @@ void launch_plain_coop_as(const double* A, const double* B, double* C, double* D
 //   twisted form's time jumps past those sizes (its wavefronts of four systems need another
 //   round: 4096 -> 5120 x 2048 11.7 -> 20.6 ms reference, 8192 -> 8448 14.0 -> 20.1 ms tiled),
 //   where the cooperative one is 0.82-0.88x its time (5120-7168 reference, 8448-12288 tiled).
-// * m >= 9 (one wavefront per SIMD): cooperative in unlocked workgroups of 4 wavefronts (5-29%
-//   faster than one) from 2 wavefronts of work per CU; below, twisted (1024 x 2048 26.4 ->
-//   13.5 ms reference, 18.5 -> 10.6 ms tiled; 4096 x 1024 stays cooperative, 15.0 vs 17.6 ms).
+// * m >= 9 (one wavefront per SIMD): twisted while its wavefronts (floor(32/m) systems each)
+//   fit one per SIMD (up to 3072 systems at m = 9, 10 on 256 CUs: 1024 x 2048 26.4 -> 13.5 ms
+//   reference, 18.5 -> 10.6 ms tiled against cooperative), cooperative in unlocked workgroups
+//   of 4 wavefronts (5-29% faster than one) above (4096 x 1024 15.0 vs 17.6 ms twisted; m = 9
+//   3456 x 2048 28.9 -> 23.7 ms reference, where the twisted wavefronts need a second round;
+//   m = 10 3072 x 2048 stays twisted, 23.8 vs 27.6 ms cooperative).
 #ifndef BTD_COOP_LOCK_FROM
 #define BTD_COOP_LOCK_FROM 2
 #endif
@@ -2461,13 +2464,15 @@ constexpr int kCoopLockFrom = (LAYOUT == LAYOUT_TILED ? 2 : 1) * BTD_COOP_LOCK_F
 template <int M, int LAYOUT>
 void launch_plain_coop(const double* A, const double* B, double* C, double* D, int nsys, int nrow,
                        int* flag, hipStream_t s) {
-    const int64_t waves = ((int64_t)nsys + kTile / M - 1) / (kTile / M), cus = device_cus();
+    const int64_t cus = device_cus();
     if constexpr (M >= 9) {
-        if (waves < 2 * cus)
+        constexpr int W = kTile / 2 / M;  // systems per twisted wavefront
+        if (((int64_t)nsys + W - 1) / W <= 4 * cus)
             launch_plain_twisted_auto<M, LAYOUT>(A, B, C, D, nsys, nrow, flag, s);
         else
             launch_plain_coop_as<M, LAYOUT, 4, false>(A, B, C, D, nsys, nrow, flag, s);
     } else {
+        const int64_t waves = ((int64_t)nsys + kTile / M - 1) / (kTile / M);
         if (waves <= (int64_t)kCoopLockFrom<LAYOUT> * cus)
             launch_plain_twisted_auto<M, LAYOUT>(A, B, C, D, nsys, nrow, flag, s);
         else if constexpr (LAYOUT == LAYOUT_TILED)
