@@ -104,8 +104,9 @@ btd_status btd_plan_create(btd_plan_t *plan, int m, int nsys, int nrow, int npar
  * and 3072 at m >= 7 (two, m or 2m lanes per system). Otherwise, for m <= 5 (both stages two lanes per partition): 1 below 128
  * rows when nsys < 8192, else about 16384 / nsys partitions (at least 2 from 8192 systems) of at
  * least 16 rows, rounded down to a power of two; for m = 6: 1 up to 128 rows, else about
- * 65536 / nsys of at least 8 rows, a power of two; for m >= 7: 1 below 128 rows, else about
- * 65536 / nsys (one wave per SIMD) of at least 8 rows. 1 whenever that is < 2. Calibrated on
+ * 65536 / nsys of at least 8 rows, a power of two; for m >= 7: 1 below 128 rows and for
+ * 1024 systems or more of at most 192 rows, else about 65536 / nsys (one wave per SIMD) of at
+ * least 8 rows. 1 whenever that is < 2. Calibrated on
  * MI355X over m = 2-8, 256-16384 systems, 64-2048 rows. Pure function (no device use). */
 int btd_auto_npart(int m, int nsys, int nrow);
 

@@ -87,6 +87,8 @@ def test_auto_npart_rule():
     assert auto(8, 2048, 2048) == 32
     assert auto(7, 4096, 2048) == 1 and auto(7, 2048, 2048) == 32 and auto(7, 700, 200) == 25
     assert auto(7, 3072, 256) == 1 and auto(10, 3072, 2048) == 1 and auto(9, 3071, 256) == 22
+    assert auto(8, 1024, 128) == 1 and auto(10, 1024, 192) == 1 and auto(8, 1024, 193) == 24
+    assert auto(8, 1023, 128) == 16
     assert auto(6, 4096, 2048) == 1 and auto(6, 2048, 2048) == 32 and auto(6, 700, 300) == 32
     assert auto(6, 1024, 128) == 1
     assert auto(2, 8192, 512) == 1 and auto(4, 8192, 512) == 1
